@@ -1,0 +1,557 @@
+// engine.hip — the reference-resident seed-and-filter engine behind the C ABI
+// (include/hspz.h). Restates hspZ's `Engine` (src/gpu/mod.rs:817-2307) on HIP:
+// one stream per engine, grow-only device buffers, on-device scans, and three
+// blocking reads per max_hits chunk (total hits, survivor count, HSP count)
+// where the reference does four to five.
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/hspz.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace hspz;
+
+namespace hspz {
+size_t dedup_and_order(hspz_segment_pair *h, size_t n);  // host.cpp
+}
+
+#define HIP_TRY(x)                                                                       \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return set_error(-(int)e_, "%s: %s", #x, hipGetErrorString(e_));             \
+    } while (0)
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    template <class T>
+    T *as() const { return (T *)p; }
+};
+
+// run.rs:278-299 batch descriptor
+struct Batch {
+    uint32_t interval;
+    int rev;
+    uint32_t lo, hi;
+};
+
+}  // namespace
+
+struct hspz_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_gate0 = nullptr, ev_gate1 = nullptr;
+    // reference side (resident)
+    DevBuf ref, index, pos, sub;
+    uint32_t ref_len = 0;
+    size_t index_len = 0;
+    int32_t sub_host[64];
+    // query side
+    DevBuf qf, qr;
+    uint32_t query_len = 0;
+    // seed slots (gpu/mod.rs:933-945)
+    DevBuf seeds[2];
+    uint32_t seed_len[2] = {0, 0};
+    // work buffers, grow-only (gpu/mod.rs:1539-1565)
+    DevBuf hit_num, tiles, total, anchors, flags, surv, hsp, done, ids2, reduced, valid;
+    uint32_t *h_small = nullptr;  // pinned scratch for counts
+    // parameters
+    uint32_t seed_size = 19;
+    int32_t xdrop = 910, hspthresh = 3000, noentropy = 0;
+    uint32_t max_hits = 0;
+    // stats
+    uint64_t launches = 0, pipeline_syncs = 0, reference_uploads = 0, query_swaps = 0;
+    uint64_t dev_bytes = 0, peak_dev_bytes = 0;
+    uint64_t gate_ns = 0, gate_hits = 0, gate_launches = 0;
+    int gate_blocks = 2048;
+};
+
+namespace {
+
+int reserve(hspz_engine *h, DevBuf &b, size_t bytes) {
+    if (bytes <= b.cap && b.p) return 0;
+    size_t want = std::max<size_t>(bytes + bytes / 8, 256);
+    if (b.p) {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        HIP_TRY(hipFree(b.p));
+        h->dev_bytes -= b.cap;
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return set_error(HSPZ_E_OOM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    b.cap = want;
+    h->dev_bytes += want;
+    h->peak_dev_bytes = std::max(h->peak_dev_bytes, h->dev_bytes);
+    return 0;
+}
+
+void release(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// Uploads a code-7-padded copy of an encoded sequence: PAD bytes either side.
+int upload_seq(hspz_engine *h, DevBuf &b, const uint8_t *src, size_t len) {
+    if (int rc = reserve(h, b, len + 2 * PAD)) return rc;
+    HIP_TRY(hipMemsetAsync(b.p, 7, len + 2 * PAD, h->stream));
+    if (len) HIP_TRY(hipMemcpyAsync(b.as<uint8_t>() + PAD, src, len, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Inclusive scan of u32[n] in place; the total lands in h->total[0].
+int scan_inclusive(hspz_engine *h, uint32_t *data, uint32_t n) {
+    uint32_t nt = std::max<uint32_t>(ceil_div(n, TILE), 1);
+    if (int rc = reserve(h, h->tiles, (size_t)nt * 4)) return rc;
+    hipLaunchKernelGGL(k_tile_sum_u32, dim3(nt), dim3(TILE_THREADS), 0, h->stream, data, n, h->tiles.as<uint32_t>());
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, h->stream, h->tiles.as<uint32_t>(), nt, h->total.as<uint32_t>());
+    hipLaunchKernelGGL(k_tile_apply_u32, dim3(nt), dim3(TILE_THREADS), 0, h->stream, data, n, h->tiles.as<uint32_t>());
+    h->launches += 3;
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Reads h->total[0] to the host (one blocking round trip).
+int read_total(hspz_engine *h, uint32_t *out) {
+    HIP_TRY(hipMemcpyAsync(h->h_small, h->total.p, 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->pipeline_syncs++;
+    *out = h->h_small[0];
+    return 0;
+}
+
+// Stable compaction of nonzero u8 flags into ids (count first, then emit).
+int compact_flags(hspz_engine *h, const uint8_t *flags, uint32_t n, DevBuf &ids, uint32_t *count) {
+    uint32_t nt = std::max<uint32_t>(ceil_div(n, TILE), 1);
+    if (int rc = reserve(h, h->tiles, (size_t)nt * 4)) return rc;
+    hipLaunchKernelGGL(k_tile_count_u8, dim3(nt), dim3(TILE_THREADS), 0, h->stream, flags, n, h->tiles.as<uint32_t>());
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, h->stream, h->tiles.as<uint32_t>(), nt, h->total.as<uint32_t>());
+    h->launches += 2;
+    HIP_TRY(hipGetLastError());
+    if (int rc = read_total(h, count)) return rc;
+    if (*count == 0) return 0;
+    if (int rc = reserve(h, ids, (size_t)*count * 4)) return rc;
+    hipLaunchKernelGGL(k_tile_emit_u8, dim3(nt), dim3(TILE_THREADS), 0, h->stream, flags, n, h->tiles.as<uint32_t>(), ids.as<uint32_t>());
+    h->launches++;
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// gpu/mod.rs:2397-2420, the KegAlign lower_bound walk. Returns -1 on the
+// reference's underflow (a seed alone beyond max_hits at index 0).
+int chunk_limits(const std::vector<uint32_t> &hit_num, uint32_t max_hits,
+                 std::vector<uint32_t> &out) {
+    uint32_t n = (uint32_t)hit_num.size();
+    uint32_t num_iter = hit_num.back() / max_hits + 1;
+    std::vector<uint32_t> limits;
+    uint32_t lim = max_hits;
+    for (uint32_t it = 0; it + 1 < num_iter; it++) {
+        uint32_t p = (uint32_t)(std::lower_bound(hit_num.begin(), hit_num.end(), lim) - hit_num.begin());
+        if (p == 0) return -1;
+        limits.push_back(p - 1);
+        lim = hit_num[p - 1] + max_hits;
+    }
+    limits.push_back(n - 1);
+    uint32_t s0 = 0, h0 = 0;
+    for (uint32_t p : limits) {
+        out.insert(out.end(), {s0, p, h0, hit_num[p]});
+        s0 = p + 1;
+        h0 = hit_num[p];
+    }
+    return 0;
+}
+
+int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_segment_pair> &res,
+                         uint32_t *num_hits_out, uint32_t *raw_out) {
+    HIP_TRY(hipSetDevice(h->device));
+    uint32_t n = h->seed_len[slot];
+    *num_hits_out = 0;
+    *raw_out = 0;
+    if (n == 0) return 0;
+    const uint8_t *q = (rev ? h->qr : h->qf).as<uint8_t>() + PAD;
+    const uint8_t *r = h->ref.as<uint8_t>() + PAD;
+    const uint64_t *seeds = h->seeds[slot].as<uint64_t>();
+
+    if (int rc = reserve(h, h->hit_num, (size_t)n * 4)) return rc;
+    hipLaunchKernelGGL(k_find_num_hits, dim3(std::min<uint32_t>(ceil_div(n, 256), 4096)), dim3(256), 0, h->stream,
+                       seeds, n, h->index.as<uint32_t>(), h->hit_num.as<uint32_t>());
+    h->launches++;
+    if (int rc = scan_inclusive(h, h->hit_num.as<uint32_t>(), n)) return rc;
+    uint32_t num_hits;
+    if (int rc = read_total(h, &num_hits)) return rc;
+    *num_hits_out = num_hits;
+    if (num_hits == 0) return 0;
+
+    std::vector<uint32_t> chunks;
+    if (num_hits <= h->max_hits) {
+        chunks = {0, n - 1, 0, num_hits};
+    } else {
+        std::vector<uint32_t> cum(n);
+        HIP_TRY(hipMemcpyAsync(cum.data(), h->hit_num.p, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        h->pipeline_syncs++;
+        if (chunk_limits(cum, h->max_hits, chunks))
+            return set_error(HSPZ_E_CHUNK, "a single seed exceeds max_hits %u", h->max_hits);
+    }
+
+    for (size_t c = 0; c < chunks.size(); c += 4) {
+        uint32_t s0 = chunks[c], s1 = chunks[c + 1], h0 = chunks[c + 2], h1 = chunks[c + 3];
+        uint32_t nseeds = s1 + 1 - s0, nh = h1 - h0;
+        if (nh == 0) continue;
+        if (int rc = reserve(h, h->anchors, (size_t)nh * 8)) return rc;
+        if (int rc = reserve(h, h->flags, (size_t)nh + 16)) return rc;
+        // find_hits selector (gpu/mod.rs:48-54): wave per seed when dense
+        if ((uint64_t)nh >= 16ull * nseeds) {
+            hipLaunchKernelGGL(k_find_hits_wave, dim3(ceil_div(nseeds, 4)), dim3(256), 0, h->stream,
+                               h->index.as<uint32_t>(), h->pos.as<uint32_t>(), seeds, h->seed_size,
+                               h->hit_num.as<uint32_t>(), h->anchors.as<uint64_t>(), s0, h0, nseeds);
+        } else {
+            hipLaunchKernelGGL(k_find_hits_thread, dim3(ceil_div(nseeds, 256)), dim3(256), 0, h->stream,
+                               h->index.as<uint32_t>(), h->pos.as<uint32_t>(), seeds, h->seed_size,
+                               h->hit_num.as<uint32_t>(), h->anchors.as<uint64_t>(), s0, h0, nseeds);
+        }
+        h->launches++;
+        uint32_t gblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(nh, GATE_CHUNK * 4), h->gate_blocks));
+        HIP_TRY(hipEventRecord(h->ev_gate0, h->stream));
+        hipLaunchKernelGGL(k_gate, dim3(gblocks), dim3(256), 0, h->stream, r, h->ref_len, q, h->query_len,
+                           h->sub.as<int32_t>(), h->xdrop, h->hspthresh, h->anchors.as<uint64_t>(), nh,
+                           h->flags.as<uint8_t>());
+        HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));
+        h->launches++;
+        HIP_TRY(hipGetLastError());
+        uint32_t nsurv;
+        if (int rc = compact_flags(h, h->flags.as<uint8_t>(), nh, h->surv, &nsurv)) return rc;
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ev_gate0, h->ev_gate1));
+        h->gate_ns += (uint64_t)((double)ms * 1e6);
+        h->gate_hits += nh;
+        h->gate_launches++;
+        if (nsurv == 0) continue;
+        if (int rc = reserve(h, h->hsp, (size_t)nsurv * 16)) return rc;
+        if (int rc = reserve(h, h->done, (size_t)nsurv + 16)) return rc;
+        hipLaunchKernelGGL(k_materialize, dim3(ceil_div(nsurv, 256)), dim3(256), 0, h->stream, r, h->ref_len, q,
+                           h->query_len, h->sub.as<int32_t>(), h->xdrop, h->hspthresh, h->noentropy,
+                           1.3862943649291992, h->anchors.as<uint64_t>(), h->surv.as<uint32_t>(), nsurv,
+                           h->hsp.as<SegmentPair>(), h->done.as<uint8_t>());
+        h->launches++;
+        HIP_TRY(hipGetLastError());
+        uint32_t nraw;
+        if (int rc = compact_flags(h, h->done.as<uint8_t>(), nsurv, h->ids2, &nraw)) return rc;
+        *raw_out += nraw;
+        if (nraw == 0) continue;
+        if (int rc = reserve(h, h->reduced, (size_t)nraw * 16)) return rc;
+        hipLaunchKernelGGL(k_gather_hsp, dim3(ceil_div(nraw, 256)), dim3(256), 0, h->stream,
+                           h->hsp.as<SegmentPair>(), h->ids2.as<uint32_t>(), nraw, h->reduced.as<SegmentPair>());
+        h->launches++;
+        size_t base = res.size();
+        res.resize(base + nraw);
+        HIP_TRY(hipMemcpyAsync(res.data() + base, h->reduced.p, (size_t)nraw * 16, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        h->pipeline_syncs++;
+        size_t kept = dedup_and_order(res.data() + base, nraw);
+        res.resize(base + kept);
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" const char *hspz_last_error(void) { return error_buffer(); }
+
+extern "C" int hspz_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+extern "C" int hspz_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes) {
+    clear_error();
+    HIP_TRY(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return 0;
+}
+
+extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_engine **out) {
+    clear_error();
+    if (!cfg || !out) return set_error(HSPZ_E_ARG, "null config or output");
+    *out = nullptr;
+    int ndev = hspz_device_count();
+    if (ndev <= 0) return set_error(HSPZ_E_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= ndev) return set_error(HSPZ_E_ARG, "device %d out of range", device);
+    if (!cfg->index_table || cfg->index_len == 0 || (cfg->index_len & (cfg->index_len - 1)))
+        return set_error(HSPZ_E_ARG, "index_table must hold 4^k entries");
+    if (!cfg->sub_mat) return set_error(HSPZ_E_ARG, "sub_mat is required (64 entries)");
+    if (cfg->ref_len >= 0xFFFFFFFFull - 2 * PAD) return set_error(HSPZ_E_ARG, "reference longer than u32");
+    if (cfg->pos_len && !cfg->pos_table) return set_error(HSPZ_E_ARG, "null pos_table");
+    if (cfg->pos_len != cfg->index_table[cfg->index_len - 1])
+        return set_error(HSPZ_E_ARG, "pos_table length %zu != last index entry %u", cfg->pos_len,
+                         cfg->index_table[cfg->index_len - 1]);
+    HIP_TRY(hipSetDevice(device));
+    hspz_engine *h = new hspz_engine();
+    h->device = device;
+    auto fail = [&](int rc) {
+        hspz_engine_free(h);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_error(-1, "stream"));
+    if (hipEventCreate(&h->ev_gate0) != hipSuccess || hipEventCreate(&h->ev_gate1) != hipSuccess)
+        return fail(set_error(-1, "events"));
+    if (hipHostMalloc((void **)&h->h_small, 64, hipHostMallocDefault) != hipSuccess)
+        return fail(set_error(HSPZ_E_OOM, "pinned scratch"));
+    h->seed_size = cfg->seed_size;
+    h->xdrop = cfg->xdrop;
+    h->hspthresh = cfg->hspthresh;
+    h->noentropy = cfg->noentropy;
+    memcpy(h->sub_host, cfg->sub_mat, sizeof h->sub_host);
+    if (cfg->max_hits) {
+        h->max_hits = cfg->max_hits;
+    } else {
+        size_t f = 0, t = 0;
+        if (hipMemGetInfo(&f, &t) != hipSuccess) return fail(set_error(-1, "hipMemGetInfo"));
+        h->max_hits = hspz_default_max_hits(t);
+    }
+    int rc;
+    if ((rc = reserve(h, h->index, cfg->index_len * 4))) return fail(rc);
+    if ((rc = reserve(h, h->pos, std::max<size_t>(cfg->pos_len, 1) * 4))) return fail(rc);
+    if ((rc = reserve(h, h->sub, 256))) return fail(rc);
+    if ((rc = reserve(h, h->total, 256))) return fail(rc);
+    if (hipMemcpy(h->index.p, cfg->index_table, cfg->index_len * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        (cfg->pos_len && hipMemcpy(h->pos.p, cfg->pos_table, cfg->pos_len * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(h->sub.p, cfg->sub_mat, 256, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_error(-1, "reference upload failed"));
+    h->index_len = cfg->index_len;
+    h->ref_len = (uint32_t)cfg->ref_len;
+    if ((rc = upload_seq(h, h->ref, cfg->ref_seq, cfg->ref_len))) return fail(rc);
+    h->reference_uploads++;
+    *out = h;
+    return 0;
+}
+
+extern "C" void hspz_engine_free(hspz_engine *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    DevBuf *bufs[] = {&h->ref, &h->index, &h->pos, &h->sub, &h->qf, &h->qr, &h->seeds[0], &h->seeds[1],
+                      &h->hit_num, &h->tiles, &h->total, &h->anchors, &h->flags, &h->surv, &h->hsp,
+                      &h->done, &h->ids2, &h->reduced, &h->valid};
+    for (DevBuf *b : bufs) release(*b);
+    if (h->h_small) (void)hipHostFree(h->h_small);
+    if (h->ev_gate0) (void)hipEventDestroy(h->ev_gate0);
+    if (h->ev_gate1) (void)hipEventDestroy(h->ev_gate1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+extern "C" int hspz_swap_query(hspz_engine *h, const uint8_t *fwd, const uint8_t *rc, size_t len) {
+    clear_error();
+    if (!h || (len && (!fwd || !rc))) return set_error(HSPZ_E_ARG, "null argument");
+    if (len >= 0xFFFFFFFFull - 2 * PAD) return set_error(HSPZ_E_ARG, "query longer than u32");
+    HIP_TRY(hipSetDevice(h->device));
+    if (int e = upload_seq(h, h->qf, fwd, len)) return e;
+    if (int e = upload_seq(h, h->qr, rc, len)) return e;
+    h->query_len = (uint32_t)len;
+    h->seed_len[0] = h->seed_len[1] = 0;
+    h->query_swaps++;
+    return 0;
+}
+
+extern "C" int hspz_upload_seeds(hspz_engine *h, int slot, const uint64_t *seeds, size_t n) {
+    clear_error();
+    if (!h || slot < 0 || slot > 1 || (n && !seeds)) return set_error(HSPZ_E_ARG, "bad slot or seeds");
+    if (n > 0xFFFFFFFFull) return set_error(HSPZ_E_ARG, "more than 2^32 seeds");
+    HIP_TRY(hipSetDevice(h->device));
+    for (size_t i = 0; i < n; i++)
+        if ((seeds[i] >> 32) >= h->index_len) return set_error(HSPZ_E_ARG, "seed %zu has k-mer beyond the index", i);
+    if (int rc = reserve(h, h->seeds[slot], std::max<size_t>(n, 1) * 8)) return rc;
+    if (n) HIP_TRY(hipMemcpyAsync(h->seeds[slot].p, seeds, n * 8, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->seed_len[slot] = (uint32_t)n;
+    return 0;
+}
+
+extern "C" int hspz_generate_seeds(hspz_engine *h, int slot, int rev, uint32_t start, uint32_t end,
+                                   const uint32_t *shape_pos, uint32_t kmer_size, int transitions,
+                                   uint32_t *n_out) {
+    clear_error();
+    if (!h || slot < 0 || slot > 1 || !shape_pos || !n_out) return set_error(HSPZ_E_ARG, "bad argument");
+    if (kmer_size < 4 || kmer_size > 15) return set_error(HSPZ_E_SHAPE, "kmer_size %u outside 4..=15", kmer_size);
+    if (((size_t)1 << (2 * kmer_size)) != h->index_len)
+        return set_error(HSPZ_E_SHAPE, "device seed shape does not match the index");
+    if (shape_pos[kmer_size - 1] + 1 > h->seed_size)
+        return set_error(HSPZ_E_SHAPE, "device seed shape does not match Engine seed size");
+    HIP_TRY(hipSetDevice(h->device));
+    *n_out = 0;
+    h->seed_len[slot] = 0;
+    uint32_t span = end > start ? end - start : 0;
+    if (span == 0) return 0;
+    if ((uint64_t)end + h->seed_size - 1 > h->query_len)
+        return set_error(HSPZ_E_RANGE, "device seed range [%u, %u) exceeds query length %u", start, end, h->query_len);
+    const uint8_t *q = (rev ? h->qr : h->qf).as<uint8_t>() + PAD;
+    if (int rc = reserve(h, h->valid, (size_t)span + 16)) return rc;
+    hipLaunchKernelGGL(k_seed_valid, dim3(std::min<uint32_t>(ceil_div(span, 256), 4096)), dim3(256), 0, h->stream,
+                       q, start, span, h->seed_size, h->valid.as<uint8_t>());
+    h->launches++;
+    uint32_t nvalid;
+    if (int rc = compact_flags(h, h->valid.as<uint8_t>(), span, h->ids2, &nvalid)) return rc;
+    uint32_t per = transitions ? 1 + kmer_size : 1;
+    uint64_t nseeds = (uint64_t)nvalid * per;
+    if (nseeds > 0xFFFFFFFFull) return set_error(HSPZ_E_ARG, "more than 2^32 seeds");
+    if (nvalid) {
+        if (int rc = reserve(h, h->seeds[slot], nseeds * 8)) return rc;
+        ShapeArg sa;
+        memset(&sa, 0, sizeof sa);
+        memcpy(sa.pos, shape_pos, kmer_size * 4);
+        sa.kmer_size = kmer_size;
+        hipLaunchKernelGGL(k_seed_scatter, dim3(ceil_div(nvalid, 256)), dim3(256), 0, h->stream, q, start,
+                           h->ids2.as<uint32_t>(), nvalid, sa, transitions, h->seeds[slot].as<uint64_t>());
+        h->launches++;
+        HIP_TRY(hipGetLastError());
+    }
+    h->seed_len[slot] = (uint32_t)nseeds;
+    *n_out = (uint32_t)nseeds;
+    return 0;
+}
+
+extern "C" int hspz_seed_and_filter(hspz_engine *h, int slot, int rev, hspz_filter_output *out) {
+    clear_error();
+    if (!h || !out || slot < 0 || slot > 1) return set_error(HSPZ_E_ARG, "bad argument");
+    memset(out, 0, sizeof *out);
+    if (h->query_len == 0)
+        return set_error(HSPZ_E_NO_QUERY, "Engine has no query: swap_query() must be called before seed_and_filter()");
+    std::vector<hspz_segment_pair> res;
+    uint32_t nh = 0, raw = 0;
+    int rc = seed_and_filter_impl(h, slot, rev, res, &nh, &raw);
+    if (rc) return rc;
+    out->num_hits = nh;
+    out->raw_hsps = raw;
+    out->n_hsps = res.size();
+    if (!res.empty()) {
+        out->hsps = (hspz_segment_pair *)aligned_alloc(16, ((res.size() * 16 + 15) / 16) * 16);
+        if (!out->hsps) return set_error(HSPZ_E_OOM, "host output allocation");
+        memcpy(out->hsps, res.data(), res.size() * 16);
+    }
+    return 0;
+}
+
+extern "C" void hspz_filter_output_free(hspz_filter_output *out) {
+    if (!out) return;
+    free(out->hsps);
+    out->hsps = nullptr;
+    out->n_hsps = 0;
+}
+
+extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
+    clear_error();
+    if (!h || !value) return set_error(HSPZ_E_ARG, "null argument");
+    switch (which) {
+    case HSPZ_STAT_MAX_HITS: *value = h->max_hits; break;
+    case HSPZ_STAT_LAUNCHES: *value = h->launches; break;
+    case HSPZ_STAT_PIPELINE_SYNCS: *value = h->pipeline_syncs; break;
+    case HSPZ_STAT_REFERENCE_UPLOADS: *value = h->reference_uploads; break;
+    case HSPZ_STAT_QUERY_SWAPS: *value = h->query_swaps; break;
+    case HSPZ_STAT_PEAK_DEVICE_BYTES: *value = h->peak_dev_bytes; break;
+    case HSPZ_STAT_GATE_NS: *value = h->gate_ns; break;
+    case HSPZ_STAT_GATE_HITS: *value = h->gate_hits; break;
+    case HSPZ_STAT_GATE_LAUNCHES: *value = h->gate_launches; break;
+    default: return set_error(HSPZ_E_ARG, "unknown stat %d", which);
+    }
+    return 0;
+}
+
+// seed_and_filter_all (run.rs:266-495) over the engine's current query:
+// batches per interval (plus chunks, then minus chunks over the mirrored
+// range), device seeding, results concatenated per interval in batch order.
+extern "C" int hspz_run_pass(hspz_engine *h, const hspz_pass_params *p, hspz_segment_pair **hsps,
+                             size_t **counts, size_t *n_intervals, hspz_pass_stats *stats) {
+    clear_error();
+    if (!h || !p || !hsps || !counts || !n_intervals) return set_error(HSPZ_E_ARG, "null argument");
+    *hsps = nullptr;
+    *counts = nullptr;
+    *n_intervals = 0;
+    if (h->query_len == 0) return set_error(HSPZ_E_NO_QUERY, "Engine has no query");
+    Shape s;
+    if (int rc = parse_shape(p->pattern ? p->pattern : "12of19", &s)) return rc;
+    if (s.size != h->seed_size) return set_error(HSPZ_E_SHAPE, "seed shape span %u != engine seed size %u", s.size, h->seed_size);
+    if (p->wga_chunk == 0 || p->interval_size == 0) return set_error(HSPZ_E_ARG, "zero chunk or interval size");
+    uint32_t bl = h->query_len;
+    std::vector<std::pair<uint32_t, uint32_t>> iv;
+    if (bl > s.size) {
+        uint32_t end_pos = bl - s.size;
+        for (uint64_t c = 0; c < end_pos; c += p->interval_size)
+            iv.push_back({(uint32_t)c, (uint32_t)std::min<uint64_t>(end_pos, c + p->interval_size)});
+    }
+    uint32_t qbl = bl > s.size ? bl - s.size : 0;
+    std::vector<Batch> batches;
+    auto add_chunks = [&](uint32_t i, int rev, uint32_t a, uint32_t b) {
+        for (uint64_t x = a; x < b; x += p->wga_chunk)
+            batches.push_back({i, rev, (uint32_t)x, (uint32_t)std::min<uint64_t>(x + p->wga_chunk, (uint64_t)b + 1)});
+    };
+    for (uint32_t i = 0; i < iv.size(); i++) {
+        if (p->plus) add_chunks(i, 0, iv[i].first, iv[i].second);
+        if (p->minus) add_chunks(i, 1, qbl - iv[i].second, qbl - iv[i].first);
+    }
+    std::vector<std::vector<hspz_segment_pair>> fw(iv.size()), rc(iv.size());
+    hspz_pass_stats st = {0, 0, 0, 0, 0};
+    for (const Batch &b : batches) {
+        uint32_t ns = 0;
+        if (int e = hspz_generate_seeds(h, 0, b.rev, b.lo, b.hi, s.pos, s.kmer_size, p->transitions, &ns)) return e;
+        st.batches++;
+        if (ns == 0) continue;
+        st.seeds += ns;
+        std::vector<hspz_segment_pair> res;
+        uint32_t nh = 0, raw = 0;
+        if (int e = seed_and_filter_impl(h, 0, b.rev, res, &nh, &raw)) return e;
+        st.hits += nh;
+        st.raw_hsps += raw;
+        auto &dst = b.rev ? rc[b.interval] : fw[b.interval];
+        dst.insert(dst.end(), res.begin(), res.end());
+    }
+    size_t total = 0;
+    for (size_t i = 0; i < iv.size(); i++) total += fw[i].size() + rc[i].size();
+    st.hsps = total;
+    hspz_segment_pair *all = (hspz_segment_pair *)aligned_alloc(16, std::max<size_t>(total, 1) * 16);
+    size_t *cnt = (size_t *)malloc(std::max<size_t>(iv.size(), 1) * 2 * sizeof(size_t));
+    if (!all || !cnt) {
+        free(all);
+        free(cnt);
+        return set_error(HSPZ_E_OOM, "host output allocation");
+    }
+    size_t at = 0;
+    for (size_t i = 0; i < iv.size(); i++) {
+        memcpy(all + at, fw[i].data(), fw[i].size() * 16);
+        at += fw[i].size();
+        memcpy(all + at, rc[i].data(), rc[i].size() * 16);
+        at += rc[i].size();
+        cnt[2 * i] = fw[i].size();
+        cnt[2 * i + 1] = rc[i].size();
+    }
+    *hsps = all;
+    *counts = cnt;
+    *n_intervals = iv.size();
+    if (stats) *stats = st;
+    return 0;
+}

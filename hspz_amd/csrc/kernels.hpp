@@ -1,0 +1,521 @@
+// kernels.hpp — gfx950 device code for the seed-and-filter path.
+//
+// Wave64 throughout. The kernels restate the reference's device semantics
+// (hspZ src/gpu/kernels.rs) in a different decomposition:
+//   * scans are on-device three-phase tile scans (no host block-sum round trip,
+//     unlike gpu/mod.rs:1621-1673);
+//   * compaction ranks come from wave ballots + mbcnt (kernels.rs:177-253 use
+//     warp ballots the same way, but over 32 lanes and with a host scan);
+//   * the score gate runs one hit per LANE with a per-wave work queue, so lanes
+//     that finish early pick up new hits instead of idling, where the reference
+//     runs one hit per warp with 32-wide scan tiles (kernels.rs:525-1057);
+//   * the materializer runs one survivor per lane with the scalar loop that the
+//     reference's host-model tests prove equal to its tiled scans
+//     (kernels.rs:1914-2557).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hspz {
+
+constexpr int WAVE = 64;
+constexpr int TILE_THREADS = 256;
+constexpr int TILE_ITEMS = 16;
+constexpr uint32_t TILE = TILE_THREADS * TILE_ITEMS;  // 4096 items per tile
+constexpr int SCAN_THREADS = 1024;
+constexpr uint32_t PAD = 64;  // bytes of padding (code 7) before and after each sequence
+
+struct SegmentPair {
+    uint32_t ref_start, query_start, len;
+    int32_t score;
+};
+
+// ----------------------------------------------------------------- helpers
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Inclusive add-scan across the wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        uint32_t up = __shfl_up(v, off, WAVE);
+        if (lane >= (uint32_t)off) v += up;
+    }
+    return v;
+}
+
+// Exclusive block scan of one u32 per thread; the block total goes to *total.
+template <int THREADS>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
+    constexpr int NW = THREADS / WAVE;
+    __shared__ uint32_t wsum[NW + 1];
+    const uint32_t lane = lane_id(), w = threadIdx.x / WAVE;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == WAVE - 1) wsum[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t s = lane < (uint32_t)NW ? wsum[lane] : 0;
+        uint32_t si = wave_incl_scan(s);
+        if (lane < (uint32_t)NW) wsum[lane] = si - s;
+        if (lane == (uint32_t)NW - 1) wsum[NW] = si;
+    }
+    __syncthreads();
+    uint32_t base = wsum[w];
+    *total = wsum[NW];
+    __syncthreads();  // wsum may be reused by the next call
+    return base + inc - v;
+}
+
+// Saturating f64 -> i32 truncation, NaN -> 0 (Rust `as i32`).
+__device__ __forceinline__ int32_t trunc_sat(double v) {
+    if (v != v) return 0;
+    if (v >= 2147483647.0) return 2147483647;
+    if (v <= -2147483648.0) return (int32_t)0x80000000;
+    return (int32_t)v;
+}
+
+__device__ __forceinline__ bool gate_keep(int32_t total, int32_t thresh) {
+    return trunc_sat((double)(float)total) >= thresh;
+}
+
+// Little-endian 8 bytes starting at an arbitrary byte address, from three
+// dword-aligned loads and v_alignbyte. The sequences carry PAD bytes on both
+// sides, so the aligned dwords around any in-range address are readable.
+__device__ __forceinline__ uint64_t load8(const uint8_t *p) {
+    uintptr_t a = (uintptr_t)p;
+    const uint32_t *b = (const uint32_t *)(a & ~(uintptr_t)3);
+    uint32_t sh = (uint32_t)(a & 3);
+    uint32_t d0 = b[0], d1 = b[1], d2 = b[2];
+    uint32_t lo = __builtin_amdgcn_alignbyte(d1, d0, sh);
+    uint32_t hi = __builtin_amdgcn_alignbyte(d2, d1, sh);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// ----------------------------------------------------------------- counts
+
+// find_num_hits (kernels.rs:70-115): bucket size of every seed's k-mer.
+__global__ void k_find_num_hits(const uint64_t *__restrict__ seeds, uint32_t n,
+                                const uint32_t *__restrict__ index, uint32_t *__restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t k = (uint32_t)(seeds[i] >> 32);
+        uint32_t end = index[k];
+        uint32_t start = k ? index[k - 1] : 0;
+        out[i] = end - start;
+    }
+}
+
+// ----------------------------------------------------------------- scans
+
+// Phase 1: per-tile sums of u32 values.
+__global__ __launch_bounds__(TILE_THREADS) void k_tile_sum_u32(const uint32_t *__restrict__ in,
+                                                               uint32_t n,
+                                                               uint32_t *__restrict__ tile_sums) {
+    uint32_t base = blockIdx.x * TILE + threadIdx.x * TILE_ITEMS;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; i++)
+        if (base + i < n) s += in[base + i];
+    uint32_t tot;
+    block_excl_scan<TILE_THREADS>(s, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// Phase 1 for u8 flags: count of nonzero bytes per tile.
+__global__ __launch_bounds__(TILE_THREADS) void k_tile_count_u8(const uint8_t *__restrict__ flags,
+                                                                uint32_t n,
+                                                                uint32_t *__restrict__ tile_sums) {
+    uint32_t base = blockIdx.x * TILE + threadIdx.x * TILE_ITEMS;
+    uint32_t s = 0;
+    if (base + TILE_ITEMS <= n) {
+        uint4 v = *(const uint4 *)(flags + base);
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) s += ((w[j] >> (8 * b)) & 0xFF) != 0;
+    } else {
+        for (int i = 0; i < TILE_ITEMS; i++)
+            if (base + i < n) s += flags[base + i] != 0;
+    }
+    uint32_t tot;
+    block_excl_scan<TILE_THREADS>(s, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// Phase 2: exclusive scan of the tile sums in one block; total -> *total.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(uint32_t *__restrict__ sums,
+                                                             uint32_t ntiles,
+                                                             uint32_t *__restrict__ total) {
+    uint32_t per = (ntiles + SCAN_THREADS - 1) / SCAN_THREADS;
+    uint32_t lo = threadIdx.x * per, hi = min(lo + per, ntiles);
+    uint32_t s = 0;
+    for (uint32_t i = lo; i < hi; i++) s += sums[i];
+    uint32_t tot;
+    uint32_t run = block_excl_scan<SCAN_THREADS>(s, &tot);
+    for (uint32_t i = lo; i < hi; i++) {
+        uint32_t v = sums[i];
+        sums[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) *total = tot;
+}
+
+// Phase 3: inclusive scan in place with the tile offsets.
+__global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_u32(uint32_t *__restrict__ data,
+                                                                 uint32_t n,
+                                                                 const uint32_t *__restrict__ offs) {
+    uint32_t base = blockIdx.x * TILE + threadIdx.x * TILE_ITEMS;
+    uint32_t v[TILE_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; i++) {
+        v[i] = base + i < n ? data[base + i] : 0;
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan<TILE_THREADS>(s, &tot) + offs[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; i++) {
+        run += v[i];
+        if (base + i < n) data[base + i] = run;
+    }
+}
+
+// Phase 3 for flags: stable compaction of the indices of nonzero flags.
+// Optionally clears the flags it read (kernels.rs:208-253 clears for reuse).
+__global__ __launch_bounds__(TILE_THREADS) void k_tile_emit_u8(const uint8_t *__restrict__ flags,
+                                                               uint32_t n,
+                                                               const uint32_t *__restrict__ offs,
+                                                               uint32_t *__restrict__ ids) {
+    uint32_t base = blockIdx.x * TILE + threadIdx.x * TILE_ITEMS;
+    uint32_t bits = 0;
+    if (base + TILE_ITEMS <= n) {
+        uint4 v = *(const uint4 *)(flags + base);
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) bits |= (uint32_t)(((w[j] >> (8 * b)) & 0xFF) != 0) << (4 * j + b);
+    } else {
+        for (int i = 0; i < TILE_ITEMS; i++)
+            if (base + i < n && flags[base + i]) bits |= 1u << i;
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan<TILE_THREADS>(__popc(bits), &tot) + offs[blockIdx.x];
+    while (bits) {
+        int i = __ffs(bits) - 1;
+        bits &= bits - 1;
+        ids[run++] = base + i;
+    }
+}
+
+// ----------------------------------------------------------------- hits
+
+// find_hits_dense (kernels.rs:421-465): one seed per thread. Anchor layout
+// (p + S) | ((j + S) << 32) at prefix - 1 - start_hit - idx (kernels.rs:448-462).
+__global__ void k_find_hits_thread(const uint32_t *__restrict__ index,
+                                   const uint32_t *__restrict__ pos,
+                                   const uint64_t *__restrict__ seeds, uint32_t seed_size,
+                                   const uint32_t *__restrict__ prefix,
+                                   uint64_t *__restrict__ anchors, uint32_t start_seed,
+                                   uint32_t start_hit, uint32_t nseeds) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseeds) return;
+    uint64_t sd = seeds[start_seed + s];
+    uint32_t k = (uint32_t)(sd >> 32);
+    uint32_t end = index[k], start = k ? index[k - 1] : 0;
+    uint64_t q = (uint64_t)((uint32_t)sd + seed_size) << 32;
+    uint32_t at = prefix[start_seed + s] - 1 - start_hit;
+    for (uint32_t j = 0; start + j < end; j++) anchors[at - j] = (uint64_t)(pos[start + j] + seed_size) | q;
+}
+
+// find_hits_dense_warp (kernels.rs:470-516) on a 64-lane wave: lanes stride the
+// bucket, so pos_table reads and anchor writes coalesce.
+__global__ __launch_bounds__(256) void k_find_hits_wave(const uint32_t *__restrict__ index,
+                                                        const uint32_t *__restrict__ pos,
+                                                        const uint64_t *__restrict__ seeds,
+                                                        uint32_t seed_size,
+                                                        const uint32_t *__restrict__ prefix,
+                                                        uint64_t *__restrict__ anchors,
+                                                        uint32_t start_seed, uint32_t start_hit,
+                                                        uint32_t nseeds) {
+    uint32_t s = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (s >= nseeds) return;
+    uint32_t lane = lane_id();
+    uint64_t sd = seeds[start_seed + s];
+    uint32_t k = (uint32_t)(sd >> 32);
+    uint32_t end = index[k], start = k ? index[k - 1] : 0;
+    uint64_t q = (uint64_t)((uint32_t)sd + seed_size) << 32;
+    uint32_t at = prefix[start_seed + s] - 1 - start_hit;
+    for (uint32_t j = lane; start + j < end; j += WAVE) anchors[at - j] = (uint64_t)(pos[start + j] + seed_size) | q;
+}
+
+// ----------------------------------------------------------------- gate
+
+// Score gate (kernels.rs:525-1057 computes the same flag): per anchor, the
+// right and left X-drop maxima; keep iff trunc(f64(f32(total))) >= thresh.
+//
+// One hit per lane. A lane walks its hit in groups of 8 cells (right phase,
+// then left phase); when a hit finishes, the lane takes the next hit from the
+// wave's queue (ballot + mbcnt ranks, wave-uniform cursor), so the wave stays
+// full until its range drains. Within a group the scalar recurrence is applied
+// branch-free with an `alive` mask: it is exactly the Appendix loop (strictly
+// greater max, stop at the first cell with max - score > xdrop, or at an edge).
+//
+// The 8x8 substitution matrix lives in LDS replicated 32 times, entry e of copy
+// b at [e * 32 + b], and lane l reads copy l % 32, so the per-cell lookups of a
+// wave spread over all banks.
+constexpr uint32_t GATE_CHUNK = 64 * 8;  // hits a wave takes from the grid at a time
+
+__global__ __launch_bounds__(256) void k_gate(const uint8_t *__restrict__ ref, uint32_t ref_len,
+                                              const uint8_t *__restrict__ qry, uint32_t qlen,
+                                              const int32_t *__restrict__ sub, int32_t xdrop,
+                                              int32_t thresh,
+                                              const uint64_t *__restrict__ anchors, uint32_t n,
+                                              uint8_t *__restrict__ flags) {
+    __shared__ int32_t s_sub[64 * 32];
+    for (uint32_t i = threadIdx.x; i < 64 * 32; i += blockDim.x) s_sub[i] = sub[i >> 5];
+    __syncthreads();
+
+    const uint32_t lane = lane_id();
+    const int32_t *tab = s_sub + (lane & 31);
+    const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
+    uint32_t chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+    uint32_t q_next = chunk * GATE_CHUNK;
+    uint32_t q_end = min(q_next + GATE_CHUNK, n);
+    if (q_next > n) q_next = q_end = n;
+
+    bool has = false;
+    uint32_t hit = 0, R = 0, Q = 0, k = 0;
+    int phase = 0;
+    int32_t c = 0, M = 0, Mr = 0;
+
+    for (;;) {
+        uint64_t need = __ballot(!has);
+        while (need) {
+            if (q_next >= q_end) {
+                chunk += nwaves;
+                uint64_t start = (uint64_t)chunk * GATE_CHUNK;
+                if (start >= n) break;
+                q_next = (uint32_t)start;
+                q_end = min(q_next + GATE_CHUNK, n);
+            }
+            uint32_t avail = q_end - q_next;
+            uint32_t rank = mbcnt(need);
+            if (!has && rank < avail) {
+                hit = q_next + rank;
+                uint64_t a = anchors[hit];
+                R = (uint32_t)a;
+                Q = (uint32_t)(a >> 32);
+                has = true;
+                phase = 0;
+                k = 0;
+                c = 0;
+                M = 0;
+            }
+            uint32_t took = min((uint32_t)__popcll(need), avail);
+            q_next += took;
+            need = __ballot(!has);
+        }
+        if (__ballot(has) == 0) break;
+        if (!has) continue;
+
+        // one group of 8 cells
+        int64_t nv;
+        uint64_t rw, qw;
+        if (phase == 0) {
+            nv = min((int64_t)ref_len - (int64_t)R, (int64_t)qlen - (int64_t)Q) - (int64_t)k;
+            rw = load8(ref + R + k);
+            qw = load8(qry + Q + k);
+        } else {
+            nv = (int64_t)min(R, Q) - (int64_t)k + 1;
+            // cells k..k+7 sit at R-k-7 .. R-k; reverse so cell i is byte i
+            rw = __builtin_bswap64(load8(ref + ((int64_t)R - (int64_t)k - 7)));
+            qw = __builtin_bswap64(load8(qry + ((int64_t)Q - (int64_t)k - 7)));
+        }
+        int nvalid = nv <= 0 ? 0 : (nv >= 8 ? 8 : (int)nv);
+        bool alive = true;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            uint32_t r = (uint32_t)(rw >> (8 * i)) & 7u;
+            uint32_t q = (uint32_t)(qw >> (8 * i)) & 7u;
+            int32_t s = tab[(r * 8 + q) << 5];
+            int32_t cn = (int32_t)((uint32_t)c + (uint32_t)s);
+            int32_t Mn = max(M, cn);
+            bool ok = alive && (i < nvalid);
+            bool drop = (int32_t)((uint32_t)Mn - (uint32_t)cn) > xdrop;
+            c = ok ? cn : c;
+            M = ok ? Mn : M;
+            alive = ok && !drop;
+        }
+        if (alive) {
+            k += 8;
+        } else if (phase == 0) {
+            Mr = M;
+            phase = 1;
+            k = 1;
+            c = 0;
+            M = 0;
+        } else {
+            int32_t total = (int32_t)((uint32_t)Mr + (uint32_t)M);
+            flags[hit] = gate_keep(total, thresh) ? 1 : 0;
+            has = false;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- materializer
+
+// Reference natural log (kernels.rs:1885-1908): atanh series on the mantissa,
+// same operation order; compiled without contraction.
+__device__ __forceinline__ double ref_ln(double x) {
+    const double LN2 = 0.6931471805599453;
+    const double SQRT2 = 1.4142135623730951;
+    uint64_t bits = (uint64_t)__double_as_longlong(x);
+    int32_t e = (int32_t)((bits >> 52) & 0x7FF) - 1023;
+    double m = __longlong_as_double((long long)((bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
+    if (m > SQRT2) {
+        m *= 0.5;
+        e += 1;
+    }
+    double s = (m - 1.0) / (m + 1.0);
+    double z = s * s;
+    double poly = 1.0 + z * (1.0 / 3.0 + z * (1.0 / 5.0 + z * (1.0 / 7.0 + z * (1.0 / 9.0 + z * (1.0 / 11.0 + z * (1.0 / 13.0 + z * (1.0 / 15.0)))))));
+    double a = 2.0 * s;
+    double b = a * poly;
+    double c = (double)e * LN2;
+    return b + c;
+}
+
+// find_hsps (kernels.rs:1063-1723) for the gate's survivors, one per lane:
+// right/left X-drop with earliest-argmax positions, entropy over the closed
+// interval when the score is inside [T, 3T], the keep test and the record.
+__global__ __launch_bounds__(256) void k_materialize(
+    const uint8_t *__restrict__ ref, uint32_t ref_len, const uint8_t *__restrict__ qry,
+    uint32_t qlen, const int32_t *__restrict__ sub, int32_t xdrop, int32_t thresh,
+    int32_t noentropy, double log4, const uint64_t *__restrict__ anchors,
+    const uint32_t *__restrict__ surv, uint32_t nsurv, SegmentPair *__restrict__ hsp,
+    uint8_t *__restrict__ done) {
+    __shared__ int32_t s_sub[64];
+    if (threadIdx.x < 64) s_sub[threadIdx.x] = sub[threadIdx.x];
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsurv) return;
+    uint64_t a = anchors[surv[i]];
+    uint32_t R = (uint32_t)a, Q = (uint32_t)(a >> 32);
+
+    int32_t c = 0, M = 0, rpos = -1;
+    for (uint32_t k = 0;; k++) {
+        if ((uint64_t)R + k >= ref_len || (uint64_t)Q + k >= qlen) break;
+        c = (int32_t)((uint32_t)c + (uint32_t)s_sub[ref[R + k] * 8 + qry[Q + k]]);
+        if (c > M) {
+            M = c;
+            rpos = (int32_t)k;
+        }
+        if ((int32_t)((uint32_t)M - (uint32_t)c) > xdrop) break;
+    }
+    int32_t Mr = M;
+    c = 0;
+    M = 0;
+    int32_t lpos = 0;
+    for (uint32_t k = 1;; k++) {
+        if (R < k || Q < k) break;
+        c = (int32_t)((uint32_t)c + (uint32_t)s_sub[ref[R - k] * 8 + qry[Q - k]]);
+        if (c > M) {
+            M = c;
+            lpos = (int32_t)k;
+        }
+        if ((int32_t)((uint32_t)M - (uint32_t)c) > xdrop) break;
+    }
+    int32_t total = (int32_t)((uint32_t)Mr + (uint32_t)M);
+    int32_t extent = (int32_t)((uint32_t)rpos + (uint32_t)lpos);
+    uint32_t left = (uint32_t)lpos;
+    double entropy = 1.0;
+    if (total >= thresh && (int64_t)total <= 3 * (int64_t)thresh && noentropy == 0) {
+        uint32_t rs = R - left, qs = Q - left;
+        int32_t positions = extent + 1;
+        uint64_t packed = 0;
+        for (int32_t p = 0; p < positions; p++) {
+            uint32_t r = ref[rs + (uint32_t)p], q = qry[qs + (uint32_t)p];
+            if (r == q && r < 4) packed += (uint64_t)1 << (16 * r);
+        }
+        int32_t cnt0 = (int32_t)(packed & 0xFFFF), cnt1 = (int32_t)((packed >> 16) & 0xFFFF);
+        int32_t cnt2 = (int32_t)((packed >> 32) & 0xFFFF), cnt3 = (int32_t)((packed >> 48) & 0xFFFF);
+        if (cnt0 + cnt1 + cnt2 + cnt3 >= 20) {
+            double denom = (double)(extent + 1);
+            int32_t cnt[4] = {cnt0, cnt1, cnt2, cnt3};
+            double acc = 0.0;
+            for (int b = 0; b < 4; b++) {
+                double p = (double)cnt[b] / denom;
+                double l = cnt[b] != 0 ? ref_ln(p) : 0.0;
+                double t = p * l;
+                acc = acc + t;
+            }
+            entropy = -acc / log4;
+        }
+    }
+    double gated = (double)(float)total * entropy;
+    if (trunc_sat(gated) >= thresh) {
+        SegmentPair o;
+        o.ref_start = R - left;
+        o.query_start = Q - left;
+        o.len = (uint32_t)extent;
+        o.score = entropy > 0.0 ? trunc_sat((double)total * entropy) : 0;
+        hsp[i] = o;
+        done[i] = 1;
+    } else {
+        done[i] = 0;
+    }
+}
+
+__global__ void k_gather_hsp(const SegmentPair *__restrict__ in, const uint32_t *__restrict__ ids,
+                             uint32_t n, SegmentPair *__restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[ids[i]];
+}
+
+// ----------------------------------------------------------------- seeding
+
+// seed_kmers (kernels.rs:273-313), validity half: a window is seedable iff all
+// seed_size codes are uppercase ACGT (codes < 4).
+__global__ void k_seed_valid(const uint8_t *__restrict__ q, uint32_t start, uint32_t span,
+                             uint32_t seed_size, uint8_t *__restrict__ valid) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < span; i += gridDim.x * blockDim.x) {
+        const uint8_t *w = q + start + i;
+        uint8_t ok = 1;
+        for (uint32_t j = 0; j < seed_size; j++) ok &= w[j] < 4;
+        valid[i] = ok;
+    }
+}
+
+struct ShapeArg {
+    uint32_t pos[32];
+    uint32_t kmer_size;
+};
+
+// scatter_seeds (kernels.rs:319-355): valid position of rank r owns slots
+// [r*per_pos, (r+1)*per_pos): the exact k-mer, then variants kmer ^ (2 << 2t).
+__global__ void k_seed_scatter(const uint8_t *__restrict__ q, uint32_t start,
+                               const uint32_t *__restrict__ ids, uint32_t nvalid, ShapeArg shape,
+                               int transitions, uint64_t *__restrict__ seeds) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nvalid) return;
+    uint32_t j = start + ids[r];
+    uint32_t kmer = 0;
+    for (uint32_t t = 0; t < shape.kmer_size; t++) kmer = (kmer << 2) + q[j + shape.pos[t]];
+    uint32_t per = transitions ? 1 + shape.kmer_size : 1;
+    uint64_t *o = seeds + (uint64_t)r * per;
+    o[0] = ((uint64_t)kmer << 32) | j;
+    if (transitions)
+        for (uint32_t t = 0; t < shape.kmer_size; t++)
+            o[1 + t] = ((uint64_t)(kmer ^ (2u << (2 * t))) << 32) | j;
+}
+
+}  // namespace hspz

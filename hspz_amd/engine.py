@@ -1,0 +1,290 @@
+"""ctypes binding of the native library (include/hspz.h).
+
+`Engine` mirrors the reference's `Engine` (hspZ src/gpu/mod.rs:817-2307):
+`Engine(...)` = Engine::new, `swap_query`, `upload_seeds`, `generate_seeds`,
+`seed_and_filter`, accessors. `run_pass` is seed_and_filter_all
+(src/run.rs:266-495). Errors raise `HspzError` carrying the library's status
+and message. There is no fallback: if libhspz.so is missing or no GPU is
+visible, construction fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhspz.so")
+
+SP_DTYPE = np.dtype([("ref_start", "<u4"), ("query_start", "<u4"), ("len", "<u4"),
+                     ("score", "<i4")])
+
+STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 3,
+        "query_swaps": 4, "peak_device_bytes": 5, "gate_ns": 6, "gate_hits": 7,
+        "gate_launches": 8}
+
+# Every symbol include/hspz.h declares (checked by tests/test_abi.py).
+EXPORTS = [
+    "hspz_engine_new", "hspz_engine_free", "hspz_swap_query", "hspz_upload_seeds",
+    "hspz_generate_seeds", "hspz_seed_and_filter", "hspz_filter_output_free", "hspz_get_stat",
+    "hspz_default_max_hits", "hspz_device_count", "hspz_device_memory", "hspz_last_error",
+    "hspz_seed_table_build", "hspz_shape_parse", "hspz_chunk_seeds", "hspz_dedup_and_order",
+    "hspz_run_pass", "hspz_free", "hspz_build_sub_mat",
+]
+
+
+class HspzError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__("hspz status %d: %s" % (status, message))
+        self.status = status
+
+
+class EngineConfig(C.Structure):
+    _fields_ = [("index_table", C.c_void_p), ("index_len", C.c_size_t),
+                ("pos_table", C.c_void_p), ("pos_len", C.c_size_t),
+                ("ref_seq", C.c_void_p), ("ref_len", C.c_size_t),
+                ("sub_mat", C.c_void_p), ("seed_size", C.c_uint32), ("xdrop", C.c_int32),
+                ("hspthresh", C.c_int32), ("noentropy", C.c_int32), ("max_hits", C.c_uint32),
+                ("hsp_blocks", C.c_uint32)]
+
+
+class FilterOutput(C.Structure):
+    _fields_ = [("hsps", C.c_void_p), ("n_hsps", C.c_size_t), ("num_hits", C.c_uint32),
+                ("raw_hsps", C.c_uint32)]
+
+
+class PassParams(C.Structure):
+    _fields_ = [("pattern", C.c_char_p), ("transitions", C.c_int32), ("plus", C.c_int32),
+                ("minus", C.c_int32), ("wga_chunk", C.c_uint32), ("interval_size", C.c_uint32)]
+
+
+class PassStats(C.Structure):
+    _fields_ = [("seeds", C.c_uint64), ("hits", C.c_uint64), ("raw_hsps", C.c_uint64),
+                ("hsps", C.c_uint64), ("batches", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Loads libhspz.so (built in-tree by hspz_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HspzError(-1, "native library missing: %s (run hspz_amd.build)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp, u32, i32, sz = C.c_void_p, C.c_uint32, C.c_int32, C.c_size_t
+        L.hspz_last_error.restype = C.c_char_p
+        L.hspz_engine_new.argtypes = [C.c_int, C.POINTER(EngineConfig), C.POINTER(vp)]
+        L.hspz_engine_free.argtypes = [vp]
+        L.hspz_engine_free.restype = None
+        L.hspz_swap_query.argtypes = [vp, vp, vp, sz]
+        L.hspz_upload_seeds.argtypes = [vp, C.c_int, vp, sz]
+        L.hspz_generate_seeds.argtypes = [vp, C.c_int, C.c_int, u32, u32, vp, u32, C.c_int,
+                                          C.POINTER(u32)]
+        L.hspz_seed_and_filter.argtypes = [vp, C.c_int, C.c_int, C.POINTER(FilterOutput)]
+        L.hspz_filter_output_free.argtypes = [C.POINTER(FilterOutput)]
+        L.hspz_filter_output_free.restype = None
+        L.hspz_get_stat.argtypes = [vp, C.c_int, C.POINTER(C.c_uint64)]
+        L.hspz_default_max_hits.argtypes = [C.c_uint64]
+        L.hspz_default_max_hits.restype = u32
+        L.hspz_device_memory.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.hspz_seed_table_build.argtypes = [vp, sz, C.c_char_p, u32, C.c_int, vp, sz,
+                                            C.POINTER(vp), C.POINTER(sz)]
+        L.hspz_shape_parse.argtypes = [C.c_char_p, C.POINTER(u32), C.POINTER(u32), vp]
+        L.hspz_chunk_seeds.argtypes = [vp, sz, C.c_char_p, C.c_int, u32, u32, C.c_int, vp,
+                                       C.POINTER(sz)]
+        L.hspz_dedup_and_order.argtypes = [vp, sz, C.POINTER(sz)]
+        L.hspz_run_pass.argtypes = [vp, C.POINTER(PassParams), C.POINTER(vp), C.POINTER(vp),
+                                    C.POINTER(sz), C.POINTER(PassStats)]
+        L.hspz_build_sub_mat.argtypes = [C.c_char_p, i32, vp, vp]
+        L.hspz_free.argtypes = [vp]
+        L.hspz_free.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(status: int):
+    if status != 0:
+        msg = lib().hspz_last_error()
+        raise HspzError(status, msg.decode() if msg else "")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ------------------------------------------------------------------ host side
+
+def shape_parse(pattern: str = "12of19"):
+    size, k = C.c_uint32(), C.c_uint32()
+    pos = np.zeros(32, dtype=np.uint32)
+    _check(lib().hspz_shape_parse(pattern.encode(), C.byref(size), C.byref(k), _ptr(pos)))
+    return int(size.value), int(k.value), pos[:k.value].copy()
+
+
+def seed_table_build(raw: bytes, pattern: str = "12of19", step: int = 1, threads: int = 8):
+    """SeedTable::build_parallel (seed.rs:207-361): (index_table, pos_table)."""
+    _, k, _ = shape_parse(pattern)
+    a = np.frombuffer(raw, dtype=np.uint8)
+    index = np.zeros(1 << (2 * k), dtype=np.uint32)
+    pp, n = C.c_void_p(), C.c_size_t()
+    _check(lib().hspz_seed_table_build(_ptr(a), len(a), pattern.encode(), step, threads,
+                                       _ptr(index), len(index), C.byref(pp), C.byref(n)))
+    pos = np.empty(n.value, dtype=np.uint32)
+    if n.value:
+        C.memmove(_ptr(pos), pp, n.value * 4)
+    lib().hspz_free(pp)
+    return index, pos
+
+
+def chunk_seeds(raw: bytes, lo: int, hi: int, pattern: str = "12of19", transitions=True,
+                threads: int = 1) -> np.ndarray:
+    _, k, _ = shape_parse(pattern)
+    a = np.frombuffer(raw, dtype=np.uint8)
+    out = np.zeros(max(hi - lo, 1) * (1 + k), dtype=np.uint64)
+    n = C.c_size_t()
+    _check(lib().hspz_chunk_seeds(_ptr(a), len(a), pattern.encode(), int(transitions), lo, hi,
+                                  threads, _ptr(out), C.byref(n)))
+    return out[:n.value].copy()
+
+
+def build_sub_mat(ambiguous: str = "", xdrop: int = 910, acgt=None) -> np.ndarray:
+    """scoring.rs:41-120; acgt = 16 ints (row-major ACGT block) or None for HOXD70."""
+    out = np.zeros(64, dtype=np.int32)
+    blk = None if acgt is None else np.ascontiguousarray(acgt, dtype=np.int32)
+    _check(lib().hspz_build_sub_mat(ambiguous.encode(), xdrop,
+                                    None if blk is None else _ptr(blk), _ptr(out)))
+    return out
+
+
+def dedup_and_order(rows: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(rows, dtype=SP_DTYPE).copy()
+    n = C.c_size_t()
+    _check(lib().hspz_dedup_and_order(_ptr(a), len(a), C.byref(n)))
+    return a[:n.value].copy()
+
+
+def default_max_hits(total_bytes: int) -> int:
+    return int(lib().hspz_default_max_hits(total_bytes))
+
+
+def device_count() -> int:
+    return int(lib().hspz_device_count())
+
+
+# ------------------------------------------------------------------ engine
+
+class Engine:
+    """Reference-resident engine on one device (gpu/mod.rs:1024-1128)."""
+
+    def __init__(self, index_table: np.ndarray, pos_table: np.ndarray, ref_enc: np.ndarray,
+                 sub_mat: np.ndarray, seed_size: int = 19, xdrop: int = 910,
+                 hspthresh: int = 3000, noentropy: bool = False, max_hits: int = 0,
+                 device: int = 0):
+        idx = np.ascontiguousarray(index_table, dtype=np.uint32)
+        pos = np.ascontiguousarray(pos_table, dtype=np.uint32)
+        ref = np.ascontiguousarray(ref_enc, dtype=np.uint8)
+        sub = np.ascontiguousarray(sub_mat, dtype=np.int32)
+        if sub.size != 64:
+            raise ValueError("sub_mat must have 64 entries")
+        cfg = EngineConfig(_ptr(idx).value, len(idx), _ptr(pos).value if len(pos) else None,
+                           len(pos), _ptr(ref).value, len(ref), _ptr(sub).value, seed_size,
+                           xdrop, hspthresh, int(noentropy), max_hits, 0)
+        h = C.c_void_p()
+        _check(lib().hspz_engine_new(device, C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.seed_size = seed_size
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hspz_engine_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def swap_query(self, fwd_enc: np.ndarray, rc_enc: np.ndarray):
+        f = np.ascontiguousarray(fwd_enc, dtype=np.uint8)
+        r = np.ascontiguousarray(rc_enc, dtype=np.uint8)
+        if len(f) != len(r):
+            raise ValueError("forward and reverse-complement blocks differ in length")
+        _check(lib().hspz_swap_query(self._h, _ptr(f), _ptr(r), len(f)))
+
+    def upload_seeds(self, slot: int, seeds: np.ndarray):
+        s = np.ascontiguousarray(seeds, dtype=np.uint64)
+        _check(lib().hspz_upload_seeds(self._h, slot, _ptr(s), len(s)))
+
+    def generate_seeds(self, slot: int, rev: bool, start: int, end: int,
+                       pattern: str = "12of19", transitions: bool = True) -> int:
+        size, k, pos = shape_parse(pattern)
+        n = C.c_uint32()
+        _check(lib().hspz_generate_seeds(self._h, slot, int(rev), start, end, _ptr(pos), k,
+                                         int(transitions), C.byref(n)))
+        return int(n.value)
+
+    def seed_and_filter(self, slot: int, rev: bool):
+        """Returns (hsps structured array, num_hits, raw_hsps)."""
+        out = FilterOutput()
+        _check(lib().hspz_seed_and_filter(self._h, slot, int(rev), C.byref(out)))
+        res = np.zeros(out.n_hsps, dtype=SP_DTYPE)
+        if out.n_hsps:
+            C.memmove(_ptr(res), out.hsps, out.n_hsps * 16)
+        lib().hspz_filter_output_free(C.byref(out))
+        return res, int(out.num_hits), int(out.raw_hsps)
+
+    def stat(self, name: str) -> int:
+        v = C.c_uint64()
+        _check(lib().hspz_get_stat(self._h, STAT[name], C.byref(v)))
+        return int(v.value)
+
+    def run_pass(self, pattern: str = "12of19", transitions: bool = True, plus: bool = True,
+                 minus: bool = True, wga_chunk: int = 250000, interval_size: int = 10000000):
+        """seed_and_filter_all (run.rs:266-495). Returns ([(fw, rc)], stats)."""
+        p = PassParams(pattern.encode(), int(transitions), int(plus), int(minus), wga_chunk,
+                       interval_size)
+        hs, cnt, ni, st = C.c_void_p(), C.c_void_p(), C.c_size_t(), PassStats()
+        _check(lib().hspz_run_pass(self._h, C.byref(p), C.byref(hs), C.byref(cnt),
+                                   C.byref(ni), C.byref(st)))
+        counts = (np.ctypeslib.as_array(C.cast(cnt, C.POINTER(C.c_size_t)),
+                                        (max(2 * ni.value, 1),))[:2 * ni.value].copy())
+        total = int(counts.sum())
+        allsp = np.zeros(total, dtype=SP_DTYPE)
+        if total:
+            C.memmove(_ptr(allsp), hs, total * 16)
+        lib().hspz_free(hs)
+        lib().hspz_free(cnt)
+        res, at = [], 0
+        for i in range(ni.value):
+            fw = allsp[at:at + counts[2 * i]]
+            at += counts[2 * i]
+            rc = allsp[at:at + counts[2 * i + 1]]
+            at += counts[2 * i + 1]
+            res.append((fw, rc))
+        return res, st.as_dict()
+
+
+def hash_pass(intervals) -> int:
+    """benchmark.rs:401-421: FNV-1a over (ref_start, query_start, len, score)
+    little-endian, intervals in order, fw then rc."""
+    h = 0xcbf29ce484222325
+    for fw, rc in intervals:
+        for arr in (fw, rc):
+            b = np.ascontiguousarray(arr, dtype=SP_DTYPE).tobytes()
+            for byte in b:
+                h ^= byte
+                h = (h * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h

@@ -1,0 +1,176 @@
+/*
+ * hspz.h — C ABI of the MI355X seed-and-filter engine.
+ *
+ * Each entry point replaces one method of the reference's `Engine`
+ * (hspZ src/gpu/mod.rs) or one host function the reference's callers use
+ * (src/seed.rs, src/hsp.rs, src/run.rs). Plain pointers and sizes only; no
+ * exceptions cross this boundary. Status codes: 0 = ok, < 0 = HIP/runtime
+ * error, > 0 = API misuse (see HSPZ_E_*). hspz_last_error() returns a
+ * thread-local message for the last failing call on the calling thread.
+ *
+ * Threading: a handle is not thread-safe. One handle per thread; several
+ * handles per process (and per device) are allowed (gpu/mod.rs:817-950,
+ * run.rs:782).
+ */
+#ifndef HSPZ_H
+#define HSPZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HSPZ_OK 0
+#define HSPZ_E_NO_QUERY 1      /* seed_and_filter before swap_query (gpu/mod.rs:1515-1519) */
+#define HSPZ_E_ARG 2           /* bad argument or size */
+#define HSPZ_E_SHAPE 3         /* seed shape mismatch (gpu/mod.rs:1320-1341) */
+#define HSPZ_E_RANGE 4         /* seed range beyond the query (gpu/mod.rs:1326-1331) */
+#define HSPZ_E_CHUNK 5         /* chunk_limits underflow (gpu/mod.rs:2405) */
+#define HSPZ_E_NO_DEVICE 6     /* no HIP device */
+#define HSPZ_E_OOM 7           /* device or host allocation failed / admission refused */
+
+/* hsp.rs:31-41: 16 bytes, align 16, same field order. */
+typedef struct hspz_segment_pair {
+    uint32_t ref_start;
+    uint32_t query_start;
+    uint32_t len;
+    int32_t score;
+} __attribute__((aligned(16))) hspz_segment_pair;
+
+/* gpu/mod.rs:1005-1021 EngineConfig. Host arrays are borrowed for the call
+ * and copied to the device (the reference uploads them in Engine::new,
+ * gpu/mod.rs:1039-1047). ref_seq is the encoded reference block
+ * (sequence.rs:66-79 alphabet, codes 0..7). */
+typedef struct hspz_engine_config {
+    const uint32_t *index_table; /* 4^kmer_size entries, inclusive bucket ends */
+    size_t index_len;
+    const uint32_t *pos_table;
+    size_t pos_len;
+    const uint8_t *ref_seq;
+    size_t ref_len;
+    const int32_t *sub_mat; /* 64 entries, 8x8 row-major (scoring.rs:41-120) */
+    uint32_t seed_size;     /* pattern span, 19 for 12of19 */
+    int32_t xdrop;
+    int32_t hspthresh;
+    int32_t noentropy;
+    uint32_t max_hits; /* 0 = derive from device memory (gpu/mod.rs:2423-2432) */
+    uint32_t hsp_blocks; /* accepted for API parity; 0 = default */
+} hspz_engine_config;
+
+/* gpu/mod.rs:57-67 FilterOutput. hsps is library-allocated; release with
+ * hspz_filter_output_free. */
+typedef struct hspz_filter_output {
+    hspz_segment_pair *hsps;
+    size_t n_hsps;
+    uint32_t num_hits;
+    uint32_t raw_hsps;
+} hspz_filter_output;
+
+typedef struct hspz_engine hspz_engine;
+
+/* ---- Engine (gpu/mod.rs) ---------------------------------------------- */
+
+/* Engine::new (gpu/mod.rs:1024-1128). */
+int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_engine **out);
+void hspz_engine_free(hspz_engine *h);
+
+/* Engine::swap_query (gpu/mod.rs:1142-1148): encoded forward and reverse-
+ * complement query blocks of equal length; copied, no ownership transfer. */
+int hspz_swap_query(hspz_engine *h, const uint8_t *fwd, const uint8_t *rc, size_t len);
+
+/* Engine::upload_seeds (gpu/mod.rs:1246-1309): slot 0 or 1; seeds are
+ * (kmer << 32) | query_position in seed-stream order (seed.rs:385-398).
+ * Copied synchronously, so the host array may be reused on return. */
+int hspz_upload_seeds(hspz_engine *h, int slot, const uint64_t *seeds, size_t n);
+
+/* Engine::generate_seeds (gpu/mod.rs:1312-1463): device seeding of query
+ * positions [start, end) on the forward (rev=0) or reverse-complement block.
+ * shape_pos lists the care positions; every care position admits a
+ * transition (seed.rs:53-56). Writes the seed count to *n_out. */
+int hspz_generate_seeds(hspz_engine *h, int slot, int rev, uint32_t start, uint32_t end,
+                        const uint32_t *shape_pos, uint32_t kmer_size, int transitions,
+                        uint32_t *n_out);
+
+/* Engine::seed_and_filter (gpu/mod.rs:1499-2178): count, scan, chunk by
+ * max_hits, expand hits, score gate, compact survivors, materialize, compact,
+ * dedup_and_order per chunk. Returns with the device drained. */
+int hspz_seed_and_filter(hspz_engine *h, int slot, int rev, hspz_filter_output *out);
+void hspz_filter_output_free(hspz_filter_output *out);
+
+/* Accessors (gpu/mod.rs:1201-1213, 2292-2306). */
+#define HSPZ_STAT_MAX_HITS 0
+#define HSPZ_STAT_LAUNCHES 1
+#define HSPZ_STAT_PIPELINE_SYNCS 2
+#define HSPZ_STAT_REFERENCE_UPLOADS 3
+#define HSPZ_STAT_QUERY_SWAPS 4
+#define HSPZ_STAT_PEAK_DEVICE_BYTES 5
+#define HSPZ_STAT_GATE_NS 6        /* cumulative score-gate kernel time (HIP events) */
+#define HSPZ_STAT_GATE_HITS 7      /* hits the score gate has scored */
+#define HSPZ_STAT_GATE_LAUNCHES 8
+int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
+
+/* resolve_max_hits / device_count / device_memory (gpu/mod.rs:2436-2463). */
+uint32_t hspz_default_max_hits(uint64_t total_device_bytes);
+int hspz_device_count(void);
+int hspz_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
+
+const char *hspz_last_error(void);
+
+/* ---- Host-side path functions (seed.rs, hsp.rs, run.rs) ---------------- */
+
+/* SeedTable::build_parallel (seed.rs:207-361), byte-identical to the serial
+ * build (seed.rs:135-180). raw is the raw reference block (seeding alphabet:
+ * uppercase ACGT only). pattern is "12of19", "14of22" or a 1/0/T string.
+ * index_table must hold 4^kmer_size entries; *pos_table is library-allocated
+ * (free with hspz_free). */
+int hspz_seed_table_build(const uint8_t *raw, size_t n, const char *pattern, uint32_t step,
+                          int threads, uint32_t *index_table, size_t index_len,
+                          uint32_t **pos_table, size_t *pos_len);
+
+/* Shape::parse (seed.rs:47-84): care positions into pos (capacity 32). */
+int hspz_shape_parse(const char *pattern, uint32_t *size, uint32_t *kmer_size, uint32_t *pos);
+
+/* chunk_seeds_parts + concat_parts (seed.rs:428-477): seeds for query
+ * positions [lo, hi) of the raw block; out holds (hi-lo)*(1+kmer_size). */
+int hspz_chunk_seeds(const uint8_t *raw, size_t n, const char *pattern, int transitions,
+                     uint32_t lo, uint32_t hi, int threads, uint64_t *out, size_t *n_out);
+
+/* build_sub_mat (scoring.rs:41-120): the 8x8 matrix over the scoring alphabet.
+ * acgt is the 4x4 ACGT block row-major, or NULL for HOXD70 (scoring.rs:26-31);
+ * ambiguous is "" / "n" / "iupac" / "<n|iupac>,<reward>,<penalty>". */
+int hspz_build_sub_mat(const char *ambiguous, int32_t xdrop, const int32_t *acgt, int32_t *out);
+
+/* dedup_and_order (hsp.rs:94-126), in place; writes the kept count. */
+int hspz_dedup_and_order(hspz_segment_pair *hsps, size_t n, size_t *n_kept);
+
+/* seed_and_filter_all (run.rs:266-495) over one query pass on the engine's
+ * current query: batches per interval (plus chunks then minus chunks), device
+ * seeding, results concatenated per interval in batch order. counts receives
+ * 2 entries per interval (fw, rc); both outputs are library-allocated
+ * (hspz_free). */
+typedef struct hspz_pass_params {
+    const char *pattern;
+    int32_t transitions;
+    int32_t plus;
+    int32_t minus;
+    uint32_t wga_chunk;     /* 250000 (cli.rs:81-82) */
+    uint32_t interval_size; /* 10000000 (cli.rs:83-84) */
+} hspz_pass_params;
+typedef struct hspz_pass_stats {
+    uint64_t seeds;
+    uint64_t hits;
+    uint64_t raw_hsps;
+    uint64_t hsps;
+    uint64_t batches;
+} hspz_pass_stats;
+int hspz_run_pass(hspz_engine *h, const hspz_pass_params *p, hspz_segment_pair **hsps,
+                  size_t **counts, size_t *n_intervals, hspz_pass_stats *stats);
+
+void hspz_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

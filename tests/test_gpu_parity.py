@@ -1,0 +1,120 @@
+"""GPU parity: the native engine (HIP kernels through the C ABI) against the
+oracle, bit-exact on every HSP list (ref_start, query_start, len, score, in
+order, per interval and strand)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from helpers import Pair, same_intervals
+from hspz_amd import engine as E
+from hspz_amd import synth
+from gen_fixtures import SCORING_ACGT
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["identical", "mismatch", "indel", "revcomp", "repeat", "ambiguous", "xdrop", "multi",
+            "softmask", "scoring"]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _check(pair, threads=16, **kw):
+    want, wst, wh = pair.oracle().run_pass(threads=threads, **kw)
+    with pair.engine() as eng:
+        got, gst = eng.run_pass(**kw)
+    assert gst["hits"] == wst["hits"], (gst, wst)
+    assert gst["raw_hsps"] == wst["raw_hsps"], (gst, wst)
+    assert same_intervals(got, want)
+    assert E.hash_pass(got) == wh
+    return got, gst
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_parity(gpu_available, name):
+    _say("case fixture %s" % name)
+    kw = {"acgt": SCORING_ACGT} if name == "scoring" else {}
+    p = Pair.fixture(name, **kw)
+    got, _ = _check(p)
+    if name == "repeat":
+        assert p.segments_digest(got) == (57, 1, "d01edd2118cf5fa5")
+    if name == "multi":
+        assert p.segments_digest(got) == (3, 2, "0da3bbda656e4f7b")
+
+
+def test_repeat_with_small_max_hits(gpu_available):
+    _say("case repeat max_hits=1000 (chunked)")
+    _check(Pair.fixture("repeat", max_hits=1000))
+
+
+def test_c1_shape_small_multi_interval(gpu_available):
+    _say("case c1-shape 400 kbp, 3 intervals, 100 kbp chunks")
+    t, q = synth.c1_pair(400_000, seed=0xC0FFEE01)
+    _check(Pair.raw(t, q), wga_chunk=100_000, interval_size=150_000)
+
+
+@pytest.mark.parametrize("ambiguous,noentropy", [("", False), ("n", False), ("iupac,5,7", True)])
+def test_noisy_sequences(gpu_available, ambiguous, noentropy):
+    _say("case noisy ambiguous=%r noentropy=%s" % (ambiguous, noentropy))
+    rng = np.random.default_rng(17)
+    t = bytearray(synth.random_bases(5, 200_000).tobytes())
+    q = bytearray(synth.mutate(np.frombuffer(bytes(t), np.uint8), 6, 0.08, 0.005).tobytes())
+    for arr in (t, q):
+        for i in rng.integers(0, len(arr), len(arr) // 100):
+            arr[i] = rng.choice(list(b"NnacgtRYKMX"))
+    _check(Pair.raw(bytes(t), bytes(q), ambiguous=ambiguous, noentropy=noentropy))
+
+
+def test_soft_masked_repeats(gpu_available):
+    _say("case c2-shape 1 Mbp with soft-masked repeats")
+    t, q = synth.c2_pair(1_000_000, seed=0xC0FFEE02)
+    _check(Pair.raw(t, q))
+
+
+def test_upload_seeds_equals_generate_seeds(gpu_available):
+    _say("case host seeds vs device seeds")
+    t, q = synth.c1_pair(200_000, seed=77)
+    p = Pair.raw(t, q)
+    o = p.oracle()
+    with p.engine() as eng:
+        for rev, (lo, hi) in [(False, (0, 60_000)), (True, (10_000, 90_000))]:
+            raw = p.q_rc_raw if rev else p.q_raw
+            seeds = E.chunk_seeds(raw, lo, hi, threads=4)
+            eng.upload_seeds(1, seeds)
+            a, nha, rawa = eng.seed_and_filter(1, rev)
+            ns = eng.generate_seeds(0, rev, lo, hi)
+            assert ns == len(seeds)
+            b, nhb, rawb = eng.seed_and_filter(0, rev)
+            want, st = o.seed_and_filter(seeds, rev)
+            assert nha == nhb == st["hits"] and rawa == rawb == st["raw_hsps"]
+            assert np.array_equal(a, want) and np.array_equal(b, want)
+
+
+def test_error_paths(gpu_available):
+    _say("case error paths")
+    p = Pair.fixture("identical")
+    o = p.oracle()
+    eng = E.Engine(o.index, o.pos, o.ref_enc, p.sub, max_hits=16711680)
+    with pytest.raises(E.HspzError) as ei:
+        eng.seed_and_filter(0, False)
+    assert ei.value.status == 1
+    eng.swap_query(o.q_enc, o.q_rc_enc)
+    with pytest.raises(E.HspzError) as ei:
+        eng.generate_seeds(0, False, 0, len(o.q_enc))
+    assert ei.value.status == 4
+    with pytest.raises(E.HspzError) as ei:
+        eng.upload_seeds(0, np.array([1 << 56], dtype=np.uint64))
+    assert ei.value.status == 2
+    assert eng.stat("max_hits") == 16711680
+    eng.close()
+
+
+def test_c1_full_size(gpu_available):
+    _say("case c1 full size: 4 Mbp x 4 Mbp")
+    t, q = synth.c1_pair(4_000_000, seed=0xC0FFEE01)
+    got, st = _check(Pair.raw(t, q))
+    assert st["hsps"] > 0
