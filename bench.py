@@ -43,7 +43,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--target-mbp", type=float, default=1000.0)
     ap.add_argument("--query-mbp", type=float, default=4.0)
-    ap.add_argument("--cpu-sample-kbp", type=int, default=200)
+    ap.add_argument("--cpu-sample-kbp", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
@@ -107,6 +107,7 @@ def main():
         log("warmup %d: %d hits, %d HSPs" % (i, st["hits"], st["hsps"]))
 
     g_ns0, g_hits0, g_l0 = eng.stat("gate_ns"), eng.stat("gate_hits"), eng.stat("gate_launches")
+    g_def0 = eng.stat("gate_deferred")
     t_start = time.perf_counter()
     stats = None
     for i in range(args.steps):
@@ -116,6 +117,7 @@ def main():
     g_ns = eng.stat("gate_ns") - g_ns0
     g_hits = eng.stat("gate_hits") - g_hits0
     g_launch = eng.stat("gate_launches") - g_l0
+    g_def = eng.stat("gate_deferred") - g_def0
     value = args.steps * query_bp / 1e6 / elapsed
     hits_per_step = stats["hits"] if stats else 0
     hsp_hash = E.hash_pass(res)
@@ -155,6 +157,7 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_gate", "avg_launch_us": round(g_ns / max(g_launch, 1) / 1e3, 2),
                 "hits_per_s": round(g_hits / (g_ns * 1e-9), 1),
+                "deferred_frac": round(g_def / max(g_hits, 1), 5),
                 "cells_per_hit": round(cells_per_hit, 3)}
 
     out = {

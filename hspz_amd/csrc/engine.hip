@@ -57,6 +57,9 @@ struct hspz_engine {
     // query side
     DevBuf qf, qr;
     uint32_t query_len = 0;
+    // packed planes + coarse exception bitmaps for the fast gate (k_gate2)
+    DevBuf ref2, refx, qf2, qfx, qr2, qrx, pairtab, slow, slow_n;
+    bool fast_gate = false;
     // seed slots (gpu/mod.rs:933-945)
     DevBuf seeds[2];
     uint32_t seed_len[2] = {0, 0};
@@ -70,7 +73,7 @@ struct hspz_engine {
     // stats
     uint64_t launches = 0, pipeline_syncs = 0, reference_uploads = 0, query_swaps = 0;
     uint64_t dev_bytes = 0, peak_dev_bytes = 0;
-    uint64_t gate_ns = 0, gate_hits = 0, gate_launches = 0;
+    uint64_t gate_ns = 0, gate_hits = 0, gate_launches = 0, gate_deferred = 0;
     int gate_blocks = 2048;
 };
 
@@ -114,6 +117,22 @@ int upload_seq(hspz_engine *h, DevBuf &b, const uint8_t *src, size_t len) {
 }
 
 uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// 2-bit plane (16 bases per word, two spare words) and coarse exception bitmap
+// (one bit per 64 bases, two spare words) of an uploaded, padded sequence.
+int build_planes(hspz_engine *h, const DevBuf &seq, uint32_t len, DevBuf &plane, DevBuf &coarse) {
+    uint32_t nw = ceil_div(len, 16) + 2, nc = ceil_div(len, 2048) + 2;
+    if (int rc = reserve(h, plane, (size_t)nw * 4)) return rc;
+    if (int rc = reserve(h, coarse, (size_t)nc * 4)) return rc;
+    const uint8_t *codes = seq.as<uint8_t>() + PAD;
+    hipLaunchKernelGGL(k_pack2, dim3(std::min<uint32_t>(ceil_div(nw, 256), 8192)), dim3(256), 0, h->stream, codes, len,
+                       plane.as<uint32_t>(), nw);
+    hipLaunchKernelGGL(k_coarse_exceptions, dim3(std::min<uint32_t>(ceil_div(nc, 256), 8192)), dim3(256), 0, h->stream,
+                       codes, len, coarse.as<uint32_t>(), nc);
+    h->launches += 2;
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
 
 // Inclusive scan of u32[n] in place; the total lands in h->total[0].
 int scan_inclusive(hspz_engine *h, uint32_t *data, uint32_t n) {
@@ -227,13 +246,31 @@ int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_seg
                                h->hit_num.as<uint32_t>(), h->anchors.as<uint64_t>(), s0, h0, nseeds);
         }
         h->launches++;
-        uint32_t gblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(nh, GATE_CHUNK * 4), h->gate_blocks));
         HIP_TRY(hipEventRecord(h->ev_gate0, h->stream));
-        hipLaunchKernelGGL(k_gate, dim3(gblocks), dim3(256), 0, h->stream, r, h->ref_len, q, h->query_len,
-                           h->sub.as<int32_t>(), h->xdrop, h->hspthresh, h->anchors.as<uint64_t>(), nh,
-                           h->flags.as<uint8_t>());
+        if (h->fast_gate) {
+            if (int rc = reserve(h, h->slow, (size_t)nh * 4)) return rc;
+            HIP_TRY(hipMemsetAsync(h->slow_n.p, 0, 4, h->stream));
+            const uint32_t *q2 = (rev ? h->qr2 : h->qf2).as<uint32_t>();
+            const uint32_t *qx = (rev ? h->qrx : h->qfx).as<uint32_t>();
+            uint32_t gblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(nh, GATE_CHUNK * 8), h->gate_blocks));
+            hipLaunchKernelGGL(k_gate2, dim3(gblocks), dim3(512), 0, h->stream, h->ref2.as<uint32_t>(),
+                               h->refx.as<uint32_t>(), h->ref_len, q2, qx, h->query_len, h->pairtab.as<uint32_t>(),
+                               h->xdrop, h->hspthresh, h->anchors.as<uint64_t>(), nh, h->flags.as<uint8_t>(),
+                               h->slow.as<uint32_t>(), h->slow_n.as<uint32_t>());
+            uint32_t sblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(nh, GATE_CHUNK * 4), 512));
+            hipLaunchKernelGGL(k_gate, dim3(sblocks), dim3(256), 0, h->stream, r, h->ref_len, q, h->query_len,
+                               h->sub.as<int32_t>(), h->xdrop, h->hspthresh, h->anchors.as<uint64_t>(), nh,
+                               h->flags.as<uint8_t>(), h->slow.as<uint32_t>(), h->slow_n.as<uint32_t>());
+            h->launches += 2;
+            HIP_TRY(hipMemcpyAsync(h->h_small + 1, h->slow_n.p, 4, hipMemcpyDeviceToHost, h->stream));
+        } else {
+            uint32_t gblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(nh, GATE_CHUNK * 4), h->gate_blocks));
+            hipLaunchKernelGGL(k_gate, dim3(gblocks), dim3(256), 0, h->stream, r, h->ref_len, q, h->query_len,
+                               h->sub.as<int32_t>(), h->xdrop, h->hspthresh, h->anchors.as<uint64_t>(), nh,
+                               h->flags.as<uint8_t>(), (const uint32_t *)nullptr, (const uint32_t *)nullptr);
+            h->launches++;
+        }
         HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));
-        h->launches++;
         HIP_TRY(hipGetLastError());
         uint32_t nsurv;
         if (int rc = compact_flags(h, h->flags.as<uint8_t>(), nh, h->surv, &nsurv)) return rc;
@@ -242,6 +279,7 @@ int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_seg
         h->gate_ns += (uint64_t)((double)ms * 1e6);
         h->gate_hits += nh;
         h->gate_launches++;
+        if (h->fast_gate) h->gate_deferred += h->h_small[1];
         if (nsurv == 0) continue;
         if (int rc = reserve(h, h->hsp, (size_t)nsurv * 16)) return rc;
         if (int rc = reserve(h, h->done, (size_t)nsurv + 16)) return rc;
@@ -347,6 +385,28 @@ extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_e
     h->ref_len = (uint32_t)cfg->ref_len;
     if ((rc = upload_seq(h, h->ref, cfg->ref_seq, cfg->ref_len))) return fail(rc);
     h->reference_uploads++;
+    // Fast gate: int16 pair table over ACGT x ACGT (scores clamped at -(xdrop+1)).
+    {
+        bool ok = cfg->xdrop >= 0 && cfg->xdrop < 32767;
+        for (int i = 0; i < 4 && ok; i++)
+            for (int j = 0; j < 4; j++) ok = ok && cfg->sub_mat[i * 8 + j] <= 32767;
+        h->fast_gate = ok;
+        if (ok) {
+            uint32_t tab[256];
+            for (uint32_t key = 0; key < 256; key++) {
+                uint32_t ra = key & 3, rb = (key >> 2) & 3, qa = (key >> 4) & 3, qb = (key >> 6) & 3;
+                int32_t lo = std::max(cfg->sub_mat[ra * 8 + qa], -(cfg->xdrop + 1));
+                int32_t hi = std::max(cfg->sub_mat[rb * 8 + qb], -(cfg->xdrop + 1));
+                tab[key] = ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16);
+            }
+            if ((rc = reserve(h, h->pairtab, sizeof tab))) return fail(rc);
+            if ((rc = reserve(h, h->slow_n, 256))) return fail(rc);
+            if (hipMemcpy(h->pairtab.p, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(set_error(-1, "pair table upload failed"));
+            if ((rc = build_planes(h, h->ref, h->ref_len, h->ref2, h->refx))) return fail(rc);
+            if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(set_error(-1, "plane build failed"));
+        }
+    }
     *out = h;
     return 0;
 }
@@ -356,6 +416,7 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     DevBuf *bufs[] = {&h->ref, &h->index, &h->pos, &h->sub, &h->qf, &h->qr, &h->seeds[0], &h->seeds[1],
+                      &h->ref2, &h->refx, &h->qf2, &h->qfx, &h->qr2, &h->qrx, &h->pairtab, &h->slow, &h->slow_n,
                       &h->hit_num, &h->tiles, &h->total, &h->anchors, &h->flags, &h->surv, &h->hsp,
                       &h->done, &h->ids2, &h->reduced, &h->valid};
     for (DevBuf *b : bufs) release(*b);
@@ -373,6 +434,10 @@ extern "C" int hspz_swap_query(hspz_engine *h, const uint8_t *fwd, const uint8_t
     HIP_TRY(hipSetDevice(h->device));
     if (int e = upload_seq(h, h->qf, fwd, len)) return e;
     if (int e = upload_seq(h, h->qr, rc, len)) return e;
+    if (h->fast_gate) {
+        if (int e = build_planes(h, h->qf, (uint32_t)len, h->qf2, h->qfx)) return e;
+        if (int e = build_planes(h, h->qr, (uint32_t)len, h->qr2, h->qrx)) return e;
+    }
     h->query_len = (uint32_t)len;
     h->seed_len[0] = h->seed_len[1] = 0;
     h->query_swaps++;
@@ -477,6 +542,7 @@ extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
     case HSPZ_STAT_GATE_NS: *value = h->gate_ns; break;
     case HSPZ_STAT_GATE_HITS: *value = h->gate_hits; break;
     case HSPZ_STAT_GATE_LAUNCHES: *value = h->gate_launches; break;
+    case HSPZ_STAT_GATE_DEFERRED: *value = h->gate_deferred; break;
     default: return set_error(HSPZ_E_ARG, "unknown stat %d", which);
     }
     return 0;

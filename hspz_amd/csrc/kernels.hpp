@@ -274,12 +274,18 @@ __global__ __launch_bounds__(256) void k_find_hits_wave(const uint32_t *__restri
 // wave spread over all banks.
 constexpr uint32_t GATE_CHUNK = 64 * 8;  // hits a wave takes from the grid at a time
 
+//
+// With `list` set, the kernel scores only the hits named there (the deferred
+// hits of k_gate2, their count read from *list_n) and writes their flags.
 __global__ __launch_bounds__(256) void k_gate(const uint8_t *__restrict__ ref, uint32_t ref_len,
                                               const uint8_t *__restrict__ qry, uint32_t qlen,
                                               const int32_t *__restrict__ sub, int32_t xdrop,
                                               int32_t thresh,
                                               const uint64_t *__restrict__ anchors, uint32_t n,
-                                              uint8_t *__restrict__ flags) {
+                                              uint8_t *__restrict__ flags,
+                                              const uint32_t *__restrict__ list,
+                                              const uint32_t *__restrict__ list_n) {
+    if (list) n = *list_n;
     __shared__ int32_t s_sub[64 * 32];
     for (uint32_t i = threadIdx.x; i < 64 * 32; i += blockDim.x) s_sub[i] = sub[i >> 5];
     __syncthreads();
@@ -311,6 +317,7 @@ __global__ __launch_bounds__(256) void k_gate(const uint8_t *__restrict__ ref, u
             uint32_t rank = mbcnt(need);
             if (!has && rank < avail) {
                 hit = q_next + rank;
+                if (list) hit = list[hit];
                 uint64_t a = anchors[hit];
                 R = (uint32_t)a;
                 Q = (uint32_t)(a >> 32);
@@ -367,6 +374,229 @@ __global__ __launch_bounds__(256) void k_gate(const uint8_t *__restrict__ ref, u
             int32_t total = (int32_t)((uint32_t)Mr + (uint32_t)M);
             flags[hit] = gate_keep(total, thresh) ? 1 : 0;
             has = false;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- gate, packed
+
+// Packed sequence planes for the fast gate: 16 bases per u32 word, 2 bits each
+// (base i of word w at bits 2i..2i+1; non-ACGT codes store 0), and a coarse
+// exception bitmap with one bit per 64-base block that holds any code >= 4.
+__global__ void k_pack2(const uint8_t *__restrict__ codes, uint32_t len,
+                        uint32_t *__restrict__ plane, uint32_t nwords) {
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x) {
+        uint32_t v = 0;
+        for (uint32_t i = 0; i < 16; i++) {
+            uint32_t p = w * 16 + i;
+            uint32_t c = p < len ? codes[p] : 0;
+            v |= (c < 4 ? c : 0) << (2 * i);
+        }
+        plane[w] = v;
+    }
+}
+
+__global__ void k_coarse_exceptions(const uint8_t *__restrict__ codes, uint32_t len,
+                                    uint32_t *__restrict__ bits, uint32_t nwords) {
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x) {
+        uint32_t v = 0;
+        for (uint32_t b = 0; b < 32; b++) {
+            uint64_t p0 = ((uint64_t)w * 32 + b) * 64;
+            bool ex = false;
+            for (uint32_t i = 0; i < 64; i++) {
+                uint64_t p = p0 + i;
+                if (p >= len || codes[p] >= 4) {
+                    ex = true;
+                    break;
+                }
+            }
+            v |= (uint32_t)ex << b;
+        }
+        bits[w] = v;
+    }
+}
+
+// Reverses the order of the sixteen 2-bit fields of x.
+__device__ __forceinline__ uint32_t rev2(uint32_t x) {
+    uint32_t b = __builtin_bitreverse32(x);
+    return ((b >> 1) & 0x55555555u) | ((b << 1) & 0xAAAAAAAAu);
+}
+
+// True when any coarse block touching bases [lo, hi] holds an exception.
+__device__ __forceinline__ bool coarse_hit(const uint32_t *__restrict__ bits, uint32_t lo, uint32_t hi) {
+    uint32_t b0 = lo >> 6, b1 = hi >> 6;
+    uint32_t w = b0 >> 5;
+    uint64_t v = (uint64_t)bits[w] | ((uint64_t)bits[w + 1] << 32);
+    v >>= (b0 & 31);
+    uint32_t nb = b1 - b0 + 1;
+    return (v & ((1ull << nb) - 1)) != 0;
+}
+
+constexpr uint32_t G2_RIGHT_WORDS = 4;  // 64 right cells in registers
+constexpr uint32_t G2_LEFT_WORDS = 7;   // 112 left cells in registers
+constexpr uint32_t G2_MARGIN = 128;     // fast path needs this many bases either side
+
+// Fast score gate over 2-bit planes. Same flag as k_gate for every hit it
+// finishes; hits it cannot finish in registers are appended to `slow` for
+// k_gate: a window that touches a non-ACGT code or a sequence end, or an
+// extension that outlives its 64 right / 112 left cells.
+//
+// One hit per lane with the per-wave queue of k_gate. When a lane takes a hit
+// it loads 12 words (192 bases) around the anchor from each plane once and
+// funnels them into two streams: the right cells in order, and the left cells
+// reversed so both directions consume 16 cells per step from word 0. Scores
+// come two cells per LDS read from a 256-entry pair table indexed by the 2-bit
+// codes of two consecutive cells of both sequences (int16 halves). Scores
+// below -(xdrop + 1) are clamped to it: such a cell drops the extension with
+// or without the clamp and never raises the maximum, so the result is
+// unchanged. The X-drop state is kept as (c, d = max - c): per cell
+// c += s, d = max(d - s, 0), stop when d > xdrop; the maximum is c + d.
+__global__ __launch_bounds__(512) void k_gate2(
+    const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
+    const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse, uint32_t qlen,
+    const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh,
+    const uint64_t *__restrict__ anchors, uint32_t n, uint8_t *__restrict__ flags,
+    uint32_t *__restrict__ slow, uint32_t *__restrict__ slow_n) {
+    __shared__ uint32_t s_tab[256 * 32];
+    for (uint32_t i = threadIdx.x; i < 256 * 32; i += blockDim.x) s_tab[i] = pairtab[i >> 5];
+    __syncthreads();
+
+    const uint32_t lane = lane_id();
+    const uint32_t *tab = s_tab + (lane & 31);
+    const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
+    uint32_t chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+    uint32_t q_next = chunk * GATE_CHUNK;
+    uint32_t q_end = min(q_next + GATE_CHUNK, n);
+    if (q_next > n) q_next = q_end = n;
+
+    bool has = false;
+    uint32_t hit = 0, grp = 0, limit = 0;
+    int phase = 0;
+    int32_t c = 0, d = 0, Mr = 0;
+    uint32_t A[G2_LEFT_WORDS], B[G2_LEFT_WORDS], LA[G2_LEFT_WORDS], LB[G2_LEFT_WORDS];
+#pragma unroll
+    for (int j = 0; j < (int)G2_LEFT_WORDS; j++) A[j] = B[j] = LA[j] = LB[j] = 0;
+
+    for (;;) {
+        uint64_t need = __ballot(!has);
+        bool fresh = false;
+        while (need) {
+            if (q_next >= q_end) {
+                chunk += nwaves;
+                uint64_t start = (uint64_t)chunk * GATE_CHUNK;
+                if (start >= n) break;
+                q_next = (uint32_t)start;
+                q_end = min(q_next + GATE_CHUNK, n);
+            }
+            uint32_t avail = q_end - q_next;
+            uint32_t rank = mbcnt(need);
+            if (!has && rank < avail) {
+                hit = q_next + rank;
+                has = true;
+                fresh = true;
+            }
+            q_next += min((uint32_t)__popcll(need), avail);
+            need = __ballot(!has);
+        }
+        if (__ballot(has) == 0) break;
+
+        bool defer = false;
+        if (fresh) {
+            uint64_t a = anchors[hit];
+            uint32_t R = (uint32_t)a, Q = (uint32_t)(a >> 32);
+            bool ok = R >= G2_MARGIN && Q >= G2_MARGIN && (uint64_t)R + G2_MARGIN <= ref_len &&
+                      (uint64_t)Q + G2_MARGIN <= qlen;
+            if (ok) ok = !coarse_hit(rcoarse, R - 16 * G2_LEFT_WORDS, R + 16 * G2_RIGHT_WORDS - 1) &&
+                         !coarse_hit(qcoarse, Q - 16 * G2_LEFT_WORDS, Q + 16 * G2_RIGHT_WORDS - 1);
+            if (!ok) {
+                defer = true;
+            } else {
+                // words wr-7 .. wr+4 of each plane; word 7 holds base R (resp. Q)
+                const uint32_t *rp = rplane + (R >> 4) - 7;
+                const uint32_t *qp = qplane + (Q >> 4) - 7;
+                uint32_t W[12], V[12];
+#pragma unroll
+                for (int j = 0; j < 12; j++) {
+                    W[j] = rp[j];
+                    V[j] = qp[j];
+                }
+                uint32_t sr = 2 * (R & 15), sq = 2 * (Q & 15);
+#pragma unroll
+                for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) {
+                    A[j] = __builtin_amdgcn_alignbit(W[8 + j], W[7 + j], sr);
+                    B[j] = __builtin_amdgcn_alignbit(V[8 + j], V[7 + j], sq);
+                }
+#pragma unroll
+                for (int j = 0; j < (int)G2_LEFT_WORDS; j++) {
+                    LA[j] = rev2(__builtin_amdgcn_alignbit(W[7 - j], W[6 - j], sr));
+                    LB[j] = rev2(__builtin_amdgcn_alignbit(V[7 - j], V[6 - j], sq));
+                }
+                phase = 0;
+                grp = 0;
+                limit = G2_RIGHT_WORDS;
+                c = 0;
+                d = 0;
+            }
+        }
+
+        if (has && !defer) {
+            uint32_t r = A[0], q = B[0];
+            uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
+            uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
+            uint32_t p[8];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                p[2 * j] = tab[((t1 >> (8 * j)) & 0xFF) << 5];
+                p[2 * j + 1] = tab[((t2 >> (8 * j)) & 0xFF) << 5];
+            }
+            bool alive = true;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                // cells 4j, 4j+1 come from p[2j]; 4j+2, 4j+3 from p[2j+1]
+                uint32_t pw = p[2 * (i >> 2) + ((i >> 1) & 1)];
+                int32_t sc = (i & 1) ? ((int32_t)pw >> 16) : (int32_t)(int16_t)(pw & 0xFFFF);
+                if (alive) {
+                    c += sc;
+                    d = max(d - sc, 0);
+                    alive = d <= xdrop;
+                }
+            }
+            if (alive) {
+                grp++;
+#pragma unroll
+                for (int j = 0; j < (int)G2_LEFT_WORDS - 1; j++) {
+                    A[j] = A[j + 1];
+                    B[j] = B[j + 1];
+                }
+                if (grp == limit) defer = true;  // outlived the register window
+            } else if (phase == 0) {
+                Mr = c + d;
+                phase = 1;
+                grp = 0;
+                limit = G2_LEFT_WORDS;
+                c = 0;
+                d = 0;
+#pragma unroll
+                for (int j = 0; j < (int)G2_LEFT_WORDS; j++) {
+                    A[j] = LA[j];
+                    B[j] = LB[j];
+                }
+            } else {
+                int32_t total = Mr + c + d;
+                flags[hit] = gate_keep(total, thresh) ? 1 : 0;
+                has = false;
+            }
+        }
+
+        uint64_t dm = __ballot(defer);
+        if (dm) {
+            uint32_t base = 0;
+            if (lane == (uint32_t)(__ffsll((unsigned long long)dm) - 1)) base = atomicAdd(slow_n, (uint32_t)__popcll(dm));
+            base = __shfl(base, __ffsll((unsigned long long)dm) - 1, WAVE);
+            if (defer) {
+                slow[base + mbcnt(dm)] = hit;
+                has = false;
+            }
         }
     }
 }

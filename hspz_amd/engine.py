@@ -22,7 +22,7 @@ SP_DTYPE = np.dtype([("ref_start", "<u4"), ("query_start", "<u4"), ("len", "<u4"
 
 STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 3,
         "query_swaps": 4, "peak_device_bytes": 5, "gate_ns": 6, "gate_hits": 7,
-        "gate_launches": 8}
+        "gate_launches": 8, "gate_deferred": 9}
 
 # Every symbol include/hspz.h declares (checked by tests/test_abi.py).
 EXPORTS = [

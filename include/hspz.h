@@ -109,6 +109,7 @@ void hspz_filter_output_free(hspz_filter_output *out);
 #define HSPZ_STAT_GATE_NS 6        /* cumulative score-gate kernel time (HIP events) */
 #define HSPZ_STAT_GATE_HITS 7      /* hits the score gate has scored */
 #define HSPZ_STAT_GATE_LAUNCHES 8
+#define HSPZ_STAT_GATE_DEFERRED 9  /* hits the packed gate handed to the byte-code gate */
 int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 
 /* resolve_max_hits / device_count / device_memory (gpu/mod.rs:2436-2463). */
