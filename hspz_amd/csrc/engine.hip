@@ -57,7 +57,7 @@ struct hspz_engine {
     // query side
     DevBuf qf, qr;
     uint32_t query_len = 0;
-    // packed planes + coarse exception bitmaps for the fast gate (k_gate2)
+    // packed planes + coarse exception bitmaps for the fast gate (k_gate_packed)
     DevBuf ref2, refx, qf2, qfx, qr2, qrx, pairtab, slow, slow_n;
     bool fast_gate = false;
     // seed slots (gpu/mod.rs:933-945)
@@ -127,7 +127,7 @@ int build_planes(hspz_engine *h, const DevBuf &seq, uint32_t len, DevBuf &plane,
     const uint8_t *codes = seq.as<uint8_t>() + PAD;
     hipLaunchKernelGGL(k_pack2, dim3(std::min<uint32_t>(ceil_div(nw, 256), 8192)), dim3(256), 0, h->stream, codes, len,
                        plane.as<uint32_t>(), nw);
-    hipLaunchKernelGGL(k_coarse_exceptions, dim3(std::min<uint32_t>(ceil_div(nc, 256), 8192)), dim3(256), 0, h->stream,
+    hipLaunchKernelGGL(k_coarse_exceptions, dim3(std::min<uint32_t>(ceil_div(nc, 8), 8192)), dim3(256), 0, h->stream,
                        codes, len, coarse.as<uint32_t>(), nc);
     h->launches += 2;
     HIP_TRY(hipGetLastError());
@@ -253,13 +253,13 @@ int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_seg
             const uint32_t *q2 = (rev ? h->qr2 : h->qf2).as<uint32_t>();
             const uint32_t *qx = (rev ? h->qrx : h->qfx).as<uint32_t>();
             uint32_t gblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(nh, GATE_CHUNK * 8), h->gate_blocks));
-            hipLaunchKernelGGL(k_gate2, dim3(gblocks), dim3(512), 0, h->stream, h->ref2.as<uint32_t>(),
+            hipLaunchKernelGGL(k_gate_packed, dim3(gblocks), dim3(512), 0, h->stream, h->ref2.as<uint32_t>(),
                                h->refx.as<uint32_t>(), h->ref_len, q2, qx, h->query_len, h->pairtab.as<uint32_t>(),
                                h->xdrop, h->hspthresh, h->anchors.as<uint64_t>(), nh, h->flags.as<uint8_t>(),
                                h->slow.as<uint32_t>(), h->slow_n.as<uint32_t>());
-            uint32_t sblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(nh, GATE_CHUNK * 4), 512));
-            hipLaunchKernelGGL(k_gate, dim3(sblocks), dim3(256), 0, h->stream, r, h->ref_len, q, h->query_len,
-                               h->sub.as<int32_t>(), h->xdrop, h->hspthresh, h->anchors.as<uint64_t>(), nh,
+            uint32_t sblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(nh, 64), 2048));
+            hipLaunchKernelGGL(k_gate_wave, dim3(sblocks), dim3(256), 0, h->stream, r, h->ref_len, q, h->query_len,
+                               h->sub.as<int32_t>(), h->xdrop, h->hspthresh, h->anchors.as<uint64_t>(),
                                h->flags.as<uint8_t>(), h->slow.as<uint32_t>(), h->slow_n.as<uint32_t>());
             h->launches += 2;
             HIP_TRY(hipMemcpyAsync(h->h_small + 1, h->slow_n.p, 4, hipMemcpyDeviceToHost, h->stream));
@@ -283,7 +283,7 @@ int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_seg
         if (nsurv == 0) continue;
         if (int rc = reserve(h, h->hsp, (size_t)nsurv * 16)) return rc;
         if (int rc = reserve(h, h->done, (size_t)nsurv + 16)) return rc;
-        hipLaunchKernelGGL(k_materialize, dim3(ceil_div(nsurv, 256)), dim3(256), 0, h->stream, r, h->ref_len, q,
+        hipLaunchKernelGGL(k_materialize, dim3(std::min<uint32_t>(ceil_div(nsurv, 4), 16384)), dim3(256), 0, h->stream, r, h->ref_len, q,
                            h->query_len, h->sub.as<int32_t>(), h->xdrop, h->hspthresh, h->noentropy,
                            1.3862943649291992, h->anchors.as<uint64_t>(), h->surv.as<uint32_t>(), nsurv,
                            h->hsp.as<SegmentPair>(), h->done.as<uint8_t>());
@@ -392,12 +392,14 @@ extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_e
             for (int j = 0; j < 4; j++) ok = ok && cfg->sub_mat[i * 8 + j] <= 32767;
         h->fast_gate = ok;
         if (ok) {
-            uint32_t tab[256];
+            // forward table, then the same with int16 halves swapped
+            uint32_t tab[512];
             for (uint32_t key = 0; key < 256; key++) {
                 uint32_t ra = key & 3, rb = (key >> 2) & 3, qa = (key >> 4) & 3, qb = (key >> 6) & 3;
                 int32_t lo = std::max(cfg->sub_mat[ra * 8 + qa], -(cfg->xdrop + 1));
                 int32_t hi = std::max(cfg->sub_mat[rb * 8 + qb], -(cfg->xdrop + 1));
                 tab[key] = ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16);
+                tab[256 + key] = ((uint32_t)hi & 0xFFFFu) | ((uint32_t)lo << 16);
             }
             if ((rc = reserve(h, h->pairtab, sizeof tab))) return fail(rc);
             if ((rc = reserve(h, h->slow_n, 256))) return fail(rc);

@@ -275,8 +275,8 @@ __global__ __launch_bounds__(256) void k_find_hits_wave(const uint32_t *__restri
 constexpr uint32_t GATE_CHUNK = 64 * 8;  // hits a wave takes from the grid at a time
 
 //
-// With `list` set, the kernel scores only the hits named there (the deferred
-// hits of k_gate2, their count read from *list_n) and writes their flags.
+// With `list` set, the kernel scores only the hits named there (their count
+// read from *list_n) and writes their flags.
 __global__ __launch_bounds__(256) void k_gate(const uint8_t *__restrict__ ref, uint32_t ref_len,
                                               const uint8_t *__restrict__ qry, uint32_t qlen,
                                               const int32_t *__restrict__ sub, int32_t xdrop,
@@ -396,30 +396,30 @@ __global__ void k_pack2(const uint8_t *__restrict__ codes, uint32_t len,
     }
 }
 
-__global__ void k_coarse_exceptions(const uint8_t *__restrict__ codes, uint32_t len,
-                                    uint32_t *__restrict__ bits, uint32_t nwords) {
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x) {
-        uint32_t v = 0;
-        for (uint32_t b = 0; b < 32; b++) {
-            uint64_t p0 = ((uint64_t)w * 32 + b) * 64;
-            bool ex = false;
-            for (uint32_t i = 0; i < 64; i++) {
-                uint64_t p = p0 + i;
-                if (p >= len || codes[p] >= 4) {
-                    ex = true;
-                    break;
-                }
-            }
-            v |= (uint32_t)ex << b;
+// One lane per 64-base block (16 dword loads), wave ballot into two bitmap
+// words. Blocks at or past the end of the sequence count as exceptions; the
+// code-7 padding after the sequence covers a final partial block.
+__global__ __launch_bounds__(256) void k_coarse_exceptions(const uint8_t *__restrict__ codes, uint32_t len,
+                                                           uint32_t *__restrict__ bits, uint32_t nwords) {
+    const uint32_t lane = lane_id();
+    const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
+    for (uint32_t w2 = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; 2 * w2 < nwords; w2 += nwaves) {
+        uint64_t blk = (uint64_t)w2 * 64 + lane;
+        uint64_t p0 = blk * 64;
+        bool ex = true;
+        if (p0 < len) {
+            const uint32_t *src = (const uint32_t *)(codes + p0);
+            uint32_t acc = 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc |= src[i];
+            ex = (acc & 0xFCFCFCFCu) != 0;
         }
-        bits[w] = v;
+        uint64_t m = __ballot(ex);
+        if (lane == 0) {
+            bits[2 * w2] = (uint32_t)m;
+            if (2 * w2 + 1 < nwords) bits[2 * w2 + 1] = (uint32_t)(m >> 32);
+        }
     }
-}
-
-// Reverses the order of the sixteen 2-bit fields of x.
-__device__ __forceinline__ uint32_t rev2(uint32_t x) {
-    uint32_t b = __builtin_bitreverse32(x);
-    return ((b >> 1) & 0x55555555u) | ((b << 1) & 0xAAAAAAAAu);
 }
 
 // True when any coarse block touching bases [lo, hi] holds an exception.
@@ -435,34 +435,55 @@ __device__ __forceinline__ bool coarse_hit(const uint32_t *__restrict__ bits, ui
 constexpr uint32_t G2_RIGHT_WORDS = 4;  // 64 right cells in registers
 constexpr uint32_t G2_LEFT_WORDS = 7;   // 112 left cells in registers
 constexpr uint32_t G2_MARGIN = 128;     // fast path needs this many bases either side
+constexpr uint32_t G2_TAB_COPIES = 16;  // LDS copies of each pair table (bank spread)
 
-// Fast score gate over 2-bit planes. Same flag as k_gate for every hit it
-// finishes; hits it cannot finish in registers are appended to `slow` for
-// k_gate: a window that touches a non-ACGT code or a sequence end, or an
-// extension that outlives its 64 right / 112 left cells.
+typedef short short2_t __attribute__((ext_vector_type(2)));
+
+// Sum of the two int16 halves of w, exact in int32.
+__device__ __forceinline__ int32_t pair_sum(uint32_t w) {
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, w), __builtin_bit_cast(short2_t, 0x00010001u), 0, false);
+}
+
+// Fast score gate over 2-bit planes (`pairtab` holds two 256-entry tables:
+// forward, then with the int16 halves swapped).
 //
 // One hit per lane with the per-wave queue of k_gate. When a lane takes a hit
 // it loads 12 words (192 bases) around the anchor from each plane once and
-// funnels them into two streams: the right cells in order, and the left cells
-// reversed so both directions consume 16 cells per step from word 0. Scores
-// come two cells per LDS read from a 256-entry pair table indexed by the 2-bit
-// codes of two consecutive cells of both sequences (int16 halves). Scores
-// below -(xdrop + 1) are clamped to it: such a cell drops the extension with
-// or without the clamp and never raises the maximum, so the result is
-// unchanged. The X-drop state is kept as (c, d = max - c): per cell
-// c += s, d = max(d - s, 0), stop when d > xdrop; the maximum is c + d.
-__global__ __launch_bounds__(512) void k_gate2(
+// funnels them into a right stream (cells R, R+1, ...) and a left stream (the
+// words below R, unreversed). Each step consumes one word pair = 16 cells:
+// both sequences' 2-bit codes of two consecutive cells form an 8-bit key into
+// an LDS pair table whose int16 halves are the two cells' scores. For the left
+// stream the key bytes are taken in reverse order and the halves-swapped table
+// is used, which walks the cells outward from R without reversing any bits.
+// Scores below -(xdrop + 1) are stored as -(xdrop + 1): such a cell ends the
+// extension either way and never raises the maximum.
+//
+// The recurrence runs on e = xdrop - (max - score): e = min(e + s, xdrop), and
+// the extension stops in the step where e first goes negative. Within that
+// step the cells after the stopping cell are still applied, so the maximum
+// taken at the end of the step, c + xdrop - e, is the running maximum over
+// every cell of the step: at least the exact X-drop maximum, and equal to it
+// unless a new maximum follows the drop inside the same 16 cells. The flag is
+// therefore an upper bound of the exact gate (never drops a hit the exact gate
+// keeps); every flagged hit is re-extended exactly by the materializer, so the
+// HSP output is unchanged.
+//
+// Hits whose windows touch a non-ACGT code or a sequence end, and extensions
+// that outlive their 64 right / 112 left cells, are appended to `slow` and
+// scored exactly by k_gate_wave.
+__global__ __launch_bounds__(512) void k_gate_packed(
     const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
     const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse, uint32_t qlen,
     const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh,
     const uint64_t *__restrict__ anchors, uint32_t n, uint8_t *__restrict__ flags,
     uint32_t *__restrict__ slow, uint32_t *__restrict__ slow_n) {
-    __shared__ uint32_t s_tab[256 * 32];
-    for (uint32_t i = threadIdx.x; i < 256 * 32; i += blockDim.x) s_tab[i] = pairtab[i >> 5];
+    __shared__ uint32_t s_tab[2 * 256 * G2_TAB_COPIES];
+    for (uint32_t i = threadIdx.x; i < 2 * 256 * G2_TAB_COPIES; i += blockDim.x) s_tab[i] = pairtab[i / G2_TAB_COPIES];
     __syncthreads();
 
     const uint32_t lane = lane_id();
-    const uint32_t *tab = s_tab + (lane & 31);
+    const uint32_t *tab_fwd = s_tab + (lane % G2_TAB_COPIES);
+    const uint32_t *tab_rev = tab_fwd + 256 * G2_TAB_COPIES;
     const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
     uint32_t chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
     uint32_t q_next = chunk * GATE_CHUNK;
@@ -471,8 +492,8 @@ __global__ __launch_bounds__(512) void k_gate2(
 
     bool has = false;
     uint32_t hit = 0, grp = 0, limit = 0;
-    int phase = 0;
-    int32_t c = 0, d = 0, Mr = 0;
+    bool left = false;
+    int32_t c = 0, e = xdrop, Mr = 0;
     uint32_t A[G2_LEFT_WORDS], B[G2_LEFT_WORDS], LA[G2_LEFT_WORDS], LB[G2_LEFT_WORDS];
 #pragma unroll
     for (int j = 0; j < (int)G2_LEFT_WORDS; j++) A[j] = B[j] = LA[j] = LB[j] = 0;
@@ -526,16 +547,17 @@ __global__ __launch_bounds__(512) void k_gate2(
                     A[j] = __builtin_amdgcn_alignbit(W[8 + j], W[7 + j], sr);
                     B[j] = __builtin_amdgcn_alignbit(V[8 + j], V[7 + j], sq);
                 }
+                // left word j: bases [R - 16(j+1), R - 16j), nearest base in the top field
 #pragma unroll
                 for (int j = 0; j < (int)G2_LEFT_WORDS; j++) {
-                    LA[j] = rev2(__builtin_amdgcn_alignbit(W[7 - j], W[6 - j], sr));
-                    LB[j] = rev2(__builtin_amdgcn_alignbit(V[7 - j], V[6 - j], sq));
+                    LA[j] = __builtin_amdgcn_alignbit(W[7 - j], W[6 - j], sr);
+                    LB[j] = __builtin_amdgcn_alignbit(V[7 - j], V[6 - j], sq);
                 }
-                phase = 0;
+                left = false;
                 grp = 0;
                 limit = G2_RIGHT_WORDS;
                 c = 0;
-                d = 0;
+                e = xdrop;
             }
         }
 
@@ -543,25 +565,27 @@ __global__ __launch_bounds__(512) void k_gate2(
             uint32_t r = A[0], q = B[0];
             uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
             uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
+            uint32_t x1 = left ? __builtin_bswap32(t2) : t1;
+            uint32_t x2 = left ? __builtin_bswap32(t1) : t2;
+            const uint32_t *tb = left ? tab_rev : tab_fwd;
             uint32_t p[8];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                p[2 * j] = tab[((t1 >> (8 * j)) & 0xFF) << 5];
-                p[2 * j + 1] = tab[((t2 >> (8 * j)) & 0xFF) << 5];
+                p[2 * j] = tb[((x1 >> (8 * j)) & 0xFF) * G2_TAB_COPIES];
+                p[2 * j + 1] = tb[((x2 >> (8 * j)) & 0xFF) * G2_TAB_COPIES];
             }
-            bool alive = true;
+            int32_t emin = e;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                // cells 4j, 4j+1 come from p[2j]; 4j+2, 4j+3 from p[2j+1]
-                uint32_t pw = p[2 * (i >> 2) + ((i >> 1) & 1)];
+                uint32_t pw = p[i >> 1];
                 int32_t sc = (i & 1) ? ((int32_t)pw >> 16) : (int32_t)(int16_t)(pw & 0xFFFF);
-                if (alive) {
-                    c += sc;
-                    d = max(d - sc, 0);
-                    alive = d <= xdrop;
-                }
+                int32_t t = e + sc;
+                e = min(t, xdrop);
+                emin = min(emin, t);
             }
-            if (alive) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) c += pair_sum(p[j]);
+            if (emin >= 0) {
                 grp++;
 #pragma unroll
                 for (int j = 0; j < (int)G2_LEFT_WORDS - 1; j++) {
@@ -569,20 +593,20 @@ __global__ __launch_bounds__(512) void k_gate2(
                     B[j] = B[j + 1];
                 }
                 if (grp == limit) defer = true;  // outlived the register window
-            } else if (phase == 0) {
-                Mr = c + d;
-                phase = 1;
+            } else if (!left) {
+                Mr = c + (xdrop - e);
+                left = true;
                 grp = 0;
                 limit = G2_LEFT_WORDS;
                 c = 0;
-                d = 0;
+                e = xdrop;
 #pragma unroll
                 for (int j = 0; j < (int)G2_LEFT_WORDS; j++) {
                     A[j] = LA[j];
                     B[j] = LB[j];
                 }
             } else {
-                int32_t total = Mr + c + d;
+                int32_t total = Mr + c + (xdrop - e);
                 flags[hit] = gate_keep(total, thresh) ? 1 : 0;
                 has = false;
             }
@@ -590,14 +614,111 @@ __global__ __launch_bounds__(512) void k_gate2(
 
         uint64_t dm = __ballot(defer);
         if (dm) {
+            uint32_t leader = (uint32_t)(__ffsll((unsigned long long)dm) - 1);
             uint32_t base = 0;
-            if (lane == (uint32_t)(__ffsll((unsigned long long)dm) - 1)) base = atomicAdd(slow_n, (uint32_t)__popcll(dm));
-            base = __shfl(base, __ffsll((unsigned long long)dm) - 1, WAVE);
+            if (lane == leader) base = atomicAdd(slow_n, (uint32_t)__popcll(dm));
+            base = __shfl(base, leader, WAVE);
             if (defer) {
                 slow[base + mbcnt(dm)] = hit;
                 has = false;
             }
         }
+    }
+}
+
+// ----------------------------------------------------------------- wave extension
+
+__device__ __forceinline__ int32_t wave_incl_scan_add_i32(int32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        int32_t up = __shfl_up(v, off, WAVE);
+        if (lane >= (uint32_t)off) v = (int32_t)((uint32_t)v + (uint32_t)up);
+    }
+    return v;
+}
+
+__device__ __forceinline__ int32_t wave_incl_scan_max_i32(int32_t v) {
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        int32_t up = __shfl_up(v, off, WAVE);  // lanes below off read their own value
+        v = up >= v ? up : v;
+    }
+    return v;
+}
+
+// One X-drop extension by a whole wave, 64 cells per tile: the reference's
+// tiled scan (kernels.rs:1279-1543) at wave width. Inclusive add-scan of the
+// cell scores, max-scan seeded with the inherited maximum, the first lane with
+// max - score > xdrop ends it; the maximum is the scan value just before that
+// lane, and its position the earliest lane reaching it (positions only when
+// WITH_POS). Right cells are R+k, Q+k (k >= 0); left cells R-k, Q-k (k >= 1).
+template <bool WITH_POS>
+__device__ __forceinline__ void wave_extend(const uint8_t *__restrict__ ref, uint32_t rlen,
+                                            const uint8_t *__restrict__ qry, uint32_t qlen,
+                                            const int32_t *s_sub, int32_t xdrop, uint32_t R,
+                                            uint32_t Q, bool left, int32_t *score_out,
+                                            int32_t *pos_out) {
+    const uint32_t lane = lane_id();
+    int64_t tile = 0;
+    int32_t prev_score = 0, prev_max = 0, prev_pos = left ? 0 : -1;
+    for (;;) {
+        int64_t off = tile + lane + (left ? 1 : 0);
+        int32_t sc = 0;
+        if (left) {
+            if ((int64_t)R >= off && (int64_t)Q >= off) sc = s_sub[ref[R - off] * 8 + qry[Q - off]];
+        } else {
+            if ((int64_t)R + off < (int64_t)rlen && (int64_t)Q + off < (int64_t)qlen)
+                sc = s_sub[ref[R + off] * 8 + qry[Q + off]];
+        }
+        int32_t ps = (int32_t)((uint32_t)wave_incl_scan_add_i32(sc) + (uint32_t)prev_score);
+        int32_t cand = ps > prev_max ? ps : prev_max;
+        int32_t scanned = wave_incl_scan_max_i32(cand);
+        uint64_t dropped = __ballot((int32_t)((uint32_t)scanned - (uint32_t)ps) > xdrop);
+        uint32_t first = dropped ? (uint32_t)(__ffsll((unsigned long long)dropped) - 1) : WAVE;
+        int32_t mx = first == 0 ? prev_max : __shfl(scanned, first - 1, WAVE);
+        int32_t mpos = prev_pos;
+        if (WITH_POS && mx != prev_max) {
+            uint64_t win = __ballot(ps == mx && lane < first);
+            mpos = (int32_t)(__ffsll((unsigned long long)win) - 1 + tile + (left ? 1 : 0));
+        }
+        int64_t last = left ? tile + WAVE : tile + WAVE - 1;
+        bool edge = left ? ((int64_t)R < last || (int64_t)Q < last)
+                         : ((int64_t)R + last >= (int64_t)rlen || (int64_t)Q + last >= (int64_t)qlen);
+        if (dropped || edge) {
+            *score_out = mx;
+            *pos_out = mpos;
+            return;
+        }
+        prev_score = __shfl(ps, WAVE - 1, WAVE);
+        prev_max = mx;
+        prev_pos = mpos;
+        tile += WAVE;
+    }
+}
+
+// Exact score gate for the hits k_gate_packed deferred: one wave per hit.
+__global__ __launch_bounds__(256) void k_gate_wave(const uint8_t *__restrict__ ref, uint32_t ref_len,
+                                                   const uint8_t *__restrict__ qry, uint32_t qlen,
+                                                   const int32_t *__restrict__ sub, int32_t xdrop,
+                                                   int32_t thresh,
+                                                   const uint64_t *__restrict__ anchors,
+                                                   uint8_t *__restrict__ flags,
+                                                   const uint32_t *__restrict__ list,
+                                                   const uint32_t *__restrict__ list_n) {
+    __shared__ int32_t s_sub[64];
+    if (threadIdx.x < 64) s_sub[threadIdx.x] = sub[threadIdx.x];
+    __syncthreads();
+    const uint32_t n = *list_n;
+    const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
+    for (uint32_t i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; i < n; i += nwaves) {
+        uint32_t hit = list[i];
+        uint64_t a = anchors[hit];
+        uint32_t R = (uint32_t)a, Q = (uint32_t)(a >> 32);
+        int32_t mr, ml, pr, pl;
+        wave_extend<false>(ref, ref_len, qry, qlen, s_sub, xdrop, R, Q, false, &mr, &pr);
+        wave_extend<false>(ref, ref_len, qry, qlen, s_sub, xdrop, R, Q, true, &ml, &pl);
+        if (lane_id() == 0) flags[hit] = gate_keep((int32_t)((uint32_t)mr + (uint32_t)ml), thresh) ? 1 : 0;
     }
 }
 
@@ -624,9 +745,11 @@ __device__ __forceinline__ double ref_ln(double x) {
     return b + c;
 }
 
-// find_hsps (kernels.rs:1063-1723) for the gate's survivors, one per lane:
-// right/left X-drop with earliest-argmax positions, entropy over the closed
-// interval when the score is inside [T, 3T], the keep test and the record.
+// find_hsps (kernels.rs:1063-1723) for the gate's survivors, one wave per
+// survivor: right and left X-drop with earliest-argmax positions (wave tiles),
+// entropy over the closed interval when the score is inside [T, 3T] (counts
+// packed in 16-bit fields of a u64, summed across the wave), the keep test and
+// the record. Lane 0 evaluates the f64 entropy in the reference's order.
 __global__ __launch_bounds__(256) void k_materialize(
     const uint8_t *__restrict__ ref, uint32_t ref_len, const uint8_t *__restrict__ qry,
     uint32_t qlen, const int32_t *__restrict__ sub, int32_t xdrop, int32_t thresh,
@@ -636,72 +759,56 @@ __global__ __launch_bounds__(256) void k_materialize(
     __shared__ int32_t s_sub[64];
     if (threadIdx.x < 64) s_sub[threadIdx.x] = sub[threadIdx.x];
     __syncthreads();
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nsurv) return;
-    uint64_t a = anchors[surv[i]];
-    uint32_t R = (uint32_t)a, Q = (uint32_t)(a >> 32);
-
-    int32_t c = 0, M = 0, rpos = -1;
-    for (uint32_t k = 0;; k++) {
-        if ((uint64_t)R + k >= ref_len || (uint64_t)Q + k >= qlen) break;
-        c = (int32_t)((uint32_t)c + (uint32_t)s_sub[ref[R + k] * 8 + qry[Q + k]]);
-        if (c > M) {
-            M = c;
-            rpos = (int32_t)k;
-        }
-        if ((int32_t)((uint32_t)M - (uint32_t)c) > xdrop) break;
-    }
-    int32_t Mr = M;
-    c = 0;
-    M = 0;
-    int32_t lpos = 0;
-    for (uint32_t k = 1;; k++) {
-        if (R < k || Q < k) break;
-        c = (int32_t)((uint32_t)c + (uint32_t)s_sub[ref[R - k] * 8 + qry[Q - k]]);
-        if (c > M) {
-            M = c;
-            lpos = (int32_t)k;
-        }
-        if ((int32_t)((uint32_t)M - (uint32_t)c) > xdrop) break;
-    }
-    int32_t total = (int32_t)((uint32_t)Mr + (uint32_t)M);
-    int32_t extent = (int32_t)((uint32_t)rpos + (uint32_t)lpos);
-    uint32_t left = (uint32_t)lpos;
-    double entropy = 1.0;
-    if (total >= thresh && (int64_t)total <= 3 * (int64_t)thresh && noentropy == 0) {
-        uint32_t rs = R - left, qs = Q - left;
-        int32_t positions = extent + 1;
-        uint64_t packed = 0;
-        for (int32_t p = 0; p < positions; p++) {
-            uint32_t r = ref[rs + (uint32_t)p], q = qry[qs + (uint32_t)p];
-            if (r == q && r < 4) packed += (uint64_t)1 << (16 * r);
-        }
-        int32_t cnt0 = (int32_t)(packed & 0xFFFF), cnt1 = (int32_t)((packed >> 16) & 0xFFFF);
-        int32_t cnt2 = (int32_t)((packed >> 32) & 0xFFFF), cnt3 = (int32_t)((packed >> 48) & 0xFFFF);
-        if (cnt0 + cnt1 + cnt2 + cnt3 >= 20) {
-            double denom = (double)(extent + 1);
-            int32_t cnt[4] = {cnt0, cnt1, cnt2, cnt3};
-            double acc = 0.0;
-            for (int b = 0; b < 4; b++) {
-                double p = (double)cnt[b] / denom;
-                double l = cnt[b] != 0 ? ref_ln(p) : 0.0;
-                double t = p * l;
-                acc = acc + t;
+    const uint32_t lane = lane_id();
+    const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
+    for (uint32_t i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; i < nsurv; i += nwaves) {
+        uint64_t a = anchors[surv[i]];
+        uint32_t R = (uint32_t)a, Q = (uint32_t)(a >> 32);
+        int32_t Mr, rpos, Ml, lpos;
+        wave_extend<true>(ref, ref_len, qry, qlen, s_sub, xdrop, R, Q, false, &Mr, &rpos);
+        wave_extend<true>(ref, ref_len, qry, qlen, s_sub, xdrop, R, Q, true, &Ml, &lpos);
+        int32_t total = (int32_t)((uint32_t)Mr + (uint32_t)Ml);
+        int32_t extent = (int32_t)((uint32_t)rpos + (uint32_t)lpos);
+        uint32_t leftx = (uint32_t)lpos;
+        double entropy = 1.0;
+        if (total >= thresh && (int64_t)total <= 3 * (int64_t)thresh && noentropy == 0) {
+            uint32_t rs = R - leftx, qs = Q - leftx;
+            int32_t positions = extent + 1;
+            uint64_t packed = 0;
+            for (int32_t p = (int32_t)lane; p < positions; p += WAVE) {
+                uint32_t r = ref[rs + (uint32_t)p], q = qry[qs + (uint32_t)p];
+                if (r == q && r < 4) packed += (uint64_t)1 << (16 * r);
             }
-            entropy = -acc / log4;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) packed += __shfl_xor(packed, off, WAVE);
+            int32_t cnt[4] = {(int32_t)(packed & 0xFFFF), (int32_t)((packed >> 16) & 0xFFFF),
+                              (int32_t)((packed >> 32) & 0xFFFF), (int32_t)((packed >> 48) & 0xFFFF)};
+            if (cnt[0] + cnt[1] + cnt[2] + cnt[3] >= 20) {
+                double denom = (double)(extent + 1);
+                double acc = 0.0;
+                for (int b = 0; b < 4; b++) {
+                    double pb = (double)cnt[b] / denom;
+                    double l = cnt[b] != 0 ? ref_ln(pb) : 0.0;
+                    double t = pb * l;
+                    acc = acc + t;
+                }
+                entropy = -acc / log4;
+            }
         }
-    }
-    double gated = (double)(float)total * entropy;
-    if (trunc_sat(gated) >= thresh) {
-        SegmentPair o;
-        o.ref_start = R - left;
-        o.query_start = Q - left;
-        o.len = (uint32_t)extent;
-        o.score = entropy > 0.0 ? trunc_sat((double)total * entropy) : 0;
-        hsp[i] = o;
-        done[i] = 1;
-    } else {
-        done[i] = 0;
+        if (lane == 0) {
+            double gated = (double)(float)total * entropy;
+            if (trunc_sat(gated) >= thresh) {
+                SegmentPair o;
+                o.ref_start = R - leftx;
+                o.query_start = Q - leftx;
+                o.len = (uint32_t)extent;
+                o.score = entropy > 0.0 ? trunc_sat((double)total * entropy) : 0;
+                hsp[i] = o;
+                done[i] = 1;
+            } else {
+                done[i] = 0;
+            }
+        }
     }
 }
 

@@ -254,7 +254,8 @@ def _codes(rng, n, p_special=0.0):
     return a
 
 
-def test_scalar_extension_equals_reference_tiles():
+@pytest.mark.parametrize("W", [32, 64])
+def test_scalar_extension_equals_reference_tiles(W):
     rng = np.random.default_rng(0x243F6A88)
     sub = O.sub_mat("", 910).astype(np.int64)
     subi = O.sub_mat("", 910)
@@ -269,8 +270,8 @@ def test_scalar_extension_equals_reference_tiles():
         R0 = int(rng.integers(0, n + 1))
         Q0 = int(min(n, max(0, R0 + rng.integers(-3, 4))))
         rs, re, ls, le, _ = O.extend(ref, qry, subi, X, R0, Q0)
-        assert (rs, re) == tiled_right(ref, qry, sub, X, R0, Q0), (trial, R0, Q0, X)
-        assert (ls, le) == tiled_left(ref, qry, sub, X, R0, Q0), (trial, R0, Q0, X)
+        assert (rs, re) == tiled_right(ref, qry, sub, X, R0, Q0, W), (trial, R0, Q0, X)
+        assert (ls, le) == tiled_left(ref, qry, sub, X, R0, Q0, W), (trial, R0, Q0, X)
 
 
 def test_extension_closed_forms():
