@@ -1,0 +1,343 @@
+// batch_host.hpp — host side of hspz_filter_batch (included by engine.hip,
+// which owns the engine struct and the buffer helpers).
+//
+// One call = one device pass over every batch of every chunk:
+//   arena upload + 2-bit planes -> batch seeding (validity, stable compaction,
+//   scatter) -> bucket sizes -> u64 inclusive scan -> per-batch max_hits
+//   chunks -> fused hit expansion + packed gate -> exact gate for deferred
+//   hits -> materializer -> compaction of kept HSPs with scope keys
+// then one read-back, per-scope dedup_and_order on host threads, and the
+// per-chunk, per-interval, fw-then-rc assembly of run.rs:278-299 / 443-451.
+#pragma once
+
+#include <thread>
+
+namespace {
+
+struct HostBatch {
+    uint32_t chunk, interval;
+    int rev;
+};
+
+// Device buffers of the batched path, grow-only, owned by the engine.
+struct BatchBufs {
+    DevBuf arena, plane, coarse, desc, poff, blk_off, blk_len, valid, ids, voff, seeds, counts, prefix,
+        tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr;
+};
+
+enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_N = 8 };
+
+}  // namespace
+
+struct hspz_batch_state {
+    BatchBufs b;
+    uint32_t cap_surv = 1u << 22, cap_slow = 1u << 22;
+};
+
+namespace {
+
+int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
+               const hspz_query_chunk *chunks, size_t n_chunks, std::vector<HostBatch> &hb,
+               std::vector<hspz_segment_pair> &hsps, std::vector<uint32_t> &scopes,
+               hspz_pass_stats *st) {
+    BatchBufs &B = h->batch->b;
+    // ---- layout: blocks (chunk c fwd = 2c, rc = 2c + 1), each after QPAD code-7 bytes
+    std::vector<uint32_t> blk_off(2 * n_chunks), blk_len(2 * n_chunks);
+    uint64_t off = QPAD;
+    for (size_t c = 0; c < n_chunks; c++)
+        for (int r = 0; r < 2; r++) {
+            blk_off[2 * c + r] = (uint32_t)off;
+            blk_len[2 * c + r] = (uint32_t)chunks[c].len;
+            off += chunks[c].len + QPAD;
+        }
+    if (off >= 0xFFFFFFFFull - 2 * PAD) return set_error(HSPZ_E_ARG, "batch arena beyond 4 GiB");
+    uint32_t arena_len = (uint32_t)off;
+    std::vector<uint8_t> arena(arena_len, 7);
+    for (size_t c = 0; c < n_chunks; c++) {
+        if (chunks[c].len) {
+            memcpy(arena.data() + blk_off[2 * c], chunks[c].fwd, chunks[c].len);
+            memcpy(arena.data() + blk_off[2 * c + 1], chunks[c].rc, chunks[c].len);
+        }
+    }
+    // ---- batches in the reference's order (run.rs:278-299), per chunk
+    std::vector<BatchDesc> desc;
+    std::vector<uint32_t> poff;
+    hb.clear();
+    uint64_t npos = 0;
+    for (size_t c = 0; c < n_chunks; c++) {
+        uint32_t bl = (uint32_t)chunks[c].len;
+        if (bl <= s.size) continue;
+        uint32_t end_pos = bl - s.size, qbl = bl - s.size;
+        uint32_t ni = 0;
+        for (uint64_t x = 0; x < end_pos; x += p->interval_size) ni++;
+        for (uint32_t i = 0; i < ni; i++) {
+            uint32_t a = (uint32_t)((uint64_t)i * p->interval_size);
+            uint32_t e = (uint32_t)std::min<uint64_t>(end_pos, (uint64_t)a + p->interval_size);
+            for (int rev = 0; rev < 2; rev++) {
+                if ((rev == 0 && !p->plus) || (rev == 1 && !p->minus)) continue;
+                uint32_t lo = rev ? qbl - e : a, hi = rev ? qbl - a : e;
+                for (uint64_t x = lo; x < hi; x += p->wga_chunk) {
+                    uint32_t y = (uint32_t)std::min<uint64_t>(x + p->wga_chunk, (uint64_t)hi + 1);
+                    desc.push_back({(uint32_t)(2 * c + rev), (uint32_t)x, y});
+                    hb.push_back({(uint32_t)c, i, rev});
+                    poff.push_back((uint32_t)npos);
+                    npos += y - x;
+                }
+            }
+        }
+    }
+    uint32_t nb = (uint32_t)desc.size();
+    poff.push_back((uint32_t)npos);
+    if (npos >= 0xFFFFFFFFull / 16) return set_error(HSPZ_E_ARG, "batch too large: %llu positions", (unsigned long long)npos);
+    if (nb == 0) return 0;
+    const uint32_t per = p->transitions ? 1 + s.kmer_size : 1;
+    const uint64_t cap_seeds = npos * per;
+    if (cap_seeds >= 0xFFFFFFFFull) return set_error(HSPZ_E_ARG, "batch too large: %llu seeds", (unsigned long long)cap_seeds);
+
+    // ---- device buffers
+    int rc;
+    if ((rc = upload_seq(h, B.arena, arena.data(), arena_len))) return rc;
+    if ((rc = build_planes(h, B.arena, arena_len, B.plane, B.coarse))) return rc;
+    if ((rc = reserve(h, B.desc, (size_t)nb * sizeof(BatchDesc)))) return rc;
+    if ((rc = reserve(h, B.poff, (size_t)(nb + 1) * 4))) return rc;
+    if ((rc = reserve(h, B.blk_off, blk_off.size() * 4))) return rc;
+    if ((rc = reserve(h, B.blk_len, blk_len.size() * 4))) return rc;
+    if ((rc = reserve(h, B.valid, npos + 16))) return rc;
+    if ((rc = reserve(h, B.ids, npos * 4 + 4))) return rc;
+    if ((rc = reserve(h, B.voff, (size_t)(nb + 1) * 4))) return rc;
+    if ((rc = reserve(h, B.seeds, cap_seeds * 8 + 8))) return rc;
+    if ((rc = reserve(h, B.counts, cap_seeds * 4 + 4))) return rc;
+    if ((rc = reserve(h, B.prefix, cap_seeds * 8 + 8))) return rc;
+    uint32_t cap_tiles = ceil_div(std::max<uint64_t>(cap_seeds, 1), TILE);
+    if ((rc = reserve(h, B.tiles, (size_t)cap_tiles * 8 + 8))) return rc;
+    if ((rc = reserve(h, B.nchunks, (size_t)nb * 4))) return rc;
+    if ((rc = reserve(h, B.starts, (size_t)nb * MAX_CHUNKS * 4))) return rc;
+    if ((rc = reserve(h, B.bhits, (size_t)nb * 8))) return rc;
+    if ((rc = reserve(h, B.ctr, CTR_N * 4))) return rc;
+    HIP_TRY(hipMemcpyAsync(B.desc.p, desc.data(), (size_t)nb * sizeof(BatchDesc), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(B.poff.p, poff.data(), (size_t)(nb + 1) * 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(B.blk_off.p, blk_off.data(), blk_off.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(B.blk_len.p, blk_len.data(), blk_len.size() * 4, hipMemcpyHostToDevice, h->stream));
+
+    const uint8_t *ar = B.arena.as<uint8_t>() + PAD;
+    uint32_t *ctr = B.ctr.as<uint32_t>();
+    ShapeArg sa;
+    memset(&sa, 0, sizeof sa);
+    memcpy(sa.pos, s.pos, s.kmer_size * 4);
+    sa.kmer_size = s.kmer_size;
+
+    for (;;) {  // one attempt per survivor/slow capacity
+        hspz_batch_state *bs = h->batch;
+        if ((rc = reserve(h, B.surv, (size_t)bs->cap_surv * sizeof(SurvRec)))) return rc;
+        if ((rc = reserve(h, B.slow, (size_t)bs->cap_slow * sizeof(SurvRec)))) return rc;
+        if ((rc = reserve(h, B.mhsp, (size_t)bs->cap_surv * 16))) return rc;
+        if ((rc = reserve(h, B.mscope, (size_t)bs->cap_surv * 4))) return rc;
+        if ((rc = reserve(h, B.keep, (size_t)bs->cap_surv + 16))) return rc;
+        if ((rc = reserve(h, B.kids, (size_t)bs->cap_surv * 4))) return rc;
+        if ((rc = reserve(h, B.ohsp, (size_t)bs->cap_surv * 16))) return rc;
+        if ((rc = reserve(h, B.oscope, (size_t)bs->cap_surv * 4))) return rc;
+
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, h->stream, ctr, (uint32_t)CTR_N, 0u);
+        // seeding
+        hipLaunchKernelGGL(k_bseed_valid, dim3(std::min<uint32_t>(ceil_div(npos, 256), 16384)), dim3(256), 0, h->stream,
+                           ar, B.blk_off.as<uint32_t>(), B.desc.as<BatchDesc>(), B.poff.as<uint32_t>(), nb,
+                           (uint32_t)npos, s.size, B.valid.as<uint8_t>());
+        uint32_t nt = std::max<uint32_t>(ceil_div(npos, TILE), 1);
+        if ((rc = reserve(h, h->tiles, (size_t)nt * 4))) return rc;
+        hipLaunchKernelGGL(k_tile_count_u8, dim3(nt), dim3(TILE_THREADS), 0, h->stream, B.valid.as<uint8_t>(),
+                           (uint32_t)npos, h->tiles.as<uint32_t>());
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, h->stream, h->tiles.as<uint32_t>(), nt,
+                           ctr + CTR_NVALID);
+        hipLaunchKernelGGL(k_tile_emit_u8, dim3(nt), dim3(TILE_THREADS), 0, h->stream, B.valid.as<uint8_t>(),
+                           (uint32_t)npos, h->tiles.as<uint32_t>(), B.ids.as<uint32_t>());
+        hipLaunchKernelGGL(k_bvoff, dim3(ceil_div(nb + 1, 256)), dim3(256), 0, h->stream, B.ids.as<uint32_t>(),
+                           ctr + CTR_NVALID, B.poff.as<uint32_t>(), nb, B.voff.as<uint32_t>());
+        hipLaunchKernelGGL(k_bseed_scatter, dim3(std::min<uint32_t>(ceil_div(npos, 256), 16384)), dim3(256), 0,
+                           h->stream, ar, B.blk_off.as<uint32_t>(), B.desc.as<BatchDesc>(), B.poff.as<uint32_t>(),
+                           nb, B.ids.as<uint32_t>(), ctr + CTR_NVALID, sa, p->transitions, B.seeds.as<uint64_t>());
+        // counts, u64 prefix, per-batch chunks
+        hipLaunchKernelGGL(k_bnum_hits, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
+                           h->stream, B.seeds.as<uint64_t>(), ctr + CTR_NVALID, per, h->index.as<uint32_t>(),
+                           B.counts.as<uint32_t>());
+        hipLaunchKernelGGL(k_tile_sum_u64, dim3(cap_tiles), dim3(TILE_THREADS), 0, h->stream, B.counts.as<uint32_t>(),
+                           ctr + CTR_NVALID, per, B.tiles.as<uint64_t>());
+        hipLaunchKernelGGL(k_scan_tiles_u64, dim3(1), dim3(SCAN_THREADS), 0, h->stream, B.tiles.as<uint64_t>(),
+                           ctr + CTR_NVALID, per);
+        hipLaunchKernelGGL(k_tile_apply_u64, dim3(cap_tiles), dim3(TILE_THREADS), 0, h->stream,
+                           B.counts.as<uint32_t>(), ctr + CTR_NVALID, per, B.tiles.as<uint64_t>(),
+                           B.prefix.as<uint64_t>());
+        hipLaunchKernelGGL(k_bchunks, dim3(ceil_div(nb, 64)), dim3(64), 0, h->stream, B.prefix.as<uint64_t>(),
+                           B.voff.as<uint32_t>(), nb, per, h->max_hits, B.nchunks.as<uint32_t>(),
+                           B.starts.as<uint32_t>(), B.bhits.as<uint64_t>(), ctr + CTR_ERR);
+        // fused expansion + packed gate, then the exact gate for deferred hits
+        HIP_TRY(hipEventRecord(h->ev_gate0, h->stream));
+        hipLaunchKernelGGL(k_bgate_packed, dim3(h->bgate_blocks), dim3(512), 0, h->stream, h->ref2.as<uint32_t>(),
+                           h->refx.as<uint32_t>(), h->ref_len, B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(),
+                           arena_len, h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.seeds.as<uint64_t>(),
+                           ctr + CTR_NVALID, per, h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->seed_size,
+                           B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(), ctr + CTR_SLOW,
+                           bs->cap_slow, ctr + CTR_ERR);
+        BlockOf bo;
+        bo.seeds = B.seeds.as<uint64_t>();
+        bo.voff = B.voff.as<uint32_t>();
+        bo.blk_off = B.blk_off.as<uint32_t>();
+        bo.blk_len = B.blk_len.as<uint32_t>();
+        bo.batches = B.desc.as<BatchDesc>();
+        bo.nb = nb;
+        bo.per = per;
+        const uint8_t *r = h->ref.as<uint8_t>() + PAD;
+        hipLaunchKernelGGL(k_bgate_wave, dim3(2048), dim3(256), 0, h->stream, r, h->ref_len, ar, bo,
+                           h->sub.as<int32_t>(), h->xdrop, h->hspthresh, B.slow.as<SurvRec>(), ctr + CTR_SLOW,
+                           bs->cap_slow, B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, ctr + CTR_ERR);
+        HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));
+        // materializer and compaction of kept records
+        hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(ceil_div(bs->cap_surv / 4 + 4, 256), 4096)), dim3(256), 0,
+                           h->stream, (uint32_t *)B.keep.p, bs->cap_surv / 4 + 4, 0u);
+        hipLaunchKernelGGL(k_bmaterialize, dim3(4096), dim3(256), 0, h->stream, r, h->ref_len, ar, bo,
+                           B.starts.as<uint32_t>(), B.nchunks.as<uint32_t>(), h->sub.as<int32_t>(), h->xdrop,
+                           h->hspthresh, h->noentropy, 1.3862943649291992, B.surv.as<SurvRec>(), ctr + CTR_SURV,
+                           bs->cap_surv, B.mhsp.as<SegmentPair>(), B.mscope.as<uint32_t>(), B.keep.as<uint8_t>());
+        uint32_t kt = ceil_div(bs->cap_surv, TILE);
+        if ((rc = reserve(h, h->tiles, (size_t)std::max(kt, nt) * 4))) return rc;
+        hipLaunchKernelGGL(k_tile_count_u8, dim3(kt), dim3(TILE_THREADS), 0, h->stream, B.keep.as<uint8_t>(),
+                           bs->cap_surv, h->tiles.as<uint32_t>());
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, h->stream, h->tiles.as<uint32_t>(), kt,
+                           ctr + CTR_KEPT);
+        hipLaunchKernelGGL(k_tile_emit_u8, dim3(kt), dim3(TILE_THREADS), 0, h->stream, B.keep.as<uint8_t>(),
+                           bs->cap_surv, h->tiles.as<uint32_t>(), B.kids.as<uint32_t>());
+        hipLaunchKernelGGL(k_gather_scoped, dim3(1024), dim3(256), 0, h->stream, B.mhsp.as<SegmentPair>(),
+                           B.mscope.as<uint32_t>(), B.kids.as<uint32_t>(), ctr + CTR_KEPT, B.ohsp.as<SegmentPair>(),
+                           B.oscope.as<uint32_t>());
+        h->launches += 22;
+        HIP_TRY(hipGetLastError());
+        uint32_t hc[CTR_N];
+        HIP_TRY(hipMemcpyAsync(h->h_small, ctr, CTR_N * 4, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        h->pipeline_syncs++;
+        memcpy(hc, h->h_small, sizeof hc);
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ev_gate0, h->ev_gate1));
+        if (hc[CTR_ERR] & 8u) {  // survivor or slow list overflowed: grow and rerun
+            bs->cap_surv = std::max(bs->cap_surv, hc[CTR_SURV]) * 2;
+            bs->cap_slow = std::max(bs->cap_slow, hc[CTR_SLOW]) * 2;
+            continue;
+        }
+        if (hc[CTR_ERR] & 1u) return set_error(HSPZ_E_CHUNK, "a single seed exceeds max_hits %u", h->max_hits);
+        if (hc[CTR_ERR] & 2u) return set_error(HSPZ_E_CHUNK, "a batch needs more than %u max_hits chunks", MAX_CHUNKS);
+        if (hc[CTR_ERR] & 4u) return set_error(HSPZ_E_ARG, "a batch has more than 2^32 hits");
+        std::vector<uint64_t> bh(nb);
+        HIP_TRY(hipMemcpy(bh.data(), B.bhits.p, (size_t)nb * 8, hipMemcpyDeviceToHost));
+        uint64_t hits = 0;
+        for (uint64_t x : bh) hits += x;
+        uint32_t nk = hc[CTR_KEPT];
+        hsps.resize(nk);
+        scopes.resize(nk);
+        if (nk) {
+            HIP_TRY(hipMemcpy(hsps.data(), B.ohsp.p, (size_t)nk * 16, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(scopes.data(), B.oscope.p, (size_t)nk * 4, hipMemcpyDeviceToHost));
+        }
+        h->gate_ns += (uint64_t)((double)ms * 1e6);
+        h->gate_hits += hits;
+        h->gate_launches++;
+        h->gate_deferred += hc[CTR_SLOW];
+        if (st) {
+            st->seeds += (uint64_t)hc[CTR_NVALID] * per;
+            st->hits += hits;
+            st->raw_hsps += nk;
+            st->batches += nb;
+        }
+        return 0;
+    }
+}
+
+}  // namespace
+
+extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, const hspz_query_chunk *chunks,
+                                 size_t n_chunks, hspz_batch_output *out, hspz_pass_stats *stats) {
+    clear_error();
+    if (!h || !p || !out || (n_chunks && !chunks)) return set_error(HSPZ_E_ARG, "null argument");
+    memset(out, 0, sizeof *out);
+    if (!h->fast_gate) return set_error(HSPZ_E_ARG, "batched path needs the packed gate (xdrop in [0, 32767), ACGT scores <= 32767)");
+    Shape s;
+    if (int rc = parse_shape(p->pattern ? p->pattern : "12of19", &s)) return rc;
+    if (s.size != h->seed_size) return set_error(HSPZ_E_SHAPE, "seed shape span %u != engine seed size %u", s.size, h->seed_size);
+    if (p->wga_chunk == 0 || p->interval_size == 0) return set_error(HSPZ_E_ARG, "zero chunk or interval size");
+    for (size_t c = 0; c < n_chunks; c++)
+        if (chunks[c].len && (!chunks[c].fwd || !chunks[c].rc)) return set_error(HSPZ_E_ARG, "chunk %zu has null data", c);
+    HIP_TRY(hipSetDevice(h->device));
+    if (!h->batch) h->batch = new hspz_batch_state();
+    hspz_pass_stats st = {0, 0, 0, 0, 0};
+    std::vector<HostBatch> hb;
+    std::vector<hspz_segment_pair> raw;
+    std::vector<uint32_t> scopes;
+    if (int rc = batch_pass(h, s, p, chunks, n_chunks, hb, raw, scopes, &st)) return rc;
+
+    // group by scope (batch * MAX_CHUNKS + chunk): counting order = output order
+    std::vector<uint32_t> order(raw.size());
+    for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return scopes[a] < scopes[b]; });
+    std::vector<hspz_segment_pair> grouped(raw.size());
+    for (size_t i = 0; i < order.size(); i++) grouped[i] = raw[order[i]];
+    std::vector<size_t> gstart;
+    for (size_t i = 0; i < order.size(); i++)
+        if (i == 0 || scopes[order[i]] != scopes[order[i - 1]]) gstart.push_back(i);
+    gstart.push_back(order.size());
+    size_t ng = gstart.size() - 1;
+    std::vector<size_t> kept(ng);
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nth; t++)
+            th.emplace_back([&, t] {
+                for (size_t g = t; g < ng; g += nth)
+                    kept[g] = dedup_and_order(grouped.data() + gstart[g], gstart[g + 1] - gstart[g]);
+            });
+        for (auto &x : th) x.join();
+    }
+    // assemble per chunk -> interval -> fw, rc (scopes ascending = batch order)
+    std::vector<size_t> civ(n_chunks, 0);
+    for (size_t c = 0; c < n_chunks; c++) {
+        uint32_t bl = (uint32_t)chunks[c].len;
+        if (bl > s.size)
+            for (uint64_t x = 0; x < bl - s.size; x += p->interval_size) civ[c]++;
+    }
+    std::vector<size_t> ibase(n_chunks + 1, 0);
+    for (size_t c = 0; c < n_chunks; c++) ibase[c + 1] = ibase[c] + civ[c];
+    size_t nlists = 2 * ibase[n_chunks];
+    std::vector<std::vector<hspz_segment_pair>> lists(nlists);
+    for (size_t g = 0; g < ng; g++) {
+        uint32_t b = scopes[order[gstart[g]]] / MAX_CHUNKS;
+        const HostBatch &x = hb[b];
+        auto &dst = lists[2 * (ibase[x.chunk] + x.interval) + x.rev];
+        dst.insert(dst.end(), grouped.begin() + gstart[g], grouped.begin() + gstart[g] + kept[g]);
+    }
+    size_t total = 0;
+    for (auto &l : lists) total += l.size();
+    st.hsps = total;
+    out->hsps = (hspz_segment_pair *)aligned_alloc(16, std::max<size_t>(total, 1) * 16);
+    out->counts = (size_t *)malloc(std::max<size_t>(nlists, 1) * sizeof(size_t));
+    out->chunk_intervals = (size_t *)malloc(std::max<size_t>(n_chunks, 1) * sizeof(size_t));
+    if (!out->hsps || !out->counts || !out->chunk_intervals) {
+        hspz_batch_output_free(out);
+        return set_error(HSPZ_E_OOM, "host output allocation");
+    }
+    size_t at = 0;
+    for (size_t i = 0; i < nlists; i++) {
+        memcpy(out->hsps + at, lists[i].data(), lists[i].size() * 16);
+        at += lists[i].size();
+        out->counts[i] = lists[i].size();
+    }
+    for (size_t c = 0; c < n_chunks; c++) out->chunk_intervals[c] = civ[c];
+    out->n_hsps = total;
+    out->n_chunks = n_chunks;
+    if (stats) *stats = st;
+    return 0;
+}
+
+extern "C" void hspz_batch_output_free(hspz_batch_output *out) {
+    if (!out) return;
+    free(out->hsps);
+    free(out->counts);
+    free(out->chunk_intervals);
+    memset(out, 0, sizeof *out);
+}

@@ -13,6 +13,7 @@
 #include "../../include/hspz.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "kernels_batch.hpp"
 
 using namespace hspz;
 
@@ -45,6 +46,8 @@ struct Batch {
 
 }  // namespace
 
+struct hspz_batch_state;  // batch_host.hpp
+
 struct hspz_engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -75,6 +78,8 @@ struct hspz_engine {
     uint64_t dev_bytes = 0, peak_dev_bytes = 0;
     uint64_t gate_ns = 0, gate_hits = 0, gate_launches = 0, gate_deferred = 0;
     int gate_blocks = 2048;
+    int bgate_blocks = 1024;  // persistent grid of the batched gate: 4 x 512-thread blocks per CU
+    hspz_batch_state *batch = nullptr;
 };
 
 namespace {
@@ -310,6 +315,8 @@ int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_seg
 
 }  // namespace
 
+#include "batch_host.hpp"
+
 // ------------------------------------------------------------------ C ABI
 
 extern "C" const char *hspz_last_error(void) { return error_buffer(); }
@@ -422,6 +429,14 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
                       &h->hit_num, &h->tiles, &h->total, &h->anchors, &h->flags, &h->surv, &h->hsp,
                       &h->done, &h->ids2, &h->reduced, &h->valid};
     for (DevBuf *b : bufs) release(*b);
+    if (h->batch) {
+        BatchBufs &B = h->batch->b;
+        DevBuf *bb[] = {&B.arena, &B.plane, &B.coarse, &B.desc, &B.poff, &B.blk_off, &B.blk_len, &B.valid, &B.ids,
+                        &B.voff, &B.seeds, &B.counts, &B.prefix, &B.tiles, &B.nchunks, &B.starts, &B.bhits, &B.surv,
+                        &B.slow, &B.mhsp, &B.mscope, &B.keep, &B.kids, &B.ohsp, &B.oscope, &B.ctr};
+        for (DevBuf *b : bb) release(*b);
+        delete h->batch;
+    }
     if (h->h_small) (void)hipHostFree(h->h_small);
     if (h->ev_gate0) (void)hipEventDestroy(h->ev_gate0);
     if (h->ev_gate1) (void)hipEventDestroy(h->ev_gate1);

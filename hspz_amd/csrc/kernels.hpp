@@ -444,54 +444,42 @@ __device__ __forceinline__ int32_t pair_sum(uint32_t w) {
     return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, w), __builtin_bit_cast(short2_t, 0x00010001u), 0, false);
 }
 
-// Fast score gate over 2-bit planes (`pairtab` holds two 256-entry tables:
-// forward, then with the int16 halves swapped).
+// Fast score gate over 2-bit planes, one hit per lane; shared by the plain
+// path (hits from the anchor array) and the batched path (hits generated from
+// seeds). `Src` supplies hits to idle lanes and receives results.
 //
-// One hit per lane with the per-wave queue of k_gate. When a lane takes a hit
-// it loads 12 words (192 bases) around the anchor from each plane once and
-// funnels them into a right stream (cells R, R+1, ...) and a left stream (the
-// words below R, unreversed). Each step consumes one word pair = 16 cells:
-// both sequences' 2-bit codes of two consecutive cells form an 8-bit key into
-// an LDS pair table whose int16 halves are the two cells' scores. For the left
-// stream the key bytes are taken in reverse order and the halves-swapped table
-// is used, which walks the cells outward from R without reversing any bits.
-// Scores below -(xdrop + 1) are stored as -(xdrop + 1): such a cell ends the
-// extension either way and never raises the maximum.
+// When a lane takes a hit it loads 12 words (192 bases) around the anchor from
+// each plane once and funnels them into a right stream (cells R, R+1, ...) and
+// a left stream (the words below R, unreversed). Each step consumes one word
+// pair = 16 cells: both sequences' 2-bit codes of two consecutive cells form
+// an 8-bit key into an LDS pair table whose int16 halves are the two cells'
+// scores. For the left stream the key bytes are taken in reverse order and the
+// halves-swapped table is used, which walks the cells outward from R without
+// reversing any bits. Scores below -(xdrop + 1) are stored as -(xdrop + 1):
+// such a cell ends the extension either way and never raises the maximum.
 //
 // The recurrence runs on e = xdrop - (max - score): e = min(e + s, xdrop), and
 // the extension stops in the step where e first goes negative. Within that
 // step the cells after the stopping cell are still applied, so the maximum
 // taken at the end of the step, c + xdrop - e, is the running maximum over
 // every cell of the step: at least the exact X-drop maximum, and equal to it
-// unless a new maximum follows the drop inside the same 16 cells. The flag is
-// therefore an upper bound of the exact gate (never drops a hit the exact gate
-// keeps); every flagged hit is re-extended exactly by the materializer, so the
-// HSP output is unchanged.
+// unless a new maximum follows the drop inside the same 16 cells. The pass
+// decision is therefore an upper bound of the exact gate (never drops a hit the
+// exact gate keeps); every passing hit is re-extended exactly by the
+// materializer, so the HSP output is unchanged.
 //
 // Hits whose windows touch a non-ACGT code or a sequence end, and extensions
-// that outlive their 64 right / 112 left cells, are appended to `slow` and
-// scored exactly by k_gate_wave.
-__global__ __launch_bounds__(512) void k_gate_packed(
-    const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
-    const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse, uint32_t qlen,
-    const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh,
-    const uint64_t *__restrict__ anchors, uint32_t n, uint8_t *__restrict__ flags,
-    uint32_t *__restrict__ slow, uint32_t *__restrict__ slow_n) {
-    __shared__ uint32_t s_tab[2 * 256 * G2_TAB_COPIES];
-    for (uint32_t i = threadIdx.x; i < 2 * 256 * G2_TAB_COPIES; i += blockDim.x) s_tab[i] = pairtab[i / G2_TAB_COPIES];
-    __syncthreads();
-
-    const uint32_t lane = lane_id();
-    const uint32_t *tab_fwd = s_tab + (lane % G2_TAB_COPIES);
-    const uint32_t *tab_rev = tab_fwd + 256 * G2_TAB_COPIES;
-    const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
-    uint32_t chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
-    uint32_t q_next = chunk * GATE_CHUNK;
-    uint32_t q_end = min(q_next + GATE_CHUNK, n);
-    if (q_next > n) q_next = q_end = n;
-
+// that outlive their 64 right / 112 left cells, are deferred to the exact
+// wave-per-hit gate.
+template <class Src>
+__device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__restrict__ rplane,
+                                                 const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
+                                                 const uint32_t *__restrict__ qplane,
+                                                 const uint32_t *__restrict__ qcoarse, uint32_t qlen,
+                                                 const uint32_t *tab_fwd, const uint32_t *tab_rev,
+                                                 int32_t xdrop, int32_t thresh) {
     bool has = false;
-    uint32_t hit = 0, grp = 0, limit = 0;
+    uint32_t R = 0, Q = 0, tag = 0, grp = 0, limit = 0;
     bool left = false;
     int32_t c = 0, e = xdrop, Mr = 0;
     uint32_t A[G2_LEFT_WORDS], B[G2_LEFT_WORDS], LA[G2_LEFT_WORDS], LB[G2_LEFT_WORDS];
@@ -499,32 +487,12 @@ __global__ __launch_bounds__(512) void k_gate_packed(
     for (int j = 0; j < (int)G2_LEFT_WORDS; j++) A[j] = B[j] = LA[j] = LB[j] = 0;
 
     for (;;) {
-        uint64_t need = __ballot(!has);
         bool fresh = false;
-        while (need) {
-            if (q_next >= q_end) {
-                chunk += nwaves;
-                uint64_t start = (uint64_t)chunk * GATE_CHUNK;
-                if (start >= n) break;
-                q_next = (uint32_t)start;
-                q_end = min(q_next + GATE_CHUNK, n);
-            }
-            uint32_t avail = q_end - q_next;
-            uint32_t rank = mbcnt(need);
-            if (!has && rank < avail) {
-                hit = q_next + rank;
-                has = true;
-                fresh = true;
-            }
-            q_next += min((uint32_t)__popcll(need), avail);
-            need = __ballot(!has);
-        }
+        src.refill(has, fresh, R, Q, tag);
         if (__ballot(has) == 0) break;
 
-        bool defer = false;
+        bool defer = false, done = false, pass = false;
         if (fresh) {
-            uint64_t a = anchors[hit];
-            uint32_t R = (uint32_t)a, Q = (uint32_t)(a >> 32);
             bool ok = R >= G2_MARGIN && Q >= G2_MARGIN && (uint64_t)R + G2_MARGIN <= ref_len &&
                       (uint64_t)Q + G2_MARGIN <= qlen;
             if (ok) ok = !coarse_hit(rcoarse, R - 16 * G2_LEFT_WORDS, R + 16 * G2_RIGHT_WORDS - 1) &&
@@ -607,23 +575,89 @@ __global__ __launch_bounds__(512) void k_gate_packed(
                 }
             } else {
                 int32_t total = Mr + c + (xdrop - e);
-                flags[hit] = gate_keep(total, thresh) ? 1 : 0;
-                has = false;
+                done = true;
+                pass = gate_keep(total, thresh);
             }
         }
+        src.emit(done, pass, tag, R, Q);
+        src.defer(defer, tag, R, Q);
+        if (done || defer) has = false;
+    }
+}
 
-        uint64_t dm = __ballot(defer);
-        if (dm) {
-            uint32_t leader = (uint32_t)(__ffsll((unsigned long long)dm) - 1);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(slow_n, (uint32_t)__popcll(dm));
-            base = __shfl(base, leader, WAVE);
-            if (defer) {
-                slow[base + mbcnt(dm)] = hit;
-                has = false;
+// Hit source over an anchor array (the plain path): per-wave queue of
+// GATE_CHUNK-hit ranges, flags written for every hit, deferred hit ids
+// appended to `slow`.
+struct AnchorSource {
+    const uint64_t *anchors;
+    uint32_t n, nwaves, chunk, q_next, q_end;
+    uint8_t *flags;
+    uint32_t *slow, *slow_n;
+
+    __device__ __forceinline__ void init() {
+        nwaves = gridDim.x * (blockDim.x / WAVE);
+        chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+        q_next = chunk * GATE_CHUNK;
+        q_end = min(q_next + GATE_CHUNK, n);
+        if (q_next > n) q_next = q_end = n;
+    }
+    __device__ __forceinline__ void refill(bool &has, bool &fresh, uint32_t &R, uint32_t &Q, uint32_t &tag) {
+        uint64_t need = __ballot(!has);
+        while (need) {
+            if (q_next >= q_end) {
+                chunk += nwaves;
+                uint64_t start = (uint64_t)chunk * GATE_CHUNK;
+                if (start >= n) break;
+                q_next = (uint32_t)start;
+                q_end = min(q_next + GATE_CHUNK, n);
             }
+            uint32_t avail = q_end - q_next;
+            uint32_t rank = mbcnt(need);
+            if (!has && rank < avail) {
+                tag = q_next + rank;
+                uint64_t a = anchors[tag];
+                R = (uint32_t)a;
+                Q = (uint32_t)(a >> 32);
+                has = true;
+                fresh = true;
+            }
+            q_next += min((uint32_t)__popcll(need), avail);
+            need = __ballot(!has);
         }
     }
+    __device__ __forceinline__ void emit(bool done, bool pass, uint32_t tag, uint32_t, uint32_t) {
+        if (done) flags[tag] = pass ? 1 : 0;
+    }
+    __device__ __forceinline__ void defer(bool d, uint32_t tag, uint32_t, uint32_t) {
+        uint64_t dm = __ballot(d);
+        if (!dm) return;
+        uint32_t leader = (uint32_t)(__ffsll((unsigned long long)dm) - 1);
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(slow_n, (uint32_t)__popcll(dm));
+        base = __shfl(base, leader, WAVE);
+        if (d) slow[base + mbcnt(dm)] = tag;
+    }
+};
+
+__global__ __launch_bounds__(512) void k_gate_packed(
+    const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
+    const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse, uint32_t qlen,
+    const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh,
+    const uint64_t *__restrict__ anchors, uint32_t n, uint8_t *__restrict__ flags,
+    uint32_t *__restrict__ slow, uint32_t *__restrict__ slow_n) {
+    __shared__ uint32_t s_tab[2 * 256 * G2_TAB_COPIES];
+    for (uint32_t i = threadIdx.x; i < 2 * 256 * G2_TAB_COPIES; i += blockDim.x) s_tab[i] = pairtab[i / G2_TAB_COPIES];
+    __syncthreads();
+    const uint32_t *tab_fwd = s_tab + (lane_id() % G2_TAB_COPIES);
+    AnchorSource src;
+    src.anchors = anchors;
+    src.n = n;
+    src.flags = flags;
+    src.slow = slow;
+    src.slow_n = slow_n;
+    src.init();
+    packed_gate_loop(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, tab_fwd,
+                     tab_fwd + 256 * G2_TAB_COPIES, xdrop, thresh);
 }
 
 // ----------------------------------------------------------------- wave extension

@@ -30,7 +30,8 @@ EXPORTS = [
     "hspz_generate_seeds", "hspz_seed_and_filter", "hspz_filter_output_free", "hspz_get_stat",
     "hspz_default_max_hits", "hspz_device_count", "hspz_device_memory", "hspz_last_error",
     "hspz_seed_table_build", "hspz_shape_parse", "hspz_chunk_seeds", "hspz_dedup_and_order",
-    "hspz_run_pass", "hspz_free", "hspz_build_sub_mat",
+    "hspz_run_pass", "hspz_free", "hspz_build_sub_mat", "hspz_filter_batch",
+    "hspz_batch_output_free",
 ]
 
 
@@ -65,6 +66,15 @@ class PassStats(C.Structure):
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class QueryChunk(C.Structure):
+    _fields_ = [("fwd", C.c_void_p), ("rc", C.c_void_p), ("len", C.c_size_t)]
+
+
+class BatchOutput(C.Structure):
+    _fields_ = [("hsps", C.c_void_p), ("n_hsps", C.c_size_t), ("counts", C.c_void_p),
+                ("chunk_intervals", C.c_void_p), ("n_chunks", C.c_size_t)]
 
 
 _lib = None
@@ -102,6 +112,10 @@ def lib():
         L.hspz_run_pass.argtypes = [vp, C.POINTER(PassParams), C.POINTER(vp), C.POINTER(vp),
                                     C.POINTER(sz), C.POINTER(PassStats)]
         L.hspz_build_sub_mat.argtypes = [C.c_char_p, i32, vp, vp]
+        L.hspz_filter_batch.argtypes = [vp, C.POINTER(PassParams), C.POINTER(QueryChunk), sz,
+                                        C.POINTER(BatchOutput), C.POINTER(PassStats)]
+        L.hspz_batch_output_free.argtypes = [C.POINTER(BatchOutput)]
+        L.hspz_batch_output_free.restype = None
         L.hspz_free.argtypes = [vp]
         L.hspz_free.restype = None
         _lib = L
@@ -274,6 +288,50 @@ class Engine:
             rc = allsp[at:at + counts[2 * i + 1]]
             at += counts[2 * i + 1]
             res.append((fw, rc))
+        return res, st.as_dict()
+
+    def filter_batch(self, chunks, pattern: str = "12of19", transitions: bool = True,
+                     plus: bool = True, minus: bool = True, wga_chunk: int = 250000,
+                     interval_size: int = 10000000):
+        """Batched path over many query blocks: chunks = [(fwd_enc, rc_enc)].
+        Returns ([per chunk: [(fw, rc) per interval]], stats); each chunk's
+        lists equal run_pass on that chunk alone."""
+        keep = []
+        arr = (QueryChunk * max(len(chunks), 1))()
+        for i, (f, r) in enumerate(chunks):
+            f = np.ascontiguousarray(f, dtype=np.uint8)
+            r = np.ascontiguousarray(r, dtype=np.uint8)
+            if len(f) != len(r):
+                raise ValueError("chunk %d: forward and reverse-complement lengths differ" % i)
+            keep += [f, r]
+            arr[i] = QueryChunk(_ptr(f).value, _ptr(r).value, len(f))
+        p = PassParams(pattern.encode(), int(transitions), int(plus), int(minus), wga_chunk,
+                       interval_size)
+        out, st = BatchOutput(), PassStats()
+        _check(lib().hspz_filter_batch(self._h, C.byref(p), arr, len(chunks), C.byref(out),
+                                       C.byref(st)))
+        try:
+            nci = (np.ctypeslib.as_array(C.cast(out.chunk_intervals, C.POINTER(C.c_size_t)),
+                                         (max(out.n_chunks, 1),))[:out.n_chunks].copy())
+            nl = 2 * int(nci.sum())
+            counts = (np.ctypeslib.as_array(C.cast(out.counts, C.POINTER(C.c_size_t)),
+                                            (max(nl, 1),))[:nl].copy())
+            allsp = np.zeros(out.n_hsps, dtype=SP_DTYPE)
+            if out.n_hsps:
+                C.memmove(_ptr(allsp), out.hsps, out.n_hsps * 16)
+        finally:
+            lib().hspz_batch_output_free(C.byref(out))
+        res, at, li = [], 0, 0
+        for c in range(len(chunks)):
+            ivs = []
+            for _ in range(int(nci[c])):
+                fw = allsp[at:at + counts[li]]
+                at += counts[li]
+                rc = allsp[at:at + counts[li + 1]]
+                at += counts[li + 1]
+                li += 2
+                ivs.append((fw, rc))
+            res.append(ivs)
         return res, st.as_dict()
 
 

@@ -169,6 +169,39 @@ typedef struct hspz_pass_stats {
 int hspz_run_pass(hspz_engine *h, const hspz_pass_params *p, hspz_segment_pair **hsps,
                   size_t **counts, size_t *n_intervals, hspz_pass_stats *stats);
 
+/* ---- Batched path ------------------------------------------------------- */
+
+/* One query block: encoded forward and reverse-complement bytes (records
+ * joined by '&', as Engine::swap_query takes them). */
+typedef struct hspz_query_chunk {
+    const uint8_t *fwd;
+    const uint8_t *rc;
+    size_t len;
+} hspz_query_chunk;
+
+/* Results per chunk, in chunk order: for chunk c and interval i, the fw list
+ * then the rc list; counts[2*(b_c + i)] and counts[2*(b_c + i) + 1] are their
+ * lengths, b_c = sum of chunk_intervals[0..c). Library-allocated; release with
+ * hspz_batch_output_free. */
+typedef struct hspz_batch_output {
+    hspz_segment_pair *hsps;
+    size_t n_hsps;
+    size_t *counts;
+    size_t *chunk_intervals;
+    size_t n_chunks;
+} hspz_batch_output;
+
+/* Batched seed-and-filter: the output for each chunk equals hspz_run_pass on
+ * that chunk alone (same batches, same max_hits dedup scopes, same order), but
+ * all batches of all chunks run in one device pass: device seeding, hit
+ * counting, per-batch max_hits chunking, a fused hit-expansion + score-gate
+ * kernel, exact materialization, then per-scope dedup_and_order on the host.
+ * No counterpart in the reference, which runs batches one by one
+ * (run.rs:266-495); its callers are the batched and serving loops. */
+int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, const hspz_query_chunk *chunks,
+                      size_t n_chunks, hspz_batch_output *out, hspz_pass_stats *stats);
+void hspz_batch_output_free(hspz_batch_output *out);
+
 void hspz_free(void *p);
 
 #ifdef __cplusplus

@@ -1,0 +1,95 @@
+"""GPU parity of the batched path: every chunk's HSP lists from one
+hspz_filter_batch call equal the oracle's run_pass on that chunk alone (and
+the plain GPU path's), bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from helpers import same_intervals
+from hspz_amd import engine as E
+from hspz_amd import sequence as S
+from hspz_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _block(records):
+    buf, chrs, bl = S.pack(records)
+    raw = buf[:bl]
+    rc, _ = S.reverse_complement(buf, chrs, bl)
+    return raw, rc
+
+
+def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy=False,
+         wga_chunk=250000, interval_size=10000000):
+    sub = O.sub_mat(ambiguous, 910)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    ref_enc = S.encode(target)
+    with E.Engine(index, pos, ref_enc, sub, max_hits=max_hits, noentropy=noentropy) as eng:
+        chunks = [(S.encode(f), S.encode(r)) for f, r in query_blocks]
+        got, st = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
+        plain = []
+        for f, r in chunks:
+            eng.swap_query(f, r)
+            plain.append(eng.run_pass(wga_chunk=wga_chunk, interval_size=interval_size)[0])
+    for i, (f, r) in enumerate(query_blocks):
+        o = O.Oracle(target, f, r, sub, max_hits=max_hits, noentropy=noentropy, index=(index, pos))
+        want, wst, wh = o.run_pass(threads=16, wga_chunk=wga_chunk, interval_size=interval_size)
+        assert same_intervals(got[i], want), "chunk %d differs from the oracle" % i
+        assert same_intervals(plain[i], want), "plain chunk %d differs from the oracle" % i
+        assert E.hash_pass(got[i]) == wh
+    return got, st
+
+
+def _chunks_from(target: bytes, n, seed, lo=2_000, hi=60_000, div=(0.02, 0.2)):
+    t = np.frombuffer(target, dtype=np.uint8)
+    pieces = synth.sample_chunks(t, seed, n, (lo * hi) ** 0.5, 0.8, lo, hi, div[0], div[1])
+    return [_block([("c%d" % i, p)]) for i, p in enumerate(pieces)]
+
+
+def test_batch_many_uneven_chunks(gpu_available):
+    _say("case batch: 24 uneven chunks vs a 600 kbp target")
+    target = synth.random_bases(0xB001, 600_000).tobytes()
+    blocks = _chunks_from(target, 24, 11)
+    # a multi-record block, a tiny block shorter than the seed and an empty-ish one
+    blocks.append(_block([("m1", target[1000:3000]), ("m2", target[5000:9000])]))
+    blocks.append(_block([("tiny", b"ACGTACGTAC")]))
+    got, st = _run(target, blocks)
+    assert st["hsps"] > 0 and st["batches"] > 0
+
+
+def test_batch_multi_interval_and_small_max_hits(gpu_available):
+    _say("case batch: multi-interval blocks, max_hits=2000 (chunked scopes)")
+    target = synth.random_bases(0xB002, 300_000).tobytes()
+    blocks = _chunks_from(target, 6, 12, lo=50_000, hi=120_000, div=(0.01, 0.05))
+    _run(target, blocks, max_hits=2000, wga_chunk=20_000, interval_size=45_000)
+
+
+def test_batch_noisy_and_soft_masked(gpu_available):
+    _say("case batch: soft-masked repeats, N runs, IUPAC codes")
+    t, q = synth.c2_pair(400_000, seed=0xB003)
+    rng = np.random.default_rng(3)
+    q = bytearray(q)
+    for i in rng.integers(0, len(q), len(q) // 200):
+        q[i] = rng.choice(list(b"NnRYX"))
+    q = bytes(q)
+    blocks = [_block([("a", q[:150_000])]), _block([("b", q[150_000:260_000]), ("c", q[260_000:])])]
+    _run(t, blocks, ambiguous="n")
+
+
+def test_batch_repeat_fixture_digest(gpu_available):
+    _say("case batch: the repeat and multi fixtures as two chunks of one call")
+    from helpers import Pair
+    for name, want in (("repeat", (57, 1, "d01edd2118cf5fa5")), ("multi", (3, 2, "0da3bbda656e4f7b"))):
+        p = Pair.fixture(name)
+        o = p.oracle()
+        with E.Engine(o.index, o.pos, o.ref_enc, p.sub, max_hits=16711680) as eng:
+            got, _ = eng.filter_batch([(o.q_enc, o.q_rc_enc), (o.q_enc, o.q_rc_enc)])
+        assert p.segments_digest(got[0]) == want and p.segments_digest(got[1]) == want
