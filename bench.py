@@ -1,21 +1,24 @@
 #!/usr/bin/env python3
-"""Seed-and-filter benchmark: query Mbp/s against a 1 Gbp resident target.
+"""Seed-and-filter benchmark: query Mbp/s against a 1 Gbp resident target,
+batched path vs the plain per-chunk path.
 
 Workload: a 1,000,000,000 bp uppercase target (iid, 41% GC, seed 0xC0FFEE05)
-indexed with 12of19 and held in HBM; the query is a set of chunks sampled from
+indexed with 12of19 and held in HBM. The query is a set of chunks sampled from
 it (clipped lognormal lengths, median 200 kbp, 10 kbp..2 Mbp, divergence
-U(5%, 25%), half reverse-complemented), packed as records of one query block.
-HOXD70, X-drop 910, threshold 3000, both strands, entropy on, max_hits pinned.
+U(5%, 25%) as substitutions plus indels at a fifteenth of that rate, half
+reverse-complemented), each chunk its own query block. HOXD70, X-drop 910,
+threshold 3000, both strands, entropy on, transitions on; max_hits is the value
+the reference derives from this card's memory (4194304 x GiB), reported.
 
-A step = one seed-and-filter pass over the whole query block: encoded query in,
-ordered HSP list out (seeding, hit counting, expansion, score gate,
-compaction, materialization, dedup). Target generation, index build and index
-upload are setup, outside the timed region.
+A step = one seed-and-filter pass over every chunk: encoded chunks in, ordered
+HSP lists out. Batched: one hspz_filter_batch call over all chunks (`value`).
+Plain: per chunk, swap_query + the reference-shaped per-batch pipeline
+(hspz_run_pass), reported beside it. Both produce the same lists (checked).
+Target generation, index build and upload are setup, outside the timed region.
 
-Prints one JSON line (rank 0, N = 1). Extra objects: `roofline` for the score
-gate kernel (HIP events around every gate launch in the timed region) and
-`cpu_baseline` (the C oracle, multi-threaded, on a bounded sample of the same
-query block, timed on this host).
+Prints one JSON line (N = 1). `roofline` covers the fused expansion + score
+gate kernel of the batched path (HIP events around its launches);
+`cpu_baseline` is the C oracle on a bounded sample of the same chunks.
 """
 from __future__ import annotations
 
@@ -29,7 +32,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-MAX_HITS = 16711680
+# Stored bytes per base the packed gate reads per sequence: a 2-bit code plus
+# one exception bit per 64 bases.
+W_PACKED = 0.25 + 1.0 / 512
 
 
 def log(msg):
@@ -45,16 +50,15 @@ def main():
     ap.add_argument("--query-mbp", type=float, default=4.0)
     ap.add_argument("--cpu-sample-kbp", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: the seed-and-filter path has no cross-GPU "
-              "step here (work units shard by reference/query bin; see the design notes). "
+        print("bench.py measures one GPU: the seed-and-filter path has no cross-GPU step "
+              "(work units shard by reference/query bin with a host-side ordered merge). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
 
     from hspz_amd import engine as E
     from hspz_amd import sequence as S
@@ -66,102 +70,120 @@ def main():
     t0 = time.time()
     tlen = int(args.target_mbp * 1e6)
     target = synth.random_bases(0xC0FFEE05, tlen)
+    traw = target.tobytes()
     log("target %d bp generated in %.1fs" % (tlen, time.time() - t0))
     t1 = time.time()
-    traw = target.tobytes()
     index, pos = E.seed_table_build(traw, "12of19", 1, args.threads)
     log("index built in %.1fs (%d positions)" % (time.time() - t1, len(pos)))
     ref_enc = S.encode(traw)
     sub = E.build_sub_mat("", 910)
 
-    # query chunks (C3 model) until query_mbp is reached
     qwant = int(args.query_mbp * 1e6)
-    chunks, total, seed = [], 0, 0xC0FFEE03
+    pieces, total, seed = [], 0, 0xC0FFEE03
     while total < qwant:
         more = synth.sample_chunks(target, seed, 16, 200_000, 0.768, 10_000, 2_000_000, 0.05, 0.25)
         seed += 1
         for c in more:
             if total >= qwant:
                 break
-            chunks.append(c)
+            pieces.append(c)
             total += len(c)
-    records = [("q%d" % i, c) for i, c in enumerate(chunks)]
-    qbuf, qchrs, qbl = S.pack(records)
-    q_raw = qbuf[:qbl]
-    q_rc, _ = S.reverse_complement(qbuf, qchrs, qbl)
-    q_enc, q_rc_enc = S.encode(q_raw), S.encode(q_rc)
-    query_bp = sum(len(c) for c in chunks)
-    log("query: %d chunks, %d bp" % (len(chunks), query_bp))
+    blocks = []
+    for i, c in enumerate(pieces):
+        buf, chrs, bl = S.pack([("q%d" % i, c)])
+        rc, _ = S.reverse_complement(buf, chrs, bl)
+        blocks.append((buf[:bl], rc))
+    enc = [(S.encode(f), S.encode(r)) for f, r in blocks]
+    query_bp = sum(len(c) for c in pieces)
+    log("query: %d chunks, %d bp" % (len(pieces), query_bp))
 
+    _, total_mem = _device_memory(E)
+    max_hits = E.default_max_hits(total_mem)
     t1 = time.time()
     eng = E.Engine(index, pos, ref_enc, sub, seed_size=19, xdrop=910, hspthresh=3000,
-                   noentropy=False, max_hits=MAX_HITS)
-    log("engine up (index + target resident) in %.1fs" % (time.time() - t1))
+                   noentropy=False, max_hits=max_hits)
+    log("engine up (index + target resident) in %.1fs, max_hits %d" % (time.time() - t1, max_hits))
 
-    def step():
-        eng.swap_query(q_enc, q_rc_enc)
-        return eng.run_pass()
+    def batched():
+        return eng.filter_batch(enc)
 
     for i in range(args.warmup):
-        res, st = step()
+        res_b, st = batched()
         log("warmup %d: %d hits, %d HSPs" % (i, st["hits"], st["hsps"]))
 
-    g_ns0, g_hits0, g_l0 = eng.stat("gate_ns"), eng.stat("gate_hits"), eng.stat("gate_launches")
-    g_def0 = eng.stat("gate_deferred")
+    g0 = {k: eng.stat(k) for k in ("gate_ns", "gate_hits", "gate_launches", "gate_deferred")}
     t_start = time.perf_counter()
-    stats = None
+    st_b = None
     for i in range(args.steps):
-        res, stats = step()
-        log("step %d done" % i)
+        res_b, st_b = batched()
+        log("batched step %d done" % i)
     elapsed = time.perf_counter() - t_start
-    g_ns = eng.stat("gate_ns") - g_ns0
-    g_hits = eng.stat("gate_hits") - g_hits0
-    g_launch = eng.stat("gate_launches") - g_l0
-    g_def = eng.stat("gate_deferred") - g_def0
+    g = {k: eng.stat(k) - g0[k] for k in g0}
     value = args.steps * query_bp / 1e6 / elapsed
-    hits_per_step = stats["hits"] if stats else 0
-    hsp_hash = E.hash_pass(res)
+    hashes_b = [E.hash_pass(r) for r in res_b]
 
-    # CPU baseline: the oracle on the first cpu_sample_kbp of the query block
-    cpu = None
-    cells_per_hit = None
-    parity_sample = None
+    plain = None
+    if not args.no_plain:
+        t_p = time.perf_counter()
+        hashes_p = []
+        for f, r in enc:
+            eng.swap_query(f, r)
+            res_p, _ = eng.run_pass()
+            hashes_p.append(E.hash_pass(res_p))
+        plain_s = time.perf_counter() - t_p
+        plain = {"value": round(query_bp / 1e6 / plain_s, 4), "unit": "Mbp/s",
+                 "ms_per_step": round(plain_s * 1e3, 2),
+                 "equals_batched": hashes_p == hashes_b}
+        log("plain per-chunk: %.3f Mbp/s; identical lists: %s" % (plain["value"], plain["equals_batched"]))
+
+    cpu, cells_per_hit, parity_sample = None, None, None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_ffi as O
-        n = min(qbl, args.cpu_sample_kbp * 1000)
-        s_raw = q_raw[:n]
-        s_rc = S.reverse_complement(s_raw + b"&", [S.Chr("s", 0, n)], n)[0]
-        orc = O.Oracle(traw, s_raw, s_rc, sub, max_hits=MAX_HITS, index=(index, pos))
-        t1 = time.perf_counter()
-        o_res, o_st, o_hash = orc.run_pass(threads=args.threads)
-        cpu_s = time.perf_counter() - t1
-        cells_per_hit = o_st["cells"] / max(o_st["hits"], 1)
-        cpu = {"value": round(n / 1e6 / cpu_s, 4), "unit": "query Mbp/s", "cores": args.threads,
-               "kind": "port",
-               "sample": "first %d bp of the query block, both strands, against the full "
-                         "target (%d hits, %.1f s)" % (n, o_st["hits"], cpu_s)}
-        eng.swap_query(S.encode(s_raw), S.encode(s_rc))
-        g_res, g_st = eng.run_pass()
-        parity_sample = (E.hash_pass(g_res) == o_hash)
-        log("cpu baseline %.3f Mbp/s on %d threads; sample parity %s" %
-            (cpu["value"], args.threads, parity_sample))
+        # bounded sample: the leading part of each of the first chunks
+        budget, sample_bp, o_time, o_hits, o_cells, ok = args.cpu_sample_kbp * 1000, 0, 0.0, 0, 0, True
+        for c in pieces:
+            if sample_bp >= budget:
+                break
+            part = c[:min(len(c), budget - sample_bp, 20_000)]
+            if len(part) <= 19:
+                continue
+            buf, chrs, bl = S.pack([("s", part)])
+            rcb, _ = S.reverse_complement(buf, chrs, bl)
+            orc = O.Oracle(traw, buf[:bl], rcb, sub, max_hits=max_hits, index=(index, pos))
+            t1 = time.perf_counter()
+            o_res, o_st, o_hash = orc.run_pass(threads=args.threads)
+            o_time += time.perf_counter() - t1
+            o_hits += o_st["hits"]
+            o_cells += o_st["cells"]
+            sample_bp += len(part)
+            g_res, _ = eng.filter_batch([(S.encode(buf[:bl]), S.encode(rcb))])
+            ok = ok and E.hash_pass(g_res[0]) == o_hash
+        cells_per_hit = o_cells / max(o_hits, 1)
+        parity_sample = ok
+        cpu = {"value": round(sample_bp / 1e6 / o_time, 5), "unit": "query Mbp/s",
+               "cores": args.threads, "kind": "port",
+               "sample": "%d bp: the leading 20 kbp of the first chunks, each its own block, both "
+                         "strands, against the full target (%d hits, %.1f s)" % (sample_bp, o_hits, o_time)}
+        log("cpu baseline %.5f Mbp/s on %d threads; sample parity %s" % (cpu["value"], args.threads, ok))
 
     roof = None
-    if g_ns > 0 and cells_per_hit is not None:
-        # score gate algorithmic bytes: 9 per hit (8 B anchor + 1 B flag) plus
-        # 2 bytes per evaluated cell (one reference and one query code byte)
-        alg_bytes = g_hits * (9.0 + 2.0 * cells_per_hit)
-        achieved = alg_bytes / (g_ns * 1e-9) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_gate", "avg_launch_us": round(g_ns / max(g_launch, 1) / 1e3, 2),
-                "hits_per_s": round(g_hits / (g_ns * 1e-9), 1),
-                "deferred_frac": round(g_def / max(g_hits, 1), 5),
-                "cells_per_hit": round(cells_per_hit, 3)}
+    if g["gate_ns"] > 0 and cells_per_hit is not None:
+        # fused expansion + gate: 4 B of pos_table per hit, plus one reference
+        # and one query cell per evaluated cell at W_PACKED bytes per base
+        per_hit = 4.0 + 2.0 * W_PACKED * cells_per_hit
+        secs = g["gate_ns"] * 1e-9
+        achieved = g["gate_hits"] * per_hit / secs / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_bgate_packed (+ k_bgate_wave for deferred hits)",
+                "avg_launch_us": round(g["gate_ns"] / max(g["gate_launches"], 1) / 1e3, 1),
+                "hits_per_s": round(g["gate_hits"] / secs, 1),
+                "deferred_frac": round(g["gate_deferred"] / max(g["gate_hits"], 1), 5),
+                "cells_per_hit": round(cells_per_hit, 3), "bytes_per_hit": round(per_hit, 2)}
 
     out = {
-        "metric": "query Mbp/s vs 1 Gbp target (plain per-chunk path)",
+        "metric": "query Mbp/s vs 1 Gbp target (batched; plain per-chunk beside it)",
         "value": round(value, 4),
         "unit": "Mbp/s",
         "n_gpus": 1,
@@ -173,17 +195,30 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "plain seed-and-filter, %d bp target resident, %d query chunks "
-                               "(%d bp), 12of19+transitions, HOXD70, X=910, T=3000, both "
-                               "strands, max_hits=%d" % (tlen, len(chunks), query_bp, MAX_HITS),
-                   "target_bp": tlen, "query_bp": query_bp, "hits_per_step": hits_per_step,
-                   "hsps_per_step": stats["hsps"] if stats else 0,
-                   "hash_pass": "%016x" % hsp_hash, "parity_sample": parity_sample},
+        "config": {"workload": "seed-and-filter of %d query chunks (%d bp, 10 kbp-2 Mbp, 5-25%% "
+                               "divergent) against a %d bp resident target; 12of19+transitions, "
+                               "HOXD70, X=910, T=3000, both strands, max_hits=%d"
+                               % (len(pieces), query_bp, tlen, max_hits),
+                   "target_bp": tlen, "query_bp": query_bp, "chunks": len(pieces),
+                   "max_hits": max_hits, "hits_per_step": st_b["hits"] if st_b else 0,
+                   "hsps_per_step": st_b["hsps"] if st_b else 0,
+                   "raw_hsps_per_step": st_b["raw_hsps"] if st_b else 0,
+                   "hash_first_chunk": "%016x" % hashes_b[0] if hashes_b else None,
+                   "parity_sample_vs_oracle": parity_sample},
+        "plain": plain,
+        "batched_over_plain": round(value / plain["value"], 3) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     eng.close()
+
+
+def _device_memory(E):
+    import ctypes as C
+    f, t = C.c_uint64(), C.c_uint64()
+    E._check(E.lib().hspz_device_memory(0, C.byref(f), C.byref(t)))
+    return int(f.value), int(t.value)
 
 
 if __name__ == "__main__":

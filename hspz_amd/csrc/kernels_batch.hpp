@@ -257,42 +257,36 @@ struct SeedSource {
     __device__ __forceinline__ void init() {
         nwaves = gridDim.x * (blockDim.x / WAVE);
         chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
-        s_cur = chunk * SEEDS_PER_CHUNK;
+        uint64_t start = (uint64_t)chunk * SEEDS_PER_CHUNK;
+        s_cur = start < nseeds ? (uint32_t)start : nseeds;
         s_end = min(s_cur + SEEDS_PER_CHUNK, nseeds);
-        if (s_cur > nseeds) s_cur = s_end = nseeds;
-        j_cur = cnt_cur = 0;
         load_seed();
     }
+    // Loads seed s_cur (or an empty stand-in past the end) without branching,
+    // so every field stays a plain wave-uniform register.
     __device__ __forceinline__ void load_seed() {
-        // state is wave-uniform; the loads are scalar
-        if (s_cur < s_end) {
-            uint64_t sd = seeds[s_cur];
-            uint32_t k = (uint32_t)(sd >> 32);
-            uint32_t b1 = index[k], b0 = k ? index[k - 1] : 0;
-            b0_cur = __builtin_amdgcn_readfirstlane(b0);
-            cnt_cur = __builtin_amdgcn_readfirstlane(b1 - b0);
-            q_cur = __builtin_amdgcn_readfirstlane((uint32_t)sd + seed_size);
-        } else {
-            cnt_cur = 0;
-        }
+        bool in = s_cur < s_end;
+        uint64_t sd = seeds[in ? s_cur : 0];
+        uint32_t k = in ? (uint32_t)(sd >> 32) : 0u;
+        uint32_t b1 = index[k], b0 = k ? index[k - 1] : 0u;
+        b0_cur = __builtin_amdgcn_readfirstlane(b0);
+        cnt_cur = __builtin_amdgcn_readfirstlane(in ? b1 - b0 : 0u);
+        q_cur = __builtin_amdgcn_readfirstlane((uint32_t)sd + seed_size);
         j_cur = 0;
     }
-    // advance to the next seed with hits; false when the grid's seeds are done
+    // Next seed with hits; false when the grid's seeds are done.
     __device__ __forceinline__ bool advance() {
         for (;;) {
             s_cur++;
-            if (s_cur >= s_end) {
-                chunk += nwaves;
-                uint64_t start = (uint64_t)chunk * SEEDS_PER_CHUNK;
-                if (start >= nseeds) {
-                    s_cur = s_end = nseeds;
-                    cnt_cur = j_cur = 0;
-                    return false;
-                }
-                s_cur = (uint32_t)start;
-                s_end = min(s_cur + SEEDS_PER_CHUNK, nseeds);
-            }
+            bool next_chunk = s_cur >= s_end;
+            uint64_t start = (uint64_t)(chunk + nwaves) * SEEDS_PER_CHUNK;
+            bool done = next_chunk && start >= nseeds;
+            chunk = next_chunk ? chunk + nwaves : chunk;
+            uint32_t ns = next_chunk ? (uint32_t)min<uint64_t>(start, nseeds) : s_cur;
+            s_cur = __builtin_amdgcn_readfirstlane(ns);
+            s_end = __builtin_amdgcn_readfirstlane(next_chunk ? min(s_cur + SEEDS_PER_CHUNK, nseeds) : s_end);
             load_seed();
+            if (done) return false;
             if (cnt_cur) return true;
         }
     }
@@ -327,8 +321,7 @@ struct SeedSource {
         uint32_t at = base + mbcnt(m);
         if (want) {
             if (at < cap) {
-                SurvRec rec = {R, Q, tag, 0};
-                list[at] = rec;
+                *(uint4 *)(list + at) = make_uint4(R, Q, tag, 0u);
             } else {
                 atomicOr(err, 8u);
             }
