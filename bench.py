@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--cpu-sample-kbp", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="time only the plain per-chunk path (profiling)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
 
@@ -106,6 +108,15 @@ def main():
 
     def batched():
         return eng.filter_batch(enc)
+
+    if args.plain_only:
+        t_p = time.perf_counter()
+        for f, r in enc:
+            eng.swap_query(f, r)
+            eng.run_pass()
+        log("plain per-chunk only: %.3f Mbp/s" % (query_bp / 1e6 / (time.perf_counter() - t_p)))
+        eng.close()
+        return
 
     for i in range(args.warmup):
         res_b, st = batched()
