@@ -257,7 +257,7 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
     clear_error();
     if (!h || !p || !out || (n_chunks && !chunks)) return set_error(HSPZ_E_ARG, "null argument");
     memset(out, 0, sizeof *out);
-    if (!h->fast_gate) return set_error(HSPZ_E_ARG, "batched path needs the packed gate (xdrop in [0, 32767), ACGT scores <= 32767)");
+    if (!h->fast_gate) return set_error(HSPZ_E_ARG, "batched path needs the packed gate (xdrop in [0, 4000], ACGT scores <= 32767)");
     Shape s;
     if (int rc = parse_shape(p->pattern ? p->pattern : "12of19", &s)) return rc;
     if (s.size != h->seed_size) return set_error(HSPZ_E_SHAPE, "seed shape span %u != engine seed size %u", s.size, h->seed_size);

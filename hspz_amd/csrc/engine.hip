@@ -394,7 +394,7 @@ extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_e
     h->reference_uploads++;
     // Fast gate: int16 pair table over ACGT x ACGT (scores clamped at -(xdrop+1)).
     {
-        bool ok = cfg->xdrop >= 0 && cfg->xdrop < 32767;
+        bool ok = cfg->xdrop >= 0 && cfg->xdrop <= G2_MAX_XDROP;
         for (int i = 0; i < 4 && ok; i++)
             for (int j = 0; j < 4; j++) ok = ok && cfg->sub_mat[i * 8 + j] <= 32767;
         h->fast_gate = ok;

@@ -433,11 +433,17 @@ __device__ __forceinline__ bool coarse_hit(const uint32_t *__restrict__ bits, ui
 }
 
 constexpr uint32_t G2_RIGHT_WORDS = 4;  // 64 right cells in registers
-constexpr uint32_t G2_LEFT_WORDS = 7;   // 112 left cells in registers
+constexpr uint32_t G2_LEFT_WORDS = 8;   // 128 left cells in registers
+constexpr int32_t G2_MAX_XDROP = 4000;  // packed int16 partial sums of 16 cells stay exact
 constexpr uint32_t G2_MARGIN = 128;     // fast path needs this many bases either side
 constexpr uint32_t G2_TAB_COPIES = 16;  // LDS copies of each pair table (bank spread)
 
 typedef short short2_t __attribute__((ext_vector_type(2)));
+
+// Lane-wise int16 add of two packed pairs (v_pk_add_i16).
+__device__ __forceinline__ uint32_t pk_add16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(short2_t, a) + __builtin_bit_cast(short2_t, b));
+}
 
 // Sum of the two int16 halves of w, exact in int32.
 __device__ __forceinline__ int32_t pair_sum(uint32_t w) {
@@ -448,10 +454,10 @@ __device__ __forceinline__ int32_t pair_sum(uint32_t w) {
 // path (hits from the anchor array) and the batched path (hits generated from
 // seeds). `Src` supplies hits to idle lanes and receives results.
 //
-// When a lane takes a hit it loads 12 words (192 bases) around the anchor from
+// When a lane takes a hit it loads 13 words (208 bases) around the anchor from
 // each plane once and funnels them into a right stream (cells R, R+1, ...) and
-// a left stream (the words below R, unreversed). Each step consumes one word
-// pair = 16 cells: both sequences' 2-bit codes of two consecutive cells form
+// a left stream (the words below R, unreversed). Each step consumes two word
+// pairs = 32 cells; per word pair (16 cells) both sequences' 2-bit codes of two consecutive cells form
 // an 8-bit key into an LDS pair table whose int16 halves are the two cells'
 // scores. For the left stream the key bytes are taken in reverse order and the
 // halves-swapped table is used, which walks the cells outward from R without
@@ -463,14 +469,15 @@ __device__ __forceinline__ int32_t pair_sum(uint32_t w) {
 // step the cells after the stopping cell are still applied, so the maximum
 // taken at the end of the step, c + xdrop - e, is the running maximum over
 // every cell of the step: at least the exact X-drop maximum, and equal to it
-// unless a new maximum follows the drop inside the same 16 cells. The pass
+// unless a new maximum follows the drop inside the same 32 cells. The pass
 // decision is therefore an upper bound of the exact gate (never drops a hit the
 // exact gate keeps); every passing hit is re-extended exactly by the
 // materializer, so the HSP output is unchanged.
 //
 // Hits whose windows touch a non-ACGT code or a sequence end, and extensions
-// that outlive their 64 right / 112 left cells, are deferred to the exact
-// wave-per-hit gate.
+// that outlive their 64 right / 128 left cells, are deferred to the exact
+// wave-per-hit gate. The 16-cell score sums use packed int16 partial sums,
+// exact while 16 * (xdrop + 1) < 32768 (xdrop <= G2_MAX_XDROP).
 template <class Src>
 __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__restrict__ rplane,
                                                  const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
@@ -481,6 +488,7 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
     bool has = false;
     uint32_t R = 0, Q = 0, tag = 0, grp = 0, limit = 0;
     bool left = false;
+    const uint32_t *tb = tab_fwd;
     int32_t c = 0, e = xdrop, Mr = 0;
     uint32_t A[G2_LEFT_WORDS], B[G2_LEFT_WORDS], LA[G2_LEFT_WORDS], LB[G2_LEFT_WORDS];
 #pragma unroll
@@ -500,72 +508,78 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
             if (!ok) {
                 defer = true;
             } else {
-                // words wr-7 .. wr+4 of each plane; word 7 holds base R (resp. Q)
-                const uint32_t *rp = rplane + (R >> 4) - 7;
-                const uint32_t *qp = qplane + (Q >> 4) - 7;
-                uint32_t W[12], V[12];
+                // words wr-8 .. wr+4 of each plane; word 8 holds base R (resp. Q)
+                const uint32_t *rp = rplane + (R >> 4) - 8;
+                const uint32_t *qp = qplane + (Q >> 4) - 8;
+                uint32_t W[13], V[13];
 #pragma unroll
-                for (int j = 0; j < 12; j++) {
+                for (int j = 0; j < 13; j++) {
                     W[j] = rp[j];
                     V[j] = qp[j];
                 }
                 uint32_t sr = 2 * (R & 15), sq = 2 * (Q & 15);
 #pragma unroll
                 for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) {
-                    A[j] = __builtin_amdgcn_alignbit(W[8 + j], W[7 + j], sr);
-                    B[j] = __builtin_amdgcn_alignbit(V[8 + j], V[7 + j], sq);
+                    A[j] = __builtin_amdgcn_alignbit(W[9 + j], W[8 + j], sr);
+                    B[j] = __builtin_amdgcn_alignbit(V[9 + j], V[8 + j], sq);
                 }
                 // left word j: bases [R - 16(j+1), R - 16j), nearest base in the top field
 #pragma unroll
                 for (int j = 0; j < (int)G2_LEFT_WORDS; j++) {
-                    LA[j] = __builtin_amdgcn_alignbit(W[7 - j], W[6 - j], sr);
-                    LB[j] = __builtin_amdgcn_alignbit(V[7 - j], V[6 - j], sq);
+                    LA[j] = __builtin_amdgcn_alignbit(W[8 - j], W[7 - j], sr);
+                    LB[j] = __builtin_amdgcn_alignbit(V[8 - j], V[7 - j], sq);
                 }
                 left = false;
+                tb = tab_fwd;
                 grp = 0;
-                limit = G2_RIGHT_WORDS;
+                limit = G2_RIGHT_WORDS / 2;
                 c = 0;
                 e = xdrop;
             }
         }
 
         if (has && !defer) {
-            uint32_t r = A[0], q = B[0];
-            uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
-            uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
-            uint32_t x1 = left ? __builtin_bswap32(t2) : t1;
-            uint32_t x2 = left ? __builtin_bswap32(t1) : t2;
-            const uint32_t *tb = left ? tab_rev : tab_fwd;
-            uint32_t p[8];
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                p[2 * j] = tb[((x1 >> (8 * j)) & 0xFF) * G2_TAB_COPIES];
-                p[2 * j + 1] = tb[((x2 >> (8 * j)) & 0xFF) * G2_TAB_COPIES];
-            }
             int32_t emin = e;
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                uint32_t pw = p[i >> 1];
-                int32_t sc = (i & 1) ? ((int32_t)pw >> 16) : (int32_t)(int16_t)(pw & 0xFFFF);
-                int32_t t = e + sc;
-                e = min(t, xdrop);
-                emin = min(emin, t);
-            }
+            for (int h = 0; h < 2; h++) {  // two words = 32 cells per step
+                uint32_t r = A[h], q = B[h];
+                uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
+                uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
+                uint32_t x1 = left ? __builtin_bswap32(t2) : t1;
+                uint32_t x2 = left ? __builtin_bswap32(t1) : t2;
+                uint32_t p[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) c += pair_sum(p[j]);
+                for (int j = 0; j < 4; j++) {
+                    p[2 * j] = tb[((x1 >> (8 * j)) & 0xFF) * G2_TAB_COPIES];
+                    p[2 * j + 1] = tb[((x2 >> (8 * j)) & 0xFF) * G2_TAB_COPIES];
+                }
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    uint32_t pw = p[i >> 1];
+                    int32_t sc = (i & 1) ? ((int32_t)pw >> 16) : (int32_t)(int16_t)(pw & 0xFFFF);
+                    int32_t t = e + sc;
+                    e = min(t, xdrop);
+                    emin = min(emin, t);
+                }
+                // sum of the 16 scores: packed int16 partial sums, then both halves
+                uint32_t acc = pk_add16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3])),
+                                        pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
+                c += pair_sum(acc);
+            }
             if (emin >= 0) {
                 grp++;
 #pragma unroll
-                for (int j = 0; j < (int)G2_LEFT_WORDS - 1; j++) {
-                    A[j] = A[j + 1];
-                    B[j] = B[j + 1];
+                for (int j = 0; j < (int)G2_LEFT_WORDS - 2; j++) {
+                    A[j] = A[j + 2];
+                    B[j] = B[j + 2];
                 }
                 if (grp == limit) defer = true;  // outlived the register window
             } else if (!left) {
                 Mr = c + (xdrop - e);
                 left = true;
+                tb = tab_rev;
                 grp = 0;
-                limit = G2_LEFT_WORDS;
+                limit = G2_LEFT_WORDS / 2;
                 c = 0;
                 e = xdrop;
 #pragma unroll
