@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--plain-only", action="store_true",
                     help="time only the plain per-chunk path (profiling)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--serve-requests", type=int, default=150,
+                    help="requests per arrival-rate arm of the serving measurement (0 = skip)")
+    ap.add_argument("--serve-rates", default="50,200,800")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -178,6 +181,11 @@ def main():
                          "strands, against the full target (%d hits, %.1f s)" % (sample_bp, o_hits, o_time)}
         log("cpu baseline %.5f Mbp/s on %d threads; sample parity %s" % (cpu["value"], args.threads, ok))
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target,
+                             [float(x) for x in args.serve_rates.split(",")], args.serve_requests)
+
     roof = None
     if g["gate_ns"] > 0 and cells_per_hit is not None:
         # fused expansion + gate: 4 B of pos_table per hit, plus one reference
@@ -220,9 +228,84 @@ def main():
         "batched_over_plain": round(value / plain["value"], 3) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     eng.close()
+
+
+def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rates, n):
+    """Serving on the resident 1 Gbp target: per arrival rate, n requests with
+    clipped lognormal sizes (median 100 kbp, sigma 1.535, 1 kbp..10 Mbp),
+    divergence U(2%, 30%), half reverse-complemented, arriving as a Poisson
+    process; two engines (same reference) serve them by continuous batching.
+    Latency = completion - arrival. Every arm also checks that the first
+    requests' lists equal the same requests processed alone."""
+    import threading
+    import numpy as np
+    eng2 = E.Engine(index, pos, ref_enc, sub, max_hits=max_hits)
+    arms = []
+    for k, rate in enumerate(rates):
+        pieces = synth.sample_chunks(target, 0xC0FFEE05 + 17 * k, n, 100_000, 1.535, 1_000,
+                                     10_000_000, 0.02, 0.30)
+        reqs = []
+        for i, p in enumerate(pieces):
+            buf, chrs, bl = S.pack([("r%d" % i, p)])
+            rc, _ = S.reverse_complement(buf, chrs, bl)
+            reqs.append((S.encode(buf[:bl]), S.encode(rc)))
+        rng = np.random.default_rng(1000 + k)
+        gaps = rng.exponential(1.0 / rate, n)
+        bases = sum(len(f) for f, _ in reqs)
+        srv = E.Server([eng, eng2], max_batch_requests=256)
+        ids = [None] * n
+
+        def submitter():
+            t0 = time.perf_counter()
+            at = 0.0
+            for i, (f, r) in enumerate(reqs):
+                at += gaps[i]
+                d = at - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                ids[i] = srv.submit(f, r)
+
+        th = threading.Thread(target=submitter)
+        t_start = time.perf_counter()
+        th.start()
+        th.join()
+        lat, last_done, first_sub, results = [], 0.0, None, []
+        for i in range(n):
+            res, tm = srv.wait(ids[i])
+            lat.append(tm["t_done"] - tm["t_submit"])
+            last_done = max(last_done, tm["t_done"])
+            first_sub = tm["t_submit"] if first_sub is None else min(first_sub, tm["t_submit"])
+            if i < 4:
+                results.append(res)
+        wall = time.perf_counter() - t_start
+        st = srv.stats()
+        srv.close()
+        same = 0
+        for i, res in enumerate(results):
+            eng.swap_query(*reqs[i])
+            alone, _ = eng.run_pass()
+            same += E.hash_pass(alone) == E.hash_pass(res)
+        lat = np.array(lat)
+        span = max(last_done - first_sub, 1e-9)
+        arms.append({"rate_per_s": rate, "requests": n, "query_bp": int(bases),
+                     "offered_mbp_s": round(bases / 1e6 / float(gaps.sum()), 3),
+                     "achieved_mbp_s": round(bases / 1e6 / span, 3),
+                     "p50_s": round(float(np.percentile(lat, 50)), 4),
+                     "p99_s": round(float(np.percentile(lat, 99)), 4),
+                     "batches": st["batches"], "max_batch_requests": st["max_batch_requests"],
+                     "alone_equals_mixed": "%d/%d" % (same, len(results)),
+                     "saturated": bool(bases / float(gaps.sum()) > bases / span * 1.05)})
+        log("serve rate %g/s: %.2f Mbp/s, p50 %.3fs p99 %.3fs, %d batches (wall %.1fs)" %
+            (rate, arms[-1]["achieved_mbp_s"], arms[-1]["p50_s"], arms[-1]["p99_s"],
+             st["batches"], wall))
+    eng2.close()
+    return {"engines": 2, "arms": arms,
+            "note": "offered load above the service rate saturates the loop; latency then "
+                    "includes queueing"}
 
 
 def _device_memory(E):

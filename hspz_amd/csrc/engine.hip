@@ -316,6 +316,7 @@ int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_seg
 }  // namespace
 
 #include "batch_host.hpp"
+#include "server.hpp"
 
 // ------------------------------------------------------------------ C ABI
 

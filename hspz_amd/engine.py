@@ -31,7 +31,8 @@ EXPORTS = [
     "hspz_default_max_hits", "hspz_device_count", "hspz_device_memory", "hspz_last_error",
     "hspz_seed_table_build", "hspz_shape_parse", "hspz_chunk_seeds", "hspz_dedup_and_order",
     "hspz_run_pass", "hspz_free", "hspz_build_sub_mat", "hspz_filter_batch",
-    "hspz_batch_output_free",
+    "hspz_batch_output_free", "hspz_server_new", "hspz_server_submit", "hspz_server_wait",
+    "hspz_request_result_free", "hspz_server_get_stats", "hspz_server_free",
 ]
 
 
@@ -77,6 +78,27 @@ class BatchOutput(C.Structure):
                 ("chunk_intervals", C.c_void_p), ("n_chunks", C.c_size_t)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("params", PassParams), ("hbm_budget", C.c_uint64),
+                ("max_batch_requests", C.c_uint32), ("max_batch_bases", C.c_uint64)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("hsps", C.c_void_p), ("n_hsps", C.c_size_t),
+                ("counts", C.c_void_p), ("n_intervals", C.c_size_t), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double), ("batch_id", C.c_uint32),
+                ("batch_requests", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("batches", "requests", "bases", "hits", "rejected",
+                                          "failed_batches", "max_batch_requests", "queued",
+                                          "in_flight_bytes")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -116,6 +138,14 @@ def lib():
                                         C.POINTER(BatchOutput), C.POINTER(PassStats)]
         L.hspz_batch_output_free.argtypes = [C.POINTER(BatchOutput)]
         L.hspz_batch_output_free.restype = None
+        L.hspz_server_new.argtypes = [C.POINTER(vp), sz, C.POINTER(ServerConfig), C.POINTER(vp)]
+        L.hspz_server_submit.argtypes = [vp, vp, vp, sz, C.POINTER(C.c_uint64)]
+        L.hspz_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(RequestResult)]
+        L.hspz_request_result_free.argtypes = [C.POINTER(RequestResult)]
+        L.hspz_request_result_free.restype = None
+        L.hspz_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+        L.hspz_server_free.argtypes = [vp]
+        L.hspz_server_free.restype = None
         L.hspz_free.argtypes = [vp]
         L.hspz_free.restype = None
         _lib = L
@@ -333,6 +363,78 @@ class Engine:
                 ivs.append((fw, rc))
             res.append(ivs)
         return res, st.as_dict()
+
+
+class Server:
+    """Continuous-batching loop (hspz_server_*) over one or more engines that
+    hold the same reference on the same device."""
+
+    def __init__(self, engines, hbm_budget: int = 0, max_batch_requests: int = 0,
+                 max_batch_bases: int = 0, pattern: str = "12of19", transitions: bool = True,
+                 plus: bool = True, minus: bool = True, wga_chunk: int = 250000,
+                 interval_size: int = 10000000):
+        self._engines = list(engines)  # keep the handles alive
+        self._pattern = pattern.encode()
+        arr = (C.c_void_p * len(self._engines))(*[e._h.value for e in self._engines])
+        cfg = ServerConfig(PassParams(self._pattern, int(transitions), int(plus), int(minus),
+                                      wga_chunk, interval_size),
+                           hbm_budget, max_batch_requests, max_batch_bases)
+        h = C.c_void_p()
+        _check(lib().hspz_server_new(arr, len(self._engines), C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def submit(self, fwd_enc: np.ndarray, rc_enc: np.ndarray) -> int:
+        f = np.ascontiguousarray(fwd_enc, dtype=np.uint8)
+        r = np.ascontiguousarray(rc_enc, dtype=np.uint8)
+        rid = C.c_uint64()
+        _check(lib().hspz_server_submit(self._h, _ptr(f), _ptr(r), len(f), C.byref(rid)))
+        return int(rid.value)
+
+    def wait(self, rid: int):
+        """Returns ([(fw, rc)] per interval, timing dict)."""
+        out = RequestResult()
+        st = lib().hspz_server_wait(self._h, rid, C.byref(out))
+        try:
+            _check(st)
+            counts = (np.ctypeslib.as_array(C.cast(out.counts, C.POINTER(C.c_size_t)),
+                                            (max(2 * out.n_intervals, 1),))[:2 * out.n_intervals].copy())
+            allsp = np.zeros(out.n_hsps, dtype=SP_DTYPE)
+            if out.n_hsps:
+                C.memmove(_ptr(allsp), out.hsps, out.n_hsps * 16)
+            timing = {"t_submit": out.t_submit, "t_start": out.t_start, "t_done": out.t_done,
+                      "batch_id": int(out.batch_id), "batch_requests": int(out.batch_requests)}
+        finally:
+            lib().hspz_request_result_free(C.byref(out))
+        res, at = [], 0
+        for i in range(len(counts) // 2):
+            fw = allsp[at:at + counts[2 * i]]
+            at += counts[2 * i]
+            rc = allsp[at:at + counts[2 * i + 1]]
+            at += counts[2 * i + 1]
+            res.append((fw, rc))
+        return res, timing
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().hspz_server_get_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hspz_server_free(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def hash_pass(intervals) -> int:

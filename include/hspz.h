@@ -202,6 +202,51 @@ int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, const hspz_quer
                       size_t n_chunks, hspz_batch_output *out, hspz_pass_stats *stats);
 void hspz_batch_output_free(hspz_batch_output *out);
 
+/* ---- Serving loop --------------------------------------------------------- */
+
+/* A continuous-batching request loop over resident engines (no counterpart in
+ * the reference; its callers are serving front ends). Each worker thread owns
+ * one engine (same reference, same device, its own stream) and repeatedly
+ * admits the longest FIFO prefix of waiting requests that fits the remaining
+ * HBM budget, runs one hspz_filter_batch over them and publishes each
+ * request's lists, which equal hspz_run_pass on that request alone. */
+typedef struct hspz_server hspz_server;
+typedef struct hspz_server_config {
+    hspz_pass_params params;
+    uint64_t hbm_budget;         /* device bytes admitted at once over all workers; 0 = 1/4 of free */
+    uint32_t max_batch_requests; /* 0 = 256 */
+    uint64_t max_batch_bases;    /* query bases per pass; 0 = 64 Mi */
+} hspz_server_config;
+
+typedef struct hspz_request_result {
+    int32_t status;      /* 0 or the failing pass's status */
+    hspz_segment_pair *hsps;
+    size_t n_hsps;
+    size_t *counts;      /* 2 per interval: fw, rc */
+    size_t n_intervals;
+    double t_submit, t_start, t_done; /* seconds since the server started */
+    uint32_t batch_id;
+    uint32_t batch_requests;
+} hspz_request_result;
+
+typedef struct hspz_server_stats {
+    uint64_t batches, requests, bases, hits, rejected, failed_batches, max_batch_requests;
+    uint64_t queued, in_flight_bytes;
+} hspz_server_stats;
+
+int hspz_server_new(hspz_engine **engines, size_t n_engines, const hspz_server_config *cfg,
+                    hspz_server **out);
+/* Queues one request (encoded fwd/rc block, copied). Refuses with HSPZ_E_OOM a
+ * request whose estimated footprint exceeds the whole budget. */
+int hspz_server_submit(hspz_server *s, const uint8_t *fwd, const uint8_t *rc, size_t len,
+                       uint64_t *request_id);
+/* Blocks until the request is done; release with hspz_request_result_free. */
+int hspz_server_wait(hspz_server *s, uint64_t request_id, hspz_request_result *out);
+void hspz_request_result_free(hspz_request_result *r);
+int hspz_server_get_stats(hspz_server *s, hspz_server_stats *out);
+/* Stops admitting new requests, finishes the queued ones, joins the workers. */
+void hspz_server_free(hspz_server *s);
+
 void hspz_free(void *p);
 
 #ifdef __cplusplus

@@ -1,0 +1,81 @@
+"""The serving loop: requests submitted together are batched (continuous
+batching over two engines) and every request's lists equal the same request
+processed alone; admission refuses a request over the HBM budget."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from helpers import same_intervals
+from hspz_amd import engine as E
+from hspz_amd import sequence as S
+from hspz_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(n_target=400_000, seed=0x5E11):
+    target = synth.random_bases(seed, n_target).tobytes()
+    sub = O.sub_mat("", 910)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    return target, sub, index, pos
+
+
+def _requests(target, n, seed):
+    t = np.frombuffer(target, dtype=np.uint8)
+    pieces = synth.sample_chunks(t, seed, n, 20_000, 1.0, 1_000, 150_000, 0.02, 0.3)
+    out = []
+    for i, p in enumerate(pieces):
+        buf, chrs, bl = S.pack([("r%d" % i, p)])
+        rc, _ = S.reverse_complement(buf, chrs, bl)
+        out.append((S.encode(buf[:bl]), S.encode(rc)))
+    return out
+
+
+def test_served_results_equal_alone(gpu_available):
+    print("case serve: 40 requests through two engines vs each alone", flush=True)
+    target, sub, index, pos = _setup()
+    reqs = _requests(target, 40, 3)
+    ref_enc = S.encode(target)
+    e1 = E.Engine(index, pos, ref_enc, sub, max_hits=16711680)
+    e2 = E.Engine(index, pos, ref_enc, sub, max_hits=16711680)
+    alone = E.Engine(index, pos, ref_enc, sub, max_hits=16711680)
+    want = []
+    for f, r in reqs:
+        alone.swap_query(f, r)
+        want.append(alone.run_pass()[0])
+    with E.Server([e1, e2], max_batch_requests=16) as srv:
+        ids = []
+        # arrivals spread over time so that admission happens while passes run
+        for i, (f, r) in enumerate(reqs):
+            ids.append(srv.submit(f, r))
+            if i % 8 == 7:
+                time.sleep(0.01)
+        got = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    for (res, timing), w in zip(got, want):
+        assert same_intervals(res, w)
+        assert timing["t_done"] >= timing["t_start"] >= timing["t_submit"] - 1e-9
+    assert st["requests"] == len(reqs) and st["failed_batches"] == 0
+    assert st["max_batch_requests"] > 1, "continuous batching merged no requests"
+    for e in (e1, e2, alone):
+        e.close()
+
+
+def test_admission_refuses_request_over_budget(gpu_available):
+    print("case serve: admission by HBM footprint", flush=True)
+    target, sub, index, pos = _setup(100_000)
+    eng = E.Engine(index, pos, S.encode(target), sub, max_hits=16711680)
+    reqs = _requests(target, 2, 5)
+    with E.Server([eng], hbm_budget=4 << 20) as srv:
+        small = np.full(1000, 0, dtype=np.uint8)
+        rid = srv.submit(small, small)
+        res, _ = srv.wait(rid)
+        with pytest.raises(E.HspzError) as ei:
+            big = np.zeros(200_000, dtype=np.uint8)
+            srv.submit(big, big)
+        assert ei.value.status == 7
+        assert srv.stats()["rejected"] == 1
+    eng.close()
