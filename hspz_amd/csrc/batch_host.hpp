@@ -29,9 +29,22 @@ enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CT
 
 }  // namespace
 
+// Everything a captured device pass depends on: if a call's key equals the
+// cached one, the instantiated graph is replayed instead of relaunching.
+struct GraphKey {
+    const void *ptrs[32];
+    uint64_t vals[16];
+    uint32_t shape[33];
+};
+
 struct hspz_batch_state {
     BatchBufs b;
     uint32_t cap_surv = 1u << 22, cap_slow = 1u << 22;
+    bool have_graph = false;
+    GraphKey key;
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};       // before, the gate, after
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    uint64_t graph_captures = 0, graph_replays = 0;
 };
 
 namespace {
@@ -137,13 +150,24 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         if ((rc = reserve(h, B.ohsp, (size_t)bs->cap_surv * 16))) return rc;
         if ((rc = reserve(h, B.oscope, (size_t)bs->cap_surv * 4))) return rc;
 
+        uint32_t nt = std::max<uint32_t>(ceil_div(npos, TILE), 1);
+        uint32_t kt = ceil_div(bs->cap_surv, TILE);
+        if ((rc = reserve(h, h->tiles, (size_t)std::max(kt, nt) * 4))) return rc;
+        BlockOf bo;
+        bo.seeds = B.seeds.as<uint64_t>();
+        bo.voff = B.voff.as<uint32_t>();
+        bo.blk_off = B.blk_off.as<uint32_t>();
+        bo.blk_len = B.blk_len.as<uint32_t>();
+        bo.batches = B.desc.as<BatchDesc>();
+        bo.nb = nb;
+        bo.per = per;
+        const uint8_t *r = h->ref.as<uint8_t>() + PAD;
+        auto enqueue_pre = [&]() {
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, h->stream, ctr, (uint32_t)CTR_N, 0u);
         // seeding
         hipLaunchKernelGGL(k_bseed_valid, dim3(std::min<uint32_t>(ceil_div(npos, 256), 16384)), dim3(256), 0, h->stream,
                            ar, B.blk_off.as<uint32_t>(), B.desc.as<BatchDesc>(), B.poff.as<uint32_t>(), nb,
                            (uint32_t)npos, s.size, B.valid.as<uint8_t>());
-        uint32_t nt = std::max<uint32_t>(ceil_div(npos, TILE), 1);
-        if ((rc = reserve(h, h->tiles, (size_t)nt * 4))) return rc;
         hipLaunchKernelGGL(k_tile_count_u8, dim3(nt), dim3(TILE_THREADS), 0, h->stream, B.valid.as<uint8_t>(),
                            (uint32_t)npos, h->tiles.as<uint32_t>());
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, h->stream, h->tiles.as<uint32_t>(), nt,
@@ -170,26 +194,19 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                            B.voff.as<uint32_t>(), nb, per, h->max_hits, B.nchunks.as<uint32_t>(),
                            B.starts.as<uint32_t>(), B.bhits.as<uint64_t>(), ctr + CTR_ERR);
         // fused expansion + packed gate, then the exact gate for deferred hits
-        HIP_TRY(hipEventRecord(h->ev_gate0, h->stream));
+        };
+        auto enqueue_gate = [&]() {
         hipLaunchKernelGGL(k_bgate_packed, dim3(h->bgate_blocks), dim3(512), 0, h->stream, h->ref2.as<uint32_t>(),
                            h->refx.as<uint32_t>(), h->ref_len, B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(),
                            arena_len, h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.seeds.as<uint64_t>(),
                            ctr + CTR_NVALID, per, h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->seed_size,
                            B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(), ctr + CTR_SLOW,
                            bs->cap_slow, ctr + CTR_ERR);
-        BlockOf bo;
-        bo.seeds = B.seeds.as<uint64_t>();
-        bo.voff = B.voff.as<uint32_t>();
-        bo.blk_off = B.blk_off.as<uint32_t>();
-        bo.blk_len = B.blk_len.as<uint32_t>();
-        bo.batches = B.desc.as<BatchDesc>();
-        bo.nb = nb;
-        bo.per = per;
-        const uint8_t *r = h->ref.as<uint8_t>() + PAD;
         hipLaunchKernelGGL(k_bgate_wave, dim3(2048), dim3(256), 0, h->stream, r, h->ref_len, ar, bo,
                            h->sub.as<int32_t>(), h->xdrop, h->hspthresh, B.slow.as<SurvRec>(), ctr + CTR_SLOW,
                            bs->cap_slow, B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, ctr + CTR_ERR);
-        HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));
+        };
+        auto enqueue_post = [&]() {
         // materializer and compaction of kept records
         hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(ceil_div(bs->cap_surv / 4 + 4, 256), 4096)), dim3(256), 0,
                            h->stream, (uint32_t *)B.keep.p, bs->cap_surv / 4 + 4, 0u);
@@ -197,8 +214,6 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                            B.starts.as<uint32_t>(), B.nchunks.as<uint32_t>(), h->sub.as<int32_t>(), h->xdrop,
                            h->hspthresh, h->noentropy, 1.3862943649291992, B.surv.as<SurvRec>(), ctr + CTR_SURV,
                            bs->cap_surv, B.mhsp.as<SegmentPair>(), B.mscope.as<uint32_t>(), B.keep.as<uint8_t>());
-        uint32_t kt = ceil_div(bs->cap_surv, TILE);
-        if ((rc = reserve(h, h->tiles, (size_t)std::max(kt, nt) * 4))) return rc;
         hipLaunchKernelGGL(k_tile_count_u8, dim3(kt), dim3(TILE_THREADS), 0, h->stream, B.keep.as<uint8_t>(),
                            bs->cap_surv, h->tiles.as<uint32_t>());
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, h->stream, h->tiles.as<uint32_t>(), kt,
@@ -208,6 +223,58 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         hipLaunchKernelGGL(k_gather_scoped, dim3(1024), dim3(256), 0, h->stream, B.mhsp.as<SegmentPair>(),
                            B.mscope.as<uint32_t>(), B.kids.as<uint32_t>(), ctr + CTR_KEPT, B.ohsp.as<SegmentPair>(),
                            B.oscope.as<uint32_t>());
+        };
+        if (h->batch_graph) {
+            GraphKey key;
+            memset(&key, 0, sizeof key);
+            const void *ptrs[] = {B.arena.p, B.plane.p, B.coarse.p, B.desc.p, B.poff.p, B.blk_off.p, B.blk_len.p,
+                                  B.valid.p, B.ids.p, B.voff.p, B.seeds.p, B.counts.p, B.prefix.p, B.tiles.p,
+                                  B.nchunks.p, B.starts.p, B.bhits.p, B.surv.p, B.slow.p, B.mhsp.p, B.mscope.p,
+                                  B.keep.p, B.kids.p, B.ohsp.p, B.oscope.p, B.ctr.p, h->tiles.p, h->ref2.p,
+                                  h->refx.p, h->pairtab.p, h->index.p, h->pos.p};
+            memcpy(key.ptrs, ptrs, sizeof ptrs);
+            uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow, cap_seeds, per, h->max_hits,
+                               (uint64_t)h->bgate_blocks, (uint64_t)p->transitions, nt, kt, cap_tiles,
+                               (uint64_t)(uintptr_t)h->ref.p, (uint64_t)(uintptr_t)h->sub.p, s.kmer_size};
+            memcpy(key.vals, vals, sizeof vals);
+            memcpy(key.shape, s.pos, sizeof s.pos);
+            key.shape[32] = s.size;
+            if (!bs->have_graph || memcmp(&key, &bs->key, sizeof key) != 0) {
+                for (int g = 0; g < 3; g++) {
+                    if (bs->exec[g]) (void)hipGraphExecDestroy(bs->exec[g]);
+                    if (bs->graph[g]) (void)hipGraphDestroy(bs->graph[g]);
+                    bs->exec[g] = nullptr;
+                    bs->graph[g] = nullptr;
+                }
+                bs->have_graph = false;
+                for (int g = 0; g < 3; g++) {
+                    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+                    if (g == 0) enqueue_pre();
+                    else if (g == 1) enqueue_gate();
+                    else enqueue_post();
+                    hipError_t ce = hipStreamEndCapture(h->stream, &bs->graph[g]);
+                    if (ce != hipSuccess) return set_error(-(int)ce, "graph capture: %s", hipGetErrorString(ce));
+                    HIP_TRY(hipGraphInstantiate(&bs->exec[g], bs->graph[g], nullptr, nullptr, 0));
+                }
+                bs->key = key;
+                bs->have_graph = true;
+                bs->graph_captures++;
+            } else {
+                bs->graph_replays++;
+            }
+            // timing events sit between the graph launches, on the stream
+            HIP_TRY(hipGraphLaunch(bs->exec[0], h->stream));
+            HIP_TRY(hipEventRecord(h->ev_gate0, h->stream));
+            HIP_TRY(hipGraphLaunch(bs->exec[1], h->stream));
+            HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));
+            HIP_TRY(hipGraphLaunch(bs->exec[2], h->stream));
+        } else {
+            enqueue_pre();
+            HIP_TRY(hipEventRecord(h->ev_gate0, h->stream));
+            enqueue_gate();
+            HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));
+            enqueue_post();
+        }
         h->launches += 22;
         HIP_TRY(hipGetLastError());
         uint32_t hc[CTR_N];
@@ -216,7 +283,10 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         h->pipeline_syncs++;
         memcpy(hc, h->h_small, sizeof hc);
         float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, h->ev_gate0, h->ev_gate1));
+        if (hipEventElapsedTime(&ms, h->ev_gate0, h->ev_gate1) != hipSuccess) {
+            (void)hipGetLastError();
+            ms = 0;
+        }
         if (hc[CTR_ERR] & 8u) {  // survivor or slow list overflowed: grow and rerun
             bs->cap_surv = std::max(bs->cap_surv, hc[CTR_SURV]) * 2;
             bs->cap_slow = std::max(bs->cap_slow, hc[CTR_SLOW]) * 2;

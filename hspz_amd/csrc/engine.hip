@@ -80,6 +80,7 @@ struct hspz_engine {
     int gate_blocks = 2048;
     int bgate_blocks = 1024;  // persistent grid of the batched gate: 4 x 512-thread blocks per CU
     hspz_batch_state *batch = nullptr;
+    bool batch_graph = true;  // capture the batched device pass in a hipGraph and replay it
 };
 
 namespace {
@@ -436,6 +437,10 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
                         &B.voff, &B.seeds, &B.counts, &B.prefix, &B.tiles, &B.nchunks, &B.starts, &B.bhits, &B.surv,
                         &B.slow, &B.mhsp, &B.mscope, &B.keep, &B.kids, &B.ohsp, &B.oscope, &B.ctr};
         for (DevBuf *b : bb) release(*b);
+        for (int g = 0; g < 3; g++) {
+            if (h->batch->exec[g]) (void)hipGraphExecDestroy(h->batch->exec[g]);
+            if (h->batch->graph[g]) (void)hipGraphDestroy(h->batch->graph[g]);
+        }
         delete h->batch;
     }
     if (h->h_small) (void)hipHostFree(h->h_small);
@@ -547,6 +552,20 @@ extern "C" void hspz_filter_output_free(hspz_filter_output *out) {
     out->n_hsps = 0;
 }
 
+extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
+    clear_error();
+    if (!h) return set_error(HSPZ_E_ARG, "null engine");
+    switch (which) {
+    case HSPZ_FLAG_BATCH_GRAPH: h->batch_graph = value != 0; break;
+    case HSPZ_FLAG_BATCH_GATE_BLOCKS:
+        if (value < 1) return set_error(HSPZ_E_ARG, "gate blocks must be >= 1");
+        h->bgate_blocks = value;
+        break;
+    default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
+    }
+    return 0;
+}
+
 extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
     clear_error();
     if (!h || !value) return set_error(HSPZ_E_ARG, "null argument");
@@ -561,6 +580,8 @@ extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
     case HSPZ_STAT_GATE_HITS: *value = h->gate_hits; break;
     case HSPZ_STAT_GATE_LAUNCHES: *value = h->gate_launches; break;
     case HSPZ_STAT_GATE_DEFERRED: *value = h->gate_deferred; break;
+    case HSPZ_STAT_GRAPH_CAPTURES: *value = h->batch ? h->batch->graph_captures : 0; break;
+    case HSPZ_STAT_GRAPH_REPLAYS: *value = h->batch ? h->batch->graph_replays : 0; break;
     default: return set_error(HSPZ_E_ARG, "unknown stat %d", which);
     }
     return 0;

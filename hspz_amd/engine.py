@@ -22,7 +22,8 @@ SP_DTYPE = np.dtype([("ref_start", "<u4"), ("query_start", "<u4"), ("len", "<u4"
 
 STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 3,
         "query_swaps": 4, "peak_device_bytes": 5, "gate_ns": 6, "gate_hits": 7,
-        "gate_launches": 8, "gate_deferred": 9}
+        "gate_launches": 8, "gate_deferred": 9, "graph_captures": 10, "graph_replays": 11}
+FLAG = {"batch_graph": 0, "batch_gate_blocks": 1}
 
 # Every symbol include/hspz.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -32,7 +33,7 @@ EXPORTS = [
     "hspz_seed_table_build", "hspz_shape_parse", "hspz_chunk_seeds", "hspz_dedup_and_order",
     "hspz_run_pass", "hspz_free", "hspz_build_sub_mat", "hspz_filter_batch",
     "hspz_batch_output_free", "hspz_server_new", "hspz_server_submit", "hspz_server_wait",
-    "hspz_request_result_free", "hspz_server_get_stats", "hspz_server_free",
+    "hspz_request_result_free", "hspz_server_get_stats", "hspz_server_free", "hspz_set_flag",
 ]
 
 
@@ -122,6 +123,7 @@ def lib():
         L.hspz_filter_output_free.argtypes = [C.POINTER(FilterOutput)]
         L.hspz_filter_output_free.restype = None
         L.hspz_get_stat.argtypes = [vp, C.c_int, C.POINTER(C.c_uint64)]
+        L.hspz_set_flag.argtypes = [vp, C.c_int, C.c_int]
         L.hspz_default_max_hits.argtypes = [C.c_uint64]
         L.hspz_default_max_hits.restype = u32
         L.hspz_device_memory.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
@@ -289,6 +291,9 @@ class Engine:
             C.memmove(_ptr(res), out.hsps, out.n_hsps * 16)
         lib().hspz_filter_output_free(C.byref(out))
         return res, int(out.num_hits), int(out.raw_hsps)
+
+    def set_flag(self, name: str, value: int):
+        _check(lib().hspz_set_flag(self._h, FLAG[name], int(value)))
 
     def stat(self, name: str) -> int:
         v = C.c_uint64()

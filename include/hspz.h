@@ -109,8 +109,16 @@ void hspz_filter_output_free(hspz_filter_output *out);
 #define HSPZ_STAT_GATE_NS 6        /* cumulative score-gate kernel time (HIP events) */
 #define HSPZ_STAT_GATE_HITS 7      /* hits the score gate has scored */
 #define HSPZ_STAT_GATE_LAUNCHES 8
-#define HSPZ_STAT_GATE_DEFERRED 9  /* hits the packed gate handed to the byte-code gate */
+#define HSPZ_STAT_GATE_DEFERRED 9  /* hits the packed gate handed to the exact wave gate */
+#define HSPZ_STAT_GRAPH_CAPTURES 10 /* batched device passes captured into a hipGraph */
+#define HSPZ_STAT_GRAPH_REPLAYS 11  /* batched device passes replayed from the cached graph */
 int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
+
+/* Engine switches (the reference's pub flags, gpu/mod.rs:836-949, have no
+ * meaning here; these are this engine's own). */
+#define HSPZ_FLAG_BATCH_GRAPH 0       /* 1 (default): capture/replay the batched pass as a hipGraph */
+#define HSPZ_FLAG_BATCH_GATE_BLOCKS 1 /* persistent grid of the batched gate, in 512-thread blocks */
+int hspz_set_flag(hspz_engine *h, int which, int value);
 
 /* resolve_max_hits / device_count / device_memory (gpu/mod.rs:2436-2463). */
 uint32_t hspz_default_max_hits(uint64_t total_device_bytes);

@@ -93,3 +93,25 @@ def test_batch_repeat_fixture_digest(gpu_available):
         with E.Engine(o.index, o.pos, o.ref_enc, p.sub, max_hits=16711680) as eng:
             got, _ = eng.filter_batch([(o.q_enc, o.q_rc_enc), (o.q_enc, o.q_rc_enc)])
         assert p.segments_digest(got[0]) == want and p.segments_digest(got[1]) == want
+
+
+def test_batch_graph_replays_match_eager(gpu_available):
+    _say("case batch: hipGraph capture once, replays x3 with eager work between, vs eager")
+    target = synth.random_bases(0xB004, 300_000).tobytes()
+    blocks = _chunks_from(target, 10, 21)
+    sub = O.sub_mat("", 910)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
+    with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
+        eng.set_flag("batch_graph", 0)
+        want, _ = eng.filter_batch(chunks)
+        eng.set_flag("batch_graph", 1)
+        c0, r0 = eng.stat("graph_captures"), eng.stat("graph_replays")
+        for k in range(3):
+            got, _ = eng.filter_batch(chunks)
+            for g, w in zip(got, want):
+                assert same_intervals(g, w), "replay %d differs from eager" % k
+            eng.swap_query(*chunks[k])          # eager work between replays
+            eng.run_pass()
+        assert eng.stat("graph_captures") - c0 == 1
+        assert eng.stat("graph_replays") - r0 == 2
