@@ -16,6 +16,10 @@ Plain: per chunk, swap_query + the reference-shaped per-batch pipeline
 (hspz_run_pass), reported beside it. Both produce the same lists (checked).
 Target generation, index build and upload are setup, outside the timed region.
 
+Before the timed region the batched gate's autotuner (hspz_autotune_batch)
+measures its launch grid on the leading ~1 Mbp of chunks and keeps the fastest
+cell; every measured cell is reported under `autotune`.
+
 Prints one JSON line (N = 1). `roofline` covers the fused expansion + score
 gate kernel of the batched path (HIP events around its launches);
 `cpu_baseline` is the C oracle on a bounded sample of the same chunks.
@@ -57,6 +61,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=150,
                     help="requests per arrival-rate arm of the serving measurement (0 = skip)")
     ap.add_argument("--serve-rates", default="50,200,800")
+    ap.add_argument("--tune-mbp", type=float, default=1.0,
+                    help="query Mbp the gate autotuner measures on before the timed region (0 = skip)")
+    ap.add_argument("--tune-reps", type=int, default=1)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -120,6 +127,33 @@ def main():
         log("plain per-chunk only: %.3f Mbp/s" % (query_bp / 1e6 / (time.perf_counter() - t_p)))
         eng.close()
         return
+
+    tune = None
+    if args.tune_mbp > 0:
+        # autotuner on the leading chunks (untimed): every cell is reported
+        sample, sbp = [], 0
+        for f, r in enc:
+            if sbp >= args.tune_mbp * 1e6:
+                break
+            sample.append((f, r))
+            sbp += len(f)
+        t1 = time.time()
+        cells, best = eng.autotune(sample, reps=args.tune_reps)
+        for c in cells:
+            log("tune step %2d threads %3d grid %5d claim %2d: gate %.2f ms + after %.2f ms "
+                "(%d hits, %d survivors)" % (c["step_cells"], c["block_threads"], c["grid_blocks"],
+                                             c["claim_seeds"], c["gate_ms"], c["after_ms"],
+                                             c["hits"], c["survivors"]))
+        log("autotuner: %d cells on %d bp in %.1fs; chose step %d threads %d grid %d claim %d"
+            % (len(cells), sbp, time.time() - t1, best["step_cells"], best["block_threads"],
+               best["grid_blocks"], best["claim_seeds"]))
+        tune = {"sample_bp": sbp, "reps": args.tune_reps,
+                "columns": ["step_cells", "block_threads", "grid_blocks", "claim_seeds", "gate_ms",
+                            "after_ms"],
+                "cells": [[c["step_cells"], c["block_threads"], c["grid_blocks"], c["claim_seeds"],
+                           round(c["gate_ms"], 3), round(c["after_ms"], 3)] for c in cells],
+                "chosen": [best["step_cells"], best["block_threads"], best["grid_blocks"],
+                           best["claim_seeds"]]}
 
     for i in range(args.warmup):
         res_b, st = batched()
@@ -229,6 +263,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "serving": serving,
+        "autotune": tune,
     }
     print(json.dumps(out), flush=True)
     eng.close()
@@ -244,6 +279,8 @@ def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rat
     import threading
     import numpy as np
     eng2 = E.Engine(index, pos, ref_enc, sub, max_hits=max_hits)
+    for name in ("batch_step_cells", "batch_block_threads", "batch_claim_seeds", "batch_gate_blocks"):
+        eng2.set_flag(name, eng.get_flag(name))  # the tuned gate configuration
     arms = []
     for k, rate in enumerate(rates):
         pieces = synth.sample_chunks(target, 0xC0FFEE05 + 17 * k, n, 100_000, 1.535, 1_000,

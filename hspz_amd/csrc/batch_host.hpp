@@ -25,7 +25,21 @@ struct BatchBufs {
         tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr;
 };
 
-enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_N = 8 };
+// Device counters of one pass; CTR_WORK is the 64-bit seed-claim counter of
+// the batched gate (two slots, 8-byte aligned).
+enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_WORK = 6, CTR_N = 8 };
+
+typedef void (*BGateFn)(const uint32_t *, const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, uint32_t,
+                        const uint32_t *, int32_t, int32_t, const uint64_t *, const uint32_t *, uint32_t,
+                        const uint32_t *, const uint32_t *, uint32_t, uint32_t, unsigned long long *, SurvRec *,
+                        uint32_t *, uint32_t, SurvRec *, uint32_t *, uint32_t, uint32_t *);
+
+// The batched gate instantiation for a step width and block size.
+BGateFn bgate_kernel(int step_cells, int threads) {
+    if (threads == 256)
+        return step_cells == 16 ? k_bgate_packed<1, 256> : step_cells == 64 ? k_bgate_packed<4, 256> : k_bgate_packed<2, 256>;
+    return step_cells == 16 ? k_bgate_packed<1, 512> : step_cells == 64 ? k_bgate_packed<4, 512> : k_bgate_packed<2, 512>;
+}
 
 }  // namespace
 
@@ -33,7 +47,7 @@ enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CT
 // cached one, the instantiated graph is replayed instead of relaunching.
 struct GraphKey {
     const void *ptrs[32];
-    uint64_t vals[16];
+    uint64_t vals[24];
     uint32_t shape[33];
 };
 
@@ -195,13 +209,22 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                            B.starts.as<uint32_t>(), B.bhits.as<uint64_t>(), ctr + CTR_ERR);
         // fused expansion + packed gate, then the exact gate for deferred hits
         };
+        BGateFn gate_fn = bgate_kernel(h->bgate_step, h->bgate_threads);
+        int grid = h->bgate_blocks;
+        if (grid == 0) {  // resident grid: occupancy of this instantiation x CUs
+            int per_cu = 0;
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gate_fn, h->bgate_threads, 0));
+            grid = std::max(per_cu, 1) * h->num_cus;
+        }
+        h->bgate_grid_used = grid;
         auto enqueue_gate = [&]() {
-        hipLaunchKernelGGL(k_bgate_packed, dim3(h->bgate_blocks), dim3(512), 0, h->stream, h->ref2.as<uint32_t>(),
+        hipLaunchKernelGGL(gate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
                            h->refx.as<uint32_t>(), h->ref_len, B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(),
                            arena_len, h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.seeds.as<uint64_t>(),
                            ctr + CTR_NVALID, per, h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->seed_size,
-                           B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(), ctr + CTR_SLOW,
-                           bs->cap_slow, ctr + CTR_ERR);
+                           (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK), B.surv.as<SurvRec>(),
+                           ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(), ctr + CTR_SLOW, bs->cap_slow,
+                           ctr + CTR_ERR);
         hipLaunchKernelGGL(k_bgate_wave, dim3(2048), dim3(256), 0, h->stream, r, h->ref_len, ar, bo,
                            h->sub.as<int32_t>(), h->xdrop, h->hspthresh, B.slow.as<SurvRec>(), ctr + CTR_SLOW,
                            bs->cap_slow, B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, ctr + CTR_ERR);
@@ -234,8 +257,9 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                   h->refx.p, h->pairtab.p, h->index.p, h->pos.p};
             memcpy(key.ptrs, ptrs, sizeof ptrs);
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow, cap_seeds, per, h->max_hits,
-                               (uint64_t)h->bgate_blocks, (uint64_t)p->transitions, nt, kt, cap_tiles,
-                               (uint64_t)(uintptr_t)h->ref.p, (uint64_t)(uintptr_t)h->sub.p, s.kmer_size};
+                               (uint64_t)grid | (uint64_t)h->bgate_claim << 32, (uint64_t)p->transitions, nt, kt,
+                               cap_tiles, (uint64_t)h->bgate_step | (uint64_t)h->bgate_threads << 32,
+                               (uint64_t)(uintptr_t)h->sub.p, s.kmer_size, (uint64_t)(uintptr_t)h->ref.p};
             memcpy(key.vals, vals, sizeof vals);
             memcpy(key.shape, s.pos, sizeof s.pos);
             key.shape[32] = s.size;
@@ -268,12 +292,14 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             HIP_TRY(hipGraphLaunch(bs->exec[1], h->stream));
             HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));
             HIP_TRY(hipGraphLaunch(bs->exec[2], h->stream));
+            HIP_TRY(hipEventRecord(h->ev_post1, h->stream));
         } else {
             enqueue_pre();
             HIP_TRY(hipEventRecord(h->ev_gate0, h->stream));
             enqueue_gate();
             HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));
             enqueue_post();
+            HIP_TRY(hipEventRecord(h->ev_post1, h->stream));
         }
         h->launches += 22;
         HIP_TRY(hipGetLastError());
@@ -282,11 +308,19 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         HIP_TRY(hipStreamSynchronize(h->stream));
         h->pipeline_syncs++;
         memcpy(hc, h->h_small, sizeof hc);
-        float ms = 0;
+        float ms = 0, post_ms = 0;
         if (hipEventElapsedTime(&ms, h->ev_gate0, h->ev_gate1) != hipSuccess) {
             (void)hipGetLastError();
             ms = 0;
         }
+        if (hipEventElapsedTime(&post_ms, h->ev_gate1, h->ev_post1) != hipSuccess) {
+            (void)hipGetLastError();
+            post_ms = 0;
+        }
+        h->last_gate_ms = ms;
+        h->last_post_ms = post_ms;
+        h->last_surv = hc[CTR_SURV];
+        h->last_deferred = hc[CTR_SLOW];
         if (hc[CTR_ERR] & 8u) {  // survivor or slow list overflowed: grow and rerun
             bs->cap_surv = std::max(bs->cap_surv, hc[CTR_SURV]) * 2;
             bs->cap_slow = std::max(bs->cap_slow, hc[CTR_SLOW]) * 2;

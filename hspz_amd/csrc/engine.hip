@@ -51,7 +51,7 @@ struct hspz_batch_state;  // batch_host.hpp
 struct hspz_engine {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_gate0 = nullptr, ev_gate1 = nullptr;
+    hipEvent_t ev_gate0 = nullptr, ev_gate1 = nullptr, ev_post1 = nullptr;
     // reference side (resident)
     DevBuf ref, index, pos, sub;
     uint32_t ref_len = 0;
@@ -78,7 +78,13 @@ struct hspz_engine {
     uint64_t dev_bytes = 0, peak_dev_bytes = 0;
     uint64_t gate_ns = 0, gate_hits = 0, gate_launches = 0, gate_deferred = 0;
     int gate_blocks = 2048;
-    int bgate_blocks = 1024;  // persistent grid of the batched gate: 4 x 512-thread blocks per CU
+    // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
+    int bgate_blocks = 0;  // persistent grid; 0 = as many blocks as are resident at once
+    int bgate_step = 32, bgate_threads = 512, bgate_claim = 16;
+    int bgate_grid_used = 0;                // grid of the last batched launch
+    int num_cus = 0;
+    double last_gate_ms = 0, last_post_ms = 0;  // event times of the last batched pass
+    uint64_t last_surv = 0, last_deferred = 0;   // its survivor and deferred-hit counts
     hspz_batch_state *batch = nullptr;
     bool batch_graph = true;  // capture the batched device pass in a hipGraph and replay it
 };
@@ -318,6 +324,7 @@ int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_seg
 
 #include "batch_host.hpp"
 #include "server.hpp"
+#include "tune_host.hpp"
 
 // ------------------------------------------------------------------ C ABI
 
@@ -365,10 +372,14 @@ extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_e
         return rc;
     };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_error(-1, "stream"));
-    if (hipEventCreate(&h->ev_gate0) != hipSuccess || hipEventCreate(&h->ev_gate1) != hipSuccess)
+    if (hipEventCreate(&h->ev_gate0) != hipSuccess || hipEventCreate(&h->ev_gate1) != hipSuccess ||
+        hipEventCreate(&h->ev_post1) != hipSuccess)
         return fail(set_error(-1, "events"));
     if (hipHostMalloc((void **)&h->h_small, 64, hipHostMallocDefault) != hipSuccess)
         return fail(set_error(HSPZ_E_OOM, "pinned scratch"));
+    if (hipDeviceGetAttribute(&h->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        h->num_cus < 1)
+        return fail(set_error(-1, "device attributes"));
     h->seed_size = cfg->seed_size;
     h->xdrop = cfg->xdrop;
     h->hspthresh = cfg->hspthresh;
@@ -446,6 +457,7 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
     if (h->h_small) (void)hipHostFree(h->h_small);
     if (h->ev_gate0) (void)hipEventDestroy(h->ev_gate0);
     if (h->ev_gate1) (void)hipEventDestroy(h->ev_gate1);
+    if (h->ev_post1) (void)hipEventDestroy(h->ev_post1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -558,9 +570,35 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
     switch (which) {
     case HSPZ_FLAG_BATCH_GRAPH: h->batch_graph = value != 0; break;
     case HSPZ_FLAG_BATCH_GATE_BLOCKS:
-        if (value < 1) return set_error(HSPZ_E_ARG, "gate blocks must be >= 1");
+        if (value < 0 || value > 65536) return set_error(HSPZ_E_ARG, "gate blocks must be in [0, 65536]");
         h->bgate_blocks = value;
         break;
+    case HSPZ_FLAG_BATCH_STEP_CELLS:
+        if (value != 16 && value != 32 && value != 64) return set_error(HSPZ_E_ARG, "step cells must be 16, 32 or 64");
+        h->bgate_step = value;
+        break;
+    case HSPZ_FLAG_BATCH_BLOCK_THREADS:
+        if (value != 256 && value != 512) return set_error(HSPZ_E_ARG, "block threads must be 256 or 512");
+        h->bgate_threads = value;
+        break;
+    case HSPZ_FLAG_BATCH_CLAIM_SEEDS:
+        if (value < 1 || value > 4096) return set_error(HSPZ_E_ARG, "claim seeds must be in [1, 4096]");
+        h->bgate_claim = value;
+        break;
+    default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
+    }
+    return 0;
+}
+
+extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
+    clear_error();
+    if (!h || !value) return set_error(HSPZ_E_ARG, "null argument");
+    switch (which) {
+    case HSPZ_FLAG_BATCH_GRAPH: *value = h->batch_graph ? 1 : 0; break;
+    case HSPZ_FLAG_BATCH_GATE_BLOCKS: *value = h->bgate_blocks ? h->bgate_blocks : h->bgate_grid_used; break;
+    case HSPZ_FLAG_BATCH_STEP_CELLS: *value = h->bgate_step; break;
+    case HSPZ_FLAG_BATCH_BLOCK_THREADS: *value = h->bgate_threads; break;
+    case HSPZ_FLAG_BATCH_CLAIM_SEEDS: *value = h->bgate_claim; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;

@@ -434,7 +434,8 @@ __device__ __forceinline__ bool coarse_hit(const uint32_t *__restrict__ bits, ui
 
 constexpr uint32_t G2_RIGHT_WORDS = 4;  // 64 right cells in registers
 constexpr uint32_t G2_LEFT_WORDS = 8;   // 128 left cells in registers
-constexpr int32_t G2_MAX_XDROP = 4000;  // packed int16 partial sums of 16 cells stay exact
+constexpr int32_t G2_MAX_XDROP = 4000;  // packed int16 partial sums of 8 cells stay exact
+constexpr int32_t G2_MAX_SCORE = 4095;  // largest ACGT score the packed sums take
 constexpr uint32_t G2_MARGIN = 128;     // fast path needs this many bases either side
 constexpr uint32_t G2_TAB_COPIES = 16;  // LDS copies of each pair table (bank spread)
 
@@ -450,41 +451,53 @@ __device__ __forceinline__ int32_t pair_sum(uint32_t w) {
     return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, w), __builtin_bit_cast(short2_t, 0x00010001u), 0, false);
 }
 
+// Coarse-bitmap test of bases [lo, hi] from the two bitmap words that hold
+// block lo >> 6 (same rule as coarse_hit, with the words already loaded).
+__device__ __forceinline__ bool coarse_bits(uint32_t w0, uint32_t w1, uint32_t lo, uint32_t hi) {
+    uint32_t b0 = lo >> 6, b1 = hi >> 6;
+    uint64_t v = ((uint64_t)w0 | ((uint64_t)w1 << 32)) >> (b0 & 31);
+    return (v & ((1ull << (b1 - b0 + 1)) - 1)) != 0;
+}
+
 // Fast score gate over 2-bit planes, one hit per lane; shared by the plain
 // path (hits from the anchor array) and the batched path (hits generated from
-// seeds). `Src` supplies hits to idle lanes and receives results.
+// seeds). `Src` supplies hits to idle lanes and receives results. `SW` is the
+// step width in plane words: each step evaluates 16 * SW cells (16, 32 or 64).
 //
 // When a lane takes a hit it loads 13 words (208 bases) around the anchor from
-// each plane once and funnels them into a right stream (cells R, R+1, ...) and
-// a left stream (the words below R, unreversed). Each step consumes two word
-// pairs = 32 cells; per word pair (16 cells) both sequences' 2-bit codes of two consecutive cells form
-// an 8-bit key into an LDS pair table whose int16 halves are the two cells'
-// scores. For the left stream the key bytes are taken in reverse order and the
-// halves-swapped table is used, which walks the cells outward from R without
-// reversing any bits. Scores below -(xdrop + 1) are stored as -(xdrop + 1):
-// such a cell ends the extension either way and never raises the maximum.
+// each plane, plus the coarse exception words of both windows, in one round
+// trip, and funnels the words into a right stream (cells R, R+1, ...) and a
+// left stream (the words below R, unreversed). Per word pair (16 cells) both
+// sequences' 2-bit codes of two consecutive cells form an 8-bit key into an
+// LDS pair table whose int16 halves are the two cells' scores. For the left
+// stream the key bytes are taken in reverse order and the halves-swapped table
+// is used, which walks the cells outward from R without reversing any bits.
+// Scores below -(xdrop + 1) are stored as -(xdrop + 1): such a cell ends the
+// extension either way and never raises the maximum.
 //
 // The recurrence runs on e = xdrop - (max - score): e = min(e + s, xdrop), and
 // the extension stops in the step where e first goes negative. Within that
 // step the cells after the stopping cell are still applied, so the maximum
 // taken at the end of the step, c + xdrop - e, is the running maximum over
 // every cell of the step: at least the exact X-drop maximum, and equal to it
-// unless a new maximum follows the drop inside the same 32 cells. The pass
+// unless a new maximum follows the drop inside the same step. The pass
 // decision is therefore an upper bound of the exact gate (never drops a hit the
-// exact gate keeps); every passing hit is re-extended exactly by the
-// materializer, so the HSP output is unchanged.
+// exact gate keeps) for every step width; every passing hit is re-extended
+// exactly by the materializer, so the HSP output does not depend on SW.
 //
 // Hits whose windows touch a non-ACGT code or a sequence end, and extensions
 // that outlive their 64 right / 128 left cells, are deferred to the exact
-// wave-per-hit gate. The 16-cell score sums use packed int16 partial sums,
-// exact while 16 * (xdrop + 1) < 32768 (xdrop <= G2_MAX_XDROP).
-template <class Src>
+// wave-per-hit gate. Each int16 half of a word's packed partial sum adds 8
+// table values in [-(xdrop + 1), G2_MAX_SCORE], exact while both bounds are
+// at most 4095.
+template <int SW, class Src>
 __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__restrict__ rplane,
                                                  const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
                                                  const uint32_t *__restrict__ qplane,
                                                  const uint32_t *__restrict__ qcoarse, uint32_t qlen,
                                                  const uint32_t *tab_fwd, const uint32_t *tab_rev,
                                                  int32_t xdrop, int32_t thresh) {
+    static_assert(SW == 1 || SW == 2 || SW == 4, "step width must divide the right window");
     bool has = false;
     uint32_t R = 0, Q = 0, tag = 0, grp = 0, limit = 0;
     bool left = false;
@@ -503,20 +516,22 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
         if (fresh) {
             bool ok = R >= G2_MARGIN && Q >= G2_MARGIN && (uint64_t)R + G2_MARGIN <= ref_len &&
                       (uint64_t)Q + G2_MARGIN <= qlen;
-            if (ok) ok = !coarse_hit(rcoarse, R - 16 * G2_LEFT_WORDS, R + 16 * G2_RIGHT_WORDS - 1) &&
-                         !coarse_hit(qcoarse, Q - 16 * G2_LEFT_WORDS, Q + 16 * G2_RIGHT_WORDS - 1);
-            if (!ok) {
-                defer = true;
-            } else {
+            if (ok) {
+                const uint32_t rlo = R - 16 * G2_LEFT_WORDS, rhi = R + 16 * G2_RIGHT_WORDS - 1;
+                const uint32_t qlo = Q - 16 * G2_LEFT_WORDS, qhi = Q + 16 * G2_RIGHT_WORDS - 1;
+                const uint32_t *rc = rcoarse + (rlo >> 11);
+                const uint32_t *qc = qcoarse + (qlo >> 11);
                 // words wr-8 .. wr+4 of each plane; word 8 holds base R (resp. Q)
                 const uint32_t *rp = rplane + (R >> 4) - 8;
                 const uint32_t *qp = qplane + (Q >> 4) - 8;
+                uint32_t cr0 = rc[0], cr1 = rc[1], cq0 = qc[0], cq1 = qc[1];
                 uint32_t W[13], V[13];
 #pragma unroll
                 for (int j = 0; j < 13; j++) {
                     W[j] = rp[j];
                     V[j] = qp[j];
                 }
+                ok = !coarse_bits(cr0, cr1, rlo, rhi) && !coarse_bits(cq0, cq1, qlo, qhi);
                 uint32_t sr = 2 * (R & 15), sq = 2 * (Q & 15);
 #pragma unroll
                 for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) {
@@ -529,10 +544,14 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
                     LA[j] = __builtin_amdgcn_alignbit(W[8 - j], W[7 - j], sr);
                     LB[j] = __builtin_amdgcn_alignbit(V[8 - j], V[7 - j], sq);
                 }
+            }
+            if (!ok) {
+                defer = true;
+            } else {
                 left = false;
                 tb = tab_fwd;
                 grp = 0;
-                limit = G2_RIGHT_WORDS / 2;
+                limit = G2_RIGHT_WORDS / SW;
                 c = 0;
                 e = xdrop;
             }
@@ -541,7 +560,7 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
         if (has && !defer) {
             int32_t emin = e;
 #pragma unroll
-            for (int h = 0; h < 2; h++) {  // two words = 32 cells per step
+            for (int h = 0; h < SW; h++) {  // SW words = 16 * SW cells per step
                 uint32_t r = A[h], q = B[h];
                 uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
                 uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
@@ -569,9 +588,9 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
             if (emin >= 0) {
                 grp++;
 #pragma unroll
-                for (int j = 0; j < (int)G2_LEFT_WORDS - 2; j++) {
-                    A[j] = A[j + 2];
-                    B[j] = B[j + 2];
+                for (int j = 0; j + SW < (int)G2_LEFT_WORDS; j++) {
+                    A[j] = A[j + SW];
+                    B[j] = B[j + SW];
                 }
                 if (grp == limit) defer = true;  // outlived the register window
             } else if (!left) {
@@ -579,7 +598,7 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
                 left = true;
                 tb = tab_rev;
                 grp = 0;
-                limit = G2_LEFT_WORDS / 2;
+                limit = G2_LEFT_WORDS / SW;
                 c = 0;
                 e = xdrop;
 #pragma unroll
@@ -670,7 +689,7 @@ __global__ __launch_bounds__(512) void k_gate_packed(
     src.slow = slow;
     src.slow_n = slow_n;
     src.init();
-    packed_gate_loop(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, tab_fwd,
+    packed_gate_loop<2>(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, tab_fwd,
                      tab_fwd + 256 * G2_TAB_COPIES, xdrop, thresh);
 }
 

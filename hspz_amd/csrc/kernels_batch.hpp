@@ -19,7 +19,6 @@ namespace hspz {
 
 constexpr uint32_t QPAD = 256;       // code-7 bytes before every block and after the last
 constexpr uint32_t MAX_CHUNKS = 64;  // max_hits chunks one batch may split into
-constexpr uint32_t SEEDS_PER_CHUNK = 16;
 
 struct BatchDesc {
     uint32_t block;  // arena block (2 * chunk + strand)
@@ -241,25 +240,40 @@ __global__ void k_bchunks(const uint64_t *__restrict__ P, const uint32_t *__rest
     nchunks[b] = num_iter;
 }
 
-// Hit source over seeds (the batched path): a wave walks SEEDS_PER_CHUNK
-// consecutive seeds at a time and hands their hits (pos_table order) to idle
-// lanes; hits are never written out. Survivors and deferred hits are appended
-// as (R, Q, seed) records.
+// Hit source over seeds (the batched path): a wave claims `claim`
+// consecutive seeds at a time from a grid-wide counter (one atomic per claim,
+// so waves that draw short buckets simply claim more often) and hands their
+// hits (pos_table order) to idle lanes; hits are never written out. Survivors
+// and deferred hits are appended as (R, Q, seed) records.
 struct SeedSource {
     const uint64_t *seeds;
     const uint32_t *index, *pos;
-    uint32_t seed_size, nseeds, nwaves;
-    uint32_t chunk, s_cur, s_end, j_cur, cnt_cur, b0_cur, q_cur;
+    unsigned long long *work;  // seeds claimed so far; zero before the launch
+    uint32_t seed_size, nseeds, claim;
+    uint32_t s_cur, s_end, j_cur, cnt_cur, b0_cur, q_cur;
+    bool dry;
     SurvRec *surv, *slow;
     uint32_t *surv_n, *slow_n, cap_surv, cap_slow;
     uint32_t *err;
 
+    // Claims the wave's next seed range [s_cur, s_end); false once every
+    // seed has been claimed.
+    __device__ __forceinline__ bool take() {
+        uint32_t leader = (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1);
+        unsigned long long v = 0;
+        if (lane_id() == leader) v = atomicAdd(work, (unsigned long long)claim);
+        uint64_t start = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+                         (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+        bool ok = start < (uint64_t)nseeds;
+        uint64_t end = start + claim;
+        s_cur = ok ? (uint32_t)start : nseeds;
+        s_end = ok && end < (uint64_t)nseeds ? (uint32_t)end : nseeds;
+        dry = !ok;
+        return ok;
+    }
     __device__ __forceinline__ void init() {
-        nwaves = gridDim.x * (blockDim.x / WAVE);
-        chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
-        uint64_t start = (uint64_t)chunk * SEEDS_PER_CHUNK;
-        s_cur = start < nseeds ? (uint32_t)start : nseeds;
-        s_end = min(s_cur + SEEDS_PER_CHUNK, nseeds);
+        dry = false;
+        take();
         load_seed();
     }
     // Loads seed s_cur (or an empty stand-in past the end) without branching,
@@ -277,16 +291,10 @@ struct SeedSource {
     // Next seed with hits; false when the grid's seeds are done.
     __device__ __forceinline__ bool advance() {
         for (;;) {
+            if (dry) return false;
             s_cur++;
-            bool next_chunk = s_cur >= s_end;
-            uint64_t start = (uint64_t)(chunk + nwaves) * SEEDS_PER_CHUNK;
-            bool done = next_chunk && start >= nseeds;
-            chunk = next_chunk ? chunk + nwaves : chunk;
-            uint32_t ns = next_chunk ? (uint32_t)min<uint64_t>(start, nseeds) : s_cur;
-            s_cur = __builtin_amdgcn_readfirstlane(ns);
-            s_end = __builtin_amdgcn_readfirstlane(next_chunk ? min(s_cur + SEEDS_PER_CHUNK, nseeds) : s_end);
+            if (s_cur >= s_end && !take()) return false;
             load_seed();
-            if (done) return false;
             if (cnt_cur) return true;
         }
     }
@@ -335,25 +343,31 @@ struct SeedSource {
     }
 };
 
-__global__ __launch_bounds__(512) void k_bgate_packed(
+// The batched gate: fused hit expansion + packed gate over a persistent grid
+// of BT-thread blocks, 16 * SW cells per step, `claim` seeds per wave claim.
+template <int SW, int BT>
+__global__ __launch_bounds__(BT) void k_bgate_packed(
     const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
     const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse, uint32_t qlen,
     const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh,
     const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ nvalid, uint32_t per,
     const uint32_t *__restrict__ index, const uint32_t *__restrict__ pos, uint32_t seed_size,
+    uint32_t claim, unsigned long long *__restrict__ work,
     SurvRec *__restrict__ surv, uint32_t *__restrict__ surv_n, uint32_t cap_surv,
     SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow,
     uint32_t *__restrict__ err) {
     __shared__ uint32_t s_tab[2 * 256 * G2_TAB_COPIES];
-    for (uint32_t i = threadIdx.x; i < 2 * 256 * G2_TAB_COPIES; i += blockDim.x) s_tab[i] = pairtab[i / G2_TAB_COPIES];
+    for (uint32_t i = threadIdx.x; i < 2 * 256 * G2_TAB_COPIES; i += BT) s_tab[i] = pairtab[i / G2_TAB_COPIES];
     __syncthreads();
     const uint32_t *tab_fwd = s_tab + (lane_id() % G2_TAB_COPIES);
     SeedSource src;
     src.seeds = seeds;
     src.index = index;
     src.pos = pos;
+    src.work = work;
     src.seed_size = seed_size;
     src.nseeds = __builtin_amdgcn_readfirstlane(*nvalid * per);
+    src.claim = claim;
     src.surv = surv;
     src.surv_n = surv_n;
     src.cap_surv = cap_surv;
@@ -362,8 +376,8 @@ __global__ __launch_bounds__(512) void k_bgate_packed(
     src.cap_slow = cap_slow;
     src.err = err;
     src.init();
-    packed_gate_loop(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, tab_fwd,
-                     tab_fwd + 256 * G2_TAB_COPIES, xdrop, thresh);
+    packed_gate_loop<SW>(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, tab_fwd,
+                         tab_fwd + 256 * G2_TAB_COPIES, xdrop, thresh);
 }
 
 // Block bounds of a seed's batch: arena offset and length of its block.

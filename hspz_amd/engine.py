@@ -23,7 +23,8 @@ SP_DTYPE = np.dtype([("ref_start", "<u4"), ("query_start", "<u4"), ("len", "<u4"
 STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 3,
         "query_swaps": 4, "peak_device_bytes": 5, "gate_ns": 6, "gate_hits": 7,
         "gate_launches": 8, "gate_deferred": 9, "graph_captures": 10, "graph_replays": 11}
-FLAG = {"batch_graph": 0, "batch_gate_blocks": 1}
+FLAG = {"batch_graph": 0, "batch_gate_blocks": 1, "batch_step_cells": 2, "batch_block_threads": 3,
+        "batch_claim_seeds": 4}
 
 # Every symbol include/hspz.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -34,6 +35,7 @@ EXPORTS = [
     "hspz_run_pass", "hspz_free", "hspz_build_sub_mat", "hspz_filter_batch",
     "hspz_batch_output_free", "hspz_server_new", "hspz_server_submit", "hspz_server_wait",
     "hspz_request_result_free", "hspz_server_get_stats", "hspz_server_free", "hspz_set_flag",
+    "hspz_get_flag", "hspz_autotune_batch",
 ]
 
 
@@ -77,6 +79,17 @@ class QueryChunk(C.Structure):
 class BatchOutput(C.Structure):
     _fields_ = [("hsps", C.c_void_p), ("n_hsps", C.c_size_t), ("counts", C.c_void_p),
                 ("chunk_intervals", C.c_void_p), ("n_chunks", C.c_size_t)]
+
+
+class TuneCell(C.Structure):
+    _fields_ = [("step_cells", C.c_uint32), ("block_threads", C.c_uint32),
+                ("grid_blocks", C.c_uint32), ("claim_seeds", C.c_uint32),
+                ("gate_ms", C.c_double), ("after_ms", C.c_double), ("hits", C.c_uint64),
+                ("survivors", C.c_uint64), ("deferred", C.c_uint64), ("status", C.c_int32)]
+
+    def as_dict(self):
+        return {k: (float(getattr(self, k)) if t is C.c_double else int(getattr(self, k)))
+                for k, t in self._fields_}
 
 
 class ServerConfig(C.Structure):
@@ -124,6 +137,10 @@ def lib():
         L.hspz_filter_output_free.restype = None
         L.hspz_get_stat.argtypes = [vp, C.c_int, C.POINTER(C.c_uint64)]
         L.hspz_set_flag.argtypes = [vp, C.c_int, C.c_int]
+        L.hspz_get_flag.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+        L.hspz_autotune_batch.argtypes = [vp, C.POINTER(PassParams), C.POINTER(QueryChunk), sz,
+                                          u32, C.POINTER(TuneCell), sz, C.POINTER(sz),
+                                          C.POINTER(TuneCell)]
         L.hspz_default_max_hits.argtypes = [C.c_uint64]
         L.hspz_default_max_hits.restype = u32
         L.hspz_device_memory.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
@@ -295,6 +312,28 @@ class Engine:
     def set_flag(self, name: str, value: int):
         _check(lib().hspz_set_flag(self._h, FLAG[name], int(value)))
 
+    def get_flag(self, name: str) -> int:
+        v = C.c_int()
+        _check(lib().hspz_get_flag(self._h, FLAG[name], C.byref(v)))
+        return int(v.value)
+
+    def autotune(self, chunks, reps: int = 1, pattern: str = "12of19", transitions: bool = True,
+                 plus: bool = True, minus: bool = True, wga_chunk: int = 250000,
+                 interval_size: int = 10000000):
+        """hspz_autotune_batch over chunks = [(fwd_enc, rc_enc)]: measures every
+        cell of the batched gate's launch grid, keeps the fastest on the engine.
+        Returns (cells, best) as dicts."""
+        arr, keep = _chunk_array(chunks)
+        p = PassParams(pattern.encode(), int(transitions), int(plus), int(minus), wga_chunk,
+                       interval_size)
+        cap = 64
+        cells = (TuneCell * cap)()
+        n, best = C.c_size_t(), TuneCell()
+        _check(lib().hspz_autotune_batch(self._h, C.byref(p), arr, len(chunks), int(reps), cells,
+                                         cap, C.byref(n), C.byref(best)))
+        del keep
+        return [cells[i].as_dict() for i in range(min(n.value, cap))], best.as_dict()
+
     def stat(self, name: str) -> int:
         v = C.c_uint64()
         _check(lib().hspz_get_stat(self._h, STAT[name], C.byref(v)))
@@ -331,15 +370,7 @@ class Engine:
         """Batched path over many query blocks: chunks = [(fwd_enc, rc_enc)].
         Returns ([per chunk: [(fw, rc) per interval]], stats); each chunk's
         lists equal run_pass on that chunk alone."""
-        keep = []
-        arr = (QueryChunk * max(len(chunks), 1))()
-        for i, (f, r) in enumerate(chunks):
-            f = np.ascontiguousarray(f, dtype=np.uint8)
-            r = np.ascontiguousarray(r, dtype=np.uint8)
-            if len(f) != len(r):
-                raise ValueError("chunk %d: forward and reverse-complement lengths differ" % i)
-            keep += [f, r]
-            arr[i] = QueryChunk(_ptr(f).value, _ptr(r).value, len(f))
+        arr, keep = _chunk_array(chunks)
         p = PassParams(pattern.encode(), int(transitions), int(plus), int(minus), wga_chunk,
                        interval_size)
         out, st = BatchOutput(), PassStats()
@@ -368,6 +399,20 @@ class Engine:
                 ivs.append((fw, rc))
             res.append(ivs)
         return res, st.as_dict()
+
+
+def _chunk_array(chunks):
+    """QueryChunk[] over [(fwd_enc, rc_enc)], plus the arrays it points into."""
+    keep = []
+    arr = (QueryChunk * max(len(chunks), 1))()
+    for i, (f, r) in enumerate(chunks):
+        f = np.ascontiguousarray(f, dtype=np.uint8)
+        r = np.ascontiguousarray(r, dtype=np.uint8)
+        if len(f) != len(r):
+            raise ValueError("chunk %d: forward and reverse-complement lengths differ" % i)
+        keep += [f, r]
+        arr[i] = QueryChunk(_ptr(f).value, _ptr(r).value, len(f))
+    return arr, keep
 
 
 class Server:

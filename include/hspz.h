@@ -116,9 +116,17 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 
 /* Engine switches (the reference's pub flags, gpu/mod.rs:836-949, have no
  * meaning here; these are this engine's own). */
-#define HSPZ_FLAG_BATCH_GRAPH 0       /* 1 (default): capture/replay the batched pass as a hipGraph */
-#define HSPZ_FLAG_BATCH_GATE_BLOCKS 1 /* persistent grid of the batched gate, in 512-thread blocks */
+#define HSPZ_FLAG_BATCH_GRAPH 0        /* 1 (default): capture/replay the batched pass as a hipGraph */
+#define HSPZ_FLAG_BATCH_GATE_BLOCKS 1  /* persistent grid of the batched gate in blocks; 0 (default) = as many
+                                          as are resident at once */
+#define HSPZ_FLAG_BATCH_STEP_CELLS 2   /* cells per gate step: 16, 32 (default) or 64 */
+#define HSPZ_FLAG_BATCH_BLOCK_THREADS 3 /* threads per gate block: 256 or 512 (default) */
+#define HSPZ_FLAG_BATCH_CLAIM_SEEDS 4  /* seeds a wave claims from the work counter at a time (1..4096,
+                                          default 16) */
 int hspz_set_flag(hspz_engine *h, int which, int value);
+/* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
+ * once a batched pass has run). */
+int hspz_get_flag(const hspz_engine *h, int which, int *value);
 
 /* resolve_max_hits / device_count / device_memory (gpu/mod.rs:2436-2463). */
 uint32_t hspz_default_max_hits(uint64_t total_device_bytes);
@@ -209,6 +217,30 @@ typedef struct hspz_batch_output {
 int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, const hspz_query_chunk *chunks,
                       size_t n_chunks, hspz_batch_output *out, hspz_pass_stats *stats);
 void hspz_batch_output_free(hspz_batch_output *out);
+
+/* One measured configuration of the batched gate. */
+typedef struct hspz_tune_cell {
+    uint32_t step_cells;    /* HSPZ_FLAG_BATCH_STEP_CELLS */
+    uint32_t block_threads; /* HSPZ_FLAG_BATCH_BLOCK_THREADS */
+    uint32_t grid_blocks;   /* HSPZ_FLAG_BATCH_GATE_BLOCKS as launched */
+    uint32_t claim_seeds;   /* HSPZ_FLAG_BATCH_CLAIM_SEEDS */
+    double gate_ms;         /* best of the repetitions: fused gate + exact gate for deferred hits */
+    double after_ms;        /* best of the repetitions: materializer and output compaction */
+    uint64_t hits, survivors, deferred;
+    int status;             /* hspz_filter_batch's status for this cell */
+} hspz_tune_cell;
+
+/* Autotuner of the batched gate (no counterpart in the reference). Runs
+ * hspz_filter_batch over `chunks` for every cell of a grid over step width
+ * {16, 32, 64} cells x block size {256, 512} x wave claim {4, 16, 64} seeds at
+ * the resident grid, then over grid sizes {1/2, 3/4, 1, 3/2, 2} x resident
+ * for the best of those, `reps` times each, and records every cell it measured
+ * in cells[0..*n_cells) (up to cap; *n_cells counts all of them). The engine
+ * keeps the cell with the least gate_ms + after_ms, which is also returned in
+ * *best. Outputs do not depend on the configuration. */
+int hspz_autotune_batch(hspz_engine *h, const hspz_pass_params *p, const hspz_query_chunk *chunks,
+                        size_t n_chunks, uint32_t reps, hspz_tune_cell *cells, size_t cap, size_t *n_cells,
+                        hspz_tune_cell *best);
 
 /* ---- Serving loop --------------------------------------------------------- */
 

@@ -115,3 +115,66 @@ def test_batch_graph_replays_match_eager(gpu_available):
             eng.run_pass()
         assert eng.stat("graph_captures") - c0 == 1
         assert eng.stat("graph_replays") - r0 == 2
+
+
+def _engine_and_chunks(seed, n_chunks, target_len=300_000):
+    target = synth.random_bases(seed, target_len).tobytes()
+    blocks = _chunks_from(target, n_chunks, seed & 0xFF)
+    sub = O.sub_mat("", 910)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
+    return target, blocks, sub, index, pos, chunks
+
+
+def test_batch_gate_configs_match(gpu_available):
+    _say("case batch: gate step/block/claim/grid variants vs the oracle")
+    target, blocks, sub, index, pos, chunks = _engine_and_chunks(0xB005, 12)
+    want = []
+    for f, r in blocks:
+        o = O.Oracle(target, f, r, sub, max_hits=16711680, index=(index, pos))
+        want.append(o.run_pass(threads=16)[0])
+    variants = [(16, 256, 1, 0), (64, 512, 4096, 0), (32, 256, 7, 3), (64, 256, 1, 1),
+                (16, 512, 64, 5000)]
+    with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
+        for step, threads, claim, blocks_n in variants:
+            _say("  step %d threads %d claim %d grid %d" % (step, threads, claim, blocks_n))
+            eng.set_flag("batch_step_cells", step)
+            eng.set_flag("batch_block_threads", threads)
+            eng.set_flag("batch_claim_seeds", claim)
+            eng.set_flag("batch_gate_blocks", blocks_n)
+            got, st = eng.filter_batch(chunks)
+            assert eng.get_flag("batch_step_cells") == step
+            if blocks_n:
+                assert eng.get_flag("batch_gate_blocks") == blocks_n
+            else:
+                assert eng.get_flag("batch_gate_blocks") >= 256
+            for i, (g, w) in enumerate(zip(got, want)):
+                assert same_intervals(g, w), "chunk %d differs at step %d claim %d" % (i, step, claim)
+        for name, bad in [("batch_step_cells", 24), ("batch_block_threads", 1024),
+                          ("batch_claim_seeds", 0), ("batch_gate_blocks", -1)]:
+            with pytest.raises(E.HspzError) as ei:
+                eng.set_flag(name, bad)
+            assert ei.value.status == 2
+
+
+def test_batch_autotune(gpu_available):
+    _say("case batch: autotuner grid, chosen cell, output unchanged")
+    target, blocks, sub, index, pos, chunks = _engine_and_chunks(0xB006, 8)
+    with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
+        before, st0 = eng.filter_batch(chunks)
+        cells, best = eng.autotune(chunks, reps=1)
+        assert len(cells) == 22
+        assert all(c["status"] == 0 for c in cells)
+        assert len({c["hits"] for c in cells}) == 1 and cells[0]["hits"] == st0["hits"]
+        assert {(c["step_cells"], c["block_threads"], c["claim_seeds"]) for c in cells[:18]} == {
+            (s, t, c) for s in (16, 32, 64) for t in (256, 512) for c in (4, 16, 64)}
+        fastest = min(c["gate_ms"] + c["after_ms"] for c in cells)
+        assert best["gate_ms"] + best["after_ms"] == fastest
+        assert eng.get_flag("batch_step_cells") == best["step_cells"]
+        assert eng.get_flag("batch_block_threads") == best["block_threads"]
+        assert eng.get_flag("batch_claim_seeds") == best["claim_seeds"]
+        after, st1 = eng.filter_batch(chunks)
+        assert eng.get_flag("batch_gate_blocks") == best["grid_blocks"]
+        for g, w in zip(after, before):
+            assert same_intervals(g, w)
+        assert st1["hits"] == st0["hits"]
