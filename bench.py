@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--tune-mbp", type=float, default=1.0,
                     help="query Mbp the gate autotuner measures on before the timed region (0 = skip)")
     ap.add_argument("--tune-reps", type=int, default=1)
+    ap.add_argument("--gate", default="",
+                    help="step_cells,block_threads,claim_seeds,grid_blocks[,order] for the batched gate "
+                         "(applied after the autotuner; use with --tune-mbp 0 to fix the configuration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -140,20 +143,29 @@ def main():
         t1 = time.time()
         cells, best = eng.autotune(sample, reps=args.tune_reps)
         for c in cells:
-            log("tune step %2d threads %3d grid %5d claim %2d: gate %.2f ms + after %.2f ms "
-                "(%d hits, %d survivors)" % (c["step_cells"], c["block_threads"], c["grid_blocks"],
-                                             c["claim_seeds"], c["gate_ms"], c["after_ms"],
-                                             c["hits"], c["survivors"]))
-        log("autotuner: %d cells on %d bp in %.1fs; chose step %d threads %d grid %d claim %d"
-            % (len(cells), sbp, time.time() - t1, best["step_cells"], best["block_threads"],
-               best["grid_blocks"], best["claim_seeds"]))
+            log("tune order %d step %2d threads %4d grid %5d claim %3d: gate %.2f ms + after %.2f ms "
+                "(%d hits, %d survivors, %d deferred)"
+                % (c["order"], c["step_cells"], c["block_threads"], c["grid_blocks"], c["claim_seeds"],
+                   c["gate_ms"], c["after_ms"], c["hits"], c["survivors"], c["deferred"]))
+        log("autotuner: %d cells on %d bp in %.1fs; chose order %d step %d threads %d grid %d claim %d"
+            % (len(cells), sbp, time.time() - t1, best["order"], best["step_cells"],
+               best["block_threads"], best["grid_blocks"], best["claim_seeds"]))
+        cols = ["order", "step_cells", "block_threads", "grid_blocks", "claim_seeds"]
         tune = {"sample_bp": sbp, "reps": args.tune_reps,
-                "columns": ["step_cells", "block_threads", "grid_blocks", "claim_seeds", "gate_ms",
-                            "after_ms"],
-                "cells": [[c["step_cells"], c["block_threads"], c["grid_blocks"], c["claim_seeds"],
-                           round(c["gate_ms"], 3), round(c["after_ms"], 3)] for c in cells],
-                "chosen": [best["step_cells"], best["block_threads"], best["grid_blocks"],
-                           best["claim_seeds"]]}
+                "order_names": ["seed-major", "k-mer-major"],
+                "columns": cols + ["gate_ms", "after_ms"],
+                "cells": [[c[k] for k in cols] + [round(c["gate_ms"], 3), round(c["after_ms"], 3)]
+                          for c in cells],
+                "chosen": [best[k] for k in cols]}
+
+    if args.gate:
+        vals = [int(x) for x in args.gate.split(",")]
+        step, thr, claim, grid = vals[:4]
+        order = vals[4] if len(vals) > 4 else 2
+        for name, v in (("batch_step_cells", step), ("batch_block_threads", thr),
+                        ("batch_claim_seeds", claim), ("batch_gate_blocks", grid), ("batch_order", order)):
+            eng.set_flag(name, v)
+        log("gate fixed: step %d threads %d claim %d grid %d order %d" % (step, thr, claim, grid, order))
 
     for i in range(args.warmup):
         res_b, st = batched()
@@ -167,6 +179,7 @@ def main():
         log("batched step %d done" % i)
     elapsed = time.perf_counter() - t_start
     g = {k: eng.stat(k) - g0[k] for k in g0}
+    order_used = eng.get_flag("batch_order_used")
     value = args.steps * query_bp / 1e6 / elapsed
     hashes_b = [E.hash_pass(r) for r in res_b]
 
@@ -229,7 +242,9 @@ def main():
         achieved = g["gate_hits"] * per_hit / secs / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_bgate_packed (+ k_bgate_wave for deferred hits)",
+                "kernel": "%s (+ k_bgate_wave for deferred hits)"
+                          % ("k_bgate_kmer" if order_used else "k_bgate_packed"),
+                "hit_order": "k-mer-major" if order_used else "seed-major",
                 "avg_launch_us": round(g["gate_ns"] / max(g["gate_launches"], 1) / 1e3, 1),
                 "hits_per_s": round(g["gate_hits"] / secs, 1),
                 "deferred_frac": round(g["gate_deferred"] / max(g["gate_hits"], 1), 5),
@@ -279,7 +294,8 @@ def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rat
     import threading
     import numpy as np
     eng2 = E.Engine(index, pos, ref_enc, sub, max_hits=max_hits)
-    for name in ("batch_step_cells", "batch_block_threads", "batch_claim_seeds", "batch_gate_blocks"):
+    for name in ("batch_step_cells", "batch_block_threads", "batch_claim_seeds", "batch_gate_blocks",
+                 "batch_order"):
         eng2.set_flag(name, eng.get_flag(name))  # the tuned gate configuration
     arms = []
     for k, rate in enumerate(rates):

@@ -22,22 +22,36 @@ struct HostBatch {
 // Device buffers of the batched path, grow-only, owned by the engine.
 struct BatchBufs {
     DevBuf arena, plane, coarse, desc, poff, blk_off, blk_len, valid, ids, voff, seeds, counts, prefix,
-        tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr;
+        tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr, gcnt, sq, skm;
 };
 
 // Device counters of one pass; CTR_WORK is the 64-bit seed-claim counter of
 // the batched gate (two slots, 8-byte aligned).
-enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_WORK = 6, CTR_N = 8 };
+enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_GROUPED = 5, CTR_WORK = 6,
+       CTR_N = 8 };
 
 typedef void (*BGateFn)(const uint32_t *, const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, uint32_t,
                         const uint32_t *, int32_t, int32_t, const uint64_t *, const uint32_t *, uint32_t,
                         const uint32_t *, const uint32_t *, uint32_t, uint32_t, unsigned long long *, SurvRec *,
                         uint32_t *, uint32_t, SurvRec *, uint32_t *, uint32_t, uint32_t *);
 
+typedef void (*KGateFn)(const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, uint32_t, const uint32_t *,
+                        int32_t, int32_t, const uint64_t *, const uint32_t *, const uint32_t *, uint32_t,
+                        const uint64_t *, const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t,
+                        unsigned long long *, SurvRec *, uint32_t *, uint32_t, SurvRec *, uint32_t *, uint32_t,
+                        uint32_t *);
+
+// The k-mer-major gate instantiation for a step width and block size.
+KGateFn kgate_kernel(int step_cells, int threads) {
+    if (threads == 1024)
+        return step_cells == 16 ? k_bgate_kmer<1, 1024> : step_cells == 64 ? k_bgate_kmer<4, 1024> : k_bgate_kmer<2, 1024>;
+    return step_cells == 16 ? k_bgate_kmer<1, 512> : step_cells == 64 ? k_bgate_kmer<4, 512> : k_bgate_kmer<2, 512>;
+}
+
 // The batched gate instantiation for a step width and block size.
 BGateFn bgate_kernel(int step_cells, int threads) {
-    if (threads == 256)
-        return step_cells == 16 ? k_bgate_packed<1, 256> : step_cells == 64 ? k_bgate_packed<4, 256> : k_bgate_packed<2, 256>;
+    if (threads == 1024)
+        return step_cells == 16 ? k_bgate_packed<1, 1024> : step_cells == 64 ? k_bgate_packed<4, 1024> : k_bgate_packed<2, 1024>;
     return step_cells == 16 ? k_bgate_packed<1, 512> : step_cells == 64 ? k_bgate_packed<4, 512> : k_bgate_packed<2, 512>;
 }
 
@@ -46,7 +60,7 @@ BGateFn bgate_kernel(int step_cells, int threads) {
 // Everything a captured device pass depends on: if a call's key equals the
 // cached one, the instantiated graph is replayed instead of relaunching.
 struct GraphKey {
-    const void *ptrs[32];
+    const void *ptrs[40];
     uint64_t vals[24];
     uint32_t shape[33];
 };
@@ -166,7 +180,16 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
 
         uint32_t nt = std::max<uint32_t>(ceil_div(npos, TILE), 1);
         uint32_t kt = ceil_div(bs->cap_surv, TILE);
-        if ((rc = reserve(h, h->tiles, (size_t)std::max(kt, nt) * 4))) return rc;
+        // hit order: k-mer-major once the batch holds about a seed per two k-mers
+        const uint32_t nkm = (uint32_t)h->index_len;
+        const bool kmer_order = h->clean.p && (h->batch_order == 1 || (h->batch_order == 2 && cap_seeds * 2 >= nkm));
+        uint32_t ntk = kmer_order ? ceil_div((uint64_t)nkm + 1, TILE) : 0;
+        if ((rc = reserve(h, h->tiles, (size_t)std::max(std::max(kt, nt), ntk) * 4))) return rc;
+        if (kmer_order) {
+            if ((rc = reserve(h, B.gcnt, ((size_t)nkm + 1) * 4))) return rc;
+            if ((rc = reserve(h, B.sq, cap_seeds * 8 + 8))) return rc;
+            if ((rc = reserve(h, B.skm, cap_seeds * 4 + 4))) return rc;
+        }
         BlockOf bo;
         bo.seeds = B.seeds.as<uint64_t>();
         bo.voff = B.voff.as<uint32_t>();
@@ -207,24 +230,54 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         hipLaunchKernelGGL(k_bchunks, dim3(ceil_div(nb, 64)), dim3(64), 0, h->stream, B.prefix.as<uint64_t>(),
                            B.voff.as<uint32_t>(), nb, per, h->max_hits, B.nchunks.as<uint32_t>(),
                            B.starts.as<uint32_t>(), B.bhits.as<uint64_t>(), ctr + CTR_ERR);
-        // fused expansion + packed gate, then the exact gate for deferred hits
+        if (kmer_order) {  // group the seeds with hits by k-mer (counting sort)
+            hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, h->stream, B.gcnt.as<uint32_t>(), nkm + 1, 0u);
+            hipLaunchKernelGGL(k_kcount, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
+                               h->stream, B.seeds.as<uint64_t>(), B.counts.as<uint32_t>(), ctr + CTR_NVALID, per,
+                               B.gcnt.as<uint32_t>());
+            hipLaunchKernelGGL(k_tile_sum_u32, dim3(ntk), dim3(TILE_THREADS), 0, h->stream, B.gcnt.as<uint32_t>(),
+                               nkm + 1, h->tiles.as<uint32_t>());
+            hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, h->stream, h->tiles.as<uint32_t>(), ntk,
+                               ctr + CTR_GROUPED);
+            hipLaunchKernelGGL(k_tile_apply_u32, dim3(ntk), dim3(TILE_THREADS), 0, h->stream, B.gcnt.as<uint32_t>(),
+                               nkm + 1, h->tiles.as<uint32_t>());
+            hipLaunchKernelGGL(k_kscatter, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
+                               h->stream, B.seeds.as<uint64_t>(), B.counts.as<uint32_t>(), ctr + CTR_NVALID, per,
+                               B.gcnt.as<uint32_t>(), B.sq.as<uint64_t>(), B.skm.as<uint32_t>());
+        }
         };
+        // fused expansion + packed gate, then the exact gate for deferred hits
         BGateFn gate_fn = bgate_kernel(h->bgate_step, h->bgate_threads);
+        KGateFn kgate_fn = kgate_kernel(h->bgate_step, h->bgate_threads);
         int grid = h->bgate_blocks;
         if (grid == 0) {  // resident grid: occupancy of this instantiation x CUs
             int per_cu = 0;
-            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gate_fn, h->bgate_threads, 0));
+            if (kmer_order)
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kgate_fn, h->bgate_threads, 0));
+            else
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gate_fn, h->bgate_threads, 0));
             grid = std::max(per_cu, 1) * h->num_cus;
         }
         h->bgate_grid_used = grid;
+        h->batch_order_used = kmer_order ? 1 : 0;
         auto enqueue_gate = [&]() {
-        hipLaunchKernelGGL(gate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
-                           h->refx.as<uint32_t>(), h->ref_len, B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(),
-                           arena_len, h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.seeds.as<uint64_t>(),
-                           ctr + CTR_NVALID, per, h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->seed_size,
-                           (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK), B.surv.as<SurvRec>(),
-                           ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(), ctr + CTR_SLOW, bs->cap_slow,
-                           ctr + CTR_ERR);
+        if (kmer_order)
+            hipLaunchKernelGGL(kgate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
+                               h->ref_len, B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(), arena_len,
+                               h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.sq.as<uint64_t>(),
+                               B.skm.as<uint32_t>(), B.gcnt.as<uint32_t>(), nkm, B.seeds.as<uint64_t>(),
+                               h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->clean.as<uint32_t>(),
+                               h->seed_size, (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK),
+                               B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(),
+                               ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
+        else
+            hipLaunchKernelGGL(gate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
+                               h->refx.as<uint32_t>(), h->ref_len, B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(),
+                               arena_len, h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.seeds.as<uint64_t>(),
+                               ctr + CTR_NVALID, per, h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->seed_size,
+                               (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK),
+                               B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(),
+                               ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
         hipLaunchKernelGGL(k_bgate_wave, dim3(2048), dim3(256), 0, h->stream, r, h->ref_len, ar, bo,
                            h->sub.as<int32_t>(), h->xdrop, h->hspthresh, B.slow.as<SurvRec>(), ctr + CTR_SLOW,
                            bs->cap_slow, B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, ctr + CTR_ERR);
@@ -254,12 +307,14 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                   B.valid.p, B.ids.p, B.voff.p, B.seeds.p, B.counts.p, B.prefix.p, B.tiles.p,
                                   B.nchunks.p, B.starts.p, B.bhits.p, B.surv.p, B.slow.p, B.mhsp.p, B.mscope.p,
                                   B.keep.p, B.kids.p, B.ohsp.p, B.oscope.p, B.ctr.p, h->tiles.p, h->ref2.p,
-                                  h->refx.p, h->pairtab.p, h->index.p, h->pos.p};
+                                  h->refx.p, h->pairtab.p, h->index.p, h->pos.p, B.gcnt.p, B.sq.p, B.skm.p,
+                                  h->clean.p};
             memcpy(key.ptrs, ptrs, sizeof ptrs);
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow, cap_seeds, per, h->max_hits,
                                (uint64_t)grid | (uint64_t)h->bgate_claim << 32, (uint64_t)p->transitions, nt, kt,
                                cap_tiles, (uint64_t)h->bgate_step | (uint64_t)h->bgate_threads << 32,
-                               (uint64_t)(uintptr_t)h->sub.p, s.kmer_size, (uint64_t)(uintptr_t)h->ref.p};
+                               (uint64_t)(uintptr_t)h->sub.p, s.kmer_size, (uint64_t)(uintptr_t)h->ref.p,
+                               (uint64_t)kmer_order, ntk};
             memcpy(key.vals, vals, sizeof vals);
             memcpy(key.shape, s.pos, sizeof s.pos);
             key.shape[32] = s.size;

@@ -80,7 +80,10 @@ struct hspz_engine {
     int gate_blocks = 2048;
     // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
     int bgate_blocks = 0;  // persistent grid; 0 = as many blocks as are resident at once
-    int bgate_step = 32, bgate_threads = 512, bgate_claim = 16;
+    int bgate_step = 64, bgate_threads = 512, bgate_claim = 64;
+    int batch_order = 2;                    // 0 seed-major, 1 k-mer-major, 2 by batch size
+    int batch_order_used = 0;               // order of the last batched pass
+    DevBuf clean;                           // pos_table-aligned clean-window bits (k-mer-major gate)
     int bgate_grid_used = 0;                // grid of the last batched launch
     int num_cus = 0;
     double last_gate_ms = 0, last_post_ms = 0;  // event times of the last batched pass
@@ -405,27 +408,39 @@ extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_e
     h->ref_len = (uint32_t)cfg->ref_len;
     if ((rc = upload_seq(h, h->ref, cfg->ref_seq, cfg->ref_len))) return fail(rc);
     h->reference_uploads++;
-    // Fast gate: int16 pair table over ACGT x ACGT (scores clamped at -(xdrop+1)).
+    // Fast gate: two-cell map table over ACGT x ACGT pairs (scores clamped at
+    // -(xdrop+1)); entry = (a = s1 + s2, b = xdrop + min(s2, 0)) as int16
+    // halves, s1 the cell met first: the lower cell of the key forward, the
+    // upper one in reverse.
     {
         bool ok = cfg->xdrop >= 0 && cfg->xdrop <= G2_MAX_XDROP;
         for (int i = 0; i < 4 && ok; i++)
-            for (int j = 0; j < 4; j++) ok = ok && cfg->sub_mat[i * 8 + j] <= 32767;
+            for (int j = 0; j < 4; j++) ok = ok && cfg->sub_mat[i * 8 + j] <= G2_MAX_SCORE;
         h->fast_gate = ok;
         if (ok) {
-            // forward table, then the same with int16 halves swapped
             uint32_t tab[512];
+            const int32_t X = cfg->xdrop;
             for (uint32_t key = 0; key < 256; key++) {
                 uint32_t ra = key & 3, rb = (key >> 2) & 3, qa = (key >> 4) & 3, qb = (key >> 6) & 3;
-                int32_t lo = std::max(cfg->sub_mat[ra * 8 + qa], -(cfg->xdrop + 1));
-                int32_t hi = std::max(cfg->sub_mat[rb * 8 + qb], -(cfg->xdrop + 1));
-                tab[key] = ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16);
-                tab[256 + key] = ((uint32_t)hi & 0xFFFFu) | ((uint32_t)lo << 16);
+                int32_t lo = std::max(cfg->sub_mat[ra * 8 + qa], -(X + 1));
+                int32_t hi = std::max(cfg->sub_mat[rb * 8 + qb], -(X + 1));
+                int32_t a = lo + hi;
+                int32_t bf = X + std::min(hi, 0), br = X + std::min(lo, 0);
+                tab[key] = ((uint32_t)a & 0xFFFFu) | ((uint32_t)bf << 16);
+                tab[256 + key] = ((uint32_t)a & 0xFFFFu) | ((uint32_t)br << 16);
             }
             if ((rc = reserve(h, h->pairtab, sizeof tab))) return fail(rc);
             if ((rc = reserve(h, h->slow_n, 256))) return fail(rc);
             if (hipMemcpy(h->pairtab.p, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess)
                 return fail(set_error(-1, "pair table upload failed"));
             if ((rc = build_planes(h, h->ref, h->ref_len, h->ref2, h->refx))) return fail(rc);
+            if (cfg->pos_len) {
+                if ((rc = reserve(h, h->clean, ((cfg->pos_len + 31) / 32 + 2) * 4))) return fail(rc);
+                hipLaunchKernelGGL(k_clean_bits, dim3(8192), dim3(256), 0, h->stream, h->pos.as<uint32_t>(),
+                                   (uint32_t)cfg->pos_len, h->refx.as<uint32_t>(), h->ref_len, h->seed_size,
+                                   h->clean.as<uint32_t>());
+                if (hipGetLastError() != hipSuccess) return fail(set_error(-1, "clean-bit build failed"));
+            }
             if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(set_error(-1, "plane build failed"));
         }
     }
@@ -438,7 +453,7 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     DevBuf *bufs[] = {&h->ref, &h->index, &h->pos, &h->sub, &h->qf, &h->qr, &h->seeds[0], &h->seeds[1],
-                      &h->ref2, &h->refx, &h->qf2, &h->qfx, &h->qr2, &h->qrx, &h->pairtab, &h->slow, &h->slow_n,
+                      &h->ref2, &h->refx, &h->qf2, &h->qfx, &h->qr2, &h->qrx, &h->pairtab, &h->slow, &h->slow_n, &h->clean,
                       &h->hit_num, &h->tiles, &h->total, &h->anchors, &h->flags, &h->surv, &h->hsp,
                       &h->done, &h->ids2, &h->reduced, &h->valid};
     for (DevBuf *b : bufs) release(*b);
@@ -578,12 +593,16 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
         h->bgate_step = value;
         break;
     case HSPZ_FLAG_BATCH_BLOCK_THREADS:
-        if (value != 256 && value != 512) return set_error(HSPZ_E_ARG, "block threads must be 256 or 512");
+        if (value != 512 && value != 1024) return set_error(HSPZ_E_ARG, "block threads must be 512 or 1024");
         h->bgate_threads = value;
         break;
     case HSPZ_FLAG_BATCH_CLAIM_SEEDS:
         if (value < 1 || value > 4096) return set_error(HSPZ_E_ARG, "claim seeds must be in [1, 4096]");
         h->bgate_claim = value;
+        break;
+    case HSPZ_FLAG_BATCH_ORDER:
+        if (value < 0 || value > 2) return set_error(HSPZ_E_ARG, "batch order must be 0, 1 or 2");
+        h->batch_order = value;
         break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
@@ -599,6 +618,8 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_STEP_CELLS: *value = h->bgate_step; break;
     case HSPZ_FLAG_BATCH_BLOCK_THREADS: *value = h->bgate_threads; break;
     case HSPZ_FLAG_BATCH_CLAIM_SEEDS: *value = h->bgate_claim; break;
+    case HSPZ_FLAG_BATCH_ORDER: *value = h->batch_order; break;
+    case HSPZ_FLAG_BATCH_ORDER_USED: *value = h->batch_order_used; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;

@@ -437,7 +437,11 @@ constexpr uint32_t G2_LEFT_WORDS = 8;   // 128 left cells in registers
 constexpr int32_t G2_MAX_XDROP = 4000;  // packed int16 partial sums of 8 cells stay exact
 constexpr int32_t G2_MAX_SCORE = 4095;  // largest ACGT score the packed sums take
 constexpr uint32_t G2_MARGIN = 128;     // fast path needs this many bases either side
-constexpr uint32_t G2_TAB_COPIES = 16;  // LDS copies of each pair table (bank spread)
+// LDS map table: per 8-bit key a 256-byte row of 32 forward then 32 reverse
+// copies of the key's 4-byte entry (64 KiB), so that one v_perm_b32 forms
+// the byte address (key << 8) | lane offset and the 32 lanes of a half wave
+// always read 32 distinct banks.
+constexpr uint32_t G2_MAP_WORDS = 256 * 64;
 
 typedef short short2_t __attribute__((ext_vector_type(2)));
 
@@ -446,10 +450,8 @@ __device__ __forceinline__ uint32_t pk_add16(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(short2_t, a) + __builtin_bit_cast(short2_t, b));
 }
 
-// Sum of the two int16 halves of w, exact in int32.
-__device__ __forceinline__ int32_t pair_sum(uint32_t w) {
-    return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, w), __builtin_bit_cast(short2_t, 0x00010001u), 0, false);
-}
+__device__ __forceinline__ int32_t lo16(uint32_t w) { return (int32_t)(int16_t)(w & 0xFFFFu); }
+__device__ __forceinline__ int32_t hi16(uint32_t w) { return (int32_t)w >> 16; }
 
 // Coarse-bitmap test of bases [lo, hi] from the two bitmap words that hold
 // block lo >> 6 (same rule as coarse_hit, with the words already loaded).
@@ -457,6 +459,19 @@ __device__ __forceinline__ bool coarse_bits(uint32_t w0, uint32_t w1, uint32_t l
     uint32_t b0 = lo >> 6, b1 = hi >> 6;
     uint64_t v = ((uint64_t)w0 | ((uint64_t)w1 << 32)) >> (b0 & 31);
     return (v & ((1ull << (b1 - b0 + 1)) - 1)) != 0;
+}
+
+// Copies the 512-entry global map table (256 forward keys, then 256 reverse)
+// into the LDS layout above.
+__device__ __forceinline__ void load_map_table(uint32_t *s_map, const uint32_t *__restrict__ tab,
+                                               uint32_t nthreads) {
+    for (uint32_t i = threadIdx.x; i < G2_MAP_WORDS; i += nthreads) s_map[i] = tab[((i >> 5) & 1) * 256 + (i >> 6)];
+    __syncthreads();
+}
+
+// Map entry at LDS byte address (key byte `sel` of x) << 8 | off.
+__device__ __forceinline__ uint32_t map_entry(const uint8_t *s_map, uint32_t x, uint32_t off, uint32_t sel) {
+    return *(const uint32_t *)(s_map + __builtin_amdgcn_perm(x, off, sel));
 }
 
 // Fast score gate over 2-bit planes, one hit per lane; shared by the plain
@@ -467,45 +482,54 @@ __device__ __forceinline__ bool coarse_bits(uint32_t w0, uint32_t w1, uint32_t l
 // When a lane takes a hit it loads 13 words (208 bases) around the anchor from
 // each plane, plus the coarse exception words of both windows, in one round
 // trip, and funnels the words into a right stream (cells R, R+1, ...) and a
-// left stream (the words below R, unreversed). Per word pair (16 cells) both
-// sequences' 2-bit codes of two consecutive cells form an 8-bit key into an
-// LDS pair table whose int16 halves are the two cells' scores. For the left
-// stream the key bytes are taken in reverse order and the halves-swapped table
-// is used, which walks the cells outward from R without reversing any bits.
-// Scores below -(xdrop + 1) are stored as -(xdrop + 1): such a cell ends the
-// extension either way and never raises the maximum.
+// left stream (the words below R, unreversed).
 //
-// The recurrence runs on e = xdrop - (max - score): e = min(e + s, xdrop), and
-// the extension stops in the step where e first goes negative. Within that
-// step the cells after the stopping cell are still applied, so the maximum
-// taken at the end of the step, c + xdrop - e, is the running maximum over
-// every cell of the step: at least the exact X-drop maximum, and equal to it
-// unless a new maximum follows the drop inside the same step. The pass
-// decision is therefore an upper bound of the exact gate (never drops a hit the
-// exact gate keeps) for every step width; every passing hit is re-extended
-// exactly by the materializer, so the HSP output does not depend on SW.
+// The X-drop recurrence runs on e = xdrop - (max - score): per cell
+// e = min(e + s, xdrop). Two consecutive cells compose to one map,
+//   e -> min(e + a, b),  a = s1 + s2,  b = xdrop + min(s2, 0),
+// so both sequences' 2-bit codes of a cell pair form an 8-bit key into an LDS
+// table of (a, b) int16 pairs, and each pair of cells costs one add and one
+// min. The left stream's words are byte-swapped when the phase starts, so its
+// key bytes come in walk order too; it takes the other nibble word first and
+// the reverse table (cells of a key swapped), which walks the cells outward
+// from R without reversing the 2-bit fields. Scores are clamped at -(xdrop + 1) first: such
+// a cell ends the extension either way and never raises the maximum.
+//
+// e is tested after every cell pair; a phase ends at the end of the first
+// step in which a test found it negative. The maximum c + xdrop - e taken
+// there is the running maximum over every cell evaluated, a superset of the
+// cells the exact scalar loop reaches before it stops, so it is at least the
+// exact X-drop maximum: the pass decision is an upper bound of the exact gate
+// (never drops a hit the exact gate keeps) for every step width. Every passing
+// hit is re-extended exactly by the materializer, so the HSP output does not
+// depend on SW.
 //
 // Hits whose windows touch a non-ACGT code or a sequence end, and extensions
 // that outlive their 64 right / 128 left cells, are deferred to the exact
-// wave-per-hit gate. Each int16 half of a word's packed partial sum adds 8
-// table values in [-(xdrop + 1), G2_MAX_SCORE], exact while both bounds are
-// at most 4095.
+// wave-per-hit gate. The score sum c adds each step's a values as packed
+// int16 partial sums of 4 entries (8 cells), exact while xdrop + 1 and the
+// largest ACGT score are at most 4095.
 template <int SW, class Src>
 __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__restrict__ rplane,
                                                  const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
                                                  const uint32_t *__restrict__ qplane,
                                                  const uint32_t *__restrict__ qcoarse, uint32_t qlen,
-                                                 const uint32_t *tab_fwd, const uint32_t *tab_rev,
-                                                 int32_t xdrop, int32_t thresh) {
+                                                 const uint8_t *s_map, int32_t xdrop, int32_t thresh) {
     static_assert(SW == 1 || SW == 2 || SW == 4, "step width must divide the right window");
+    // perm selector of key byte j: byte 1 = byte j of the key word, byte 0 = the lane offset, bytes 3, 2 = 0
+    constexpr uint32_t SEL[4] = {0x0C0C0400u, 0x0C0C0500u, 0x0C0C0600u, 0x0C0C0700u};
     bool has = false;
     uint32_t R = 0, Q = 0, tag = 0, grp = 0, limit = 0;
     bool left = false;
-    const uint32_t *tb = tab_fwd;
+    const uint32_t copy_off = (lane_id() & 31) * 4;
+    uint32_t off = copy_off;
     int32_t c = 0, e = xdrop, Mr = 0;
-    uint32_t A[G2_LEFT_WORDS], B[G2_LEFT_WORDS], LA[G2_LEFT_WORDS], LB[G2_LEFT_WORDS];
+    // right-phase words A/B, left-phase words LA/LB (each phase consumes its own)
+    uint32_t A[G2_RIGHT_WORDS], B[G2_RIGHT_WORDS], LA[G2_LEFT_WORDS], LB[G2_LEFT_WORDS];
 #pragma unroll
-    for (int j = 0; j < (int)G2_LEFT_WORDS; j++) A[j] = B[j] = LA[j] = LB[j] = 0;
+    for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) A[j] = B[j] = 0;
+#pragma unroll
+    for (int j = 0; j < (int)G2_LEFT_WORDS; j++) LA[j] = LB[j] = 0;
 
     for (;;) {
         bool fresh = false;
@@ -521,35 +545,48 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
                 const uint32_t qlo = Q - 16 * G2_LEFT_WORDS, qhi = Q + 16 * G2_RIGHT_WORDS - 1;
                 const uint32_t *rc = rcoarse + (rlo >> 11);
                 const uint32_t *qc = qcoarse + (qlo >> 11);
-                // words wr-8 .. wr+4 of each plane; word 8 holds base R (resp. Q)
-                const uint32_t *rp = rplane + (R >> 4) - 8;
-                const uint32_t *qp = qplane + (Q >> 4) - 8;
+                // plane words w-8 .. w+4 around word w = R >> 4 (resp. Q >> 4), loaded
+                // straight into the stream registers: A[k] = word w+k, k < 4, and
+                // LA[k] = word w-k, k < 8, plus the two end words
+                const uint32_t *rp = rplane + (R >> 4);
+                const uint32_t *qp = qplane + (Q >> 4);
                 uint32_t cr0 = rc[0], cr1 = rc[1], cq0 = qc[0], cq1 = qc[1];
-                uint32_t W[13], V[13];
+                uint32_t rend = rp[G2_RIGHT_WORDS], qend = qp[G2_RIGHT_WORDS];
+                uint32_t lend = rp[-(int)G2_LEFT_WORDS], mend = qp[-(int)G2_LEFT_WORDS];
 #pragma unroll
-                for (int j = 0; j < 13; j++) {
-                    W[j] = rp[j];
-                    V[j] = qp[j];
+                for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) {
+                    A[k] = rp[k];
+                    B[k] = qp[k];
+                }
+#pragma unroll
+                for (int k = 0; k < (int)G2_LEFT_WORDS; k++) {
+                    LA[k] = rp[-k];
+                    LB[k] = qp[-k];
                 }
                 ok = !coarse_bits(cr0, cr1, rlo, rhi) && !coarse_bits(cq0, cq1, qlo, qhi);
                 uint32_t sr = 2 * (R & 15), sq = 2 * (Q & 15);
+                // right word k: bases [R + 16k, R + 16k + 16), first base in the low field
 #pragma unroll
-                for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) {
-                    A[j] = __builtin_amdgcn_alignbit(W[9 + j], W[8 + j], sr);
-                    B[j] = __builtin_amdgcn_alignbit(V[9 + j], V[8 + j], sq);
+                for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) {
+                    uint32_t rn = k + 1 < (int)G2_RIGHT_WORDS ? A[k + 1] : rend;
+                    uint32_t qn = k + 1 < (int)G2_RIGHT_WORDS ? B[k + 1] : qend;
+                    A[k] = __builtin_amdgcn_alignbit(rn, A[k], sr);
+                    B[k] = __builtin_amdgcn_alignbit(qn, B[k], sq);
                 }
-                // left word j: bases [R - 16(j+1), R - 16j), nearest base in the top field
+                // left word k: bases [R - 16(k+1), R - 16k), nearest base in the top field
 #pragma unroll
-                for (int j = 0; j < (int)G2_LEFT_WORDS; j++) {
-                    LA[j] = __builtin_amdgcn_alignbit(W[8 - j], W[7 - j], sr);
-                    LB[j] = __builtin_amdgcn_alignbit(V[8 - j], V[7 - j], sq);
+                for (int k = 0; k < (int)G2_LEFT_WORDS; k++) {
+                    uint32_t rn = k + 1 < (int)G2_LEFT_WORDS ? LA[k + 1] : lend;
+                    uint32_t qn = k + 1 < (int)G2_LEFT_WORDS ? LB[k + 1] : mend;
+                    LA[k] = __builtin_amdgcn_alignbit(LA[k], rn, sr);
+                    LB[k] = __builtin_amdgcn_alignbit(LB[k], qn, sq);
                 }
             }
             if (!ok) {
                 defer = true;
             } else {
                 left = false;
-                tb = tab_fwd;
+                off = copy_off;
                 grp = 0;
                 limit = G2_RIGHT_WORDS / SW;
                 c = 0;
@@ -561,50 +598,56 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
             int32_t emin = e;
 #pragma unroll
             for (int h = 0; h < SW; h++) {  // SW words = 16 * SW cells per step
-                uint32_t r = A[h], q = B[h];
+                uint32_t r = left ? LA[h] : A[h], q = left ? LB[h] : B[h];
+                // key bytes: t1 byte k = cells 4k, 4k+1; t2 byte k = cells 4k+2, 4k+3
+                // (left words are byte-swapped, so t2 byte k holds the nearer pair)
                 uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
                 uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
-                uint32_t x1 = left ? __builtin_bswap32(t2) : t1;
-                uint32_t x2 = left ? __builtin_bswap32(t1) : t2;
+                uint32_t x1 = left ? t2 : t1, x2 = left ? t1 : t2;
                 uint32_t p[8];
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    p[2 * j] = tb[((x1 >> (8 * j)) & 0xFF) * G2_TAB_COPIES];
-                    p[2 * j + 1] = tb[((x2 >> (8 * j)) & 0xFF) * G2_TAB_COPIES];
+                    p[2 * j] = map_entry(s_map, x1, off, SEL[j]);
+                    p[2 * j + 1] = map_entry(s_map, x2, off, SEL[j]);
                 }
 #pragma unroll
-                for (int i = 0; i < 16; i++) {
-                    uint32_t pw = p[i >> 1];
-                    int32_t sc = (i & 1) ? ((int32_t)pw >> 16) : (int32_t)(int16_t)(pw & 0xFFFF);
-                    int32_t t = e + sc;
-                    e = min(t, xdrop);
-                    emin = min(emin, t);
+                for (int i = 0; i < 8; i += 2) {
+                    int32_t e1 = min(e + lo16(p[i]), hi16(p[i]));
+                    e = min(e1 + lo16(p[i + 1]), hi16(p[i + 1]));
+                    emin = min(emin, min(e1, e));
                 }
-                // sum of the 16 scores: packed int16 partial sums, then both halves
-                uint32_t acc = pk_add16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3])),
-                                        pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
-                c += pair_sum(acc);
+                // score sum: the a halves of 4 entries at a time, exact in int16
+                c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3]))) +
+                     lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
             }
             if (emin >= 0) {
                 grp++;
+                if (!left) {
 #pragma unroll
-                for (int j = 0; j + SW < (int)G2_LEFT_WORDS; j++) {
-                    A[j] = A[j + SW];
-                    B[j] = B[j + SW];
+                    for (int j = 0; j + SW < (int)G2_RIGHT_WORDS; j++) {
+                        A[j] = A[j + SW];
+                        B[j] = B[j + SW];
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j + SW < (int)G2_LEFT_WORDS; j++) {
+                        LA[j] = LA[j + SW];
+                        LB[j] = LB[j + SW];
+                    }
                 }
                 if (grp == limit) defer = true;  // outlived the register window
             } else if (!left) {
                 Mr = c + (xdrop - e);
                 left = true;
-                tb = tab_rev;
+                off = copy_off + 128;
                 grp = 0;
                 limit = G2_LEFT_WORDS / SW;
                 c = 0;
                 e = xdrop;
 #pragma unroll
                 for (int j = 0; j < (int)G2_LEFT_WORDS; j++) {
-                    A[j] = LA[j];
-                    B[j] = LB[j];
+                    LA[j] = __builtin_bswap32(LA[j]);
+                    LB[j] = __builtin_bswap32(LB[j]);
                 }
             } else {
                 int32_t total = Mr + c + (xdrop - e);
@@ -678,10 +721,8 @@ __global__ __launch_bounds__(512) void k_gate_packed(
     const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh,
     const uint64_t *__restrict__ anchors, uint32_t n, uint8_t *__restrict__ flags,
     uint32_t *__restrict__ slow, uint32_t *__restrict__ slow_n) {
-    __shared__ uint32_t s_tab[2 * 256 * G2_TAB_COPIES];
-    for (uint32_t i = threadIdx.x; i < 2 * 256 * G2_TAB_COPIES; i += blockDim.x) s_tab[i] = pairtab[i / G2_TAB_COPIES];
-    __syncthreads();
-    const uint32_t *tab_fwd = s_tab + (lane_id() % G2_TAB_COPIES);
+    __shared__ uint32_t s_map[G2_MAP_WORDS];
+    load_map_table(s_map, pairtab, blockDim.x);
     AnchorSource src;
     src.anchors = anchors;
     src.n = n;
@@ -689,8 +730,8 @@ __global__ __launch_bounds__(512) void k_gate_packed(
     src.slow = slow;
     src.slow_n = slow_n;
     src.init();
-    packed_gate_loop<2>(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, tab_fwd,
-                     tab_fwd + 256 * G2_TAB_COPIES, xdrop, thresh);
+    packed_gate_loop<2>(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, (const uint8_t *)s_map, xdrop,
+                        thresh);
 }
 
 // ----------------------------------------------------------------- wave extension

@@ -345,8 +345,10 @@ struct SeedSource {
 
 // The batched gate: fused hit expansion + packed gate over a persistent grid
 // of BT-thread blocks, 16 * SW cells per step, `claim` seeds per wave claim.
+// Two blocks share a CU (the LDS table is 64 KiB); at 1024 threads that is 8
+// waves per SIMD, which caps the kernel at 64 VGPRs.
 template <int SW, int BT>
-__global__ __launch_bounds__(BT) void k_bgate_packed(
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_packed(
     const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
     const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse, uint32_t qlen,
     const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh,
@@ -356,10 +358,8 @@ __global__ __launch_bounds__(BT) void k_bgate_packed(
     SurvRec *__restrict__ surv, uint32_t *__restrict__ surv_n, uint32_t cap_surv,
     SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow,
     uint32_t *__restrict__ err) {
-    __shared__ uint32_t s_tab[2 * 256 * G2_TAB_COPIES];
-    for (uint32_t i = threadIdx.x; i < 2 * 256 * G2_TAB_COPIES; i += BT) s_tab[i] = pairtab[i / G2_TAB_COPIES];
-    __syncthreads();
-    const uint32_t *tab_fwd = s_tab + (lane_id() % G2_TAB_COPIES);
+    __shared__ uint32_t s_map[G2_MAP_WORDS];
+    load_map_table(s_map, pairtab, BT);
     SeedSource src;
     src.seeds = seeds;
     src.index = index;
@@ -376,8 +376,389 @@ __global__ __launch_bounds__(BT) void k_bgate_packed(
     src.cap_slow = cap_slow;
     src.err = err;
     src.init();
-    packed_gate_loop<SW>(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, tab_fwd,
-                         tab_fwd + 256 * G2_TAB_COPIES, xdrop, thresh);
+    packed_gate_loop<SW>(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, (const uint8_t *)s_map, xdrop,
+                         thresh);
+}
+
+// ---------------------------------------------------------------- k-mer-major order
+//
+// The seed-major walk above gives every hit its own random reference window
+// (one or two 128-byte lines per hit, past L2). Against a 1 Gbp target a
+// multi-megabase batch holds several query seeds per k-mer, so the batched
+// gate can instead group seeds by k-mer (a counting sort) and give each lane
+// one reference position of the k-mer's bucket: the lane keeps that window in
+// registers and walks every query seed of the group, so a reference window is
+// fetched once per (k-mer, position) instead of once per hit.
+
+// Per pos_table entry: 1 when the anchor R = pos + seed_size has the packed
+// gate's margins and no exception code in its window (the reference-side
+// test of packed_gate_loop, done once per engine).
+__global__ void k_clean_bits(const uint32_t *__restrict__ pos, uint32_t n, const uint32_t *__restrict__ rcoarse,
+                             uint32_t ref_len, uint32_t seed_size, uint32_t *__restrict__ bits) {
+    const uint32_t lane = lane_id();
+    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x / WAVE);
+    const uint32_t nwords = (n + 31) / 32;
+    for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * WAVE; base < n;
+         base += nwaves * WAVE) {
+        uint64_t i = base + lane;
+        bool ok = false;
+        if (i < n) {
+            uint32_t R = pos[i] + seed_size;
+            ok = R >= G2_MARGIN && (uint64_t)R + G2_MARGIN <= ref_len &&
+                 !coarse_hit(rcoarse, R - 16 * G2_LEFT_WORDS, R + 16 * G2_RIGHT_WORDS - 1);
+        }
+        uint64_t m = __ballot(ok);
+        uint32_t w = (uint32_t)(base / 32);
+        if (lane == 0) bits[w] = (uint32_t)m;
+        if (lane == 1 && w + 1 < nwords) bits[w + 1] = (uint32_t)(m >> 32);
+    }
+}
+
+// Group sizes: seeds with at least one hit, counted per k-mer.
+__global__ void k_kcount(const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ counts,
+                         const uint32_t *__restrict__ nvalid, uint32_t per, uint32_t *__restrict__ gcnt) {
+    const uint32_t n = *nvalid * per;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (counts[i]) atomicAdd(&gcnt[(uint32_t)(seeds[i] >> 32)], 1u);
+}
+
+// After an inclusive scan of gcnt, every seed with hits takes a slot of its
+// group: sq = (seed index << 32) | arena position, skm = k-mer. gcnt[k] ends
+// as the start of group k (and gcnt[nk] stays the number of grouped seeds).
+__global__ void k_kscatter(const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ counts,
+                           const uint32_t *__restrict__ nvalid, uint32_t per, uint32_t *__restrict__ gcnt,
+                           uint64_t *__restrict__ sq, uint32_t *__restrict__ skm) {
+    const uint32_t n = *nvalid * per;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (!counts[i]) continue;
+        uint64_t sd = seeds[i];
+        uint32_t k = (uint32_t)(sd >> 32);
+        uint32_t slot = atomicSub(&gcnt[k], 1u) - 1u;
+        sq[slot] = ((uint64_t)i << 32) | (uint32_t)sd;
+        skm[slot] = k;
+    }
+}
+
+// Word g * SW + h of a stream (g = step within the phase), by selects.
+template <int SW, int N>
+__device__ __forceinline__ uint32_t pick_word(const uint32_t (&w)[N], uint32_t g, int h) {
+    uint32_t v = w[h];
+#pragma unroll
+    for (int gg = 1; gg < N / SW; gg++) v = g == (uint32_t)gg ? w[gg * SW + h] : v;
+    return v;
+}
+
+// K-mer-major hit source: waves claim `claim` consecutive grouped seeds at a
+// time and own the groups that start inside the claim; a group's bucket
+// entries go to idle lanes as items (one reference position each), and a
+// lane walks its item's group of query seeds one hit at a time, with the next
+// seed's record loaded a hit ahead.
+struct KmerSource {
+    const uint64_t *sq, *seeds;
+    const uint32_t *skm, *goff, *index, *pos, *clean;
+    unsigned long long *work;
+    uint32_t total, claim, seed_size;
+    uint32_t p_cur, p_end, g0, g1, b0, n, j, q0, t0;
+    bool dry;
+    SurvRec *surv, *slow;
+    uint32_t *surv_n, *slow_n, cap_surv, cap_slow;
+    uint32_t *err;
+
+    __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+    // Claims the next range of grouped seeds; p_cur = first group start in it.
+    __device__ __forceinline__ bool take() {
+        uint32_t leader = (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1);
+        unsigned long long v = 0;
+        if (lane_id() == leader) v = atomicAdd(work, (unsigned long long)claim);
+        uint64_t start = ((uint64_t)rfl((uint32_t)(v >> 32)) << 32) | (uint64_t)rfl((uint32_t)v);
+        if (start >= (uint64_t)total) {
+            dry = true;
+            return false;
+        }
+        uint64_t end = start + claim;
+        p_end = end < (uint64_t)total ? (uint32_t)end : total;
+        uint32_t st = (uint32_t)start;
+        uint32_t k = rfl(skm[st]), kp = rfl(skm[st ? st - 1 : 0]);
+        if (st == 0) kp = ~0u;
+        p_cur = k == kp ? rfl(goff[k + 1]) : st;
+        return true;
+    }
+    __device__ __forceinline__ void init() {
+        dry = false;
+        p_cur = p_end = 0;
+        n = j = 0;
+    }
+    // Next group of the claims; false when they are exhausted.
+    __device__ __forceinline__ bool next_group() {
+        for (;;) {
+            if (dry) return false;
+            if (p_cur >= p_end) {
+                if (!take()) return false;
+                continue;
+            }
+            uint32_t k = rfl(skm[p_cur]);
+            g0 = p_cur;
+            g1 = rfl(goff[k + 1]);
+            p_cur = g1;
+            uint32_t bb1 = index[k], bb0 = index[k ? k - 1 : 0];
+            if (k == 0) bb0 = 0;
+            b0 = rfl(bb0);
+            n = rfl(bb1 - bb0);
+            j = 0;
+            uint64_t f = sq[g0];
+            q0 = rfl((uint32_t)f + seed_size);
+            t0 = rfl((uint32_t)(f >> 32));
+            if (n) return true;
+        }
+    }
+    // A lane without a hit takes its item's next seed, or else a new item.
+    __device__ __forceinline__ void refill(bool &has, bool &item, bool &fresh_r, bool &fresh_q, uint32_t &R,
+                                           uint32_t &Q, uint32_t &tag, uint32_t &s_cur, uint32_t &s_end,
+                                           uint64_t &next, bool &rok) {
+        if (!has && item) {
+            if (s_cur < s_end) {
+                Q = (uint32_t)next + seed_size;
+                tag = (uint32_t)(next >> 32);
+                s_cur++;
+                if (s_cur < s_end) next = sq[s_cur];
+                has = true;
+                fresh_q = true;
+            } else {
+                item = false;
+            }
+        }
+        uint64_t need = __ballot(!has);
+        while (need) {
+            if (j >= n) {
+                if (!next_group()) break;
+                continue;
+            }
+            uint32_t avail = n - j;
+            uint32_t rank = mbcnt(need);
+            if (!has && rank < avail) {
+                uint32_t e = b0 + j + rank;
+                R = pos[e] + seed_size;
+                rok = ((clean[e >> 5] >> (e & 31)) & 1u) != 0;
+                Q = q0;
+                tag = t0;
+                s_cur = g0 + 1;
+                s_end = g1;
+                if (s_cur < s_end) next = sq[s_cur];
+                has = true;
+                item = true;
+                fresh_r = true;
+                fresh_q = true;
+            }
+            j += min((uint32_t)__popcll(need), avail);
+            need = __ballot(!has);
+        }
+    }
+    __device__ __forceinline__ void emit(bool done, bool pass, uint32_t tag, uint32_t R, uint32_t Q) {
+        SeedSource::append(done && pass, surv, surv_n, cap_surv, err, R, Q, tag);
+    }
+    __device__ __forceinline__ void defer(bool d, uint32_t tag, uint32_t R, uint32_t Q) {
+        SeedSource::append(d, slow, slow_n, cap_slow, err, R, Q, tag);
+    }
+};
+
+// The packed gate over k-mer-major items: the same X-drop core and upper-bound
+// argument as packed_gate_loop; reference words are prepared once per item
+// (A0, LA0) and query words once per hit. At 64-cell steps no stream is
+// shifted (each step picks its words by step index), so the reference window
+// survives for the item's next hit; narrower steps shift working copies that
+// every hit restores from A0, LA0. Left-stream words are byte-swapped at
+// preparation.
+template <int SW, class Src>
+__device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restrict__ rplane, uint32_t ref_len,
+                                               const uint32_t *__restrict__ qplane,
+                                               const uint32_t *__restrict__ qcoarse, uint32_t qlen,
+                                               const uint8_t *s_map, int32_t xdrop, int32_t thresh) {
+    static_assert(SW == 1 || SW == 2 || SW == 4, "step width must divide the right window");
+    constexpr uint32_t SEL[4] = {0x0C0C0400u, 0x0C0C0500u, 0x0C0C0600u, 0x0C0C0700u};
+    bool has = false, item = false, rok = false, left = false;
+    uint32_t R = 0, Q = 0, tag = 0, s_cur = 0, s_end = 0, grp = 0, limit = 0;
+    uint64_t next = 0;
+    const uint32_t copy_off = (lane_id() & 31) * 4;
+    uint32_t off = copy_off;
+    int32_t c = 0, e = xdrop, Mr = 0;
+    constexpr bool SHIFT = SW < 4;
+    constexpr int WR = SHIFT ? (int)G2_RIGHT_WORDS : 1, WL = SHIFT ? (int)G2_LEFT_WORDS : 1;
+    uint32_t A0[G2_RIGHT_WORDS], LA0[G2_LEFT_WORDS], B[G2_RIGHT_WORDS], LB[G2_LEFT_WORDS];
+    uint32_t A[WR], LA[WL];  // working copies of the reference streams (narrow steps only)
+#pragma unroll
+    for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) A0[j] = B[j] = 0;
+#pragma unroll
+    for (int j = 0; j < (int)G2_LEFT_WORDS; j++) LA0[j] = LB[j] = 0;
+#pragma unroll
+    for (int j = 0; j < WR; j++) A[j] = 0;
+#pragma unroll
+    for (int j = 0; j < WL; j++) LA[j] = 0;
+
+    for (;;) {
+        bool fresh_r = false, fresh_q = false;
+        src.refill(has, item, fresh_r, fresh_q, R, Q, tag, s_cur, s_end, next, rok);
+        if (__ballot(has) == 0) break;
+
+        bool defer = false, done = false, pass = false;
+        if (fresh_r && rok) {
+            const uint32_t *rp = rplane + (R >> 4);
+            uint32_t rend = rp[G2_RIGHT_WORDS], lend = rp[-(int)G2_LEFT_WORDS];
+#pragma unroll
+            for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) A0[k] = rp[k];
+#pragma unroll
+            for (int k = 0; k < (int)G2_LEFT_WORDS; k++) LA0[k] = rp[-k];
+            uint32_t sr = 2 * (R & 15);
+#pragma unroll
+            for (int k = 0; k < (int)G2_RIGHT_WORDS; k++)
+                A0[k] = __builtin_amdgcn_alignbit(k + 1 < (int)G2_RIGHT_WORDS ? A0[k + 1] : rend, A0[k], sr);
+#pragma unroll
+            for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
+                LA0[k] = __builtin_bswap32(
+                    __builtin_amdgcn_alignbit(LA0[k], k + 1 < (int)G2_LEFT_WORDS ? LA0[k + 1] : lend, sr));
+        }
+        if (fresh_q) {
+            bool ok = rok && Q >= G2_MARGIN && (uint64_t)Q + G2_MARGIN <= qlen;
+            if (ok) {
+                const uint32_t qlo = Q - 16 * G2_LEFT_WORDS, qhi = Q + 16 * G2_RIGHT_WORDS - 1;
+                const uint32_t *qc = qcoarse + (qlo >> 11);
+                const uint32_t *qp = qplane + (Q >> 4);
+                uint32_t cq0 = qc[0], cq1 = qc[1];
+                uint32_t qend = qp[G2_RIGHT_WORDS], mend = qp[-(int)G2_LEFT_WORDS];
+#pragma unroll
+                for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) B[k] = qp[k];
+#pragma unroll
+                for (int k = 0; k < (int)G2_LEFT_WORDS; k++) LB[k] = qp[-k];
+                ok = !coarse_bits(cq0, cq1, qlo, qhi);
+                uint32_t sq = 2 * (Q & 15);
+#pragma unroll
+                for (int k = 0; k < (int)G2_RIGHT_WORDS; k++)
+                    B[k] = __builtin_amdgcn_alignbit(k + 1 < (int)G2_RIGHT_WORDS ? B[k + 1] : qend, B[k], sq);
+#pragma unroll
+                for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
+                    LB[k] = __builtin_bswap32(
+                        __builtin_amdgcn_alignbit(LB[k], k + 1 < (int)G2_LEFT_WORDS ? LB[k + 1] : mend, sq));
+            }
+            if (!ok) {
+                defer = true;
+            } else {
+                left = false;
+                off = copy_off;
+                grp = 0;
+                limit = G2_RIGHT_WORDS / SW;
+                c = 0;
+                e = xdrop;
+                if constexpr (SHIFT) {
+#pragma unroll
+                    for (int k = 0; k < WR; k++) A[k] = A0[k];
+#pragma unroll
+                    for (int k = 0; k < WL; k++) LA[k] = LA0[k];
+                }
+            }
+        }
+
+        if (has && !defer) {
+            int32_t emin = e;
+#pragma unroll
+            for (int h = 0; h < SW; h++) {
+                uint32_t r, q;
+                if constexpr (SHIFT) {
+                    r = left ? LA[h] : A[h];
+                    q = left ? LB[h] : B[h];
+                } else {
+                    r = left ? pick_word<SW>(LA0, grp, h) : A0[h];
+                    q = left ? pick_word<SW>(LB, grp, h) : B[h];
+                }
+                uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
+                uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
+                uint32_t x1 = left ? t2 : t1, x2 = left ? t1 : t2;
+                uint32_t p[8];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    p[2 * j] = map_entry(s_map, x1, off, SEL[j]);
+                    p[2 * j + 1] = map_entry(s_map, x2, off, SEL[j]);
+                }
+#pragma unroll
+                for (int i = 0; i < 8; i += 2) {
+                    int32_t e1 = min(e + lo16(p[i]), hi16(p[i]));
+                    e = min(e1 + lo16(p[i + 1]), hi16(p[i + 1]));
+                    emin = min(emin, min(e1, e));
+                }
+                c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3]))) +
+                     lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
+            }
+            if (emin >= 0) {
+                grp++;
+                if constexpr (SHIFT) {
+                    if (!left) {
+#pragma unroll
+                        for (int j = 0; j + SW < WR; j++) {
+                            A[j] = A[j + SW];
+                            B[j] = B[j + SW];
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j + SW < WL; j++) {
+                            LA[j] = LA[j + SW];
+                            LB[j] = LB[j + SW];
+                        }
+                    }
+                }
+                if (grp == limit) defer = true;  // outlived the register window
+            } else if (!left) {
+                Mr = c + (xdrop - e);
+                left = true;
+                off = copy_off + 128;
+                grp = 0;
+                limit = G2_LEFT_WORDS / SW;
+                c = 0;
+                e = xdrop;
+            } else {
+                int32_t total = Mr + c + (xdrop - e);
+                done = true;
+                pass = gate_keep(total, thresh);
+            }
+        }
+        src.emit(done, pass, tag, R, Q);
+        src.defer(defer, tag, R, Q);
+        if (done || defer) has = false;
+    }
+}
+
+// The batched gate in k-mer-major order (same launch shape as k_bgate_packed).
+template <int SW, int BT>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_kmer(
+    const uint32_t *__restrict__ rplane, uint32_t ref_len, const uint32_t *__restrict__ qplane,
+    const uint32_t *__restrict__ qcoarse, uint32_t qlen, const uint32_t *__restrict__ pairtab, int32_t xdrop,
+    int32_t thresh, const uint64_t *__restrict__ sq, const uint32_t *__restrict__ skm,
+    const uint32_t *__restrict__ goff, uint32_t nk, const uint64_t *__restrict__ seeds,
+    const uint32_t *__restrict__ index, const uint32_t *__restrict__ pos, const uint32_t *__restrict__ clean,
+    uint32_t seed_size, uint32_t claim, unsigned long long *__restrict__ work, SurvRec *__restrict__ surv,
+    uint32_t *__restrict__ surv_n, uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n,
+    uint32_t cap_slow, uint32_t *__restrict__ err) {
+    __shared__ uint32_t s_map[G2_MAP_WORDS];
+    load_map_table(s_map, pairtab, BT);
+    KmerSource src;
+    src.sq = sq;
+    src.seeds = seeds;
+    src.skm = skm;
+    src.goff = goff;
+    src.index = index;
+    src.pos = pos;
+    src.clean = clean;
+    src.work = work;
+    src.total = __builtin_amdgcn_readfirstlane(goff[nk]);
+    src.claim = claim;
+    src.seed_size = seed_size;
+    src.surv = surv;
+    src.surv_n = surv_n;
+    src.cap_surv = cap_surv;
+    src.slow = slow;
+    src.slow_n = slow_n;
+    src.cap_slow = cap_slow;
+    src.err = err;
+    src.init();
+    kmer_gate_loop<SW>(src, rplane, ref_len, qplane, qcoarse, qlen, (const uint8_t *)s_map, xdrop, thresh);
 }
 
 // Block bounds of a seed's batch: arena offset and length of its block.

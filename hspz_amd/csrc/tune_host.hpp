@@ -1,14 +1,15 @@
 // tune_host.hpp — autotuner of the batched gate (hspz_autotune_batch in
 // include/hspz.h; included by engine.hip after batch_host.hpp).
 //
-// The batched gate has four launch knobs that change its speed and never its
-// output: cells per step (16/32/64, the width of the score tile a lane
-// evaluates between drop tests), threads per block (256/512), seeds per wave
-// claim (the granule of the grid-wide work counter; at ~60 hits per seed
-// against a 1 Gbp target a claim of 4/16/64 seeds is ~240/960/3840 hits per
-// wave), and the persistent grid. The tuner runs the whole batched pass over
-// the caller's chunks for each cell, takes the gate and materializer times
-// from the pass's own stream events, and keeps the fastest cell.
+// The batched gate has five knobs that change its speed and never its
+// output: the hit order (seed-major, or k-mer-major with reference windows
+// reused across a k-mer's query seeds), cells per step (16/32/64, the width of
+// the score tile a lane evaluates between drop tests), threads per block
+// (512/1024, sharing one 64 KiB LDS map table), seeds per wave claim (the
+// granule of the grid-wide work counter), and the persistent grid. The tuner
+// runs the whole batched pass over the caller's chunks for each cell, takes
+// the gate and materializer times from the pass's own stream events, and
+// keeps the fastest cell.
 #pragma once
 
 #include <cmath>
@@ -16,16 +17,19 @@
 namespace {
 
 struct TuneKnobs {
-    int step, threads, claim, blocks;
+    int step, threads, claim, blocks, order;
 };
 
-TuneKnobs tune_get(const hspz_engine *h) { return {h->bgate_step, h->bgate_threads, h->bgate_claim, h->bgate_blocks}; }
+TuneKnobs tune_get(const hspz_engine *h) {
+    return {h->bgate_step, h->bgate_threads, h->bgate_claim, h->bgate_blocks, h->batch_order};
+}
 
 void tune_set(hspz_engine *h, const TuneKnobs &k) {
     h->bgate_step = k.step;
     h->bgate_threads = k.threads;
     h->bgate_claim = k.claim;
     h->bgate_blocks = k.blocks;
+    h->batch_order = k.order;
 }
 
 }  // namespace
@@ -47,6 +51,7 @@ extern "C" int hspz_autotune_batch(hspz_engine *h, const hspz_pass_params *p, co
         c.step_cells = (uint32_t)k.step;
         c.block_threads = (uint32_t)k.threads;
         c.claim_seeds = (uint32_t)k.claim;
+        c.order = (uint32_t)k.order;
         double best_sum = HUGE_VAL;
         for (uint32_t r = 0; r < reps; r++) {
             hspz_batch_output out;
@@ -80,12 +85,16 @@ extern "C" int hspz_autotune_batch(hspz_engine *h, const hspz_pass_params *p, co
         }
         return bi;
     };
+    // stage 0: hit order (seed-major, k-mer-major) at the engine's knobs
+    for (int order = 0; order < 2; order++) measure({saved.step, saved.threads, saved.claim, 0, order});
+    long bi = best_index();
+    const int order = bi >= 0 ? knobs[bi].order : saved.order;
     // stage 1: step x block x claim at the resident grid
-    const int steps[] = {16, 32, 64}, threads[] = {256, 512}, claims[] = {4, 16, 64};
+    const int steps[] = {16, 32, 64}, threads[] = {512, 1024}, claims[] = {16, 64, 256};
     for (int st : steps)
         for (int th : threads)
-            for (int cl : claims) measure({st, th, cl, 0});
-    long bi = best_index();
+            for (int cl : claims) measure({st, th, cl, 0, order});
+    bi = best_index();
     // stage 2: grid sizes around the resident grid of the best cell
     if (bi >= 0) {
         TuneKnobs k = knobs[bi];

@@ -24,7 +24,7 @@ STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 
         "query_swaps": 4, "peak_device_bytes": 5, "gate_ns": 6, "gate_hits": 7,
         "gate_launches": 8, "gate_deferred": 9, "graph_captures": 10, "graph_replays": 11}
 FLAG = {"batch_graph": 0, "batch_gate_blocks": 1, "batch_step_cells": 2, "batch_block_threads": 3,
-        "batch_claim_seeds": 4}
+        "batch_claim_seeds": 4, "batch_order": 5, "batch_order_used": 6}
 
 # Every symbol include/hspz.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -83,7 +83,7 @@ class BatchOutput(C.Structure):
 
 class TuneCell(C.Structure):
     _fields_ = [("step_cells", C.c_uint32), ("block_threads", C.c_uint32),
-                ("grid_blocks", C.c_uint32), ("claim_seeds", C.c_uint32),
+                ("grid_blocks", C.c_uint32), ("claim_seeds", C.c_uint32), ("order", C.c_uint32),
                 ("gate_ms", C.c_double), ("after_ms", C.c_double), ("hits", C.c_uint64),
                 ("survivors", C.c_uint64), ("deferred", C.c_uint64), ("status", C.c_int32)]
 

@@ -119,10 +119,15 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 #define HSPZ_FLAG_BATCH_GRAPH 0        /* 1 (default): capture/replay the batched pass as a hipGraph */
 #define HSPZ_FLAG_BATCH_GATE_BLOCKS 1  /* persistent grid of the batched gate in blocks; 0 (default) = as many
                                           as are resident at once */
-#define HSPZ_FLAG_BATCH_STEP_CELLS 2   /* cells per gate step: 16, 32 (default) or 64 */
-#define HSPZ_FLAG_BATCH_BLOCK_THREADS 3 /* threads per gate block: 256 or 512 (default) */
+#define HSPZ_FLAG_BATCH_STEP_CELLS 2   /* cells per gate step: 16, 32 or 64 (default) */
+#define HSPZ_FLAG_BATCH_BLOCK_THREADS 3 /* threads per gate block: 512 (default) or 1024 */
 #define HSPZ_FLAG_BATCH_CLAIM_SEEDS 4  /* seeds a wave claims from the work counter at a time (1..4096,
-                                          default 16) */
+                                          default 64) */
+#define HSPZ_FLAG_BATCH_ORDER 5        /* hit order of the batched gate: 0 seed-major, 1 k-mer-major (a
+                                          reference window is reused by every query seed of its k-mer),
+                                          2 (default) k-mer-major when the batch holds at least one seed
+                                          per two k-mers */
+#define HSPZ_FLAG_BATCH_ORDER_USED 6   /* read-only: the order of the last batched pass */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */
@@ -224,6 +229,7 @@ typedef struct hspz_tune_cell {
     uint32_t block_threads; /* HSPZ_FLAG_BATCH_BLOCK_THREADS */
     uint32_t grid_blocks;   /* HSPZ_FLAG_BATCH_GATE_BLOCKS as launched */
     uint32_t claim_seeds;   /* HSPZ_FLAG_BATCH_CLAIM_SEEDS */
+    uint32_t order;         /* HSPZ_FLAG_BATCH_ORDER (0 or 1) */
     double gate_ms;         /* best of the repetitions: fused gate + exact gate for deferred hits */
     double after_ms;        /* best of the repetitions: materializer and output compaction */
     uint64_t hits, survivors, deferred;
@@ -231,8 +237,9 @@ typedef struct hspz_tune_cell {
 } hspz_tune_cell;
 
 /* Autotuner of the batched gate (no counterpart in the reference). Runs
- * hspz_filter_batch over `chunks` for every cell of a grid over step width
- * {16, 32, 64} cells x block size {256, 512} x wave claim {4, 16, 64} seeds at
+ * hspz_filter_batch over `chunks` for both hit orders at the engine's knobs,
+ * then, in the faster order, for every cell of a grid over step width
+ * {16, 32, 64} cells x block size {512, 1024} x wave claim {16, 64, 256} seeds at
  * the resident grid, then over grid sizes {1/2, 3/4, 1, 3/2, 2} x resident
  * for the best of those, `reps` times each, and records every cell it measured
  * in cells[0..*n_cells) (up to cap; *n_cells counts all of them). The engine

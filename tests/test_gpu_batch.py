@@ -35,6 +35,13 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
     with E.Engine(index, pos, ref_enc, sub, max_hits=max_hits, noentropy=noentropy) as eng:
         chunks = [(S.encode(f), S.encode(r)) for f, r in query_blocks]
         got, st = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
+        for order in (0, 1):  # both hit orders of the batched gate give the same lists
+            eng.set_flag("batch_order", order)
+            again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
+            assert eng.get_flag("batch_order_used") == order
+            for i, (a, g) in enumerate(zip(again, got)):
+                assert same_intervals(a, g), "chunk %d differs in hit order %d" % (i, order)
+        eng.set_flag("batch_order", 2)
         plain = []
         for f, r in chunks:
             eng.swap_query(f, r)
@@ -133,16 +140,19 @@ def test_batch_gate_configs_match(gpu_available):
     for f, r in blocks:
         o = O.Oracle(target, f, r, sub, max_hits=16711680, index=(index, pos))
         want.append(o.run_pass(threads=16)[0])
-    variants = [(16, 256, 1, 0), (64, 512, 4096, 0), (32, 256, 7, 3), (64, 256, 1, 1),
-                (16, 512, 64, 5000)]
+    variants = [(16, 1024, 1, 0, 0), (64, 512, 4096, 0, 0), (32, 1024, 7, 3, 0), (64, 1024, 1, 1, 0),
+                (16, 512, 64, 5000, 0), (64, 1024, 1, 0, 1), (32, 512, 3, 2, 1), (16, 1024, 4096, 0, 1),
+                (64, 512, 64, 7, 1), (32, 1024, 1, 5000, 1)]
     with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
-        for step, threads, claim, blocks_n in variants:
-            _say("  step %d threads %d claim %d grid %d" % (step, threads, claim, blocks_n))
+        for step, threads, claim, blocks_n, order in variants:
+            _say("  step %d threads %d claim %d grid %d order %d" % (step, threads, claim, blocks_n, order))
+            eng.set_flag("batch_order", order)
             eng.set_flag("batch_step_cells", step)
             eng.set_flag("batch_block_threads", threads)
             eng.set_flag("batch_claim_seeds", claim)
             eng.set_flag("batch_gate_blocks", blocks_n)
             got, st = eng.filter_batch(chunks)
+            assert eng.get_flag("batch_order_used") == order
             assert eng.get_flag("batch_step_cells") == step
             if blocks_n:
                 assert eng.get_flag("batch_gate_blocks") == blocks_n
@@ -150,8 +160,9 @@ def test_batch_gate_configs_match(gpu_available):
                 assert eng.get_flag("batch_gate_blocks") >= 256
             for i, (g, w) in enumerate(zip(got, want)):
                 assert same_intervals(g, w), "chunk %d differs at step %d claim %d" % (i, step, claim)
-        for name, bad in [("batch_step_cells", 24), ("batch_block_threads", 1024),
-                          ("batch_claim_seeds", 0), ("batch_gate_blocks", -1)]:
+        for name, bad in [("batch_step_cells", 24), ("batch_block_threads", 256),
+                          ("batch_claim_seeds", 0), ("batch_gate_blocks", -1), ("batch_order", 3),
+                          ("batch_order_used", 1)]:
             with pytest.raises(E.HspzError) as ei:
                 eng.set_flag(name, bad)
             assert ei.value.status == 2
@@ -163,16 +174,19 @@ def test_batch_autotune(gpu_available):
     with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
         before, st0 = eng.filter_batch(chunks)
         cells, best = eng.autotune(chunks, reps=1)
-        assert len(cells) == 22
+        assert len(cells) == 24
         assert all(c["status"] == 0 for c in cells)
         assert len({c["hits"] for c in cells}) == 1 and cells[0]["hits"] == st0["hits"]
-        assert {(c["step_cells"], c["block_threads"], c["claim_seeds"]) for c in cells[:18]} == {
-            (s, t, c) for s in (16, 32, 64) for t in (256, 512) for c in (4, 16, 64)}
+        assert [c["order"] for c in cells[:2]] == [0, 1]
+        assert len({c["order"] for c in cells[2:]}) == 1
+        assert {(c["step_cells"], c["block_threads"], c["claim_seeds"]) for c in cells[2:20]} == {
+            (s, t, c) for s in (16, 32, 64) for t in (512, 1024) for c in (16, 64, 256)}
         fastest = min(c["gate_ms"] + c["after_ms"] for c in cells)
         assert best["gate_ms"] + best["after_ms"] == fastest
         assert eng.get_flag("batch_step_cells") == best["step_cells"]
         assert eng.get_flag("batch_block_threads") == best["block_threads"]
         assert eng.get_flag("batch_claim_seeds") == best["claim_seeds"]
+        assert eng.get_flag("batch_order") == best["order"]
         after, st1 = eng.filter_batch(chunks)
         assert eng.get_flag("batch_gate_blocks") == best["grid_blocks"]
         for g, w in zip(after, before):
