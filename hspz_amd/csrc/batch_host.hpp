@@ -22,7 +22,7 @@ struct HostBatch {
 // Device buffers of the batched path, grow-only, owned by the engine.
 struct BatchBufs {
     DevBuf arena, plane, coarse, desc, poff, blk_off, blk_len, valid, ids, voff, seeds, counts, prefix,
-        tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr, gcnt, sq, skm;
+        tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr, gcnt, sq, skm, qrec;
 };
 
 // Device counters of one pass; CTR_WORK is the 64-bit seed-claim counter of
@@ -35,11 +35,10 @@ typedef void (*BGateFn)(const uint32_t *, const uint32_t *, uint32_t, const uint
                         const uint32_t *, const uint32_t *, uint32_t, uint32_t, unsigned long long *, SurvRec *,
                         uint32_t *, uint32_t, SurvRec *, uint32_t *, uint32_t, uint32_t *);
 
-typedef void (*KGateFn)(const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, uint32_t, const uint32_t *,
-                        int32_t, int32_t, const uint64_t *, const uint32_t *, const uint32_t *, uint32_t,
-                        const uint64_t *, const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t,
-                        unsigned long long *, SurvRec *, uint32_t *, uint32_t, SurvRec *, uint32_t *, uint32_t,
-                        uint32_t *);
+typedef void (*KGateFn)(const uint32_t *, const uint32_t *, int32_t, int32_t, const QRec *, const uint32_t *,
+                        const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, const uint32_t *, uint32_t,
+                        uint32_t, unsigned long long *, SurvRec *, uint32_t *, uint32_t, SurvRec *, uint32_t *,
+                        uint32_t, uint32_t *);
 
 // The k-mer-major gate instantiation for a step width and block size.
 KGateFn kgate_kernel(int step_cells, int threads) {
@@ -189,6 +188,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             if ((rc = reserve(h, B.gcnt, ((size_t)nkm + 1) * 4))) return rc;
             if ((rc = reserve(h, B.sq, cap_seeds * 8 + 8))) return rc;
             if ((rc = reserve(h, B.skm, cap_seeds * 4 + 4))) return rc;
+            if ((rc = reserve(h, B.qrec, cap_seeds * sizeof(QRec) + sizeof(QRec)))) return rc;
         }
         BlockOf bo;
         bo.seeds = B.seeds.as<uint64_t>();
@@ -244,6 +244,9 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             hipLaunchKernelGGL(k_kscatter, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
                                h->stream, B.seeds.as<uint64_t>(), B.counts.as<uint32_t>(), ctr + CTR_NVALID, per,
                                B.gcnt.as<uint32_t>(), B.sq.as<uint64_t>(), B.skm.as<uint32_t>());
+            hipLaunchKernelGGL(k_qrecords, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
+                               h->stream, B.sq.as<uint64_t>(), B.gcnt.as<uint32_t>() + nkm, B.plane.as<uint32_t>(),
+                               B.coarse.as<uint32_t>(), arena_len, h->seed_size, B.qrec.as<QRec>());
         }
         };
         // fused expansion + packed gate, then the exact gate for deferred hits
@@ -263,9 +266,8 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         auto enqueue_gate = [&]() {
         if (kmer_order)
             hipLaunchKernelGGL(kgate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
-                               h->ref_len, B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(), arena_len,
-                               h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.sq.as<uint64_t>(),
-                               B.skm.as<uint32_t>(), B.gcnt.as<uint32_t>(), nkm, B.seeds.as<uint64_t>(),
+                               h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
+                               B.skm.as<uint32_t>(), B.gcnt.as<uint32_t>(), nkm,
                                h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->clean.as<uint32_t>(),
                                h->seed_size, (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK),
                                B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(),
@@ -308,7 +310,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                   B.nchunks.p, B.starts.p, B.bhits.p, B.surv.p, B.slow.p, B.mhsp.p, B.mscope.p,
                                   B.keep.p, B.kids.p, B.ohsp.p, B.oscope.p, B.ctr.p, h->tiles.p, h->ref2.p,
                                   h->refx.p, h->pairtab.p, h->index.p, h->pos.p, B.gcnt.p, B.sq.p, B.skm.p,
-                                  h->clean.p};
+                                  h->clean.p, B.qrec.p};
             memcpy(key.ptrs, ptrs, sizeof ptrs);
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow, cap_seeds, per, h->max_hits,
                                (uint64_t)grid | (uint64_t)h->bgate_claim << 32, (uint64_t)p->transitions, nt, kt,

@@ -439,6 +439,55 @@ __global__ void k_kscatter(const uint64_t *__restrict__ seeds, const uint32_t *_
     }
 }
 
+// Query record of one grouped seed: its right and left query streams as the
+// k-mer-major gate consumes them (aligned to Q, left words byte-swapped),
+// the anchor Q, the seed index and whether the query window is clean.
+struct QRec {
+    uint32_t right[G2_RIGHT_WORDS];
+    uint32_t left[G2_LEFT_WORDS];
+    uint32_t Q, tag, ok, pad;
+};
+static_assert(sizeof(QRec) == 64, "query records are four 16-byte loads");
+
+// One record per grouped seed (after k_kscatter), so that a hit's query side
+// costs four loads instead of a window load, alignment and exception test.
+__global__ void k_qrecords(const uint64_t *__restrict__ sq, const uint32_t *__restrict__ total_p,
+                           const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse,
+                           uint32_t qlen, uint32_t seed_size, QRec *__restrict__ recs) {
+    const uint32_t total = *total_p;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+        uint64_t f = sq[p];
+        uint32_t Q = (uint32_t)f + seed_size;
+        QRec r;
+        r.Q = Q;
+        r.tag = (uint32_t)(f >> 32);
+        r.pad = 0;
+        bool ok = Q >= G2_MARGIN && (uint64_t)Q + G2_MARGIN <= qlen;
+        uint32_t W[13];
+#pragma unroll
+        for (int k = 0; k < 13; k++) W[k] = 0;
+        if (ok) {
+            const uint32_t qlo = Q - 16 * G2_LEFT_WORDS, qhi = Q + 16 * G2_RIGHT_WORDS - 1;
+            const uint32_t *qc = qcoarse + (qlo >> 11);
+            ok = !coarse_bits(qc[0], qc[1], qlo, qhi);
+            const uint32_t *qp = qplane + (Q >> 4) - G2_LEFT_WORDS;
+#pragma unroll
+            for (int k = 0; k < 13; k++) W[k] = qp[k];  // word 8 holds base Q
+        }
+        uint32_t sq2 = 2 * (Q & 15);
+#pragma unroll
+        for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) r.right[k] = __builtin_amdgcn_alignbit(W[9 + k], W[8 + k], sq2);
+#pragma unroll
+        for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
+            r.left[k] = __builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sq2));
+        r.ok = ok ? 1u : 0u;
+        uint4 *o = (uint4 *)(recs + p);
+        const uint4 *s = (const uint4 *)&r;
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = s[k];
+    }
+}
+
 // Word g * SW + h of a stream (g = step within the phase), by selects.
 template <int SW, int N>
 __device__ __forceinline__ uint32_t pick_word(const uint32_t (&w)[N], uint32_t g, int h) {
@@ -451,14 +500,12 @@ __device__ __forceinline__ uint32_t pick_word(const uint32_t (&w)[N], uint32_t g
 // K-mer-major hit source: waves claim `claim` consecutive grouped seeds at a
 // time and own the groups that start inside the claim; a group's bucket
 // entries go to idle lanes as items (one reference position each), and a
-// lane walks its item's group of query seeds one hit at a time, with the next
-// seed's record loaded a hit ahead.
+// lane walks its item's group of query records one hit at a time.
 struct KmerSource {
-    const uint64_t *sq, *seeds;
     const uint32_t *skm, *goff, *index, *pos, *clean;
     unsigned long long *work;
     uint32_t total, claim, seed_size;
-    uint32_t p_cur, p_end, g0, g1, b0, n, j, q0, t0;
+    uint32_t p_cur, p_end, g0, g1, b0, n, j;
     bool dry;
     SurvRec *surv, *slow;
     uint32_t *surv_n, *slow_n, cap_surv, cap_slow;
@@ -506,22 +553,15 @@ struct KmerSource {
             b0 = rfl(bb0);
             n = rfl(bb1 - bb0);
             j = 0;
-            uint64_t f = sq[g0];
-            q0 = rfl((uint32_t)f + seed_size);
-            t0 = rfl((uint32_t)(f >> 32));
             if (n) return true;
         }
     }
-    // A lane without a hit takes its item's next seed, or else a new item.
+    // A lane without a hit takes its item's next query record (index s_cur),
+    // or else a new item.
     __device__ __forceinline__ void refill(bool &has, bool &item, bool &fresh_r, bool &fresh_q, uint32_t &R,
-                                           uint32_t &Q, uint32_t &tag, uint32_t &s_cur, uint32_t &s_end,
-                                           uint64_t &next, bool &rok) {
+                                           uint32_t &s_cur, uint32_t &s_end, bool &rok) {
         if (!has && item) {
             if (s_cur < s_end) {
-                Q = (uint32_t)next + seed_size;
-                tag = (uint32_t)(next >> 32);
-                s_cur++;
-                if (s_cur < s_end) next = sq[s_cur];
                 has = true;
                 fresh_q = true;
             } else {
@@ -540,11 +580,8 @@ struct KmerSource {
                 uint32_t e = b0 + j + rank;
                 R = pos[e] + seed_size;
                 rok = ((clean[e >> 5] >> (e & 31)) & 1u) != 0;
-                Q = q0;
-                tag = t0;
-                s_cur = g0 + 1;
+                s_cur = g0;
                 s_end = g1;
-                if (s_cur < s_end) next = sq[s_cur];
                 has = true;
                 item = true;
                 fresh_r = true;
@@ -570,15 +607,13 @@ struct KmerSource {
 // every hit restores from A0, LA0. Left-stream words are byte-swapped at
 // preparation.
 template <int SW, class Src>
-__device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restrict__ rplane, uint32_t ref_len,
-                                               const uint32_t *__restrict__ qplane,
-                                               const uint32_t *__restrict__ qcoarse, uint32_t qlen,
-                                               const uint8_t *s_map, int32_t xdrop, int32_t thresh) {
+__device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restrict__ rplane,
+                                               const QRec *__restrict__ recs, const uint8_t *s_map, int32_t xdrop,
+                                               int32_t thresh) {
     static_assert(SW == 1 || SW == 2 || SW == 4, "step width must divide the right window");
     constexpr uint32_t SEL[4] = {0x0C0C0400u, 0x0C0C0500u, 0x0C0C0600u, 0x0C0C0700u};
     bool has = false, item = false, rok = false, left = false;
     uint32_t R = 0, Q = 0, tag = 0, s_cur = 0, s_end = 0, grp = 0, limit = 0;
-    uint64_t next = 0;
     const uint32_t copy_off = (lane_id() & 31) * 4;
     uint32_t off = copy_off;
     int32_t c = 0, e = xdrop, Mr = 0;
@@ -597,7 +632,7 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
 
     for (;;) {
         bool fresh_r = false, fresh_q = false;
-        src.refill(has, item, fresh_r, fresh_q, R, Q, tag, s_cur, s_end, next, rok);
+        src.refill(has, item, fresh_r, fresh_q, R, s_cur, s_end, rok);
         if (__ballot(has) == 0) break;
 
         bool defer = false, done = false, pass = false;
@@ -618,27 +653,15 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
                     __builtin_amdgcn_alignbit(LA0[k], k + 1 < (int)G2_LEFT_WORDS ? LA0[k + 1] : lend, sr));
         }
         if (fresh_q) {
-            bool ok = rok && Q >= G2_MARGIN && (uint64_t)Q + G2_MARGIN <= qlen;
-            if (ok) {
-                const uint32_t qlo = Q - 16 * G2_LEFT_WORDS, qhi = Q + 16 * G2_RIGHT_WORDS - 1;
-                const uint32_t *qc = qcoarse + (qlo >> 11);
-                const uint32_t *qp = qplane + (Q >> 4);
-                uint32_t cq0 = qc[0], cq1 = qc[1];
-                uint32_t qend = qp[G2_RIGHT_WORDS], mend = qp[-(int)G2_LEFT_WORDS];
-#pragma unroll
-                for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) B[k] = qp[k];
-#pragma unroll
-                for (int k = 0; k < (int)G2_LEFT_WORDS; k++) LB[k] = qp[-k];
-                ok = !coarse_bits(cq0, cq1, qlo, qhi);
-                uint32_t sq = 2 * (Q & 15);
-#pragma unroll
-                for (int k = 0; k < (int)G2_RIGHT_WORDS; k++)
-                    B[k] = __builtin_amdgcn_alignbit(k + 1 < (int)G2_RIGHT_WORDS ? B[k + 1] : qend, B[k], sq);
-#pragma unroll
-                for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
-                    LB[k] = __builtin_bswap32(
-                        __builtin_amdgcn_alignbit(LB[k], k + 1 < (int)G2_LEFT_WORDS ? LB[k + 1] : mend, sq));
-            }
+            const uint4 *rec = (const uint4 *)(recs + s_cur);
+            s_cur++;
+            uint4 v0 = rec[0], v1 = rec[1], v2 = rec[2], v3 = rec[3];
+            B[0] = v0.x, B[1] = v0.y, B[2] = v0.z, B[3] = v0.w;
+            LB[0] = v1.x, LB[1] = v1.y, LB[2] = v1.z, LB[3] = v1.w;
+            LB[4] = v2.x, LB[5] = v2.y, LB[6] = v2.z, LB[7] = v2.w;
+            Q = v3.x;
+            tag = v3.y;
+            bool ok = rok && v3.z != 0;
             if (!ok) {
                 defer = true;
             } else {
@@ -728,10 +751,9 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
 // The batched gate in k-mer-major order (same launch shape as k_bgate_packed).
 template <int SW, int BT>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_kmer(
-    const uint32_t *__restrict__ rplane, uint32_t ref_len, const uint32_t *__restrict__ qplane,
-    const uint32_t *__restrict__ qcoarse, uint32_t qlen, const uint32_t *__restrict__ pairtab, int32_t xdrop,
-    int32_t thresh, const uint64_t *__restrict__ sq, const uint32_t *__restrict__ skm,
-    const uint32_t *__restrict__ goff, uint32_t nk, const uint64_t *__restrict__ seeds,
+    const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop,
+    int32_t thresh, const QRec *__restrict__ recs, const uint32_t *__restrict__ skm,
+    const uint32_t *__restrict__ goff, uint32_t nk,
     const uint32_t *__restrict__ index, const uint32_t *__restrict__ pos, const uint32_t *__restrict__ clean,
     uint32_t seed_size, uint32_t claim, unsigned long long *__restrict__ work, SurvRec *__restrict__ surv,
     uint32_t *__restrict__ surv_n, uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n,
@@ -739,8 +761,6 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     __shared__ uint32_t s_map[G2_MAP_WORDS];
     load_map_table(s_map, pairtab, BT);
     KmerSource src;
-    src.sq = sq;
-    src.seeds = seeds;
     src.skm = skm;
     src.goff = goff;
     src.index = index;
@@ -758,7 +778,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.cap_slow = cap_slow;
     src.err = err;
     src.init();
-    kmer_gate_loop<SW>(src, rplane, ref_len, qplane, qcoarse, qlen, (const uint8_t *)s_map, xdrop, thresh);
+    kmer_gate_loop<SW>(src, rplane, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
 // Block bounds of a seed's batch: arena offset and length of its block.

@@ -64,10 +64,11 @@ struct hspz_server {
 namespace {
 
 // Device bytes one request needs inside a batched pass: per query base both
-// strands' positions (valid flag 1 + id 4) and up to 13 seeds each
-// (seed 8 + count 4 + prefix 8), the arena and its planes, plus a fixed
-// share of the per-pass survivor lists.
-uint64_t request_footprint(size_t len) { return (uint64_t)len * (2 * (5 + 13 * 20) + 3) + (1ull << 20); }
+// strands' positions (valid flag 1 + id 4) and up to 13 seeds each (seed 8 +
+// count 4 + prefix 8, and for the k-mer-major order the grouped record 8 +
+// k-mer 4 + query record 64), the arena and its planes, plus a fixed share of
+// the per-pass survivor lists.
+uint64_t request_footprint(size_t len) { return (uint64_t)len * (2 * (5 + 13 * 96) + 3) + (1ull << 20); }
 
 void server_worker(hspz_server *s, size_t w) {
     hspz_engine *h = s->engines[w];
