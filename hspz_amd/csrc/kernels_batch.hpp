@@ -439,8 +439,16 @@ __global__ void k_kscatter(const uint64_t *__restrict__ seeds, const uint32_t *_
     }
 }
 
+// Swaps the two nibbles of every byte: applied to byte-swapped left words, it
+// brings a byte's nearer cell pair into the low nibble, so the left stream
+// forms its keys exactly like the right one.
+__device__ __forceinline__ uint32_t nib_swap(uint32_t x) {
+    return ((x >> 4) & 0x0F0F0F0Fu) | ((x << 4) & 0xF0F0F0F0u);
+}
+
 // Query record of one grouped seed: its right and left query streams as the
-// k-mer-major gate consumes them (aligned to Q, left words byte-swapped),
+// k-mer-major gate consumes them (aligned to Q, left words byte- and
+// nibble-swapped),
 // the anchor Q, the seed index and whether the query window is clean.
 struct QRec {
     uint32_t right[G2_RIGHT_WORDS];
@@ -479,7 +487,7 @@ __global__ void k_qrecords(const uint64_t *__restrict__ sq, const uint32_t *__re
         for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) r.right[k] = __builtin_amdgcn_alignbit(W[9 + k], W[8 + k], sq2);
 #pragma unroll
         for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
-            r.left[k] = __builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sq2));
+            r.left[k] = nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sq2)));
         r.ok = ok ? 1u : 0u;
         uint4 *o = (uint4 *)(recs + p);
         const uint4 *s = (const uint4 *)&r;
@@ -604,8 +612,8 @@ struct KmerSource {
 // (A0, LA0) and query words once per hit. At 64-cell steps no stream is
 // shifted (each step picks its words by step index), so the reference window
 // survives for the item's next hit; narrower steps shift working copies that
-// every hit restores from A0, LA0. Left-stream words are byte-swapped at
-// preparation.
+// every hit restores from A0, LA0. Left-stream words are byte- and
+// nibble-swapped at preparation, so both phases form keys the same way.
 template <int SW, class Src>
 __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restrict__ rplane,
                                                const QRec *__restrict__ recs, const uint8_t *s_map, int32_t xdrop,
@@ -649,8 +657,8 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
                 A0[k] = __builtin_amdgcn_alignbit(k + 1 < (int)G2_RIGHT_WORDS ? A0[k + 1] : rend, A0[k], sr);
 #pragma unroll
             for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
-                LA0[k] = __builtin_bswap32(
-                    __builtin_amdgcn_alignbit(LA0[k], k + 1 < (int)G2_LEFT_WORDS ? LA0[k + 1] : lend, sr));
+                LA0[k] = nib_swap(__builtin_bswap32(
+                    __builtin_amdgcn_alignbit(LA0[k], k + 1 < (int)G2_LEFT_WORDS ? LA0[k + 1] : lend, sr)));
         }
         if (fresh_q) {
             const uint4 *rec = (const uint4 *)(recs + s_cur);
@@ -692,14 +700,15 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
                     r = left ? pick_word<SW>(LA0, grp, h) : A0[h];
                     q = left ? pick_word<SW>(LB, grp, h) : B[h];
                 }
+                // key bytes in walk order for both phases (left words are
+                // byte- and nibble-swapped): t1 byte k = cell pair 2k, t2 byte k = pair 2k+1
                 uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
                 uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
-                uint32_t x1 = left ? t2 : t1, x2 = left ? t1 : t2;
                 uint32_t p[8];
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    p[2 * j] = map_entry(s_map, x1, off, SEL[j]);
-                    p[2 * j + 1] = map_entry(s_map, x2, off, SEL[j]);
+                    p[2 * j] = map_entry(s_map, t1, off, SEL[j]);
+                    p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
                 }
 #pragma unroll
                 for (int i = 0; i < 8; i += 2) {
