@@ -171,7 +171,8 @@ def main():
         res_b, st = batched()
         log("warmup %d: %d hits, %d HSPs" % (i, st["hits"], st["hsps"]))
 
-    g0 = {k: eng.stat(k) for k in ("gate_ns", "gate_hits", "gate_launches", "gate_deferred")}
+    g0 = {k: eng.stat(k) for k in ("gate_ns", "gate_hits", "gate_launches", "gate_deferred", "batch_prep_ns",
+                                   "batch_device_ns", "batch_read_ns", "batch_post_ns")}
     t_start = time.perf_counter()
     st_b = None
     for i in range(args.steps):
@@ -275,6 +276,9 @@ def main():
                    "parity_sample_vs_oracle": parity_sample},
         "plain": plain,
         "batched_over_plain": round(value / plain["value"], 3) if plain else None,
+        "batched_ms_per_step_by_part": {k[6:-3]: round(g[k] / 1e6 / args.steps, 2)
+                                        for k in ("batch_prep_ns", "batch_device_ns", "batch_read_ns",
+                                                  "batch_post_ns")},
         "roofline": roof,
         "cpu_baseline": cpu,
         "serving": serving,

@@ -10,6 +10,7 @@
 // per-chunk, per-interval, fw-then-rc assembly of run.rs:278-299 / 443-451.
 #pragma once
 
+#include <chrono>
 #include <thread>
 
 namespace {
@@ -81,6 +82,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                std::vector<hspz_segment_pair> &hsps, std::vector<uint32_t> &scopes,
                hspz_pass_stats *st) {
     BatchBufs &B = h->batch->b;
+    const auto t_begin = std::chrono::steady_clock::now();
     // ---- layout: blocks (chunk c fwd = 2c, rc = 2c + 1), each after QPAD code-7 bytes
     std::vector<uint32_t> blk_off(2 * n_chunks), blk_len(2 * n_chunks);
     uint64_t off = QPAD;
@@ -166,6 +168,14 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
     memcpy(sa.pos, s.pos, s.kmer_size * 4);
     sa.kmer_size = s.kmer_size;
 
+    {
+        // start the survivor and deferred-hit lists at a size the batch is
+        // likely to need (~4% and ~20% of its seed capacity against a 1 Gbp
+        // target), so that large batches rarely pay an overflow rerun
+        hspz_batch_state *bs = h->batch;
+        bs->cap_surv = (uint32_t)std::max<uint64_t>(bs->cap_surv, std::min<uint64_t>(cap_seeds / 16, 1u << 28));
+        bs->cap_slow = (uint32_t)std::max<uint64_t>(bs->cap_slow, std::min<uint64_t>(cap_seeds / 4, 1u << 28));
+    }
     for (;;) {  // one attempt per survivor/slow capacity
         hspz_batch_state *bs = h->batch;
         if ((rc = reserve(h, B.surv, (size_t)bs->cap_surv * sizeof(SurvRec)))) return rc;
@@ -302,6 +312,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                            B.mscope.as<uint32_t>(), B.kids.as<uint32_t>(), ctr + CTR_KEPT, B.ohsp.as<SegmentPair>(),
                            B.oscope.as<uint32_t>());
         };
+        const auto t_launch = std::chrono::steady_clock::now();
         if (h->batch_graph) {
             GraphKey key;
             memset(&key, 0, sizeof key);
@@ -321,6 +332,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             memcpy(key.shape, s.pos, sizeof s.pos);
             key.shape[32] = s.size;
             if (!bs->have_graph || memcmp(&key, &bs->key, sizeof key) != 0) {
+                std::lock_guard<std::mutex> lk(capture_mutex());
                 for (int g = 0; g < 3; g++) {
                     if (bs->exec[g]) (void)hipGraphExecDestroy(bs->exec[g]);
                     if (bs->graph[g]) (void)hipGraphDestroy(bs->graph[g]);
@@ -363,6 +375,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         uint32_t hc[CTR_N];
         HIP_TRY(hipMemcpyAsync(h->h_small, ctr, CTR_N * 4, hipMemcpyDeviceToHost, h->stream));
         HIP_TRY(hipStreamSynchronize(h->stream));
+        const auto t_synced = std::chrono::steady_clock::now();
         h->pipeline_syncs++;
         memcpy(hc, h->h_small, sizeof hc);
         float ms = 0, post_ms = 0;
@@ -387,16 +400,24 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         if (hc[CTR_ERR] & 2u) return set_error(HSPZ_E_CHUNK, "a batch needs more than %u max_hits chunks", MAX_CHUNKS);
         if (hc[CTR_ERR] & 4u) return set_error(HSPZ_E_ARG, "a batch has more than 2^32 hits");
         std::vector<uint64_t> bh(nb);
-        HIP_TRY(hipMemcpy(bh.data(), B.bhits.p, (size_t)nb * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpyAsync(bh.data(), B.bhits.p, (size_t)nb * 8, hipMemcpyDeviceToHost, h->stream));
         uint64_t hits = 0;
         for (uint64_t x : bh) hits += x;
         uint32_t nk = hc[CTR_KEPT];
         hsps.resize(nk);
         scopes.resize(nk);
         if (nk) {
-            HIP_TRY(hipMemcpy(hsps.data(), B.ohsp.p, (size_t)nk * 16, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(scopes.data(), B.oscope.p, (size_t)nk * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpyAsync(hsps.data(), B.ohsp.p, (size_t)nk * 16, hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(hipMemcpyAsync(scopes.data(), B.oscope.p, (size_t)nk * 4, hipMemcpyDeviceToHost, h->stream));
         }
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        const auto t_read = std::chrono::steady_clock::now();
+        auto ns = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+            return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(b - a).count();
+        };
+        h->batch_prep_ns += ns(t_begin, t_launch);
+        h->batch_device_ns += ns(t_launch, t_synced);
+        h->batch_read_ns += ns(t_synced, t_read);
         h->gate_ns += (uint64_t)((double)ms * 1e6);
         h->gate_hits += hits;
         h->gate_launches++;
@@ -432,6 +453,7 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
     std::vector<hspz_segment_pair> raw;
     std::vector<uint32_t> scopes;
     if (int rc = batch_pass(h, s, p, chunks, n_chunks, hb, raw, scopes, &st)) return rc;
+    const auto t_post = std::chrono::steady_clock::now();
 
     // group by scope (batch * MAX_CHUNKS + chunk): counting order = output order
     std::vector<uint32_t> order(raw.size());
@@ -491,6 +513,9 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
     for (size_t c = 0; c < n_chunks; c++) out->chunk_intervals[c] = civ[c];
     out->n_hsps = total;
     out->n_chunks = n_chunks;
+    h->batch_post_ns += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                            std::chrono::steady_clock::now() - t_post)
+                            .count();
     if (stats) *stats = st;
     return 0;
 }

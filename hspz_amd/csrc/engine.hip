@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/hspz.h"
@@ -63,6 +64,7 @@ struct hspz_engine {
     // packed planes + coarse exception bitmaps for the fast gate (k_gate_packed)
     DevBuf ref2, refx, qf2, qfx, qr2, qrx, pairtab, slow, slow_n;
     bool fast_gate = false;
+    uint32_t unsafe_codes = 0xFFFFFFFFu;  // perm table: byte k = 0xFF when code 4 + k needs the exact gate
     // seed slots (gpu/mod.rs:933-945)
     DevBuf seeds[2];
     uint32_t seed_len[2] = {0, 0};
@@ -77,6 +79,9 @@ struct hspz_engine {
     uint64_t launches = 0, pipeline_syncs = 0, reference_uploads = 0, query_swaps = 0;
     uint64_t dev_bytes = 0, peak_dev_bytes = 0;
     uint64_t gate_ns = 0, gate_hits = 0, gate_launches = 0, gate_deferred = 0;
+    // wall time of hspz_filter_batch: host layout and uploads, device pass
+    // (launch to drained stream), read-back, host dedup and assembly
+    uint64_t batch_prep_ns = 0, batch_device_ns = 0, batch_read_ns = 0, batch_post_ns = 0;
     int gate_blocks = 2048;
     // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
     int bgate_blocks = 0;  // persistent grid; 0 = as many blocks as are resident at once
@@ -94,8 +99,17 @@ struct hspz_engine {
 
 namespace {
 
+// Serializes hipGraph captures with the calls that must not overlap another
+// thread's capture (allocation and release of device memory): engines of a
+// serving loop capture and grow their buffers from different host threads.
+std::mutex &capture_mutex() {
+    static std::mutex mu;
+    return mu;
+}
+
 int reserve(hspz_engine *h, DevBuf &b, size_t bytes) {
     if (bytes <= b.cap && b.p) return 0;
+    std::lock_guard<std::mutex> lk(capture_mutex());
     size_t want = std::max<size_t>(bytes + bytes / 8, 256);
     if (b.p) {
         HIP_TRY(hipStreamSynchronize(h->stream));
@@ -143,7 +157,7 @@ int build_planes(hspz_engine *h, const DevBuf &seq, uint32_t len, DevBuf &plane,
     hipLaunchKernelGGL(k_pack2, dim3(std::min<uint32_t>(ceil_div(nw, 256), 8192)), dim3(256), 0, h->stream, codes, len,
                        plane.as<uint32_t>(), nw);
     hipLaunchKernelGGL(k_coarse_exceptions, dim3(std::min<uint32_t>(ceil_div(nc, 8), 8192)), dim3(256), 0, h->stream,
-                       codes, len, coarse.as<uint32_t>(), nc);
+                       codes, len, coarse.as<uint32_t>(), nc, h->unsafe_codes);
     h->launches += 2;
     HIP_TRY(hipGetLastError());
     return 0;
@@ -420,6 +434,14 @@ extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_e
         if (ok) {
             uint32_t tab[512];
             const int32_t X = cfg->xdrop;
+            // non-ACGT codes that can score above -(X+1) against some code
+            h->unsafe_codes = 0;
+            for (int c = 4; c < 8; c++) {
+                bool unsafe = false;
+                for (int j = 0; j < 8; j++)
+                    unsafe = unsafe || cfg->sub_mat[c * 8 + j] > -(X + 1) || cfg->sub_mat[j * 8 + c] > -(X + 1);
+                if (unsafe) h->unsafe_codes |= 0xFFu << (8 * (c - 4));
+            }
             for (uint32_t key = 0; key < 256; key++) {
                 uint32_t ra = key & 3, rb = (key >> 2) & 3, qa = (key >> 4) & 3, qb = (key >> 6) & 3;
                 int32_t lo = std::max(cfg->sub_mat[ra * 8 + qa], -(X + 1));
@@ -641,6 +663,10 @@ extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
     case HSPZ_STAT_GATE_DEFERRED: *value = h->gate_deferred; break;
     case HSPZ_STAT_GRAPH_CAPTURES: *value = h->batch ? h->batch->graph_captures : 0; break;
     case HSPZ_STAT_GRAPH_REPLAYS: *value = h->batch ? h->batch->graph_replays : 0; break;
+    case HSPZ_STAT_BATCH_PREP_NS: *value = h->batch_prep_ns; break;
+    case HSPZ_STAT_BATCH_DEVICE_NS: *value = h->batch_device_ns; break;
+    case HSPZ_STAT_BATCH_READ_NS: *value = h->batch_read_ns; break;
+    case HSPZ_STAT_BATCH_POST_NS: *value = h->batch_post_ns; break;
     default: return set_error(HSPZ_E_ARG, "unknown stat %d", which);
     }
     return 0;

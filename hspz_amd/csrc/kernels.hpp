@@ -382,7 +382,16 @@ __global__ __launch_bounds__(256) void k_gate(const uint8_t *__restrict__ ref, u
 
 // Packed sequence planes for the fast gate: 16 bases per u32 word, 2 bits each
 // (base i of word w at bits 2i..2i+1; non-ACGT codes store 0), and a coarse
-// exception bitmap with one bit per 64-base block that holds any code >= 4.
+// exception bitmap with one bit per 64-base block that holds an unsafe code.
+//
+// The fast gate only has to be an upper bound of the exact gate. A non-ACGT
+// code whose scores against every code are at most -(xdrop + 1) forces an
+// X-drop wherever it occurs, and the packed table never scores a cell below
+// -(xdrop + 1); evaluating such a cell as whatever its 2-bit stand-in scores
+// can only raise prefix sums and delay the drop, so the maximum stays at least
+// the exact one. Only the other non-ACGT codes (IUPAC X at FILL, N under
+// --ambiguous) need the exact gate; the separator, soft-masked and N codes of
+// the default matrix do not.
 __global__ void k_pack2(const uint8_t *__restrict__ codes, uint32_t len,
                         uint32_t *__restrict__ plane, uint32_t nwords) {
     for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x) {
@@ -397,10 +406,12 @@ __global__ void k_pack2(const uint8_t *__restrict__ codes, uint32_t len,
 }
 
 // One lane per 64-base block (16 dword loads), wave ballot into two bitmap
-// words. Blocks at or past the end of the sequence count as exceptions; the
-// code-7 padding after the sequence covers a final partial block.
+// words. `unsafe` holds 0xFF in byte k when code 4 + k is unsafe, so one
+// v_perm_b32 per word looks up its four codes. Blocks at or past the end of
+// the sequence count as exceptions.
 __global__ __launch_bounds__(256) void k_coarse_exceptions(const uint8_t *__restrict__ codes, uint32_t len,
-                                                           uint32_t *__restrict__ bits, uint32_t nwords) {
+                                                           uint32_t *__restrict__ bits, uint32_t nwords,
+                                                           uint32_t unsafe) {
     const uint32_t lane = lane_id();
     const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
     for (uint32_t w2 = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; 2 * w2 < nwords; w2 += nwaves) {
@@ -411,8 +422,8 @@ __global__ __launch_bounds__(256) void k_coarse_exceptions(const uint8_t *__rest
             const uint32_t *src = (const uint32_t *)(codes + p0);
             uint32_t acc = 0;
 #pragma unroll
-            for (int i = 0; i < 16; i++) acc |= src[i];
-            ex = (acc & 0xFCFCFCFCu) != 0;
+            for (int i = 0; i < 16; i++) acc |= __builtin_amdgcn_perm(unsafe, 0u, src[i] & 0x07070707u);
+            ex = acc != 0;
         }
         uint64_t m = __ballot(ex);
         if (lane == 0) {

@@ -327,13 +327,8 @@ struct SeedSource {
         if (lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
         base = __shfl(base, leader, WAVE);
         uint32_t at = base + mbcnt(m);
-        if (want) {
-            if (at < cap) {
-                *(uint4 *)(list + at) = make_uint4(R, Q, tag, 0u);
-            } else {
-                atomicOr(err, 8u);
-            }
-        }
+        if (want && at < cap) *(uint4 *)(list + at) = make_uint4(R, Q, tag, 0u);
+        if (lane_id() == leader && (uint64_t)base + (uint64_t)__popcll(m) > cap) atomicOr(err, 8u);
     }
     __device__ __forceinline__ void emit(bool done, bool pass, uint32_t tag, uint32_t R, uint32_t Q) {
         append(done && pass, surv, surv_n, cap_surv, err, R, Q, tag);

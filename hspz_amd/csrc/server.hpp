@@ -30,6 +30,7 @@ struct ServerReq {
 
 struct ServerDone {
     int status = 0;
+    std::string msg;  // the worker's error text when status != 0
     std::vector<hspz_segment_pair> hsps;
     std::vector<size_t> counts;
     double t_submit = 0, t_start = 0, t_done = 0;
@@ -116,6 +117,7 @@ void server_worker(hspz_server *s, size_t w) {
         hspz_batch_output out;
         hspz_pass_stats st;
         int rc = hspz_filter_batch(h, &s->params, chunks.data(), chunks.size(), &out, &st);
+        std::string msg = rc ? std::string(hspz_last_error()) : std::string();
         double t_done = s->now();
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -123,6 +125,7 @@ void server_worker(hspz_server *s, size_t w) {
             for (size_t i = 0; i < batch.size(); i++) {
                 ServerDone d;
                 d.status = rc;
+                d.msg = msg;
                 d.t_submit = batch[i].t_submit;
                 d.t_start = t_start;
                 d.t_done = t_done;
@@ -232,7 +235,8 @@ extern "C" int hspz_server_wait(hspz_server *s, uint64_t id, hspz_request_result
     if (!out->hsps || !out->counts) return set_error(HSPZ_E_OOM, "host output allocation");
     memcpy(out->hsps, d.hsps.data(), d.hsps.size() * 16);
     memcpy(out->counts, d.counts.data(), d.counts.size() * sizeof(size_t));
-    return d.status;
+    if (d.status) return set_error(d.status, "request %llu failed in its batch: %s", (unsigned long long)id, d.msg.c_str());
+    return 0;
 }
 
 extern "C" void hspz_request_result_free(hspz_request_result *r) {

@@ -22,7 +22,8 @@ SP_DTYPE = np.dtype([("ref_start", "<u4"), ("query_start", "<u4"), ("len", "<u4"
 
 STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 3,
         "query_swaps": 4, "peak_device_bytes": 5, "gate_ns": 6, "gate_hits": 7,
-        "gate_launches": 8, "gate_deferred": 9, "graph_captures": 10, "graph_replays": 11}
+        "gate_launches": 8, "gate_deferred": 9, "graph_captures": 10, "graph_replays": 11,
+        "batch_prep_ns": 12, "batch_device_ns": 13, "batch_read_ns": 14, "batch_post_ns": 15}
 FLAG = {"batch_graph": 0, "batch_gate_blocks": 1, "batch_step_cells": 2, "batch_block_threads": 3,
         "batch_claim_seeds": 4, "batch_order": 5, "batch_order_used": 6}
 

@@ -112,6 +112,11 @@ void hspz_filter_output_free(hspz_filter_output *out);
 #define HSPZ_STAT_GATE_DEFERRED 9  /* hits the packed gate handed to the exact wave gate */
 #define HSPZ_STAT_GRAPH_CAPTURES 10 /* batched device passes captured into a hipGraph */
 #define HSPZ_STAT_GRAPH_REPLAYS 11  /* batched device passes replayed from the cached graph */
+/* Cumulative wall time of hspz_filter_batch, in ns, by part: */
+#define HSPZ_STAT_BATCH_PREP_NS 12   /* host layout of the arena and batches, uploads */
+#define HSPZ_STAT_BATCH_DEVICE_NS 13 /* device pass, launch to drained stream */
+#define HSPZ_STAT_BATCH_READ_NS 14   /* read-back of counters and kept HSPs */
+#define HSPZ_STAT_BATCH_POST_NS 15   /* host dedup_and_order per scope and assembly */
 int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 
 /* Engine switches (the reference's pub flags, gpu/mod.rs:836-949, have no

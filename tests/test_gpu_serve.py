@@ -79,3 +79,36 @@ def test_admission_refuses_request_over_budget(gpu_available):
         assert ei.value.status == 7
         assert srv.stats()["rejected"] == 1
     eng.close()
+
+
+def test_served_growing_batches_equal_alone(gpu_available):
+    print("case serve: growing batches (both hit orders, buffer growth during captures)", flush=True)
+    target, sub, index, pos = _setup(600_000, seed=0x5E12)
+    t = np.frombuffer(target, dtype=np.uint8)
+    pieces = synth.sample_chunks(t, 9, 24, 150_000, 0.8, 20_000, 500_000, 0.02, 0.2)
+    reqs = []
+    for i, p in enumerate(pieces):
+        buf, chrs, bl = S.pack([("g%d" % i, p)])
+        rc, _ = S.reverse_complement(buf, chrs, bl)
+        reqs.append((S.encode(buf[:bl]), S.encode(rc)))
+    ref_enc = S.encode(target)
+    e1 = E.Engine(index, pos, ref_enc, sub, max_hits=16711680)
+    e2 = E.Engine(index, pos, ref_enc, sub, max_hits=16711680)
+    alone = E.Engine(index, pos, ref_enc, sub, max_hits=16711680)
+    want = []
+    for f, r in reqs:
+        alone.swap_query(f, r)
+        want.append(alone.run_pass()[0])
+    with E.Server([e1, e2], max_batch_requests=12) as srv:
+        ids = []
+        for i, (f, r) in enumerate(sorted(reqs, key=lambda x: len(x[0]))):
+            ids.append((i, srv.submit(f, r)))
+        order = sorted(range(len(reqs)), key=lambda k: len(reqs[k][0]))
+        got = {order[i]: srv.wait(rid)[0] for i, rid in ids}
+        st = srv.stats()
+    for k, w in enumerate(want):
+        assert same_intervals(got[k], w), "request %d differs from alone" % k
+    assert st["failed_batches"] == 0
+    assert max(e.get_flag("batch_order_used") for e in (e1, e2)) == 1 or st["max_batch_requests"] > 1
+    for e in (e1, e2, alone):
+        e.close()
