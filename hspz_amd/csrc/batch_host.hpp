@@ -13,6 +13,8 @@
 #include <chrono>
 #include <thread>
 
+#include "dedup_dev.hpp"
+
 namespace {
 
 struct HostBatch {
@@ -73,14 +75,81 @@ struct hspz_batch_state {
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};       // before, the gate, after
     hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
     uint64_t graph_captures = 0, graph_replays = 0;
+    // device dedup_and_order scratch
+    DevBuf dd_key_a, dd_key_b, dd_id_a, dd_id_b, dd_keep, dd_kids, dd_tmp;
 };
 
 namespace {
 
+// Stable radix sort of (key, u32 value) pairs on the engine's stream.
+template <class K>
+int dd_sort(hspz_engine *h, DevBuf &tmp, const K *ki, K *ko, const uint32_t *vi, uint32_t *vo, uint32_t n) {
+    size_t bytes = 0;
+    HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, ki, ko, vi, vo, n, 0, 8 * sizeof(K), h->stream));
+    if (int rc = reserve(h, tmp, bytes + 256)) return rc;
+    HIP_TRY(rocprim::radix_sort_pairs(tmp.p, bytes, ki, ko, vi, vo, n, 0, 8 * sizeof(K), h->stream));
+    return 0;
+}
+
+// dedup_and_order of every scope of the pass's kept list (ohsp/oscope, n
+// records) on the device; the result, grouped by ascending scope and in the
+// reference's output order within each scope, lands in mhsp/mscope and its
+// length in *kept.
+int device_dedup(hspz_engine *h, BatchBufs &B, uint32_t n, uint32_t *kept) {
+    hspz_batch_state *bs = h->batch;
+    int rc;
+    if ((rc = reserve(h, bs->dd_key_a, (size_t)n * 8 + 8)) || (rc = reserve(h, bs->dd_key_b, (size_t)n * 8 + 8)) ||
+        (rc = reserve(h, bs->dd_id_a, (size_t)n * 4 + 4)) || (rc = reserve(h, bs->dd_id_b, (size_t)n * 4 + 4)) ||
+        (rc = reserve(h, bs->dd_keep, (size_t)n + 16)) || (rc = reserve(h, bs->dd_kids, (size_t)n * 4 + 4)))
+        return rc;
+    uint32_t nt = std::max<uint32_t>(ceil_div(n, TILE), 1);
+    if ((rc = reserve(h, h->tiles, (size_t)nt * 4))) return rc;
+    const SegmentPair *hsp = B.ohsp.as<SegmentPair>();
+    const uint32_t *scope = B.oscope.as<uint32_t>();
+    uint64_t *ka = bs->dd_key_a.as<uint64_t>(), *kb = bs->dd_key_b.as<uint64_t>();
+    uint32_t *ia = bs->dd_id_a.as<uint32_t>(), *ib = bs->dd_id_b.as<uint32_t>();
+    uint32_t *k32a = (uint32_t *)ka, *k32b = (uint32_t *)kb;
+    auto grid = [](uint32_t m) { return dim3(std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(m, 256), 4096))); };
+    // order 1 over all records: (scope, diagonal, ref_start, len, -score)
+    hipLaunchKernelGGL(k_dd_key_len_score, grid(n), dim3(256), 0, h->stream, hsp, nullptr, n, ka, ia);
+    if ((rc = dd_sort<uint64_t>(h, bs->dd_tmp, ka, kb, ia, ib, n))) return rc;
+    hipLaunchKernelGGL(k_dd_key_ref, grid(n), dim3(256), 0, h->stream, hsp, ib, n, k32a);
+    if ((rc = dd_sort<uint32_t>(h, bs->dd_tmp, k32a, k32b, ib, ia, n))) return rc;
+    hipLaunchKernelGGL(k_dd_key_scope, grid(n), dim3(256), 0, h->stream, hsp, scope, ia, n, 1, ka);
+    if ((rc = dd_sort<uint64_t>(h, bs->dd_tmp, ka, kb, ia, ib, n))) return rc;
+    // containment against the previous record, compaction
+    hipLaunchKernelGGL(k_dd_mark, grid(n), dim3(256), 0, h->stream, hsp, scope, ib, n, bs->dd_keep.as<uint8_t>());
+    hipLaunchKernelGGL(k_tile_count_u8, dim3(nt), dim3(TILE_THREADS), 0, h->stream, bs->dd_keep.as<uint8_t>(), n,
+                       h->tiles.as<uint32_t>());
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, h->stream, h->tiles.as<uint32_t>(), nt,
+                       h->total.as<uint32_t>());
+    hipLaunchKernelGGL(k_tile_emit_u8, dim3(nt), dim3(TILE_THREADS), 0, h->stream, bs->dd_keep.as<uint8_t>(), n,
+                       h->tiles.as<uint32_t>(), bs->dd_kids.as<uint32_t>());
+    HIP_TRY(hipGetLastError());
+    uint32_t k = 0;
+    if ((rc = read_total(h, &k))) return rc;
+    if (k > n) return set_error(HSPZ_E_ARG, "device dedup kept %u of %u records", k, n);
+    hipLaunchKernelGGL(k_dd_compose, grid(k), dim3(256), 0, h->stream, ib, bs->dd_kids.as<uint32_t>(), k, ia);
+    // order 2 over the kept records: (scope, query_start, ref_start, len, -score)
+    hipLaunchKernelGGL(k_dd_key_len_score, grid(k), dim3(256), 0, h->stream, hsp, ia, k, ka, ib);
+    if ((rc = dd_sort<uint64_t>(h, bs->dd_tmp, ka, kb, ib, ia, k))) return rc;
+    hipLaunchKernelGGL(k_dd_key_ref, grid(k), dim3(256), 0, h->stream, hsp, ia, k, k32a);
+    if ((rc = dd_sort<uint32_t>(h, bs->dd_tmp, k32a, k32b, ia, ib, k))) return rc;
+    hipLaunchKernelGGL(k_dd_key_scope, grid(k), dim3(256), 0, h->stream, hsp, scope, ib, k, 0, ka);
+    if ((rc = dd_sort<uint64_t>(h, bs->dd_tmp, ka, kb, ib, ia, k))) return rc;
+    hipLaunchKernelGGL(k_dd_gather, grid(k), dim3(256), 0, h->stream, hsp, scope, ia, k, B.mhsp.as<SegmentPair>(),
+                       B.mscope.as<uint32_t>());
+    HIP_TRY(hipGetLastError());
+    h->launches += 17;
+    *kept = k;
+    return 0;
+}
+
 int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                const hspz_query_chunk *chunks, size_t n_chunks, std::vector<HostBatch> &hb,
                std::vector<hspz_segment_pair> &hsps, std::vector<uint32_t> &scopes,
-               hspz_pass_stats *st) {
+               hspz_pass_stats *st, bool *deduped) {
+    *deduped = false;
     BatchBufs &B = h->batch->b;
     const auto t_begin = std::chrono::steady_clock::now();
     // ---- layout: blocks (chunk c fwd = 2c, rc = 2c + 1), each after QPAD code-7 bytes
@@ -403,12 +472,19 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         HIP_TRY(hipMemcpyAsync(bh.data(), B.bhits.p, (size_t)nb * 8, hipMemcpyDeviceToHost, h->stream));
         uint64_t hits = 0;
         for (uint64_t x : bh) hits += x;
-        uint32_t nk = hc[CTR_KEPT];
-        hsps.resize(nk);
-        scopes.resize(nk);
-        if (nk) {
-            HIP_TRY(hipMemcpyAsync(hsps.data(), B.ohsp.p, (size_t)nk * 16, hipMemcpyDeviceToHost, h->stream));
-            HIP_TRY(hipMemcpyAsync(scopes.data(), B.oscope.p, (size_t)nk * 4, hipMemcpyDeviceToHost, h->stream));
+        uint32_t nk = hc[CTR_KEPT], nout = nk;
+        const void *src_hsp = B.ohsp.p, *src_scope = B.oscope.p;
+        if (nk && h->device_dedup) {
+            if ((rc = device_dedup(h, B, nk, &nout))) return rc;
+            src_hsp = B.mhsp.p;
+            src_scope = B.mscope.p;
+            *deduped = true;
+        }
+        hsps.resize(nout);
+        scopes.resize(nout);
+        if (nout) {
+            HIP_TRY(hipMemcpyAsync(hsps.data(), src_hsp, (size_t)nout * 16, hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(hipMemcpyAsync(scopes.data(), src_scope, (size_t)nout * 4, hipMemcpyDeviceToHost, h->stream));
         }
         HIP_TRY(hipStreamSynchronize(h->stream));
         const auto t_read = std::chrono::steady_clock::now();
@@ -452,23 +528,36 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
     std::vector<HostBatch> hb;
     std::vector<hspz_segment_pair> raw;
     std::vector<uint32_t> scopes;
-    if (int rc = batch_pass(h, s, p, chunks, n_chunks, hb, raw, scopes, &st)) return rc;
+    bool deduped = false;
+    if (int rc = batch_pass(h, s, p, chunks, n_chunks, hb, raw, scopes, &st, &deduped)) return rc;
     const auto t_post = std::chrono::steady_clock::now();
 
-    // group by scope (batch * MAX_CHUNKS + chunk): counting order = output order
+    // group by scope (batch * MAX_CHUNKS + chunk): ascending scope = output order
     std::vector<uint32_t> order(raw.size());
-    for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
-    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return scopes[a] < scopes[b]; });
-    std::vector<hspz_segment_pair> grouped(raw.size());
-    for (size_t i = 0; i < order.size(); i++) grouped[i] = raw[order[i]];
+    std::vector<hspz_segment_pair> grouped;
+    if (deduped) {  // already grouped by ascending scope and ordered within each
+        for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+        grouped.swap(raw);
+    } else {  // counting sort by scope, stable
+        uint32_t smax = 0;
+        for (uint32_t sc : scopes) smax = std::max(smax, sc);
+        std::vector<uint32_t> cnt(raw.empty() ? 1 : (size_t)smax + 2, 0);
+        for (uint32_t sc : scopes) cnt[sc + 1]++;
+        for (size_t i = 1; i < cnt.size(); i++) cnt[i] += cnt[i - 1];
+        for (uint32_t i = 0; i < order.size(); i++) order[cnt[scopes[i]]++] = i;
+        grouped.resize(raw.size());
+        for (size_t i = 0; i < order.size(); i++) grouped[i] = raw[order[i]];
+    }
     std::vector<size_t> gstart;
     for (size_t i = 0; i < order.size(); i++)
         if (i == 0 || scopes[order[i]] != scopes[order[i - 1]]) gstart.push_back(i);
     gstart.push_back(order.size());
     size_t ng = gstart.size() - 1;
     std::vector<size_t> kept(ng);
-    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    {
+    if (deduped) {
+        for (size_t g = 0; g < ng; g++) kept[g] = gstart[g + 1] - gstart[g];
+    } else {
+        unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         std::vector<std::thread> th;
         for (unsigned t = 0; t < nth; t++)
             th.emplace_back([&, t] {

@@ -95,6 +95,7 @@ struct hspz_engine {
     uint64_t last_surv = 0, last_deferred = 0;   // its survivor and deferred-hit counts
     hspz_batch_state *batch = nullptr;
     bool batch_graph = true;  // capture the batched device pass in a hipGraph and replay it
+    bool device_dedup = true; // dedup_and_order of the batched path on the device
 };
 
 namespace {
@@ -483,8 +484,13 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
         BatchBufs &B = h->batch->b;
         DevBuf *bb[] = {&B.arena, &B.plane, &B.coarse, &B.desc, &B.poff, &B.blk_off, &B.blk_len, &B.valid, &B.ids,
                         &B.voff, &B.seeds, &B.counts, &B.prefix, &B.tiles, &B.nchunks, &B.starts, &B.bhits, &B.surv,
-                        &B.slow, &B.mhsp, &B.mscope, &B.keep, &B.kids, &B.ohsp, &B.oscope, &B.ctr};
+                        &B.slow, &B.mhsp, &B.mscope, &B.keep, &B.kids, &B.ohsp, &B.oscope, &B.ctr, &B.gcnt,
+                        &B.sq, &B.skm, &B.qrec};
         for (DevBuf *b : bb) release(*b);
+        hspz_batch_state *bs = h->batch;
+        DevBuf *dd[] = {&bs->dd_key_a, &bs->dd_key_b, &bs->dd_id_a, &bs->dd_id_b, &bs->dd_keep, &bs->dd_kids,
+                        &bs->dd_tmp};
+        for (DevBuf *b : dd) release(*b);
         for (int g = 0; g < 3; g++) {
             if (h->batch->exec[g]) (void)hipGraphExecDestroy(h->batch->exec[g]);
             if (h->batch->graph[g]) (void)hipGraphDestroy(h->batch->graph[g]);
@@ -626,6 +632,7 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
         if (value < 0 || value > 2) return set_error(HSPZ_E_ARG, "batch order must be 0, 1 or 2");
         h->batch_order = value;
         break;
+    case HSPZ_FLAG_BATCH_DEVICE_DEDUP: h->device_dedup = value != 0; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;
@@ -642,6 +649,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_CLAIM_SEEDS: *value = h->bgate_claim; break;
     case HSPZ_FLAG_BATCH_ORDER: *value = h->batch_order; break;
     case HSPZ_FLAG_BATCH_ORDER_USED: *value = h->batch_order_used; break;
+    case HSPZ_FLAG_BATCH_DEVICE_DEDUP: *value = h->device_dedup ? 1 : 0; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;

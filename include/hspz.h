@@ -133,6 +133,8 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           2 (default) k-mer-major when the batch holds at least one seed
                                           per two k-mers */
 #define HSPZ_FLAG_BATCH_ORDER_USED 6   /* read-only: the order of the last batched pass */
+#define HSPZ_FLAG_BATCH_DEVICE_DEDUP 7 /* 1 (default): dedup_and_order of the batched path on the device
+                                          (stable radix sorts of all scopes at once); 0: on host threads */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

@@ -42,6 +42,11 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
             for i, (a, g) in enumerate(zip(again, got)):
                 assert same_intervals(a, g), "chunk %d differs in hit order %d" % (i, order)
         eng.set_flag("batch_order", 2)
+        eng.set_flag("batch_device_dedup", 0)  # host dedup_and_order gives the same lists
+        again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
+        for i, (a, g) in enumerate(zip(again, got)):
+            assert same_intervals(a, g), "chunk %d differs between device and host dedup" % i
+        eng.set_flag("batch_device_dedup", 1)
         plain = []
         for f, r in chunks:
             eng.swap_query(f, r)
