@@ -56,7 +56,8 @@ struct hspz_engine {
     // reference side (resident)
     DevBuf ref, index, pos, sub;
     uint32_t ref_len = 0;
-    size_t index_len = 0;
+    size_t index_len = 0, pos_len = 0;
+    uint64_t index_build_ns = 0;  // device index build (hspz_engine_new_device_index)
     int32_t sub_host[64];
     // query side
     DevBuf qf, qr;
@@ -343,6 +344,74 @@ int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_seg
 #include "batch_host.hpp"
 #include "server.hpp"
 #include "tune_host.hpp"
+#include "index_dev.hpp"
+
+namespace {
+
+// SeedTable::build on the device (index_dev.hpp): keys, one stable radix sort
+// over the k-mer bits, bucket bounds, positions. Temporaries are released.
+int device_index_build(hspz_engine *h, const Shape &s, uint32_t step) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t st = step, len = h->ref_len;
+    const uint64_t offset = (s.size + 1) % st, start = st - offset;
+    const uint64_t nsteps = (len + offset) > s.size ? (len + offset - s.size) / st : 0;
+    const uint32_t nk = 1u << (2 * s.kmer_size);
+    if (nsteps >= 0xFFFFFFFFull) return set_error(HSPZ_E_ARG, "more than 2^32 indexed positions");
+    int rc;
+    if ((rc = reserve(h, h->index, (size_t)nk * 4))) return rc;
+    h->index_len = nk;
+    DevBuf keys, sorted, tmp;
+    auto done = [&](int code) {
+        (void)hipStreamSynchronize(h->stream);
+        release(keys);
+        release(sorted);
+        release(tmp);
+        return code;
+    };
+    hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, h->stream, h->index.as<uint32_t>(), nk, 0u);
+    uint32_t nvalid = 0;
+    if (nsteps) {
+        if ((rc = reserve(h, keys, nsteps * 8)) || (rc = reserve(h, sorted, nsteps * 8))) return done(rc);
+        ShapeArg sa;
+        memset(&sa, 0, sizeof sa);
+        memcpy(sa.pos, s.pos, s.kmer_size * 4);
+        sa.kmer_size = s.kmer_size;
+        const dim3 grid((uint32_t)std::min<uint64_t>((nsteps + 255) / 256, 65536));
+        hipLaunchKernelGGL(k_index_keys, grid, dim3(256), 0, h->stream, h->ref.as<uint8_t>() + PAD, start, st,
+                           nsteps, sa, s.size, nk, keys.as<uint64_t>());
+        const unsigned end_bit = 32 + 2 * s.kmer_size + 1;
+        size_t bytes = 0;
+        hipError_t e = rocprim::radix_sort_keys(nullptr, bytes, keys.as<uint64_t>(), sorted.as<uint64_t>(),
+                                                (size_t)nsteps, 32, end_bit, h->stream);
+        if (e != hipSuccess) return done(set_error(-(int)e, "index sort: %s", hipGetErrorString(e)));
+        if ((rc = reserve(h, tmp, bytes + 256))) return done(rc);
+        e = rocprim::radix_sort_keys(tmp.p, bytes, keys.as<uint64_t>(), sorted.as<uint64_t>(), (size_t)nsteps, 32,
+                                     end_bit, h->stream);
+        if (e != hipSuccess) return done(set_error(-(int)e, "index sort: %s", hipGetErrorString(e)));
+        hipLaunchKernelGGL(k_index_bounds, grid, dim3(256), 0, h->stream, sorted.as<uint64_t>(), nsteps, nk,
+                           h->index.as<uint32_t>());
+        if (hipGetLastError() != hipSuccess) return done(set_error(-1, "index bounds kernel failed"));
+        HIP_TRY(hipMemcpyAsync(h->h_small, h->index.as<uint32_t>() + (nk - 1), 4, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        nvalid = h->h_small[0];
+    }
+    if ((rc = reserve(h, h->pos, std::max<size_t>(nvalid, 1) * 4))) return done(rc);
+    if (nvalid) {
+        const dim3 grid((uint32_t)std::min<uint64_t>((nvalid + 255) / 256, 65536));
+        hipLaunchKernelGGL(k_index_pos, grid, dim3(256), 0, h->stream, sorted.as<uint64_t>(), (uint64_t)nvalid,
+                           h->pos.as<uint32_t>());
+        if (hipGetLastError() != hipSuccess) return done(set_error(-1, "index position kernel failed"));
+    }
+    h->pos_len = nvalid;
+    h->launches += 5;
+    rc = done(0);
+    h->index_build_ns += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                             std::chrono::steady_clock::now() - t0)
+                             .count();
+    return rc;
+}
+
+}  // namespace
 
 // ------------------------------------------------------------------ C ABI
 
@@ -367,21 +436,32 @@ extern "C" int hspz_device_memory(int device, uint64_t *free_bytes, uint64_t *to
     return 0;
 }
 
-extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_engine **out) {
+namespace {
+int device_index_build(hspz_engine *h, const Shape &s, uint32_t step);
+
+// Engine::new (gpu/mod.rs:1024-1128). With `build` set, the index is built
+// on the device from the reference (cfg's index arrays are ignored).
+int engine_create(int device, const hspz_engine_config *cfg, const Shape *build, uint32_t build_step,
+                  hspz_engine **out) {
     clear_error();
     if (!cfg || !out) return set_error(HSPZ_E_ARG, "null config or output");
     *out = nullptr;
     int ndev = hspz_device_count();
     if (ndev <= 0) return set_error(HSPZ_E_NO_DEVICE, "no HIP device visible");
     if (device < 0 || device >= ndev) return set_error(HSPZ_E_ARG, "device %d out of range", device);
-    if (!cfg->index_table || cfg->index_len == 0 || (cfg->index_len & (cfg->index_len - 1)))
-        return set_error(HSPZ_E_ARG, "index_table must hold 4^k entries");
+    if (!build) {
+        if (!cfg->index_table || cfg->index_len == 0 || (cfg->index_len & (cfg->index_len - 1)))
+            return set_error(HSPZ_E_ARG, "index_table must hold 4^k entries");
+        if (cfg->pos_len && !cfg->pos_table) return set_error(HSPZ_E_ARG, "null pos_table");
+        if (cfg->pos_len != cfg->index_table[cfg->index_len - 1])
+            return set_error(HSPZ_E_ARG, "pos_table length %zu != last index entry %u", cfg->pos_len,
+                             cfg->index_table[cfg->index_len - 1]);
+    } else if (build->size != cfg->seed_size) {
+        return set_error(HSPZ_E_SHAPE, "seed shape span %u != seed_size %u", build->size, cfg->seed_size);
+    }
     if (!cfg->sub_mat) return set_error(HSPZ_E_ARG, "sub_mat is required (64 entries)");
     if (cfg->ref_len >= 0xFFFFFFFFull - 2 * PAD) return set_error(HSPZ_E_ARG, "reference longer than u32");
-    if (cfg->pos_len && !cfg->pos_table) return set_error(HSPZ_E_ARG, "null pos_table");
-    if (cfg->pos_len != cfg->index_table[cfg->index_len - 1])
-        return set_error(HSPZ_E_ARG, "pos_table length %zu != last index entry %u", cfg->pos_len,
-                         cfg->index_table[cfg->index_len - 1]);
+    if (cfg->ref_len && !cfg->ref_seq) return set_error(HSPZ_E_ARG, "null ref_seq");
     HIP_TRY(hipSetDevice(device));
     hspz_engine *h = new hspz_engine();
     h->device = device;
@@ -411,18 +491,25 @@ extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_e
         h->max_hits = hspz_default_max_hits(t);
     }
     int rc;
-    if ((rc = reserve(h, h->index, cfg->index_len * 4))) return fail(rc);
-    if ((rc = reserve(h, h->pos, std::max<size_t>(cfg->pos_len, 1) * 4))) return fail(rc);
     if ((rc = reserve(h, h->sub, 256))) return fail(rc);
     if ((rc = reserve(h, h->total, 256))) return fail(rc);
-    if (hipMemcpy(h->index.p, cfg->index_table, cfg->index_len * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        (cfg->pos_len && hipMemcpy(h->pos.p, cfg->pos_table, cfg->pos_len * 4, hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemcpy(h->sub.p, cfg->sub_mat, 256, hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(h->sub.p, cfg->sub_mat, 256, hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_error(-1, "reference upload failed"));
-    h->index_len = cfg->index_len;
     h->ref_len = (uint32_t)cfg->ref_len;
     if ((rc = upload_seq(h, h->ref, cfg->ref_seq, cfg->ref_len))) return fail(rc);
     h->reference_uploads++;
+    if (build) {
+        if ((rc = device_index_build(h, *build, build_step))) return fail(rc);
+    } else {
+        if ((rc = reserve(h, h->index, cfg->index_len * 4))) return fail(rc);
+        if ((rc = reserve(h, h->pos, std::max<size_t>(cfg->pos_len, 1) * 4))) return fail(rc);
+        if (hipMemcpy(h->index.p, cfg->index_table, cfg->index_len * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            (cfg->pos_len &&
+             hipMemcpy(h->pos.p, cfg->pos_table, cfg->pos_len * 4, hipMemcpyHostToDevice) != hipSuccess))
+            return fail(set_error(-1, "index upload failed"));
+        h->index_len = cfg->index_len;
+        h->pos_len = cfg->pos_len;
+    }
     // Fast gate: two-cell map table over ACGT x ACGT pairs (scores clamped at
     // -(xdrop+1)); entry = (a = s1 + s2, b = xdrop + min(s2, 0)) as int16
     // halves, s1 the cell met first: the lower cell of the key forward, the
@@ -457,10 +544,10 @@ extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_e
             if (hipMemcpy(h->pairtab.p, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess)
                 return fail(set_error(-1, "pair table upload failed"));
             if ((rc = build_planes(h, h->ref, h->ref_len, h->ref2, h->refx))) return fail(rc);
-            if (cfg->pos_len) {
-                if ((rc = reserve(h, h->clean, ((cfg->pos_len + 31) / 32 + 2) * 4))) return fail(rc);
+            if (h->pos_len) {
+                if ((rc = reserve(h, h->clean, ((h->pos_len + 31) / 32 + 2) * 4))) return fail(rc);
                 hipLaunchKernelGGL(k_clean_bits, dim3(8192), dim3(256), 0, h->stream, h->pos.as<uint32_t>(),
-                                   (uint32_t)cfg->pos_len, h->refx.as<uint32_t>(), h->ref_len, h->seed_size,
+                                   (uint32_t)h->pos_len, h->refx.as<uint32_t>(), h->ref_len, h->seed_size,
                                    h->clean.as<uint32_t>());
                 if (hipGetLastError() != hipSuccess) return fail(set_error(-1, "clean-bit build failed"));
             }
@@ -468,6 +555,37 @@ extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_e
         }
     }
     *out = h;
+    return 0;
+}
+}  // namespace
+
+extern "C" int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_engine **out) {
+    return engine_create(device, cfg, nullptr, 0, out);
+}
+
+extern "C" int hspz_engine_new_device_index(int device, const hspz_engine_config *cfg, const char *pattern,
+                                            uint32_t step, hspz_engine **out) {
+    clear_error();
+    Shape s;
+    if (int rc = parse_shape(pattern ? pattern : "12of19", &s)) return rc;
+    return engine_create(device, cfg, &s, step ? step : 1, out);
+}
+
+extern "C" int hspz_engine_index(const hspz_engine *h, uint32_t *index_table, size_t index_len,
+                                 uint32_t **pos_table, size_t *pos_len) {
+    clear_error();
+    if (!h || !index_table || !pos_table || !pos_len) return set_error(HSPZ_E_ARG, "null argument");
+    if (index_len != h->index_len) return set_error(HSPZ_E_ARG, "index_table must hold %zu entries", h->index_len);
+    HIP_TRY(hipSetDevice(h->device));
+    uint32_t *pos = (uint32_t *)malloc(std::max<size_t>(h->pos_len, 1) * 4);
+    if (!pos) return set_error(HSPZ_E_OOM, "pos_table allocation failed");
+    if (hipMemcpy(index_table, h->index.p, index_len * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        (h->pos_len && hipMemcpy(pos, h->pos.p, h->pos_len * 4, hipMemcpyDeviceToHost) != hipSuccess)) {
+        free(pos);
+        return set_error(-1, "index read-back failed");
+    }
+    *pos_table = pos;
+    *pos_len = h->pos_len;
     return 0;
 }
 
@@ -675,6 +793,9 @@ extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
     case HSPZ_STAT_BATCH_DEVICE_NS: *value = h->batch_device_ns; break;
     case HSPZ_STAT_BATCH_READ_NS: *value = h->batch_read_ns; break;
     case HSPZ_STAT_BATCH_POST_NS: *value = h->batch_post_ns; break;
+    case HSPZ_STAT_INDEX_BUILD_NS: *value = h->index_build_ns; break;
+    case HSPZ_STAT_POS_LEN: *value = h->pos_len; break;
+    case HSPZ_STAT_INDEX_LEN: *value = h->index_len; break;
     default: return set_error(HSPZ_E_ARG, "unknown stat %d", which);
     }
     return 0;

@@ -23,7 +23,8 @@ SP_DTYPE = np.dtype([("ref_start", "<u4"), ("query_start", "<u4"), ("len", "<u4"
 STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 3,
         "query_swaps": 4, "peak_device_bytes": 5, "gate_ns": 6, "gate_hits": 7,
         "gate_launches": 8, "gate_deferred": 9, "graph_captures": 10, "graph_replays": 11,
-        "batch_prep_ns": 12, "batch_device_ns": 13, "batch_read_ns": 14, "batch_post_ns": 15}
+        "batch_prep_ns": 12, "batch_device_ns": 13, "batch_read_ns": 14, "batch_post_ns": 15,
+        "index_build_ns": 16, "pos_len": 17, "index_len": 18}
 FLAG = {"batch_graph": 0, "batch_gate_blocks": 1, "batch_step_cells": 2, "batch_block_threads": 3,
         "batch_claim_seeds": 4, "batch_order": 5, "batch_order_used": 6, "batch_device_dedup": 7}
 
@@ -36,7 +37,7 @@ EXPORTS = [
     "hspz_run_pass", "hspz_free", "hspz_build_sub_mat", "hspz_filter_batch",
     "hspz_batch_output_free", "hspz_server_new", "hspz_server_submit", "hspz_server_wait",
     "hspz_request_result_free", "hspz_server_get_stats", "hspz_server_free", "hspz_set_flag",
-    "hspz_get_flag", "hspz_autotune_batch",
+    "hspz_get_flag", "hspz_autotune_batch", "hspz_engine_new_device_index", "hspz_engine_index",
 ]
 
 
@@ -128,6 +129,9 @@ def lib():
         L.hspz_last_error.restype = C.c_char_p
         L.hspz_engine_new.argtypes = [C.c_int, C.POINTER(EngineConfig), C.POINTER(vp)]
         L.hspz_engine_free.argtypes = [vp]
+        L.hspz_engine_new_device_index.argtypes = [C.c_int, C.POINTER(EngineConfig), C.c_char_p, u32,
+                                                   C.POINTER(vp)]
+        L.hspz_engine_index.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.hspz_engine_free.restype = None
         L.hspz_swap_query.argtypes = [vp, vp, vp, sz]
         L.hspz_upload_seeds.argtypes = [vp, C.c_int, vp, sz]
@@ -246,23 +250,43 @@ def device_count() -> int:
 class Engine:
     """Reference-resident engine on one device (gpu/mod.rs:1024-1128)."""
 
-    def __init__(self, index_table: np.ndarray, pos_table: np.ndarray, ref_enc: np.ndarray,
+    def __init__(self, index_table, pos_table, ref_enc: np.ndarray,
                  sub_mat: np.ndarray, seed_size: int = 19, xdrop: int = 910,
                  hspthresh: int = 3000, noentropy: bool = False, max_hits: int = 0,
-                 device: int = 0):
-        idx = np.ascontiguousarray(index_table, dtype=np.uint32)
-        pos = np.ascontiguousarray(pos_table, dtype=np.uint32)
+                 device: int = 0, device_index: str = None, index_step: int = 1):
+        """index_table/pos_table: the host index (hspz_engine_new); or None and
+        device_index = the seed pattern, to build the index on the device
+        (hspz_engine_new_device_index)."""
         ref = np.ascontiguousarray(ref_enc, dtype=np.uint8)
         sub = np.ascontiguousarray(sub_mat, dtype=np.int32)
         if sub.size != 64:
             raise ValueError("sub_mat must have 64 entries")
-        cfg = EngineConfig(_ptr(idx).value, len(idx), _ptr(pos).value if len(pos) else None,
-                           len(pos), _ptr(ref).value, len(ref), _ptr(sub).value, seed_size,
-                           xdrop, hspthresh, int(noentropy), max_hits, 0)
         h = C.c_void_p()
-        _check(lib().hspz_engine_new(device, C.byref(cfg), C.byref(h)))
+        if device_index is not None:
+            cfg = EngineConfig(None, 0, None, 0, _ptr(ref).value, len(ref), _ptr(sub).value, seed_size,
+                               xdrop, hspthresh, int(noentropy), max_hits, 0)
+            _check(lib().hspz_engine_new_device_index(device, C.byref(cfg), device_index.encode(),
+                                                      index_step, C.byref(h)))
+        else:
+            idx = np.ascontiguousarray(index_table, dtype=np.uint32)
+            pos = np.ascontiguousarray(pos_table, dtype=np.uint32)
+            cfg = EngineConfig(_ptr(idx).value, len(idx), _ptr(pos).value if len(pos) else None,
+                               len(pos), _ptr(ref).value, len(ref), _ptr(sub).value, seed_size,
+                               xdrop, hspthresh, int(noentropy), max_hits, 0)
+            _check(lib().hspz_engine_new(device, C.byref(cfg), C.byref(h)))
         self._h = h
         self.seed_size = seed_size
+
+    def index(self):
+        """The engine's resident index copied to the host: (index_table, pos_table)."""
+        idx = np.zeros(self.stat("index_len"), dtype=np.uint32)
+        p, ln = C.c_void_p(), C.c_size_t()
+        _check(lib().hspz_engine_index(self._h, _ptr(idx), len(idx), C.byref(p), C.byref(ln)))
+        pos = np.zeros(ln.value, dtype=np.uint32)
+        if ln.value:
+            C.memmove(_ptr(pos), p, ln.value * 4)
+        lib().hspz_free(p)
+        return idx, pos
 
     def close(self):
         if getattr(self, "_h", None):

@@ -74,6 +74,18 @@ typedef struct hspz_engine hspz_engine;
 
 /* Engine::new (gpu/mod.rs:1024-1128). */
 int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_engine **out);
+/* Engine::new with the reference index built on the device and kept there
+ * (SeedTable::build, seed.rs:126-373: the same positions, buckets and order as
+ * hspz_seed_table_build with the same pattern and step). cfg->index_table,
+ * index_len, pos_table and pos_len are ignored; cfg->seed_size must equal the
+ * pattern's span. */
+int hspz_engine_new_device_index(int device, const hspz_engine_config *cfg, const char *pattern, uint32_t step,
+                                 hspz_engine **out);
+/* Copies the engine's resident index to the host: index_len must be 4^k;
+ * *pos_table is library-allocated (hspz_free). No counterpart in the
+ * reference; it lets a caller reuse or check a device-built index. */
+int hspz_engine_index(const hspz_engine *h, uint32_t *index_table, size_t index_len, uint32_t **pos_table,
+                      size_t *pos_len);
 void hspz_engine_free(hspz_engine *h);
 
 /* Engine::swap_query (gpu/mod.rs:1142-1148): encoded forward and reverse-
@@ -117,6 +129,9 @@ void hspz_filter_output_free(hspz_filter_output *out);
 #define HSPZ_STAT_BATCH_DEVICE_NS 13 /* device pass, launch to drained stream */
 #define HSPZ_STAT_BATCH_READ_NS 14   /* read-back of counters and kept HSPs */
 #define HSPZ_STAT_BATCH_POST_NS 15   /* host dedup_and_order per scope and assembly */
+#define HSPZ_STAT_INDEX_BUILD_NS 16  /* device index build of hspz_engine_new_device_index */
+#define HSPZ_STAT_POS_LEN 17         /* indexed positions held by the engine */
+#define HSPZ_STAT_INDEX_LEN 18       /* index_table entries (4^k) held by the engine */
 int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 
 /* Engine switches (the reference's pub flags, gpu/mod.rs:836-949, have no

@@ -9,6 +9,7 @@ import pytest
 import oracle_ffi as O
 from helpers import Pair, same_intervals
 from hspz_amd import engine as E
+from hspz_amd import sequence as S
 from hspz_amd import synth
 from gen_fixtures import SCORING_ACGT
 
@@ -118,3 +119,44 @@ def test_c1_full_size(gpu_available):
     t, q = synth.c1_pair(4_000_000, seed=0xC0FFEE01)
     got, st = _check(Pair.raw(t, q))
     assert st["hsps"] > 0
+
+
+@pytest.mark.parametrize("case", ["random", "softmasked_n", "step3", "tiny"])
+def test_device_index_equals_host(gpu_available, case):
+    """hspz_engine_new_device_index builds the same index as the host
+    SeedTable::build (seed.rs:126-373) and the oracle, and the engine it makes
+    gives the same HSP lists as one built from the host index."""
+    print("case device index: %s" % case, flush=True)
+    step = 3 if case == "step3" else 1
+    if case == "tiny":
+        target = b"AAAAACGTAAAAACGTTTTTTTTTTTTTTTTTTTTTTTGGGGA"
+    elif case == "softmasked_n":
+        t, _ = synth.c2_pair(300_000, seed=0xD1)
+        t = bytearray(t)
+        rng = np.random.default_rng(5)
+        for i in rng.integers(0, len(t) - 50, 300):
+            t[i:i + rng.integers(1, 40)] = b"N" * len(t[i:i + rng.integers(1, 40)])
+        target = bytes(t)
+    else:
+        target = synth.random_bases(0xD2, 500_000).tobytes()
+    sub = O.sub_mat("", 910)
+    h_index, h_pos = E.seed_table_build(target, "12of19", step, 8)
+    o_index, o_pos = O.seed_table(target, O.shape("12of19"), step)
+    assert np.array_equal(h_index, o_index) and np.array_equal(h_pos, o_pos)
+    with E.Engine(None, None, S.encode(target), sub, max_hits=16711680, device_index="12of19",
+                  index_step=step) as eng:
+        d_index, d_pos = eng.index()
+        assert eng.stat("pos_len") == len(h_pos)
+        assert np.array_equal(d_index, h_index), "index_table differs"
+        assert np.array_equal(d_pos, h_pos), "pos_table differs"
+        if case == "random":
+            q = synth.mutate(np.frombuffer(target[100_000:200_000], dtype=np.uint8), 7, 0.1, 0.005)
+            buf, chrs, bl = S.pack([("q", bytes(q))])
+            rc, _ = S.reverse_complement(buf, chrs, bl)
+            fw, rv = S.encode(buf[:bl]), S.encode(rc)
+            eng.swap_query(fw, rv)
+            got, _ = eng.run_pass()
+            with E.Engine(h_index, h_pos, S.encode(target), sub, max_hits=16711680) as ref_eng:
+                ref_eng.swap_query(fw, rv)
+                want, _ = ref_eng.run_pass()
+            assert same_intervals(got, want)
