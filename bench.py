@@ -30,6 +30,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -40,6 +41,20 @@ HBM_PEAK_GBS = 8000.0
 # Stored bytes per base the packed gate reads per sequence: a 2-bit code plus
 # one exception bit per 64 bases.
 W_PACKED = 0.25 + 1.0 / 512
+
+
+def profiled_traffic(hits_per_launch, order_used):
+    """HBM bytes per launch of the batched gate (k_bgate_kmer + k_bgate_wave),
+    FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from the committed rocprofv3 counter
+    summary of this same workload; None when this run's launch is not that one
+    (another hit order or hit count)."""
+    path = os.path.join(ROOT, "profiles", "kmer_gate_1gbp_target.md")
+    if order_used != 1 or not os.path.exists(path):
+        return None
+    m = re.search(r"traffic-line: \S+ hits=(\d+) .*bytes_per_launch=(\d+)", open(path).read())
+    if not m or int(m.group(1)) != hits_per_launch:
+        return None
+    return int(m.group(2))
 
 
 def log(msg):
@@ -245,8 +260,13 @@ def main():
         per_hit = 4.0 + 2.0 * W_PACKED * cells_per_hit
         secs = g["gate_ns"] * 1e-9
         achieved = g["gate_hits"] * per_hit / secs / 1e9
+        traffic = profiled_traffic(st_b["hits"] if st_b else 0, order_used)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/kmer_gate_1gbp_target.md (FETCH_SIZE x2.0 + WRITE_SIZE, "
+                                  "same workload)" if traffic else None,
+                "binding_limit": "VALU issue (~91% of VALU issue capacity, profiles/kmer_gate_1gbp_target.md)"
+                                 if order_used else "L2-miss bandwidth of random reference lines",
                 "kernel": "%s (+ k_bgate_wave for deferred hits)"
                           % ("k_bgate_kmer" if order_used else "k_bgate_packed"),
                 "hit_order": "k-mer-major" if order_used else "seed-major",
