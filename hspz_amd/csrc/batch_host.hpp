@@ -39,19 +39,26 @@ typedef void (*BGateFn)(const uint32_t *, const uint32_t *, uint32_t, const uint
                         uint32_t *, uint32_t, SurvRec *, uint32_t *, uint32_t, uint32_t *);
 
 typedef void (*KGateFn)(const uint32_t *, const uint32_t *, int32_t, int32_t, const QRec *, const uint32_t *,
-                        const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, const uint32_t *, uint32_t,
-                        uint32_t, unsigned long long *, SurvRec *, uint32_t *, uint32_t, SurvRec *, uint32_t *,
-                        uint32_t, uint32_t *);
+                        const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, const uint32_t *,
+                        const uint4 *, uint32_t, uint32_t, unsigned long long *, SurvRec *, uint32_t *, uint32_t,
+                        SurvRec *, uint32_t *, uint32_t, uint32_t *);
 
 // The k-mer-major gate instantiation for a step width and block size.
 KGateFn kgate_kernel(int step_cells, int threads) {
     if (threads == 1024)
-        return step_cells == 16 ? k_bgate_kmer<1, 1024> : step_cells == 64 ? k_bgate_kmer<4, 1024> : k_bgate_kmer<2, 1024>;
-    return step_cells == 16 ? k_bgate_kmer<1, 512> : step_cells == 64 ? k_bgate_kmer<4, 512> : k_bgate_kmer<2, 512>;
+        return step_cells == 16    ? k_bgate_kmer<1, 1024>
+               : step_cells == 64  ? k_bgate_kmer<4, 1024>
+               : step_cells == 128 ? k_bgate_kmer<8, 1024>
+                                   : k_bgate_kmer<2, 1024>;
+    return step_cells == 16    ? k_bgate_kmer<1, 512>
+           : step_cells == 64  ? k_bgate_kmer<4, 512>
+           : step_cells == 128 ? k_bgate_kmer<8, 512>
+                               : k_bgate_kmer<2, 512>;
 }
 
 // The batched gate instantiation for a step width and block size.
 BGateFn bgate_kernel(int step_cells, int threads) {
+    if (step_cells == 128) step_cells = 64;  // the merged-phase variant exists in k-mer-major order only
     if (threads == 1024)
         return step_cells == 16 ? k_bgate_packed<1, 1024> : step_cells == 64 ? k_bgate_packed<4, 1024> : k_bgate_packed<2, 1024>;
     return step_cells == 16 ? k_bgate_packed<1, 512> : step_cells == 64 ? k_bgate_packed<4, 512> : k_bgate_packed<2, 512>;
@@ -80,6 +87,28 @@ struct hspz_batch_state {
 };
 
 namespace {
+
+// Builds the engine's reference records (k_ref_records) on first use, when
+// HSPZ_FLAG_REF_RECORDS is on and device memory leaves room for them and a
+// margin; otherwise the k-mer-major gate reads its windows from the plane.
+int ensure_ref_records(hspz_engine *h) {
+    if (!h->use_rrec || h->rrec.p || h->rrec_declined || h->pos_len == 0) return 0;
+    size_t need = h->pos_len * 48;
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    if (f < need + (8ull << 30)) {
+        h->rrec_declined = true;
+        return 0;
+    }
+    if (int rc = reserve(h, h->rrec, need)) return rc;
+    hipLaunchKernelGGL(k_ref_records, dim3(std::min<uint32_t>(ceil_div(h->pos_len, 256), 65536)), dim3(256), 0,
+                       h->stream, h->pos.as<uint32_t>(), (uint32_t)h->pos_len, h->clean.as<uint32_t>(),
+                       h->ref2.as<uint32_t>(), h->seed_size, h->rrec.as<uint4>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->launches++;
+    return 0;
+}
 
 // Stable radix sort of (key, u32 value) pairs on the engine's stream.
 template <class K>
@@ -268,6 +297,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             if ((rc = reserve(h, B.sq, cap_seeds * 8 + 8))) return rc;
             if ((rc = reserve(h, B.skm, cap_seeds * 4 + 4))) return rc;
             if ((rc = reserve(h, B.qrec, cap_seeds * sizeof(QRec) + sizeof(QRec)))) return rc;
+            if ((rc = ensure_ref_records(h))) return rc;
         }
         BlockOf bo;
         bo.seeds = B.seeds.as<uint64_t>();
@@ -348,7 +378,8 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
                                B.skm.as<uint32_t>(), B.gcnt.as<uint32_t>(), nkm,
                                h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->clean.as<uint32_t>(),
-                               h->seed_size, (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK),
+                               h->rrec.p ? h->rrec.as<uint4>() : nullptr, h->seed_size, (uint32_t)h->bgate_claim,
+                               (unsigned long long *)(ctr + CTR_WORK),
                                B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(),
                                ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
         else
@@ -390,7 +421,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                   B.nchunks.p, B.starts.p, B.bhits.p, B.surv.p, B.slow.p, B.mhsp.p, B.mscope.p,
                                   B.keep.p, B.kids.p, B.ohsp.p, B.oscope.p, B.ctr.p, h->tiles.p, h->ref2.p,
                                   h->refx.p, h->pairtab.p, h->index.p, h->pos.p, B.gcnt.p, B.sq.p, B.skm.p,
-                                  h->clean.p, B.qrec.p};
+                                  h->clean.p, B.qrec.p, h->rrec.p};
             memcpy(key.ptrs, ptrs, sizeof ptrs);
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow, cap_seeds, per, h->max_hits,
                                (uint64_t)grid | (uint64_t)h->bgate_claim << 32, (uint64_t)p->transitions, nt, kt,

@@ -86,10 +86,12 @@ struct hspz_engine {
     int gate_blocks = 2048;
     // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
     int bgate_blocks = 0;  // persistent grid; 0 = as many blocks as are resident at once
-    int bgate_step = 64, bgate_threads = 512, bgate_claim = 64;
+    int bgate_step = 128, bgate_threads = 512, bgate_claim = 256;
     int batch_order = 2;                    // 0 seed-major, 1 k-mer-major, 2 by batch size
     int batch_order_used = 0;               // order of the last batched pass
     DevBuf clean;                           // pos_table-aligned clean-window bits (k-mer-major gate)
+    DevBuf rrec;                            // pos_table-ordered reference records (k_ref_records)
+    bool use_rrec = false, rrec_declined = false;
     int bgate_grid_used = 0;                // grid of the last batched launch
     int num_cus = 0;
     double last_gate_ms = 0, last_post_ms = 0;  // event times of the last batched pass
@@ -594,7 +596,7 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     DevBuf *bufs[] = {&h->ref, &h->index, &h->pos, &h->sub, &h->qf, &h->qr, &h->seeds[0], &h->seeds[1],
-                      &h->ref2, &h->refx, &h->qf2, &h->qfx, &h->qr2, &h->qrx, &h->pairtab, &h->slow, &h->slow_n, &h->clean,
+                      &h->ref2, &h->refx, &h->qf2, &h->qfx, &h->qr2, &h->qrx, &h->pairtab, &h->slow, &h->slow_n, &h->clean, &h->rrec,
                       &h->hit_num, &h->tiles, &h->total, &h->anchors, &h->flags, &h->surv, &h->hsp,
                       &h->done, &h->ids2, &h->reduced, &h->valid};
     for (DevBuf *b : bufs) release(*b);
@@ -735,7 +737,8 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
         h->bgate_blocks = value;
         break;
     case HSPZ_FLAG_BATCH_STEP_CELLS:
-        if (value != 16 && value != 32 && value != 64) return set_error(HSPZ_E_ARG, "step cells must be 16, 32 or 64");
+        if (value != 16 && value != 32 && value != 64 && value != 128)
+            return set_error(HSPZ_E_ARG, "step cells must be 16, 32, 64 or 128");
         h->bgate_step = value;
         break;
     case HSPZ_FLAG_BATCH_BLOCK_THREADS:
@@ -751,6 +754,15 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
         h->batch_order = value;
         break;
     case HSPZ_FLAG_BATCH_DEVICE_DEDUP: h->device_dedup = value != 0; break;
+    case HSPZ_FLAG_REF_RECORDS:
+        h->use_rrec = value != 0;
+        if (!h->use_rrec && h->rrec.p) {
+            (void)hipStreamSynchronize(h->stream);
+            std::lock_guard<std::mutex> lk(capture_mutex());
+            release(h->rrec);
+        }
+        h->rrec_declined = false;
+        break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;
@@ -768,6 +780,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_ORDER: *value = h->batch_order; break;
     case HSPZ_FLAG_BATCH_ORDER_USED: *value = h->batch_order_used; break;
     case HSPZ_FLAG_BATCH_DEVICE_DEDUP: *value = h->device_dedup ? 1 : 0; break;
+    case HSPZ_FLAG_REF_RECORDS: *value = h->rrec.p ? 2 : h->use_rrec ? 1 : 0; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;

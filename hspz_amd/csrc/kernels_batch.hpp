@@ -385,6 +385,13 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
 // registers and walks every query seed of the group, so a reference window is
 // fetched once per (k-mer, position) instead of once per hit.
 
+// Swaps the two nibbles of every byte: applied to byte-swapped left words, it
+// brings a byte's nearer cell pair into the low nibble, so the left stream
+// forms its keys exactly like the right one.
+__device__ __forceinline__ uint32_t nib_swap(uint32_t x) {
+    return ((x >> 4) & 0x0F0F0F0Fu) | ((x << 4) & 0xF0F0F0F0u);
+}
+
 // Per pos_table entry: 1 when the anchor R = pos + seed_size has the packed
 // gate's margins and no exception code in its window (the reference-side
 // test of packed_gate_loop, done once per engine).
@@ -406,6 +413,37 @@ __global__ void k_clean_bits(const uint32_t *__restrict__ pos, uint32_t n, const
         uint32_t w = (uint32_t)(base / 32);
         if (lane == 0) bits[w] = (uint32_t)m;
         if (lane == 1 && w + 1 < nwords) bits[w + 1] = (uint32_t)(m >> 32);
+    }
+}
+
+// Reference records in pos_table order (one per indexed position, 48 bytes):
+// the k-mer-major gate's right and left reference streams of the anchor
+// R = pos + seed_size, aligned to R, left words byte- and nibble-swapped
+// (zeros where the clean bit is 0), so that taking an item costs one
+// coalesced load instead of a position load followed by a random window load.
+__global__ void k_ref_records(const uint32_t *__restrict__ pos, uint32_t n, const uint32_t *__restrict__ clean,
+                              const uint32_t *__restrict__ rplane, uint32_t seed_size, uint4 *__restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t A[G2_RIGHT_WORDS + G2_LEFT_WORDS];
+#pragma unroll
+        for (int k = 0; k < (int)(G2_RIGHT_WORDS + G2_LEFT_WORDS); k++) A[k] = 0;
+        if ((clean[i >> 5] >> (i & 31)) & 1u) {
+            uint32_t R = pos[i] + seed_size;
+            const uint32_t *rp = rplane + (R >> 4) - G2_LEFT_WORDS;
+            uint32_t W[13];
+#pragma unroll
+            for (int k = 0; k < 13; k++) W[k] = rp[k];  // word 8 holds base R
+            uint32_t sr = 2 * (R & 15);
+#pragma unroll
+            for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) A[k] = __builtin_amdgcn_alignbit(W[9 + k], W[8 + k], sr);
+#pragma unroll
+            for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
+                A[G2_RIGHT_WORDS + k] = nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sr)));
+        }
+        uint4 *o = out + 3 * (size_t)i;
+        o[0] = make_uint4(A[0], A[1], A[2], A[3]);
+        o[1] = make_uint4(A[4], A[5], A[6], A[7]);
+        o[2] = make_uint4(A[8], A[9], A[10], A[11]);
     }
 }
 
@@ -434,12 +472,6 @@ __global__ void k_kscatter(const uint64_t *__restrict__ seeds, const uint32_t *_
     }
 }
 
-// Swaps the two nibbles of every byte: applied to byte-swapped left words, it
-// brings a byte's nearer cell pair into the low nibble, so the left stream
-// forms its keys exactly like the right one.
-__device__ __forceinline__ uint32_t nib_swap(uint32_t x) {
-    return ((x >> 4) & 0x0F0F0F0Fu) | ((x << 4) & 0xF0F0F0F0u);
-}
 
 // Query record of one grouped seed: its right and left query streams as the
 // k-mer-major gate consumes them (aligned to Q, left words byte- and
@@ -562,7 +594,7 @@ struct KmerSource {
     // A lane without a hit takes its item's next query record (index s_cur),
     // or else a new item.
     __device__ __forceinline__ void refill(bool &has, bool &item, bool &fresh_r, bool &fresh_q, uint32_t &R,
-                                           uint32_t &s_cur, uint32_t &s_end, bool &rok) {
+                                           uint32_t &s_cur, uint32_t &s_end, bool &rok, uint32_t &ridx) {
         if (!has && item) {
             if (s_cur < s_end) {
                 has = true;
@@ -583,6 +615,7 @@ struct KmerSource {
                 uint32_t e = b0 + j + rank;
                 R = pos[e] + seed_size;
                 rok = ((clean[e >> 5] >> (e & 31)) & 1u) != 0;
+                ridx = e;
                 s_cur = g0;
                 s_end = g1;
                 has = true;
@@ -635,7 +668,8 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
 
     for (;;) {
         bool fresh_r = false, fresh_q = false;
-        src.refill(has, item, fresh_r, fresh_q, R, s_cur, s_end, rok);
+        uint32_t ridx = 0;
+        src.refill(has, item, fresh_r, fresh_q, R, s_cur, s_end, rok, ridx);
         if (__ballot(has) == 0) break;
 
         bool defer = false, done = false, pass = false;
@@ -752,6 +786,147 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
     }
 }
 
+// min(a, b, c) as one v_min3_i32 (the compiler otherwise builds min trees
+// that take an extra instruction per pair).
+__device__ __forceinline__ int32_t min3_i32(int32_t a, int32_t b, int32_t c) {
+    int32_t d;
+    asm volatile("v_min3_i32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
+// One 64-cell stretch (4 word pairs) of a stream through the map table at
+// lane offset `off`: updates e, the score sum c, and the minimum of e over
+// the cell pairs.
+// With N = 8 words, `hi` picks words 4..7 instead of 0..3 (per lane).
+template <int N>
+__device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t (&rw)[N], const uint32_t (&qw)[N],
+                                            bool hi, uint32_t off, int32_t &e, int32_t &c, int32_t &emin) {
+    constexpr uint32_t SEL[4] = {0x0C0C0400u, 0x0C0C0500u, 0x0C0C0600u, 0x0C0C0700u};
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        uint32_t r = rw[h], q = qw[h];
+        if constexpr (N == 8) {
+            r = hi ? rw[4 + h] : r;
+            q = hi ? qw[4 + h] : q;
+        }
+        uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
+        uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
+        uint32_t p[8];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            p[2 * j] = map_entry(s_map, t1, off, SEL[j]);
+            p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            int32_t e1 = min(e + lo16(p[i]), hi16(p[i]));
+            e = min(e1 + lo16(p[i + 1]), hi16(p[i + 1]));
+            emin = min3_i32(emin, e1, e);
+        }
+        c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3]))) +
+             lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
+        // finish this word before the next one's reads: deferring the sums
+        // keeps every table entry of the stretch live
+        asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
+    }
+}
+
+// The k-mer-major gate with merged phases: one iteration evaluates a fresh
+// hit's 64 right cells and its first 64 left cells; a hit whose left
+// extension outlives them takes a second iteration for left cells 64..127.
+// It evaluates exactly the cells of kmer_gate_loop<4> and reaches the same
+// decisions, with about half the iterations per hit (the refill and
+// bookkeeping of an iteration are paid once per hit) and both table offsets
+// fixed per stretch.
+template <class Src>
+__device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__restrict__ rplane,
+                                                 const uint4 *__restrict__ rrec, const QRec *__restrict__ recs,
+                                                 const uint8_t *s_map, int32_t xdrop, int32_t thresh) {
+    bool has = false, item = false, rok = false, cont = false;
+    uint32_t R = 0, Q = 0, tag = 0, s_cur = 0, s_end = 0;
+    const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
+    int32_t Mr = 0, el_keep = 0, cl_keep = 0;
+    uint32_t A0[G2_RIGHT_WORDS], LA0[G2_LEFT_WORDS], B[G2_RIGHT_WORDS], LB[G2_LEFT_WORDS];
+#pragma unroll
+    for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) A0[j] = B[j] = 0;
+#pragma unroll
+    for (int j = 0; j < (int)G2_LEFT_WORDS; j++) LA0[j] = LB[j] = 0;
+
+    for (;;) {
+        bool fresh_r = false, fresh_q = false;
+        uint32_t ridx = 0;
+        src.refill(has, item, fresh_r, fresh_q, R, s_cur, s_end, rok, ridx);
+        if (__ballot(has) == 0) break;
+
+        bool defer = false, done = false, pass = false;
+        if (rrec && fresh_r && rok) {  // the item's precomputed reference record
+            const uint4 *rr = rrec + 3 * (size_t)ridx;
+            uint4 w0 = rr[0], w1 = rr[1], w2 = rr[2];
+            A0[0] = w0.x, A0[1] = w0.y, A0[2] = w0.z, A0[3] = w0.w;
+            LA0[0] = w1.x, LA0[1] = w1.y, LA0[2] = w1.z, LA0[3] = w1.w;
+            LA0[4] = w2.x, LA0[5] = w2.y, LA0[6] = w2.z, LA0[7] = w2.w;
+        } else if (fresh_r && rok) {
+            const uint32_t *rp = rplane + (R >> 4);
+            uint32_t rend = rp[G2_RIGHT_WORDS], lend = rp[-(int)G2_LEFT_WORDS];
+#pragma unroll
+            for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) A0[k] = rp[k];
+#pragma unroll
+            for (int k = 0; k < (int)G2_LEFT_WORDS; k++) LA0[k] = rp[-k];
+            uint32_t sr = 2 * (R & 15);
+#pragma unroll
+            for (int k = 0; k < (int)G2_RIGHT_WORDS; k++)
+                A0[k] = __builtin_amdgcn_alignbit(k + 1 < (int)G2_RIGHT_WORDS ? A0[k + 1] : rend, A0[k], sr);
+#pragma unroll
+            for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
+                LA0[k] = nib_swap(__builtin_bswap32(
+                    __builtin_amdgcn_alignbit(LA0[k], k + 1 < (int)G2_LEFT_WORDS ? LA0[k + 1] : lend, sr)));
+        }
+        if (fresh_q) {
+            const uint4 *rec = (const uint4 *)(recs + s_cur);
+            s_cur++;
+            uint4 v0 = rec[0], v1 = rec[1], v2 = rec[2], v3 = rec[3];
+            B[0] = v0.x, B[1] = v0.y, B[2] = v0.z, B[3] = v0.w;
+            LB[0] = v1.x, LB[1] = v1.y, LB[2] = v1.z, LB[3] = v1.w;
+            LB[4] = v2.x, LB[5] = v2.y, LB[6] = v2.z, LB[7] = v2.w;
+            Q = v3.x;
+            tag = v3.y;
+            cont = false;
+            if (!(rok && v3.z != 0)) defer = true;
+        }
+
+        if (has && !defer) {
+            // right cells 0..63 (used by fresh hits only)
+            int32_t er = xdrop, cr = 0, eminr = xdrop;
+            map_stretch<4>(s_map, A0, B, false, off_fwd, er, cr, eminr);
+            // left cells 0..63, or 64..127 for a continuing hit
+            int32_t el = cont ? el_keep : xdrop, cl = cont ? cl_keep : 0, eminl = el;
+            map_stretch<8>(s_map, LA0, LB, cont, off_rev, el, cl, eminl);
+            if (!cont) {
+                if (eminr >= 0) defer = true;  // the right extension outlived its 64 cells
+                else Mr = cr + (xdrop - er);
+            }
+            if (!defer) {
+                if (eminl < 0) {
+                    done = true;
+                    pass = gate_keep(Mr + cl + (xdrop - el), thresh);
+                } else if (cont) {
+                    defer = true;  // the left extension outlived its 128 cells
+                } else {
+                    cont = true;
+                    el_keep = el;
+                    cl_keep = cl;
+                }
+            }
+        }
+        src.emit(done, pass, tag, R, Q);
+        src.defer(defer, tag, R, Q);
+        if (done || defer) {
+            has = false;
+            cont = false;
+        }
+    }
+}
+
 // The batched gate in k-mer-major order (same launch shape as k_bgate_packed).
 template <int SW, int BT>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_kmer(
@@ -759,7 +934,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     int32_t thresh, const QRec *__restrict__ recs, const uint32_t *__restrict__ skm,
     const uint32_t *__restrict__ goff, uint32_t nk,
     const uint32_t *__restrict__ index, const uint32_t *__restrict__ pos, const uint32_t *__restrict__ clean,
-    uint32_t seed_size, uint32_t claim, unsigned long long *__restrict__ work, SurvRec *__restrict__ surv,
+    const uint4 *__restrict__ rrec, uint32_t seed_size, uint32_t claim, unsigned long long *__restrict__ work,
+    SurvRec *__restrict__ surv,
     uint32_t *__restrict__ surv_n, uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n,
     uint32_t cap_slow, uint32_t *__restrict__ err) {
     __shared__ uint32_t s_map[G2_MAP_WORDS];
@@ -782,7 +958,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.cap_slow = cap_slow;
     src.err = err;
     src.init();
-    kmer_gate_loop<SW>(src, rplane, recs, (const uint8_t *)s_map, xdrop, thresh);
+    if constexpr (SW == 8) kmer_gate_merged(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+    else kmer_gate_loop<SW>(src, rplane, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
 // Block bounds of a seed's batch: arena offset and length of its block.

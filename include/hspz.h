@@ -139,10 +139,12 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 #define HSPZ_FLAG_BATCH_GRAPH 0        /* 1 (default): capture/replay the batched pass as a hipGraph */
 #define HSPZ_FLAG_BATCH_GATE_BLOCKS 1  /* persistent grid of the batched gate in blocks; 0 (default) = as many
                                           as are resident at once */
-#define HSPZ_FLAG_BATCH_STEP_CELLS 2   /* cells per gate step: 16, 32 or 64 (default) */
+#define HSPZ_FLAG_BATCH_STEP_CELLS 2   /* cells per gate step: 16, 32, 64, or 128 (default): 64 right + 64
+                                          left cells of a hit in one step (k-mer-major order; seed-major
+                                          runs it as 64) */
 #define HSPZ_FLAG_BATCH_BLOCK_THREADS 3 /* threads per gate block: 512 (default) or 1024 */
 #define HSPZ_FLAG_BATCH_CLAIM_SEEDS 4  /* seeds a wave claims from the work counter at a time (1..4096,
-                                          default 64) */
+                                          default 256) */
 #define HSPZ_FLAG_BATCH_ORDER 5        /* hit order of the batched gate: 0 seed-major, 1 k-mer-major (a
                                           reference window is reused by every query seed of its k-mer),
                                           2 (default) k-mer-major when the batch holds at least one seed
@@ -150,6 +152,10 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 #define HSPZ_FLAG_BATCH_ORDER_USED 6   /* read-only: the order of the last batched pass */
 #define HSPZ_FLAG_BATCH_DEVICE_DEDUP 7 /* 1 (default): dedup_and_order of the batched path on the device
                                           (stable radix sorts of all scopes at once); 0: on host threads */
+#define HSPZ_FLAG_REF_RECORDS 8        /* 1: the merged-step k-mer-major gate reads 48-byte reference
+                                          records in pos_table order, built on first use when device
+                                          memory allows (48 B per indexed position); 0 (default): it reads
+                                          the 2-bit plane. hspz_get_flag reports 2 once they are built */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

@@ -147,7 +147,8 @@ def test_batch_gate_configs_match(gpu_available):
         want.append(o.run_pass(threads=16)[0])
     variants = [(16, 1024, 1, 0, 0), (64, 512, 4096, 0, 0), (32, 1024, 7, 3, 0), (64, 1024, 1, 1, 0),
                 (16, 512, 64, 5000, 0), (64, 1024, 1, 0, 1), (32, 512, 3, 2, 1), (16, 1024, 4096, 0, 1),
-                (64, 512, 64, 7, 1), (32, 1024, 1, 5000, 1)]
+                (64, 512, 64, 7, 1), (32, 1024, 1, 5000, 1), (128, 1024, 1, 0, 1), (128, 512, 256, 3, 1),
+                (128, 1024, 64, 0, 0)]
     with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
         for step, threads, claim, blocks_n, order in variants:
             _say("  step %d threads %d claim %d grid %d order %d" % (step, threads, claim, blocks_n, order))
@@ -179,13 +180,13 @@ def test_batch_autotune(gpu_available):
     with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
         before, st0 = eng.filter_batch(chunks)
         cells, best = eng.autotune(chunks, reps=1)
-        assert len(cells) == 24
+        assert len(cells) == 30
         assert all(c["status"] == 0 for c in cells)
         assert len({c["hits"] for c in cells}) == 1 and cells[0]["hits"] == st0["hits"]
         assert [c["order"] for c in cells[:2]] == [0, 1]
         assert len({c["order"] for c in cells[2:]}) == 1
-        assert {(c["step_cells"], c["block_threads"], c["claim_seeds"]) for c in cells[2:20]} == {
-            (s, t, c) for s in (16, 32, 64) for t in (512, 1024) for c in (16, 64, 256)}
+        assert {(c["step_cells"], c["block_threads"], c["claim_seeds"]) for c in cells[2:26]} == {
+            (s, t, c) for s in (16, 32, 64, 128) for t in (512, 1024) for c in (16, 64, 256)}
         fastest = min(c["gate_ms"] + c["after_ms"] for c in cells)
         assert best["gate_ms"] + best["after_ms"] == fastest
         assert eng.get_flag("batch_step_cells") == best["step_cells"]
