@@ -265,7 +265,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": "profiles/kmer_gate_1gbp_target.md (FETCH_SIZE x2.0 + WRITE_SIZE, "
                                   "same workload)" if traffic else None,
-                "binding_limit": "VALU issue (~91% of VALU issue capacity, profiles/kmer_gate_1gbp_target.md)"
+                "binding_limit": "per-iteration load round trips with VALU at 68% of issue "
+                                 "(profiles/kmer_gate_1gbp_target.md)"
                                  if order_used else "L2-miss bandwidth of random reference lines",
                 "kernel": "%s (+ k_bgate_wave for deferred hits)"
                           % ("k_bgate_kmer" if order_used else "k_bgate_packed"),
