@@ -53,19 +53,29 @@ class Chr:
     len: int
 
 
+# ASCII whitespace as Rust's u8::is_ascii_whitespace defines it (no \x0b).
+_WS = b" \t\n\x0c\r"
+
+
 def read_fasta(path: str) -> list[tuple[str, bytes]]:
-    """Name = first whitespace token after '>', whitespace dropped from bases."""
+    """sequence.rs:346-414: name = first whitespace token after '>' (invalid
+    UTF-8 replaced), whitespace dropped from sequence lines, lines before the
+    first header ignored."""
     records: list[tuple[str, bytes]] = []
     name, parts = None, []
     with open(path, "rb") as fh:
-        for line in fh:
-            if line.startswith(b">"):
-                if name is not None:
-                    records.append((name, b"".join(parts)))
-                toks = line[1:].split()
-                name, parts = (toks[0].decode() if toks else ""), []
-            else:
-                parts.append(b"".join(line.split()))
+        data = fh.read()
+    for line in data.split(b"\n"):
+        if line.startswith(b">"):
+            if name is not None:
+                records.append((name, b"".join(parts)))
+            head = line[1:]
+            for w in _WS:
+                head = head.replace(bytes([w]), b" ")
+            toks = [t for t in head.split(b" ") if t]
+            name, parts = (toks[0].decode("utf-8", "replace") if toks else ""), []
+        elif name is not None:
+            parts.append(line.translate(None, _WS))
     if name is not None:
         records.append((name, b"".join(parts)))
     return records

@@ -8,10 +8,10 @@ from hspz_amd import engine as E
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
+def declared_symbols(only=None):
     names = set()
     for h in os.listdir(os.path.join(ROOT, "include")):
-        if not h.endswith(".h"):
+        if not h.endswith(".h") or (only and h != only):
             continue
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
@@ -28,7 +28,9 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_binding_export_list_matches_header():
-    assert set(E.EXPORTS) <= declared_symbols()
+    from hspz_amd import seqio
+    assert set(E.EXPORTS) | set(seqio.EXPORTS) <= declared_symbols()
+    assert set(seqio.EXPORTS) == declared_symbols("hspz_io.h")
 
 
 def test_device_count_is_callable_without_gpu():
