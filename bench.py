@@ -10,10 +10,12 @@ reverse-complemented), each chunk its own query block. HOXD70, X-drop 910,
 threshold 3000, both strands, entropy on, transitions on; max_hits is the value
 the reference derives from this card's memory (4194304 x GiB), reported.
 
-A step = one seed-and-filter pass over every chunk: encoded chunks in, ordered
-HSP lists out. Batched: one hspz_filter_batch call over all chunks (`value`).
-Plain: per chunk, swap_query + the reference-shaped per-batch pipeline
-(hspz_run_pass), reported beside it. Both produce the same lists (checked).
+A step = one seed-and-filter pass over every chunk: raw query blocks in (the
+bytes a FASTA record packs to), ordered HSP lists out; the reverse complement
+and the encoding of both strands run on the device inside the step. Batched:
+one hspz_filter_batch call over all chunks (`value`). Plain: per chunk,
+hspz_swap_query_raw + the reference-shaped per-batch pipeline (hspz_run_pass),
+reported beside it. Both produce the same lists (checked).
 Target generation and the device build of the resident index are setup, outside
 the timed region (the index build time is reported).
 
@@ -23,7 +25,10 @@ cell; every measured cell is reported under `autotune`.
 
 Prints one JSON line (N = 1). `roofline` covers the fused expansion + score
 gate kernel of the batched path (HIP events around its launches);
-`cpu_baseline` is the C oracle on a bounded sample of the same chunks.
+`cpu_baseline` is the C oracle on a bounded sample of the same chunks. `io`
+times the host paths around the engine outside the timed region: writing the
+step's lists as .segments (directory, -D partitions, .tar.gz) and reading a
+FASTA / gzipped FASTA slice of the target.
 """
 from __future__ import annotations
 
@@ -80,6 +85,7 @@ def main():
     ap.add_argument("--tune-mbp", type=float, default=1.0,
                     help="query Mbp the gate autotuner measures on before the timed region (0 = skip)")
     ap.add_argument("--tune-reps", type=int, default=1)
+    ap.add_argument("--no-io", action="store_true", help="skip the host input/output timing")
     ap.add_argument("--gate", default="",
                     help="step_cells,block_threads,claim_seeds,grid_blocks[,order] for the batched gate "
                          "(applied after the autotuner; use with --tune-mbp 0 to fix the configuration)")
@@ -103,7 +109,8 @@ def main():
     target = synth.random_bases(0xC0FFEE05, tlen)
     traw = target.tobytes()
     log("target %d bp generated in %.1fs" % (tlen, time.time() - t0))
-    ref_enc = S.encode(traw)
+    from hspz_amd import seqio as X
+    ref_enc = X.encode(traw)
     sub = E.build_sub_mat("", 910)
 
     qwant = int(args.query_mbp * 1e6)
@@ -116,12 +123,9 @@ def main():
                 break
             pieces.append(c)
             total += len(c)
-    blocks = []
-    for i, c in enumerate(pieces):
-        buf, chrs, bl = S.pack([("q%d" % i, c)])
-        rc, _ = S.reverse_complement(buf, chrs, bl)
-        blocks.append((buf[:bl], rc))
-    enc = [(S.encode(f), S.encode(r)) for f, r in blocks]
+    # raw query blocks (one record each); the device encodes both strands
+    raws = [S.pack([("q%d" % i, c)])[0][:len(c)] for i, c in enumerate(pieces)]
+    chunks = [(r, None) for r in raws]
     query_bp = sum(len(c) for c in pieces)
     log("query: %d chunks, %d bp" % (len(pieces), query_bp))
 
@@ -139,12 +143,12 @@ def main():
         index, pos = eng.index()  # for the oracle's sample
 
     def batched():
-        return eng.filter_batch(enc)
+        return eng.filter_batch(chunks)
 
     if args.plain_only:
         t_p = time.perf_counter()
-        for f, r in enc:
-            eng.swap_query(f, r)
+        for r in raws:
+            eng.swap_query_raw(r)
             eng.run_pass()
         log("plain per-chunk only: %.3f Mbp/s" % (query_bp / 1e6 / (time.perf_counter() - t_p)))
         eng.close()
@@ -154,11 +158,11 @@ def main():
     if args.tune_mbp > 0:
         # autotuner on the leading chunks (untimed): every cell is reported
         sample, sbp = [], 0
-        for f, r in enc:
+        for r in raws:
             if sbp >= args.tune_mbp * 1e6:
                 break
-            sample.append((f, r))
-            sbp += len(f)
+            sample.append((r, None))
+            sbp += len(r)
         t1 = time.time()
         cells, best = eng.autotune(sample, reps=args.tune_reps)
         for c in cells:
@@ -207,8 +211,8 @@ def main():
     if not args.no_plain:
         t_p = time.perf_counter()
         hashes_p = []
-        for f, r in enc:
-            eng.swap_query(f, r)
+        for r in raws:
+            eng.swap_query_raw(r)
             res_p, _ = eng.run_pass()
             hashes_p.append(E.hash_pass(res_p))
         plain_s = time.perf_counter() - t_p
@@ -238,7 +242,7 @@ def main():
             o_hits += o_st["hits"]
             o_cells += o_st["cells"]
             sample_bp += len(part)
-            g_res, _ = eng.filter_batch([(S.encode(buf[:bl]), S.encode(rcb))])
+            g_res, _ = eng.filter_batch([(buf[:bl], None)])
             ok = ok and E.hash_pass(g_res[0]) == o_hash
         cells_per_hit = o_cells / max(o_hits, 1)
         parity_sample = ok
@@ -247,6 +251,10 @@ def main():
                "sample": "%d bp: the leading 20 kbp of the first chunks, each its own block, both "
                          "strands, against the full target (%d hits, %.1f s)" % (sample_bp, o_hits, o_time)}
         log("cpu baseline %.5f Mbp/s on %d threads; sample parity %s" % (cpu["value"], args.threads, ok))
+
+    io = None
+    if not args.no_io:
+        io = io_measure(res_b, pieces, traw)
 
     serving = None
     if args.serve_requests > 0:
@@ -310,6 +318,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "autotune": tune,
+        "io": io,
     }
     print(json.dumps(out), flush=True)
     eng.close()
@@ -332,26 +341,22 @@ def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rat
     for k, rate in enumerate(rates):
         pieces = synth.sample_chunks(target, 0xC0FFEE05 + 17 * k, n, 100_000, 1.535, 1_000,
                                      10_000_000, 0.02, 0.30)
-        reqs = []
-        for i, p in enumerate(pieces):
-            buf, chrs, bl = S.pack([("r%d" % i, p)])
-            rc, _ = S.reverse_complement(buf, chrs, bl)
-            reqs.append((S.encode(buf[:bl]), S.encode(rc)))
+        reqs = [S.pack([("r%d" % i, p)])[0][:len(p)] for i, p in enumerate(pieces)]  # raw blocks
         rng = np.random.default_rng(1000 + k)
         gaps = rng.exponential(1.0 / rate, n)
-        bases = sum(len(f) for f, _ in reqs)
+        bases = sum(len(r) for r in reqs)
         srv = E.Server([eng, eng2], max_batch_requests=256)
         ids = [None] * n
 
         def submitter():
             t0 = time.perf_counter()
             at = 0.0
-            for i, (f, r) in enumerate(reqs):
+            for i, r in enumerate(reqs):
                 at += gaps[i]
                 d = at - (time.perf_counter() - t0)
                 if d > 0:
                     time.sleep(d)
-                ids[i] = srv.submit(f, r)
+                ids[i] = srv.submit(r)
 
         th = threading.Thread(target=submitter)
         t_start = time.perf_counter()
@@ -370,7 +375,7 @@ def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rat
         srv.close()
         same = 0
         for i, res in enumerate(results):
-            eng.swap_query(*reqs[i])
+            eng.swap_query_raw(reqs[i])
             alone, _ = eng.run_pass()
             same += E.hash_pass(alone) == E.hash_pass(res)
         lat = np.array(lat)
@@ -390,6 +395,60 @@ def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rat
     return {"engines": 2, "arms": arms,
             "note": "offered load above the service rate saturates the loop; latency then "
                     "includes queueing"}
+
+
+def io_measure(results, pieces, traw):
+    """Host paths around the engine, timed outside the step: the step's lists
+    written as .segments files (one work unit per chunk, the target as one
+    record) into a directory, with -D partitions, and into one .tar.gz; and the
+    native reader over 100 Mbp of the target as 60-column FASTA and 20 Mbp of it
+    gzipped."""
+    import gzip
+    import shutil
+    import tempfile
+    import numpy as np
+    from hspz_amd import seqio as X
+    d = tempfile.mkdtemp(prefix="hspz_bench_io_")
+    out = {}
+    try:
+        r_chrs = [X.Chr("target", 0, len(traw))]
+        q_chrs = [[X.Chr("q%d" % i, 0, len(c))] for i, c in enumerate(pieces)]
+        n_hsps = sum(len(a) for res in results for fw, rc in res for a in (fw, rc))
+        for name, tarball, diag in (("directory", None, False), ("directory_D", None, True),
+                                    ("tar_gz", "", False)):
+            t0 = time.perf_counter()
+            em = X.Emitter(os.path.join(d, name), tarball=tarball, diagonal=diag)
+            for i, res in enumerate(results):
+                em.emit_unit(res, r_chrs, q_chrs[i], q_chrs[i], query_bin=i)
+            rep = em.finish()
+            ms = (time.perf_counter() - t0) * 1e3
+            out["emit_" + name] = {"ms": round(ms, 2), "files": rep["files"], "hsps": n_hsps,
+                                   "bytes_in": rep["bytes_in"], "bytes_out": rep["bytes_out"],
+                                   "mb_s": round(rep["bytes_in"] / 1e3 / max(ms, 1e-6), 1)}
+        for name, mbp, gz in (("read_fasta", 100, False), ("read_fasta_gz", 20, True)):
+            body = np.frombuffer(traw[:mbp * 1_000_000 // 60 * 60], dtype=np.uint8).reshape(-1, 60)
+            text = b">target\n" + np.hstack([body, np.full((len(body), 1), 10, np.uint8)]).tobytes()
+            path = os.path.join(d, name + (".fa.gz" if gz else ".fa"))
+            with open(path, "wb") as fh:
+                fh.write(gzip.compress(text, compresslevel=1) if gz else text)
+            t0 = time.perf_counter()
+            s = X.SeqSet.read(path)
+            ms = (time.perf_counter() - t0) * 1e3
+            ok = s.block_len == len(body) * 60 and s.raw[:1000] == traw[:1000]
+            out[name] = {"ms": round(ms, 1), "bases": s.block_len, "file_bytes": s.bytes_read,
+                         "mb_s": round(len(text) / 1e3 / ms, 1), "ok": bool(ok)}
+            if not gz:
+                t0 = time.perf_counter()
+                X.encode(s.raw)
+                ms = (time.perf_counter() - t0) * 1e3
+                out["encode_host"] = {"ms": round(ms, 1), "mb_s": round(s.block_len / 1e3 / ms, 1)}
+            s.close()
+        log("io: emit %s ms (dir) / %s ms (-D) / %s ms (tar.gz); FASTA read %s MB/s, gz %s MB/s"
+            % (out["emit_directory"]["ms"], out["emit_directory_D"]["ms"], out["emit_tar_gz"]["ms"],
+               out["read_fasta"]["mb_s"], out["read_fasta_gz"]["mb_s"]))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
 
 
 def _device_memory(E):

@@ -192,11 +192,22 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         }
     if (off >= 0xFFFFFFFFull - 2 * PAD) return set_error(HSPZ_E_ARG, "batch arena beyond 4 GiB");
     uint32_t arena_len = (uint32_t)off;
-    std::vector<uint8_t> arena(arena_len, 7);
+    // encoded chunks are copied into a host image of the arena; raw chunks
+    // (rc == NULL) are concatenated and encoded into it on the device
+    bool any_encoded = false;
+    std::vector<uint8_t> arena, rawcat;
+    std::vector<EncSeg> segs;
     for (size_t c = 0; c < n_chunks; c++) {
-        if (chunks[c].len) {
+        if (!chunks[c].len) continue;
+        if (chunks[c].rc) {
+            if (!any_encoded) arena.assign(arena_len, 7);
+            any_encoded = true;
             memcpy(arena.data() + blk_off[2 * c], chunks[c].fwd, chunks[c].len);
             memcpy(arena.data() + blk_off[2 * c + 1], chunks[c].rc, chunks[c].len);
+        } else {
+            encode_segments(segs, (uint32_t)rawcat.size(), (uint32_t)chunks[c].len, blk_off[2 * c],
+                            blk_off[2 * c + 1]);
+            rawcat.insert(rawcat.end(), chunks[c].fwd, chunks[c].fwd + chunks[c].len);
         }
     }
     // ---- batches in the reference's order (run.rs:278-299), per chunk
@@ -236,7 +247,15 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
 
     // ---- device buffers
     int rc;
-    if ((rc = upload_seq(h, B.arena, arena.data(), arena_len))) return rc;
+    if (any_encoded) {
+        if ((rc = upload_seq(h, B.arena, arena.data(), arena_len))) return rc;
+    } else {
+        if ((rc = reserve(h, B.arena, (size_t)arena_len + 2 * PAD))) return rc;
+        HIP_TRY(hipMemsetAsync(B.arena.p, 7, (size_t)arena_len + 2 * PAD, h->stream));
+    }
+    if ((rc = encode_on_device(h, rawcat.data(), rawcat.size(), segs, B.arena.as<uint8_t>() + PAD,
+                               B.arena.as<uint8_t>() + PAD)))
+        return rc;
     if ((rc = build_planes(h, B.arena, arena_len, B.plane, B.coarse))) return rc;
     if ((rc = reserve(h, B.desc, (size_t)nb * sizeof(BatchDesc)))) return rc;
     if ((rc = reserve(h, B.poff, (size_t)(nb + 1) * 4))) return rc;
@@ -552,7 +571,7 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
     if (s.size != h->seed_size) return set_error(HSPZ_E_SHAPE, "seed shape span %u != engine seed size %u", s.size, h->seed_size);
     if (p->wga_chunk == 0 || p->interval_size == 0) return set_error(HSPZ_E_ARG, "zero chunk or interval size");
     for (size_t c = 0; c < n_chunks; c++)
-        if (chunks[c].len && (!chunks[c].fwd || !chunks[c].rc)) return set_error(HSPZ_E_ARG, "chunk %zu has null data", c);
+        if (chunks[c].len && !chunks[c].fwd) return set_error(HSPZ_E_ARG, "chunk %zu has null data", c);
     HIP_TRY(hipSetDevice(h->device));
     if (!h->batch) h->batch = new hspz_batch_state();
     hspz_pass_stats st = {0, 0, 0, 0, 0};

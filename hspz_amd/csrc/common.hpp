@@ -17,6 +17,11 @@ struct Shape {
 
 int parse_shape(const char *spec, Shape *s);
 
+// io.cpp: the scoring alphabet of every byte (sequence.rs:66-79) and the
+// IUPAC complement of every byte (sequence.rs:80-86), 256 entries each.
+const uint8_t *encoding_table();
+const uint8_t *complement_table();
+
 // Thread-local last-error text behind hspz_last_error().
 inline char *error_buffer() {
     static thread_local char buf[512];

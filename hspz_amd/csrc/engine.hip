@@ -15,6 +15,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "kernels_batch.hpp"
+#include "encode_dev.hpp"
 
 using namespace hspz;
 
@@ -62,6 +63,9 @@ struct hspz_engine {
     // query side
     DevBuf qf, qr;
     uint32_t query_len = 0;
+    // device encoding of raw query bytes (k_encode_segments): the encode and
+    // encode-of-complement tables, the raw staging buffer, the segment list
+    DevBuf enc_tabs, rawq, encseg;
     // packed planes + coarse exception bitmaps for the fast gate (k_gate_packed)
     DevBuf ref2, refx, qf2, qfx, qr2, qrx, pairtab, slow, slow_n;
     bool fast_gate = false;
@@ -341,6 +345,22 @@ int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_seg
     return 0;
 }
 
+// Uploads raw bytes and runs k_encode_segments over `segs` into fwd / rc
+// (device pointers); the raw staging and segment buffers are the engine's.
+int encode_on_device(hspz_engine *h, const uint8_t *raw, size_t n, const std::vector<EncSeg> &segs, uint8_t *fwd,
+                     uint8_t *rc) {
+    if (segs.empty()) return 0;
+    if (int e = reserve(h, h->rawq, std::max<size_t>(n, 1))) return e;
+    if (int e = reserve(h, h->encseg, segs.size() * sizeof(EncSeg))) return e;
+    HIP_TRY(hipMemcpyAsync(h->rawq.p, raw, n, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->encseg.p, segs.data(), segs.size() * sizeof(EncSeg), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_encode_segments, dim3((uint32_t)segs.size()), dim3(256), 0, h->stream,
+                       h->rawq.as<uint8_t>(), h->encseg.as<EncSeg>(), h->enc_tabs.as<uint8_t>(), fwd, rc);
+    h->launches++;
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 }  // namespace
 
 #include "batch_host.hpp"
@@ -480,6 +500,14 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
     if (hipDeviceGetAttribute(&h->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         h->num_cus < 1)
         return fail(set_error(-1, "device attributes"));
+    {
+        uint8_t tabs[512];
+        const uint8_t *enc = encoding_table(), *comp = complement_table();
+        for (int i = 0; i < 256; i++) tabs[i] = enc[i], tabs[256 + i] = enc[comp[i]];
+        if (int rc = reserve(h, h->enc_tabs, sizeof tabs)) return fail(rc);
+        if (hipMemcpy(h->enc_tabs.p, tabs, sizeof tabs, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_error(-1, "encoding table upload failed"));
+    }
     h->seed_size = cfg->seed_size;
     h->xdrop = cfg->xdrop;
     h->hspthresh = cfg->hspthresh;
@@ -595,7 +623,8 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    DevBuf *bufs[] = {&h->ref, &h->index, &h->pos, &h->sub, &h->qf, &h->qr, &h->seeds[0], &h->seeds[1],
+    DevBuf *bufs[] = {&h->ref, &h->index, &h->pos, &h->sub, &h->qf, &h->qr, &h->enc_tabs, &h->rawq, &h->encseg,
+                      &h->seeds[0], &h->seeds[1],
                       &h->ref2, &h->refx, &h->qf2, &h->qfx, &h->qr2, &h->qrx, &h->pairtab, &h->slow, &h->slow_n, &h->clean, &h->rrec,
                       &h->hit_num, &h->tiles, &h->total, &h->anchors, &h->flags, &h->surv, &h->hsp,
                       &h->done, &h->ids2, &h->reduced, &h->valid};
@@ -639,6 +668,42 @@ extern "C" int hspz_swap_query(hspz_engine *h, const uint8_t *fwd, const uint8_t
     h->query_len = (uint32_t)len;
     h->seed_len[0] = h->seed_len[1] = 0;
     h->query_swaps++;
+    return 0;
+}
+
+extern "C" int hspz_swap_query_raw(hspz_engine *h, const uint8_t *raw, size_t len) {
+    clear_error();
+    if (!h || (len && !raw)) return set_error(HSPZ_E_ARG, "null argument");
+    if (len >= 0xFFFFFFFFull - 2 * PAD) return set_error(HSPZ_E_ARG, "query longer than u32");
+    HIP_TRY(hipSetDevice(h->device));
+    for (DevBuf *b : {&h->qf, &h->qr}) {
+        if (int e = reserve(h, *b, len + 2 * PAD)) return e;
+        HIP_TRY(hipMemsetAsync(b->p, 7, len + 2 * PAD, h->stream));
+    }
+    std::vector<EncSeg> segs;
+    encode_segments(segs, 0, (uint32_t)len, 0, 0);
+    if (int e = encode_on_device(h, raw, len, segs, h->qf.as<uint8_t>() + PAD, h->qr.as<uint8_t>() + PAD)) return e;
+    if (h->fast_gate) {
+        if (int e = build_planes(h, h->qf, (uint32_t)len, h->qf2, h->qfx)) return e;
+        if (int e = build_planes(h, h->qr, (uint32_t)len, h->qr2, h->qrx)) return e;
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->query_len = (uint32_t)len;
+    h->seed_len[0] = h->seed_len[1] = 0;
+    h->query_swaps++;
+    return 0;
+}
+
+extern "C" int hspz_engine_query(const hspz_engine *h, uint8_t *fwd, uint8_t *rc, size_t len) {
+    clear_error();
+    if (!h || (len && (!fwd || !rc))) return set_error(HSPZ_E_ARG, "null argument");
+    if (len != h->query_len) return set_error(HSPZ_E_ARG, "query holds %u bases, not %zu", h->query_len, len);
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (len) {
+        HIP_TRY(hipMemcpy(fwd, h->qf.as<uint8_t>() + PAD, len, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(rc, h->qr.as<uint8_t>() + PAD, len, hipMemcpyDeviceToHost));
+    }
     return 0;
 }
 

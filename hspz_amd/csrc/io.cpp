@@ -14,6 +14,8 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <memory>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -21,8 +23,40 @@
 #include "../../include/hspz_io.h"
 #include "common.hpp"
 
+// Growable byte buffer that leaves new bytes uninitialised (the readers write
+// every byte they append).
+struct ByteBuf {
+    uint8_t *p = nullptr;
+    size_t n = 0, cap = 0;
+    ByteBuf() = default;
+    ByteBuf(const ByteBuf &) = delete;
+    ByteBuf &operator=(const ByteBuf &) = delete;
+    ~ByteBuf() { free(p); }
+    bool reserve(size_t want) {
+        if (want <= cap) return true;
+        uint8_t *q = (uint8_t *)realloc(p, want);
+        if (!q) return false;
+        p = q;
+        cap = want;
+        return true;
+    }
+    // room for k more bytes at p + n (size grows only via commit)
+    uint8_t *room(size_t k) {
+        if (n + k > cap && !reserve(std::max(n + k, cap + cap / 2 + 4096))) throw std::bad_alloc();
+        return p + n;
+    }
+    void commit(size_t k) { n += k; }
+    void push(uint8_t b) {
+        *room(1) = b;
+        n++;
+    }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+    uint8_t *data() const { return p; }
+};
+
 struct hspz_seqset {
-    std::vector<uint8_t> buf;
+    ByteBuf buf;
     std::vector<std::string> names;
     std::vector<uint64_t> starts;
     std::vector<uint32_t> lens;
@@ -30,7 +64,6 @@ struct hspz_seqset {
     size_t block_len = 0;
     int32_t format = HSPZ_FORMAT_FASTA;
     uint64_t bytes_read = 0;
-    uint64_t digest = 0;
 };
 
 namespace hspz {
@@ -112,7 +145,7 @@ struct Packer {
     void end() {
         if (!open) return;
         s->lens.push_back((uint32_t)(s->buf.size() - s->starts.back()));  // `seq.len() as u32`
-        s->buf.push_back(SEP);
+        s->buf.push(SEP);
         open = false;
     }
 };
@@ -121,6 +154,11 @@ void finish(hspz_seqset *s, bool packed) {
     s->chrs.resize(s->names.size());
     for (size_t i = 0; i < s->names.size(); i++) s->chrs[i] = {s->names[i].c_str(), s->starts[i], s->lens[i]};
     if (packed) s->block_len = s->buf.empty() ? 0 : s->buf.size() - 1;
+}
+
+// Genome::digest (sequence.rs:289-306): FNV-1a over each record's name, start
+// (usize, little-endian) and len (u32), then the buffer, then block_len.
+uint64_t set_digest(const hspz_seqset *s) {
     uint64_t h = FNV_OFFSET;
     for (size_t i = 0; i < s->names.size(); i++) {
         h = fnv1a(s->names[i].data(), s->names[i].size(), h);
@@ -131,7 +169,7 @@ void finish(hspz_seqset *s, bool packed) {
     }
     h = fnv1a(s->buf.data(), s->buf.size(), h);
     uint64_t bl = s->block_len;
-    s->digest = fnv1a(&bl, 8, h);
+    return fnv1a(&bl, 8, h);
 }
 
 // One FASTA line (sequence.rs:395-414): '>' starts a record named by the first
@@ -147,16 +185,24 @@ inline void fasta_line(Packer &pk, const uint8_t *line, size_t n) {
         return;
     }
     if (!pk.open || !n) return;
-    std::vector<uint8_t> &buf = pk.s->buf;
-    size_t at = buf.size();
-    buf.resize(at + n);
-    uint8_t *o = buf.data() + at;
-    size_t k = 0;
+    uint8_t *o = pk.s->buf.room(n);
+    // a sequence line rarely holds whitespace: copy it whole while testing,
+    // and compact it only when the test fires
+    uint8_t ws = 0;
     for (size_t i = 0; i < n; i++) {
-        o[k] = line[i];
-        k += !ascii_ws(line[i]);
+        uint8_t c = line[i];
+        o[i] = c;
+        ws |= (uint8_t)((c == ' ') | (c == '\t') | (c == '\n') | (c == '\x0C') | (c == '\r'));
     }
-    buf.resize(at + k);
+    size_t k = n;
+    if (ws) {
+        k = 0;
+        for (size_t i = 0; i < n; i++) {
+            o[k] = line[i];
+            k += !ascii_ws(line[i]);
+        }
+    }
+    pk.s->buf.commit(k);
 }
 
 // Streams a FASTA body through fasta_line; `read` fills a block and returns
@@ -194,16 +240,17 @@ int fasta_stream(Packer &pk, Read read) {
 int read_fasta_plain(const char *path, Packer &pk) {
     FILE *f = fopen(path, "rb");
     if (!f) return set_error(HSPZ_E_IO, "%s: cannot open", path);
+    std::unique_ptr<FILE, int (*)(FILE *)> closer(f, fclose);
     if (fseek(f, 0, SEEK_END) == 0) {
         long sz = ftell(f);
-        if (sz > 0) pk.s->buf.reserve((size_t)sz + 1);
+        if (sz > 0 && !pk.s->buf.reserve((size_t)sz + 1))
+            return set_error(HSPZ_E_OOM, "%s: no memory for %ld bytes", path, sz);
         fseek(f, 0, SEEK_SET);
     }
     int rc = fasta_stream(pk, [&](uint8_t *b, size_t cap) -> long {
         size_t n = fread(b, 1, cap, f);
         return ferror(f) ? -1 : (long)n;
     });
-    fclose(f);
     return rc ? set_error(HSPZ_E_IO, "%s: read error", path) : 0;
 }
 
@@ -211,6 +258,7 @@ int read_fasta_plain(const char *path, Packer &pk) {
 int read_fasta_gz(const char *path, Packer &pk) {
     gzFile g = gzopen(path, "rb");
     if (!g) return set_error(HSPZ_E_IO, "%s: cannot open", path);
+    std::unique_ptr<gzFile_s, int (*)(gzFile)> closer(g, gzclose);
     gzbuffer(g, 1u << 20);
     int rc = fasta_stream(pk, [&](uint8_t *b, size_t cap) -> long {
         int n = gzread(g, b, (unsigned)std::min<size_t>(cap, 1u << 30));
@@ -219,7 +267,6 @@ int read_fasta_gz(const char *path, Packer &pk) {
     int err = 0;
     const char *msg = gzerror(g, &err);
     std::string m = msg ? msg : "";
-    gzclose(g);
     if (rc || (err != Z_OK && err != Z_BUF_ERROR)) return set_error(HSPZ_E_IO, "%s: gzip: %s", path, m.c_str());
     return 0;
 }
@@ -306,14 +353,10 @@ int read_2bit(const char *path, const std::vector<uint8_t> &file, Packer &pk) {
         size_t bytes = ((size_t)dna + 3) / 4;
         if (r.bad || r.at + bytes > n) return set_error(HSPZ_E_IO, "%s: %s: 2bit record truncated", path, names[i].c_str());
         pk.begin(names[i]);
-        std::vector<uint8_t> &buf = pk.s->buf;
-        size_t at = buf.size();
-        buf.resize(at + bytes * 4);
-        uint8_t *o = buf.data() + at;
+        uint8_t *o = pk.s->buf.room(bytes * 4);
         const uint8_t *dp = p + r.at;
         for (size_t j = 0; j < bytes; j++) memcpy(o + 4 * j, &quad[dp[j]], 4);
-        buf.resize(at + dna);
-        o = buf.data() + at;
+        pk.s->buf.commit(dna);
         for (size_t j = 0; j < ns.size(); j++) {
             uint64_t a = std::min<uint64_t>(ns[j], dna), b = std::min<uint64_t>((uint64_t)ns[j] + nl[j], dna);
             memset(o + a, 'N', b - a);
@@ -375,6 +418,9 @@ struct Encoding {
 };
 const Encoding ENCODING;
 
+const uint8_t *encoding_table() { return ENCODING.t; }
+const uint8_t *complement_table() { return COMPLEMENT.t; }
+
 template <class F>
 void parallel_ranges(size_t n, int threads, F f) {
     unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -393,8 +439,7 @@ void parallel_ranges(size_t n, int threads, F f) {
 
 using namespace hspz;
 
-extern "C" int hspz_seqset_read(const char *path, const char *prefix, hspz_seqset **out) {
-    clear_error();
+static int seqset_read(const char *path, const char *prefix, hspz_seqset **out) {
     if (!path || !out) return set_error(HSPZ_E_ARG, "null argument");
     *out = nullptr;
     uint8_t head[4] = {0, 0, 0, 0};
@@ -405,8 +450,8 @@ extern "C" int hspz_seqset_read(const char *path, const char *prefix, hspz_seqse
         nh = fread(head, 1, 4, f);
         fclose(f);
     }
-    hspz_seqset *s = new hspz_seqset;
-    Packer pk{s, prefix ? prefix : ""};
+    std::unique_ptr<hspz_seqset> s(new hspz_seqset);
+    Packer pk{s.get(), prefix ? prefix : ""};
     uint32_t le = (uint32_t)head[0] | (uint32_t)head[1] << 8 | (uint32_t)head[2] << 16 | (uint32_t)head[3] << 24;
     int rc;
     if (nh >= 4 && (le == 0x1A412743u || __builtin_bswap32(le) == 0x1A412743u)) {
@@ -421,10 +466,7 @@ extern "C" int hspz_seqset_read(const char *path, const char *prefix, hspz_seqse
         s->format = HSPZ_FORMAT_FASTA;
         rc = read_fasta_plain(path, pk);
     }
-    if (rc) {
-        delete s;
-        return rc;
-    }
+    if (rc) return rc;
     FILE *f = fopen(path, "rb");
     if (f) {
         fseek(f, 0, SEEK_END);
@@ -432,36 +474,38 @@ extern "C" int hspz_seqset_read(const char *path, const char *prefix, hspz_seqse
         s->bytes_read = sz > 0 ? (uint64_t)sz : 0;
         fclose(f);
     }
-    finish(s, true);
-    *out = s;
+    finish(s.get(), true);
+    *out = s.release();
     return 0;
 }
 
-extern "C" int hspz_seqset_pack(size_t n, const char *const *names, const uint8_t *const *seqs, const size_t *lens,
-                                const char *prefix, hspz_seqset **out) {
-    clear_error();
+static int seqset_pack(size_t n, const char *const *names, const uint8_t *const *seqs, const size_t *lens,
+                       const char *prefix, hspz_seqset **out) {
     if (!out || (n && (!names || !seqs || !lens))) return set_error(HSPZ_E_ARG, "null argument");
-    hspz_seqset *s = new hspz_seqset;
-    Packer pk{s, prefix ? prefix : ""};
+    std::unique_ptr<hspz_seqset> s(new hspz_seqset);
+    Packer pk{s.get(), prefix ? prefix : ""};
     size_t total = 0;
     for (size_t i = 0; i < n; i++) total += lens[i] + 1;
-    s->buf.reserve(total);
+    if (!s->buf.reserve(total)) return set_error(HSPZ_E_OOM, "no memory for %zu bytes", total);
     for (size_t i = 0; i < n; i++) {
         pk.begin(names[i] ? names[i] : "");
-        if (lens[i]) s->buf.insert(s->buf.end(), seqs[i], seqs[i] + lens[i]);
+        if (lens[i]) {
+            memcpy(s->buf.room(lens[i]), seqs[i], lens[i]);
+            s->buf.commit(lens[i]);
+        }
         pk.end();
     }
-    finish(s, true);
-    *out = s;
+    finish(s.get(), true);
+    *out = s.release();
     return 0;
 }
 
-extern "C" int hspz_seqset_reverse_complement(const hspz_seqset *s, hspz_seqset **out) {
-    clear_error();
+static int seqset_rc(const hspz_seqset *s, hspz_seqset **out) {
     if (!s || !out) return set_error(HSPZ_E_ARG, "null argument");
-    hspz_seqset *r = new hspz_seqset;
+    std::unique_ptr<hspz_seqset> r(new hspz_seqset);
     const size_t n = s->block_len;
-    r->buf.resize(n);
+    if (!r->buf.reserve(std::max<size_t>(n, 1))) return set_error(HSPZ_E_OOM, "no memory for %zu bytes", n);
+    r->buf.commit(n);
     const uint8_t *src = s->buf.data();
     uint8_t *dst = r->buf.data();
     parallel_ranges(n, 0, [&](size_t a, size_t b) {
@@ -475,9 +519,33 @@ extern "C" int hspz_seqset_reverse_complement(const hspz_seqset *s, hspz_seqset 
     r->block_len = n;
     r->format = s->format;
     r->bytes_read = s->bytes_read;
-    finish(r, false);
-    *out = r;
+    finish(r.get(), false);
+    *out = r.release();
     return 0;
+}
+
+// No exception crosses the ABI: an allocation failure becomes HSPZ_E_OOM.
+template <class F>
+static int guarded(F f) {
+    clear_error();
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return set_error(HSPZ_E_OOM, "host allocation failed");
+    }
+}
+
+extern "C" int hspz_seqset_read(const char *path, const char *prefix, hspz_seqset **out) {
+    return guarded([&] { return seqset_read(path, prefix, out); });
+}
+
+extern "C" int hspz_seqset_pack(size_t n, const char *const *names, const uint8_t *const *seqs, const size_t *lens,
+                                const char *prefix, hspz_seqset **out) {
+    return guarded([&] { return seqset_pack(n, names, seqs, lens, prefix, out); });
+}
+
+extern "C" int hspz_seqset_reverse_complement(const hspz_seqset *s, hspz_seqset **out) {
+    return guarded([&] { return seqset_rc(s, out); });
 }
 
 extern "C" int hspz_seqset_get(const hspz_seqset *s, hspz_seqset_info *info) {
@@ -490,9 +558,10 @@ extern "C" int hspz_seqset_get(const hspz_seqset *s, hspz_seqset_info *info) {
     info->n_chrs = s->chrs.size();
     info->format = s->format;
     info->bytes_read = s->bytes_read;
-    info->digest = s->digest;
     return 0;
 }
+
+extern "C" uint64_t hspz_seqset_digest(const hspz_seqset *s) { return s ? set_digest(s) : 0; }
 
 extern "C" void hspz_seqset_free(hspz_seqset *s) { delete s; }
 

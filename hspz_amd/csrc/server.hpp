@@ -24,6 +24,7 @@ namespace {
 struct ServerReq {
     uint64_t id;
     std::vector<uint8_t> fwd, rc;
+    bool raw;  // fwd holds raw bytes; the device encodes both strands
     uint64_t footprint;
     double t_submit;
 };
@@ -113,7 +114,7 @@ void server_worker(hspz_server *s, size_t w) {
         double t_start = s->now();
         std::vector<hspz_query_chunk> chunks(batch.size());
         for (size_t i = 0; i < batch.size(); i++)
-            chunks[i] = {batch[i].fwd.data(), batch[i].rc.data(), batch[i].fwd.size()};
+            chunks[i] = {batch[i].fwd.data(), batch[i].raw ? nullptr : batch[i].rc.data(), batch[i].fwd.size()};
         hspz_batch_output out;
         hspz_pass_stats st;
         int rc = hspz_filter_batch(h, &s->params, chunks.data(), chunks.size(), &out, &st);
@@ -191,7 +192,7 @@ extern "C" int hspz_server_new(hspz_engine **engines, size_t n_engines, const hs
 extern "C" int hspz_server_submit(hspz_server *s, const uint8_t *fwd, const uint8_t *rc, size_t len,
                                   uint64_t *request_id) {
     clear_error();
-    if (!s || !request_id || (len && (!fwd || !rc))) return set_error(HSPZ_E_ARG, "null argument");
+    if (!s || !request_id || (len && !fwd)) return set_error(HSPZ_E_ARG, "null argument");
     uint64_t fp = request_footprint(len);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stopping) return set_error(HSPZ_E_ARG, "server is shutting down");
@@ -203,7 +204,8 @@ extern "C" int hspz_server_submit(hspz_server *s, const uint8_t *fwd, const uint
     ServerReq r;
     r.id = s->next_id++;
     r.fwd.assign(fwd, fwd + len);
-    r.rc.assign(rc, rc + len);
+    r.raw = rc == nullptr;
+    if (rc) r.rc.assign(rc, rc + len);
     r.footprint = fp;
     r.t_submit = s->now();
     *request_id = r.id;

@@ -39,6 +39,7 @@ EXPORTS = [
     "hspz_batch_output_free", "hspz_server_new", "hspz_server_submit", "hspz_server_wait",
     "hspz_request_result_free", "hspz_server_get_stats", "hspz_server_free", "hspz_set_flag",
     "hspz_get_flag", "hspz_autotune_batch", "hspz_engine_new_device_index", "hspz_engine_index",
+    "hspz_swap_query_raw", "hspz_engine_query",
 ]
 
 
@@ -135,6 +136,8 @@ def lib():
         L.hspz_engine_index.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.hspz_engine_free.restype = None
         L.hspz_swap_query.argtypes = [vp, vp, vp, sz]
+        L.hspz_swap_query_raw.argtypes = [vp, vp, sz]
+        L.hspz_engine_query.argtypes = [vp, vp, vp, sz]
         L.hspz_upload_seeds.argtypes = [vp, C.c_int, vp, sz]
         L.hspz_generate_seeds.argtypes = [vp, C.c_int, C.c_int, u32, u32, vp, u32, C.c_int,
                                           C.POINTER(u32)]
@@ -312,6 +315,21 @@ class Engine:
         if len(f) != len(r):
             raise ValueError("forward and reverse-complement blocks differ in length")
         _check(lib().hspz_swap_query(self._h, _ptr(f), _ptr(r), len(f)))
+        self._query_len = len(f)
+
+    def swap_query_raw(self, raw):
+        """hspz_swap_query_raw: the raw query block; both strands are encoded
+        on the device."""
+        a = _u8(raw)
+        _check(lib().hspz_swap_query_raw(self._h, _ptr(a), len(a)))
+        self._query_len = len(a)
+
+    def query(self):
+        """The engine's encoded (fwd, rc) query strands, copied back."""
+        n = int(getattr(self, "_query_len", 0))
+        f, r = np.empty(n, dtype=np.uint8), np.empty(n, dtype=np.uint8)
+        _check(lib().hspz_engine_query(self._h, _ptr(f), _ptr(r), n))
+        return f, r
 
     def upload_seeds(self, slot: int, seeds: np.ndarray):
         s = np.ascontiguousarray(seeds, dtype=np.uint64)
@@ -427,16 +445,27 @@ class Engine:
         return res, st.as_dict()
 
 
+def _u8(x) -> np.ndarray:
+    if isinstance(x, (bytes, bytearray, memoryview)):
+        return np.frombuffer(bytes(x), dtype=np.uint8)
+    return np.ascontiguousarray(x, dtype=np.uint8)
+
+
 def _chunk_array(chunks):
-    """QueryChunk[] over [(fwd_enc, rc_enc)], plus the arrays it points into."""
+    """QueryChunk[] over [(fwd_enc, rc_enc)] or [(raw, None)] (the device then
+    encodes both strands), plus the arrays it points into."""
     keep = []
     arr = (QueryChunk * max(len(chunks), 1))()
     for i, (f, r) in enumerate(chunks):
-        f = np.ascontiguousarray(f, dtype=np.uint8)
-        r = np.ascontiguousarray(r, dtype=np.uint8)
+        f = _u8(f)
+        keep.append(f)
+        if r is None:
+            arr[i] = QueryChunk(_ptr(f).value, None, len(f))
+            continue
+        r = _u8(r)
         if len(f) != len(r):
             raise ValueError("chunk %d: forward and reverse-complement lengths differ" % i)
-        keep += [f, r]
+        keep.append(r)
         arr[i] = QueryChunk(_ptr(f).value, _ptr(r).value, len(f))
     return arr, keep
 
@@ -459,11 +488,13 @@ class Server:
         _check(lib().hspz_server_new(arr, len(self._engines), C.byref(cfg), C.byref(h)))
         self._h = h
 
-    def submit(self, fwd_enc: np.ndarray, rc_enc: np.ndarray) -> int:
-        f = np.ascontiguousarray(fwd_enc, dtype=np.uint8)
-        r = np.ascontiguousarray(rc_enc, dtype=np.uint8)
+    def submit(self, fwd_enc, rc_enc=None) -> int:
+        """(fwd_enc, rc_enc) encoded strands, or (raw, None): the raw block."""
+        f = _u8(fwd_enc)
+        r = None if rc_enc is None else _u8(rc_enc)
         rid = C.c_uint64()
-        _check(lib().hspz_server_submit(self._h, _ptr(f), _ptr(r), len(f), C.byref(rid)))
+        _check(lib().hspz_server_submit(self._h, _ptr(f), None if r is None else _ptr(r), len(f),
+                                        C.byref(rid)))
         return int(rid.value)
 
     def wait(self, rid: int):

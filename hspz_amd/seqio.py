@@ -28,7 +28,7 @@ REC_DTYPE = np.dtype([("r_chr", "<u4"), ("q_chr", "<u4"), ("r_start", "<u8"), ("
 # Every symbol include/hspz_io.h declares (checked by tests/test_abi.py).
 EXPORTS = [
     "hspz_seqset_read", "hspz_seqset_pack", "hspz_seqset_reverse_complement", "hspz_seqset_get",
-    "hspz_seqset_free", "hspz_encode", "hspz_intervals", "hspz_records", "hspz_render_records",
+    "hspz_seqset_free", "hspz_seqset_digest", "hspz_encode", "hspz_intervals", "hspz_records", "hspz_render_records",
     "hspz_render_segments", "hspz_partition_chunk_size", "hspz_partition_split", "hspz_hash_pass",
     "hspz_hash_segments", "hspz_emitter_new", "hspz_emit_unit", "hspz_emitter_finish",
 ]
@@ -41,7 +41,7 @@ class ChrT(C.Structure):
 class SeqSetInfo(C.Structure):
     _fields_ = [("buf", C.c_void_p), ("buf_len", C.c_size_t), ("block_len", C.c_size_t),
                 ("chrs", C.POINTER(ChrT)), ("n_chrs", C.c_size_t), ("format", C.c_int32),
-                ("bytes_read", C.c_uint64), ("digest", C.c_uint64)]
+                ("bytes_read", C.c_uint64)]
 
 
 class OutputReport(C.Structure):
@@ -67,6 +67,8 @@ def lib():
         L.hspz_seqset_reverse_complement.argtypes = [vp, P(vp)]
         L.hspz_seqset_get.argtypes = [vp, P(SeqSetInfo)]
         L.hspz_seqset_free.argtypes = [vp]
+        L.hspz_seqset_digest.argtypes = [vp]
+        L.hspz_seqset_digest.restype = u64
         L.hspz_encode.argtypes = [vp, sz, C.c_int, vp]
         L.hspz_intervals.argtypes = [sz, u32, u32, vp, sz, P(sz)]
         L.hspz_records.argtypes = [vp, sz, vp, sz, vp, sz, C.c_char, vp]
@@ -128,13 +130,25 @@ class SeqSet:
         self._h = handle
         info = SeqSetInfo()
         _check(lib().hspz_seqset_get(self._h, C.byref(info)))
-        self.buf = C.string_at(info.buf, info.buf_len) if info.buf_len else b""
+        self._info = info
+        self._buf = None
         self.block_len = int(info.block_len)
         self.chrs = [Chr(info.chrs[i].name.decode("utf-8"), int(info.chrs[i].start), int(info.chrs[i].len))
                      for i in range(info.n_chrs)]
         self.format = FORMAT[int(info.format)]
         self.bytes_read = int(info.bytes_read)
-        self.digest = int(info.digest)
+
+    @property
+    def buf(self) -> bytes:
+        """The packed bytes (copied out of the library on first use)."""
+        if self._buf is None:
+            i = self._info
+            self._buf = C.string_at(i.buf, i.buf_len) if i.buf_len else b""
+        return self._buf
+
+    @property
+    def digest(self) -> int:
+        return int(lib().hspz_seqset_digest(self._h))
 
     @classmethod
     def read(cls, path: str, prefix: str = "") -> "SeqSet":

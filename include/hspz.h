@@ -91,6 +91,16 @@ void hspz_engine_free(hspz_engine *h);
 /* Engine::swap_query (gpu/mod.rs:1142-1148): encoded forward and reverse-
  * complement query blocks of equal length; copied, no ownership transfer. */
 int hspz_swap_query(hspz_engine *h, const uint8_t *fwd, const uint8_t *rc, size_t len);
+/* Engine::swap_query from the raw query block (records joined by '&', as
+ * pack makes it): the reverse complement and the encoding of both strands
+ * (the caller's per-bin host step, run.rs:92-97, sequence.rs:212-230 and
+ * :66-79) run on the device from one upload of the raw bytes. Same result as
+ * hspz_swap_query(encode(raw), encode(reverse_complement(raw))). */
+int hspz_swap_query_raw(hspz_engine *h, const uint8_t *raw, size_t len);
+/* Copies the engine's encoded query strands back to the host (len must be the
+ * swapped length). No counterpart in the reference; it lets a caller check
+ * the device encoding. */
+int hspz_engine_query(const hspz_engine *h, uint8_t *fwd, uint8_t *rc, size_t len);
 
 /* Engine::upload_seeds (gpu/mod.rs:1246-1309): slot 0 or 1; seeds are
  * (kmer << 32) | query_position in seed-stream order (seed.rs:385-398).
@@ -221,7 +231,9 @@ int hspz_run_pass(hspz_engine *h, const hspz_pass_params *p, hspz_segment_pair *
 /* ---- Batched path ------------------------------------------------------- */
 
 /* One query block: encoded forward and reverse-complement bytes (records
- * joined by '&', as Engine::swap_query takes them). */
+ * joined by '&', as Engine::swap_query takes them), or with rc = NULL the raw
+ * forward block, whose two strands are then encoded on the device as
+ * hspz_swap_query_raw does. */
 typedef struct hspz_query_chunk {
     const uint8_t *fwd;
     const uint8_t *rc;
@@ -311,7 +323,8 @@ typedef struct hspz_server_stats {
 
 int hspz_server_new(hspz_engine **engines, size_t n_engines, const hspz_server_config *cfg,
                     hspz_server **out);
-/* Queues one request (encoded fwd/rc block, copied). Refuses with HSPZ_E_OOM a
+/* Queues one request (encoded fwd/rc block, or with rc = NULL the raw block;
+ * copied). Refuses with HSPZ_E_OOM a
  * request whose estimated footprint exceeds the whole budget. */
 int hspz_server_submit(hspz_server *s, const uint8_t *fwd, const uint8_t *rc, size_t len,
                        uint64_t *request_id);

@@ -51,7 +51,6 @@ typedef struct hspz_seqset_info {
     size_t n_chrs;
     int32_t format;       /* HSPZ_FORMAT_* of the file it was read from */
     uint64_t bytes_read;  /* on-disk size of that file, 0 for packed records */
-    uint64_t digest;      /* Genome::digest (sequence.rs:289-306) */
 } hspz_seqset_info;
 
 /* read_records + pack (sequence.rs:238-248, :185-205): FASTA (the reference's
@@ -67,6 +66,10 @@ int hspz_seqset_pack(size_t n, const char *const *names, const uint8_t *const *s
  * reverse order. */
 int hspz_seqset_reverse_complement(const hspz_seqset *s, hspz_seqset **out);
 int hspz_seqset_get(const hspz_seqset *s, hspz_seqset_info *info);
+/* Genome::digest (sequence.rs:289-306): FNV-1a over the record table (name,
+ * start as u64, len as u32, little-endian), the buffer and block_len. Equal
+ * sets from different formats have equal digests. */
+uint64_t hspz_seqset_digest(const hspz_seqset *s);
 void hspz_seqset_free(hspz_seqset *s);
 
 /* encode (sequence.rs:66-79): raw bytes to the scoring alphabet (A0 C1 G2 T3,
