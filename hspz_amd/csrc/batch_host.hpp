@@ -56,6 +56,12 @@ KGateFn kgate_kernel(int step_cells, int threads) {
                                : k_bgate_kmer<2, 512>;
 }
 
+// The lookahead gate (k_bgate_ahead) applies to merged 128-cell steps in
+// 512-thread blocks and needs the reference records.
+bool lookahead_wanted(const hspz_engine *h) {
+    return h->lookahead && h->bgate_step == 128 && h->bgate_threads == 512;
+}
+
 // The batched gate instantiation for a step width and block size.
 BGateFn bgate_kernel(int step_cells, int threads) {
     if (step_cells == 128) step_cells = 64;  // the merged-phase variant exists in k-mer-major order only
@@ -88,11 +94,12 @@ struct hspz_batch_state {
 
 namespace {
 
-// Builds the engine's reference records (k_ref_records) on first use, when
-// HSPZ_FLAG_REF_RECORDS is on and device memory leaves room for them and a
-// margin; otherwise the k-mer-major gate reads its windows from the plane.
-int ensure_ref_records(hspz_engine *h) {
-    if (!h->use_rrec || h->rrec.p || h->rrec_declined || h->pos_len == 0) return 0;
+// Builds the engine's reference records (k_ref_records) on first use, when a
+// gate wants them (HSPZ_FLAG_REF_RECORDS, or the lookahead gate) and device
+// memory leaves room for them and a margin; otherwise the k-mer-major gate
+// reads its windows from the plane.
+int ensure_ref_records(hspz_engine *h, bool want) {
+    if (!want || h->rrec.p || h->rrec_declined || h->pos_len == 0) return 0;
     size_t need = h->pos_len * 48;
     size_t f = 0, t = 0;
     HIP_TRY(hipMemGetInfo(&f, &t));
@@ -316,7 +323,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             if ((rc = reserve(h, B.sq, cap_seeds * 8 + 8))) return rc;
             if ((rc = reserve(h, B.skm, cap_seeds * 4 + 4))) return rc;
             if ((rc = reserve(h, B.qrec, cap_seeds * sizeof(QRec) + sizeof(QRec)))) return rc;
-            if ((rc = ensure_ref_records(h))) return rc;
+            if ((rc = ensure_ref_records(h, h->use_rrec || lookahead_wanted(h)))) return rc;
         }
         BlockOf bo;
         bo.seeds = B.seeds.as<uint64_t>();
@@ -380,6 +387,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         // fused expansion + packed gate, then the exact gate for deferred hits
         BGateFn gate_fn = bgate_kernel(h->bgate_step, h->bgate_threads);
         KGateFn kgate_fn = kgate_kernel(h->bgate_step, h->bgate_threads);
+        if (kmer_order && lookahead_wanted(h) && h->rrec.p) kgate_fn = k_bgate_ahead;
         int grid = h->bgate_blocks;
         if (grid == 0) {  // resident grid: occupancy of this instantiation x CUs
             int per_cu = 0;
@@ -446,7 +454,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                (uint64_t)grid | (uint64_t)h->bgate_claim << 32, (uint64_t)p->transitions, nt, kt,
                                cap_tiles, (uint64_t)h->bgate_step | (uint64_t)h->bgate_threads << 32,
                                (uint64_t)(uintptr_t)h->sub.p, s.kmer_size, (uint64_t)(uintptr_t)h->ref.p,
-                               (uint64_t)kmer_order, ntk};
+                               (uint64_t)kmer_order, ntk, (uint64_t)(uintptr_t)(const void *)kgate_fn};
             memcpy(key.vals, vals, sizeof vals);
             memcpy(key.shape, s.pos, sizeof s.pos);
             key.shape[32] = s.size;

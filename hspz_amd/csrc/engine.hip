@@ -96,6 +96,7 @@ struct hspz_engine {
     DevBuf clean;                           // pos_table-aligned clean-window bits (k-mer-major gate)
     DevBuf rrec;                            // pos_table-ordered reference records (k_ref_records)
     bool use_rrec = false, rrec_declined = false;
+    bool lookahead = true;                  // k_bgate_ahead for merged steps (needs rrec)
     int bgate_grid_used = 0;                // grid of the last batched launch
     int num_cus = 0;
     double last_gate_ms = 0, last_post_ms = 0;  // event times of the last batched pass
@@ -819,6 +820,7 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
         h->batch_order = value;
         break;
     case HSPZ_FLAG_BATCH_DEVICE_DEDUP: h->device_dedup = value != 0; break;
+    case HSPZ_FLAG_BATCH_LOOKAHEAD: h->lookahead = value != 0; break;
     case HSPZ_FLAG_REF_RECORDS:
         h->use_rrec = value != 0;
         if (!h->use_rrec && h->rrec.p) {
@@ -846,6 +848,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_ORDER_USED: *value = h->batch_order_used; break;
     case HSPZ_FLAG_BATCH_DEVICE_DEDUP: *value = h->device_dedup ? 1 : 0; break;
     case HSPZ_FLAG_REF_RECORDS: *value = h->rrec.p ? 2 : h->use_rrec ? 1 : 0; break;
+    case HSPZ_FLAG_BATCH_LOOKAHEAD: *value = h->lookahead ? 1 : 0; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;

@@ -627,6 +627,30 @@ struct KmerSource {
             need = __ballot(!has);
         }
     }
+    // Hands each lane with `want` the next item of the current group(s): its
+    // pos_table entry e and its group's query-record range [s0, s1). Returns
+    // whether this lane got one (false for every lane once the work is drained).
+    __device__ __forceinline__ bool claim_item(bool want, uint32_t &e, uint32_t &s0, uint32_t &s1) {
+        bool got = false;
+        uint64_t need = __ballot(want);
+        while (need) {
+            if (j >= n) {
+                if (!next_group()) break;
+                continue;
+            }
+            uint32_t avail = n - j;
+            uint32_t rank = mbcnt(need);
+            if (want && !got && rank < avail) {
+                e = b0 + j + rank;
+                s0 = g0;
+                s1 = g1;
+                got = true;
+            }
+            j += min((uint32_t)__popcll(need), avail);
+            need = __ballot(want && !got);
+        }
+        return got;
+    }
     __device__ __forceinline__ void emit(bool done, bool pass, uint32_t tag, uint32_t R, uint32_t Q) {
         SeedSource::append(done && pass, surv, surv_n, cap_surv, err, R, Q, tag);
     }
@@ -925,6 +949,201 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
             cont = false;
         }
     }
+}
+
+// kmer_gate_merged with every global load issued one iteration before its
+// data is used, so a wave computes one hit while the loads for its next hit
+// and next item are in flight. Each lane holds, besides its current item and
+// hit: the next item (its reference record, R, clean bit and query-record
+// range), the item claimed in the last iteration (loads in flight), and the
+// query record of its next fresh hit (prefetched). The load instructions of
+// an iteration are issued unconditionally (idle lanes read entry 0), so the
+// counter wait at the top of the next iteration covers exactly them. Results
+// of a finished hit are appended at the top of the next iteration. Hits,
+// their cells and decisions are those of kmer_gate_merged; only the order in
+// which lanes take items differs. Reads the reference records (k_ref_records).
+__device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__restrict__ rrec,
+                                                const QRec *__restrict__ recs, const uint8_t *s_map, int32_t xdrop,
+                                                int32_t thresh) {
+    const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
+    // current item and hit
+    bool cur = false, rok = false, has = false, cont = false, qok = false;
+    uint32_t R = 0, s_cur = 0, s_end = 0, Q = 0, tag = 0;
+    int32_t Mr = 0, el_keep = 0, cl_keep = 0;
+    uint32_t A0[G2_RIGHT_WORDS], LA0[G2_LEFT_WORDS], B[G2_RIGHT_WORDS], LB[G2_LEFT_WORDS];
+    // next item, and the item whose loads are in flight
+    bool nxt = false, nrok = false, fly = false;
+    uint32_t nR = 0, ns0 = 0, ns1 = 0, fsh = 0, fs0 = 0, fs1 = 0;
+    uint32_t NA[G2_RIGHT_WORDS + G2_LEFT_WORDS];
+    uint4 T0 = make_uint4(0, 0, 0, 0), T1 = T0, T2 = T0;
+    uint32_t tpos = 0, tclean = 0;
+    // prefetched query record of the next fresh hit
+    bool pv = false;
+    uint32_t pidx = 0;
+    uint4 P0 = T0, P1 = T0, P2 = T0, P3 = T0;
+    // the hit finished in the previous iteration, appended at the top of this one
+    bool e_pass = false, e_defer = false;
+    uint32_t eR = 0, eQ = 0, etag = 0;
+#pragma unroll
+    for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) A0[j] = B[j] = 0;
+#pragma unroll
+    for (int j = 0; j < (int)G2_LEFT_WORDS; j++) LA0[j] = LB[j] = 0;
+#pragma unroll
+    for (int j = 0; j < (int)(G2_RIGHT_WORDS + G2_LEFT_WORDS); j++) NA[j] = 0;
+
+    for (;;) {
+        // The record's pad word is read nowhere; keeping it live to here stops
+        // the compiler from reusing its register while the load is in flight
+        // (which would stall the hit's cells on that load).
+        asm volatile("" ::"v"(P3.w));
+        // 1. results of the previous iteration's loads and hit
+        if (fly) {
+            NA[0] = T0.x, NA[1] = T0.y, NA[2] = T0.z, NA[3] = T0.w;
+            NA[4] = T1.x, NA[5] = T1.y, NA[6] = T1.z, NA[7] = T1.w;
+            NA[8] = T2.x, NA[9] = T2.y, NA[10] = T2.z, NA[11] = T2.w;
+            nR = tpos + src.seed_size;
+            nrok = ((tclean >> fsh) & 1u) != 0;
+            ns0 = fs0;
+            ns1 = fs1;
+            nxt = true;
+            fly = false;
+        }
+        src.emit(true, e_pass, etag, eR, eQ);
+        src.defer(e_defer, etag, eR, eQ);
+        e_pass = e_defer = false;
+
+        // 2. a lane without a hit starts its item's next one, promoting the
+        //    next item when the current one is used up
+        bool defer = false;
+        if (!has) {
+            if (cur && s_cur >= s_end) cur = false;
+            if (!cur && nxt) {
+#pragma unroll
+                for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) A0[k] = NA[k];
+#pragma unroll
+                for (int k = 0; k < (int)G2_LEFT_WORDS; k++) LA0[k] = NA[G2_RIGHT_WORDS + k];
+                R = nR;
+                rok = nrok;
+                s_cur = ns0;
+                s_end = ns1;
+                cur = true;
+                nxt = false;
+            }
+            if (cur) {
+                uint4 v0 = P0, v1 = P1, v2 = P2, v3 = P3;
+                if (!(pv && pidx == s_cur)) {  // not prefetched (kept for safety; the schedule prefetches it)
+                    const uint4 *rec = (const uint4 *)(recs + s_cur);
+                    v0 = rec[0], v1 = rec[1], v2 = rec[2], v3 = rec[3];
+                }
+                B[0] = v0.x, B[1] = v0.y, B[2] = v0.z, B[3] = v0.w;
+                LB[0] = v1.x, LB[1] = v1.y, LB[2] = v1.z, LB[3] = v1.w;
+                LB[4] = v2.x, LB[5] = v2.y, LB[6] = v2.z, LB[7] = v2.w;
+                Q = v3.x;
+                tag = v3.y;
+                qok = v3.z != 0;
+                s_cur++;
+                has = true;
+                cont = false;
+                if (!(rok && qok)) defer = true;
+            }
+        }
+
+        // 3. lanes without a next item claim one
+        uint32_t ce = 0, c0 = 0, c1 = 0;
+        bool got = src.claim_item(!nxt && !fly, ce, c0, c1);
+        bool pending = cur && s_cur < s_end;
+        if (__ballot(has || pending || nxt || got || e_pass || e_defer) == 0) break;
+
+        // 4. this iteration's loads, for every lane: the claimed item's record,
+        //    pos entry and clean word; the query record of the next fresh hit
+        {
+            uint32_t e = got ? ce : 0u;
+            const uint4 *rr = rrec + 3 * (size_t)e;
+            T0 = rr[0], T1 = rr[1], T2 = rr[2];
+            tpos = src.pos[e];
+            tclean = src.clean[e >> 5];
+            fsh = e & 31;
+            fs0 = c0;
+            fs1 = c1;
+            fly = got;
+            uint32_t idx = pending ? s_cur : nxt ? ns0 : c0;
+            pv = pending || nxt || got;
+            pidx = idx;
+            const uint4 *rec = (const uint4 *)(recs + (pv ? idx : 0u));
+            P0 = rec[0], P1 = rec[1], P2 = rec[2], P3 = rec[3];
+        }
+
+        // 5. the current hit: right cells 0..63 and left cells 0..63, or left
+        //    cells 64..127 for a continuing hit (as kmer_gate_merged)
+        bool done = false, pass = false;
+        if (has && !defer) {
+            int32_t er = xdrop, cr = 0, eminr = xdrop;
+            map_stretch<4>(s_map, A0, B, false, off_fwd, er, cr, eminr);
+            int32_t el = cont ? el_keep : xdrop, cl = cont ? cl_keep : 0, eminl = el;
+            map_stretch<8>(s_map, LA0, LB, cont, off_rev, el, cl, eminl);
+            if (!cont) {
+                if (eminr >= 0) defer = true;
+                else Mr = cr + (xdrop - er);
+            }
+            if (!defer) {
+                if (eminl < 0) {
+                    done = true;
+                    pass = gate_keep(Mr + cl + (xdrop - el), thresh);
+                } else if (cont) {
+                    defer = true;
+                } else {
+                    cont = true;
+                    el_keep = el;
+                    cl_keep = cl;
+                }
+            }
+        }
+        if (has && (done || defer)) {
+            e_pass = done && pass;
+            e_defer = defer;
+            eR = R;
+            eQ = Q;
+            etag = tag;
+            has = false;
+            cont = false;
+        }
+    }
+}
+
+// The lookahead k-mer-major gate (kmer_gate_ahead); same arguments as
+// k_bgate_kmer, rrec required. Four waves per SIMD (512-thread blocks, two per
+// CU) leave it the 128 VGPRs its in-flight state needs.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_bgate_ahead(
+    const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop,
+    int32_t thresh, const QRec *__restrict__ recs, const uint32_t *__restrict__ skm,
+    const uint32_t *__restrict__ goff, uint32_t nk,
+    const uint32_t *__restrict__ index, const uint32_t *__restrict__ pos, const uint32_t *__restrict__ clean,
+    const uint4 *__restrict__ rrec, uint32_t seed_size, uint32_t claim, unsigned long long *__restrict__ work,
+    SurvRec *__restrict__ surv,
+    uint32_t *__restrict__ surv_n, uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n,
+    uint32_t cap_slow, uint32_t *__restrict__ err) {
+    __shared__ uint32_t s_map[G2_MAP_WORDS];
+    load_map_table(s_map, pairtab, 512);
+    KmerSource src;
+    src.skm = skm;
+    src.goff = goff;
+    src.index = index;
+    src.pos = pos;
+    src.clean = clean;
+    src.work = work;
+    src.total = __builtin_amdgcn_readfirstlane(goff[nk]);
+    src.claim = claim;
+    src.seed_size = seed_size;
+    src.surv = surv;
+    src.surv_n = surv_n;
+    src.cap_surv = cap_surv;
+    src.slow = slow;
+    src.slow_n = slow_n;
+    src.cap_slow = cap_slow;
+    src.err = err;
+    src.init();
+    (void)rplane;
+    kmer_gate_ahead(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
 // The batched gate in k-mer-major order (same launch shape as k_bgate_packed).

@@ -27,7 +27,7 @@ STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 
         "index_build_ns": 16, "pos_len": 17, "index_len": 18}
 FLAG = {"batch_graph": 0, "batch_gate_blocks": 1, "batch_step_cells": 2, "batch_block_threads": 3,
         "batch_claim_seeds": 4, "batch_order": 5, "batch_order_used": 6, "batch_device_dedup": 7,
-        "ref_records": 8}
+        "ref_records": 8, "batch_lookahead": 9}
 
 # Every symbol include/hspz.h declares (checked by tests/test_abi.py).
 EXPORTS = [

@@ -166,6 +166,11 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           records in pos_table order, built on first use when device
                                           memory allows (48 B per indexed position); 0 (default): it reads
                                           the 2-bit plane. hspz_get_flag reports 2 once they are built */
+#define HSPZ_FLAG_BATCH_LOOKAHEAD 9    /* 1 (default): with 128-cell steps and 512-thread blocks, the
+                                          k-mer-major gate issues each hit's and each item's loads one
+                                          iteration ahead of their use; it reads the reference records
+                                          (built on first use when device memory allows, as
+                                          HSPZ_FLAG_REF_RECORDS builds them). 0: the plain merged gate */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

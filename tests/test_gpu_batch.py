@@ -145,13 +145,18 @@ def test_batch_gate_configs_match(gpu_available):
     for f, r in blocks:
         o = O.Oracle(target, f, r, sub, max_hits=16711680, index=(index, pos))
         want.append(o.run_pass(threads=16)[0])
-    variants = [(16, 1024, 1, 0, 0), (64, 512, 4096, 0, 0), (32, 1024, 7, 3, 0), (64, 1024, 1, 1, 0),
-                (16, 512, 64, 5000, 0), (64, 1024, 1, 0, 1), (32, 512, 3, 2, 1), (16, 1024, 4096, 0, 1),
-                (64, 512, 64, 7, 1), (32, 1024, 1, 5000, 1), (128, 1024, 1, 0, 1), (128, 512, 256, 3, 1),
-                (128, 1024, 64, 0, 0)]
+    # (step, threads, claim, grid, order, lookahead): 128-cell steps in 512-thread blocks run the
+    # lookahead gate unless it is switched off
+    variants = [(16, 1024, 1, 0, 0, 1), (64, 512, 4096, 0, 0, 1), (32, 1024, 7, 3, 0, 1), (64, 1024, 1, 1, 0, 1),
+                (16, 512, 64, 5000, 0, 1), (64, 1024, 1, 0, 1, 1), (32, 512, 3, 2, 1, 1), (16, 1024, 4096, 0, 1, 1),
+                (64, 512, 64, 7, 1, 1), (32, 1024, 1, 5000, 1, 1), (128, 1024, 1, 0, 1, 1), (128, 512, 256, 3, 1, 0),
+                (128, 1024, 64, 0, 0, 1), (128, 512, 256, 0, 1, 1), (128, 512, 1, 5000, 1, 1),
+                (128, 512, 4096, 1, 1, 1), (128, 512, 16, 7, 1, 1)]
     with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
-        for step, threads, claim, blocks_n, order in variants:
-            _say("  step %d threads %d claim %d grid %d order %d" % (step, threads, claim, blocks_n, order))
+        for step, threads, claim, blocks_n, order, ahead in variants:
+            _say("  step %d threads %d claim %d grid %d order %d lookahead %d"
+                 % (step, threads, claim, blocks_n, order, ahead))
+            eng.set_flag("batch_lookahead", ahead)
             eng.set_flag("batch_order", order)
             eng.set_flag("batch_step_cells", step)
             eng.set_flag("batch_block_threads", threads)
@@ -164,6 +169,8 @@ def test_batch_gate_configs_match(gpu_available):
                 assert eng.get_flag("batch_gate_blocks") == blocks_n
             else:
                 assert eng.get_flag("batch_gate_blocks") >= 256
+            if step == 128 and threads == 512 and order == 1 and ahead:
+                assert eng.get_flag("ref_records") == 2  # the lookahead gate's records were built
             for i, (g, w) in enumerate(zip(got, want)):
                 assert same_intervals(g, w), "chunk %d differs at step %d claim %d" % (i, step, claim)
         for name, bad in [("batch_step_cells", 24), ("batch_block_threads", 256),
