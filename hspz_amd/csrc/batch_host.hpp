@@ -25,7 +25,8 @@ struct HostBatch {
 // Device buffers of the batched path, grow-only, owned by the engine.
 struct BatchBufs {
     DevBuf arena, plane, coarse, desc, poff, blk_off, blk_len, valid, ids, voff, seeds, counts, prefix,
-        tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr, gcnt, sq, skm, qrec;
+        tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr, gcnt, sq, skm, qrec,
+        kvalo, ktmp;
 };
 
 // Device counters of one pass; CTR_WORK is the 64-bit seed-claim counter of
@@ -85,8 +86,9 @@ struct hspz_batch_state {
     uint32_t cap_surv = 1u << 22, cap_slow = 1u << 22;
     bool have_graph = false;
     GraphKey key;
-    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};       // before, the gate, after
-    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    // before the grouping sort, after it, the gate, after the gate
+    hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
     uint64_t graph_captures = 0, graph_replays = 0;
     // device dedup_and_order scratch
     DevBuf dd_key_a, dd_key_b, dd_id_a, dd_id_b, dd_keep, dd_kids, dd_tmp;
@@ -320,10 +322,20 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         if ((rc = reserve(h, h->tiles, (size_t)std::max(std::max(kt, nt), ntk) * 4))) return rc;
         if (kmer_order) {
             if ((rc = reserve(h, B.gcnt, ((size_t)nkm + 1) * 4))) return rc;
-            if ((rc = reserve(h, B.sq, cap_seeds * 8 + 8))) return rc;
+            if ((rc = reserve(h, B.sq, cap_seeds * 8 + 8))) return rc;  // sort keys and values (u32 each)
+            if ((rc = reserve(h, B.kvalo, cap_seeds * 4 + 4))) return rc;
             if ((rc = reserve(h, B.skm, cap_seeds * 4 + 4))) return rc;
             if ((rc = reserve(h, B.qrec, cap_seeds * sizeof(QRec) + sizeof(QRec)))) return rc;
             if ((rc = ensure_ref_records(h, h->use_rrec || lookahead_wanted(h)))) return rc;
+        }
+        uint32_t *kkey = kmer_order ? B.sq.as<uint32_t>() : nullptr;
+        uint32_t *kval = kmer_order ? kkey + cap_seeds : nullptr;
+        const int key_bits = 32 - __builtin_clz(nkm);  // keys are in [0, nk]
+        if (kmer_order) {  // sort temp storage before any capture
+            size_t bytes = 0;
+            HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint32_t>(),
+                                              (uint32_t)cap_seeds, 0, key_bits, h->stream));
+            if ((rc = reserve(h, B.ktmp, bytes + 256))) return rc;
         }
         BlockOf bo;
         bo.seeds = B.seeds.as<uint64_t>();
@@ -365,24 +377,32 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         hipLaunchKernelGGL(k_bchunks, dim3(ceil_div(nb, 64)), dim3(64), 0, h->stream, B.prefix.as<uint64_t>(),
                            B.voff.as<uint32_t>(), nb, per, h->max_hits, B.nchunks.as<uint32_t>(),
                            B.starts.as<uint32_t>(), B.bhits.as<uint64_t>(), ctr + CTR_ERR);
-        if (kmer_order) {  // group the seeds with hits by k-mer (counting sort)
-            hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, h->stream, B.gcnt.as<uint32_t>(), nkm + 1, 0u);
-            hipLaunchKernelGGL(k_kcount, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
+        if (kmer_order)  // keys of the grouping sort: k-mer for seeds with hits, nk for the rest
+            hipLaunchKernelGGL(k_kkeys, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
                                h->stream, B.seeds.as<uint64_t>(), B.counts.as<uint32_t>(), ctr + CTR_NVALID, per,
-                               B.gcnt.as<uint32_t>());
-            hipLaunchKernelGGL(k_tile_sum_u32, dim3(ntk), dim3(TILE_THREADS), 0, h->stream, B.gcnt.as<uint32_t>(),
-                               nkm + 1, h->tiles.as<uint32_t>());
-            hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, h->stream, h->tiles.as<uint32_t>(), ntk,
-                               ctr + CTR_GROUPED);
-            hipLaunchKernelGGL(k_tile_apply_u32, dim3(ntk), dim3(TILE_THREADS), 0, h->stream, B.gcnt.as<uint32_t>(),
-                               nkm + 1, h->tiles.as<uint32_t>());
-            hipLaunchKernelGGL(k_kscatter, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
-                               h->stream, B.seeds.as<uint64_t>(), B.counts.as<uint32_t>(), ctr + CTR_NVALID, per,
-                               B.gcnt.as<uint32_t>(), B.sq.as<uint64_t>(), B.skm.as<uint32_t>());
+                               (uint32_t)cap_seeds, nkm, kkey, kval);
+        };
+        // k-mer grouping: a stable radix sort of all seed slots by key (run
+        // eagerly between the graphs: rocPRIM issues its own fills), then the
+        // group starts and one query record per grouped seed
+        auto enqueue_sort = [&]() -> int {
+            if (!kmer_order) return 0;
+            size_t bytes = 0;
+            HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint32_t>(),
+                                              (uint32_t)cap_seeds, 0, key_bits, h->stream));
+            if (int e = reserve(h, B.ktmp, bytes + 256)) return e;
+            HIP_TRY(rocprim::radix_sort_pairs(B.ktmp.p, bytes, kkey, B.skm.as<uint32_t>(), kval,
+                                              B.kvalo.as<uint32_t>(), (uint32_t)cap_seeds, 0, key_bits, h->stream));
+            return 0;
+        };
+        auto enqueue_group = [&]() {
+            if (!kmer_order) return;
+            hipLaunchKernelGGL(k_kbounds, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
+                               h->stream, B.skm.as<uint32_t>(), (uint32_t)cap_seeds, nkm, B.gcnt.as<uint32_t>());
             hipLaunchKernelGGL(k_qrecords, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
-                               h->stream, B.sq.as<uint64_t>(), B.gcnt.as<uint32_t>() + nkm, B.plane.as<uint32_t>(),
-                               B.coarse.as<uint32_t>(), arena_len, h->seed_size, B.qrec.as<QRec>());
-        }
+                               h->stream, B.kvalo.as<uint32_t>(), B.seeds.as<uint64_t>(), B.gcnt.as<uint32_t>() + nkm,
+                               B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(), arena_len, h->seed_size,
+                               B.qrec.as<QRec>());
         };
         // fused expansion + packed gate, then the exact gate for deferred hits
         BGateFn gate_fn = bgate_kernel(h->bgate_step, h->bgate_threads);
@@ -448,7 +468,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                   B.nchunks.p, B.starts.p, B.bhits.p, B.surv.p, B.slow.p, B.mhsp.p, B.mscope.p,
                                   B.keep.p, B.kids.p, B.ohsp.p, B.oscope.p, B.ctr.p, h->tiles.p, h->ref2.p,
                                   h->refx.p, h->pairtab.p, h->index.p, h->pos.p, B.gcnt.p, B.sq.p, B.skm.p,
-                                  h->clean.p, B.qrec.p, h->rrec.p};
+                                  h->clean.p, B.qrec.p, h->rrec.p, B.kvalo.p};
             memcpy(key.ptrs, ptrs, sizeof ptrs);
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow, cap_seeds, per, h->max_hits,
                                (uint64_t)grid | (uint64_t)h->bgate_claim << 32, (uint64_t)p->transitions, nt, kt,
@@ -460,17 +480,18 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             key.shape[32] = s.size;
             if (!bs->have_graph || memcmp(&key, &bs->key, sizeof key) != 0) {
                 std::lock_guard<std::mutex> lk(capture_mutex());
-                for (int g = 0; g < 3; g++) {
+                for (int g = 0; g < 4; g++) {
                     if (bs->exec[g]) (void)hipGraphExecDestroy(bs->exec[g]);
                     if (bs->graph[g]) (void)hipGraphDestroy(bs->graph[g]);
                     bs->exec[g] = nullptr;
                     bs->graph[g] = nullptr;
                 }
                 bs->have_graph = false;
-                for (int g = 0; g < 3; g++) {
+                for (int g = 0; g < 4; g++) {
                     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
                     if (g == 0) enqueue_pre();
-                    else if (g == 1) enqueue_gate();
+                    else if (g == 1) enqueue_group();
+                    else if (g == 2) enqueue_gate();
                     else enqueue_post();
                     hipError_t ce = hipStreamEndCapture(h->stream, &bs->graph[g]);
                     if (ce != hipSuccess) return set_error(-(int)ce, "graph capture: %s", hipGetErrorString(ce));
@@ -484,13 +505,17 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             }
             // timing events sit between the graph launches, on the stream
             HIP_TRY(hipGraphLaunch(bs->exec[0], h->stream));
-            HIP_TRY(hipEventRecord(h->ev_gate0, h->stream));
+            if ((rc = enqueue_sort())) return rc;
             HIP_TRY(hipGraphLaunch(bs->exec[1], h->stream));
-            HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));
+            HIP_TRY(hipEventRecord(h->ev_gate0, h->stream));
             HIP_TRY(hipGraphLaunch(bs->exec[2], h->stream));
+            HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));
+            HIP_TRY(hipGraphLaunch(bs->exec[3], h->stream));
             HIP_TRY(hipEventRecord(h->ev_post1, h->stream));
         } else {
             enqueue_pre();
+            if ((rc = enqueue_sort())) return rc;
+            enqueue_group();
             HIP_TRY(hipEventRecord(h->ev_gate0, h->stream));
             enqueue_gate();
             HIP_TRY(hipEventRecord(h->ev_gate1, h->stream));

@@ -447,32 +447,6 @@ __global__ void k_ref_records(const uint32_t *__restrict__ pos, uint32_t n, cons
     }
 }
 
-// Group sizes: seeds with at least one hit, counted per k-mer.
-__global__ void k_kcount(const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ counts,
-                         const uint32_t *__restrict__ nvalid, uint32_t per, uint32_t *__restrict__ gcnt) {
-    const uint32_t n = *nvalid * per;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        if (counts[i]) atomicAdd(&gcnt[(uint32_t)(seeds[i] >> 32)], 1u);
-}
-
-// After an inclusive scan of gcnt, every seed with hits takes a slot of its
-// group: sq = (seed index << 32) | arena position, skm = k-mer. gcnt[k] ends
-// as the start of group k (and gcnt[nk] stays the number of grouped seeds).
-__global__ void k_kscatter(const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ counts,
-                           const uint32_t *__restrict__ nvalid, uint32_t per, uint32_t *__restrict__ gcnt,
-                           uint64_t *__restrict__ sq, uint32_t *__restrict__ skm) {
-    const uint32_t n = *nvalid * per;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (!counts[i]) continue;
-        uint64_t sd = seeds[i];
-        uint32_t k = (uint32_t)(sd >> 32);
-        uint32_t slot = atomicSub(&gcnt[k], 1u) - 1u;
-        sq[slot] = ((uint64_t)i << 32) | (uint32_t)sd;
-        skm[slot] = k;
-    }
-}
-
-
 // Query record of one grouped seed: its right and left query streams as the
 // k-mer-major gate consumes them (aligned to Q, left words byte- and
 // nibble-swapped),
@@ -484,18 +458,46 @@ struct QRec {
 };
 static_assert(sizeof(QRec) == 64, "query records are four 16-byte loads");
 
-// One record per grouped seed (after k_kscatter), so that a hit's query side
-// costs four loads instead of a window load, alignment and exception test.
-__global__ void k_qrecords(const uint64_t *__restrict__ sq, const uint32_t *__restrict__ total_p,
+// Sort keys of the k-mer grouping: every seed slot below cap gets its k-mer
+// when the seed has hits and nk otherwise (so the seeds without hits sort
+// last), and its own index as the value.
+__global__ void k_kkeys(const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ counts,
+                        const uint32_t *__restrict__ nvalid, uint32_t per, uint32_t cap, uint32_t nk,
+                        uint32_t *__restrict__ key, uint32_t *__restrict__ val) {
+    const uint32_t n = *nvalid * per;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
+        key[i] = i < n && counts[i] ? (uint32_t)(seeds[i] >> 32) : nk;
+        val[i] = i;
+    }
+}
+
+// Group starts from the sorted keys: goff[k] = the first slot whose key is at
+// least k, for k in [0, nk] (goff[nk] is the number of grouped seeds). Slot j
+// writes the k-mers in (key[j - 1], key[j]], and the last slot those after it.
+__global__ void k_kbounds(const uint32_t *__restrict__ key, uint32_t n, uint32_t nk, uint32_t *__restrict__ goff) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        uint32_t kj = key[j];
+        uint32_t lo = j ? key[j - 1] + 1 : 0;
+        for (uint32_t k = lo; k <= kj; k++) goff[k] = j;
+        if (j + 1 == n)
+            for (uint32_t k = kj + 1; k <= nk; k++) goff[k] = n;
+    }
+}
+
+// One record per grouped seed, so that a hit's query side costs four loads
+// instead of a window load, alignment and exception test. order[p] is the
+// seed index at grouped slot p.
+__global__ void k_qrecords(const uint32_t *__restrict__ order, const uint64_t *__restrict__ seeds,
+                           const uint32_t *__restrict__ total_p,
                            const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse,
                            uint32_t qlen, uint32_t seed_size, QRec *__restrict__ recs) {
     const uint32_t total = *total_p;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
-        uint64_t f = sq[p];
-        uint32_t Q = (uint32_t)f + seed_size;
+        uint32_t i = order[p];
+        uint32_t Q = (uint32_t)seeds[i] + seed_size;
         QRec r;
         r.Q = Q;
-        r.tag = (uint32_t)(f >> 32);
+        r.tag = i;
         r.pad = 0;
         bool ok = Q >= G2_MARGIN && (uint64_t)Q + G2_MARGIN <= qlen;
         uint32_t W[13];
