@@ -437,7 +437,8 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK),
                                B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(),
                                ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
-        hipLaunchKernelGGL(k_bgate_wave, dim3(2048), dim3(256), 0, h->stream, r, h->ref_len, ar, bo,
+        hipLaunchKernelGGL(k_bgate_wave, dim3(2048), dim3(256), 0, h->stream, r, h->ref_len, ar, arena_len,
+                           h->sep_drops ? 1 : 0, bo,
                            h->sub.as<int32_t>(), h->xdrop, h->hspthresh, B.slow.as<SurvRec>(), ctr + CTR_SLOW,
                            bs->cap_slow, B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, ctr + CTR_ERR);
         };

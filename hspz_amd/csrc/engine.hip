@@ -97,6 +97,7 @@ struct hspz_engine {
     DevBuf rrec;                            // pos_table-ordered reference records (k_ref_records)
     bool use_rrec = false, rrec_declined = false;
     bool lookahead = true;                  // k_bgate_ahead for merged steps (needs rrec)
+    bool sep_drops = false;                 // code 7 scores <= -(xdrop+1) against every code
     int bgate_grid_used = 0;                // grid of the last batched launch
     int num_cus = 0;
     double last_gate_ms = 0, last_post_ms = 0;  // event times of the last batched pass
@@ -553,6 +554,10 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
         if (ok) {
             uint32_t tab[512];
             const int32_t X = cfg->xdrop;
+            // the separator forces a drop against every code (as build_sub_mat's -10 * xdrop does)
+            h->sep_drops = true;
+            for (int j = 0; j < 8; j++)
+                h->sep_drops = h->sep_drops && cfg->sub_mat[7 * 8 + j] <= -(X + 1) && cfg->sub_mat[j * 8 + 7] <= -(X + 1);
             // non-ACGT codes that can score above -(X+1) against some code
             h->unsafe_codes = 0;
             for (int c = 4; c < 8; c++) {

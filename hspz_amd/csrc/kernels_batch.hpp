@@ -1194,9 +1194,72 @@ struct BlockOf {
     }
 };
 
-// Exact gate for deferred hits (one wave per record), chunk-local bounds.
+// One 64-cell tile of wave_extend's recurrence (without positions): folds the
+// tile's cell scores into the running score and maximum; true when the
+// extension ends in this tile (a drop, or its last cell at a sequence edge),
+// with the final maximum in prev_max.
+__device__ __forceinline__ bool wave_tile(int32_t sc, bool edge, int32_t xdrop, int32_t &prev_score,
+                                          int32_t &prev_max) {
+    int32_t ps = (int32_t)((uint32_t)wave_incl_scan_add_i32(sc) + (uint32_t)prev_score);
+    int32_t cand = ps > prev_max ? ps : prev_max;
+    int32_t scanned = wave_incl_scan_max_i32(cand);
+    uint64_t dropped = __ballot((int32_t)((uint32_t)scanned - (uint32_t)ps) > xdrop);
+    uint32_t first = dropped ? (uint32_t)(__ffsll((unsigned long long)dropped) - 1) : WAVE;
+    prev_max = first == 0 ? prev_max : __shfl(scanned, first - 1, WAVE);
+    if (dropped || edge) return true;
+    prev_score = __shfl(ps, WAVE - 1, WAVE);
+    return false;
+}
+
+// The maximum of one X-drop extension (= wave_extend<false>'s score) with the
+// bytes of its first P tiles loaded at once, so that an extension within
+// 64 * P cells costs one memory round trip; later tiles load one at a time.
+template <int P>
+__device__ __forceinline__ int32_t wave_extend_pre(const uint8_t *__restrict__ ref, uint32_t rlen,
+                                                   const uint8_t *__restrict__ qry, uint32_t qlen,
+                                                   const int32_t *s_sub, int32_t xdrop, uint32_t R, uint32_t Q,
+                                                   bool left) {
+    const uint32_t lane = lane_id();
+    auto cell = [&](int64_t tile, uint8_t &rb, uint8_t &qb) -> bool {
+        int64_t off = tile + lane + (left ? 1 : 0);
+        bool v = left ? ((int64_t)R >= off && (int64_t)Q >= off)
+                      : ((int64_t)R + off < (int64_t)rlen && (int64_t)Q + off < (int64_t)qlen);
+        rb = v ? ref[left ? R - off : R + off] : 0;
+        qb = v ? qry[left ? Q - off : Q + off] : 0;
+        return v;
+    };
+    auto at_edge = [&](int64_t tile) -> bool {
+        int64_t last = left ? tile + WAVE : tile + WAVE - 1;
+        return left ? ((int64_t)R < last || (int64_t)Q < last)
+                    : ((int64_t)R + last >= (int64_t)rlen || (int64_t)Q + last >= (int64_t)qlen);
+    };
+    uint8_t rb[P], qb[P];
+    bool v[P];
+#pragma unroll
+    for (int t = 0; t < P; t++) v[t] = cell((int64_t)t * WAVE, rb[t], qb[t]);
+    int32_t ps = 0, pm = 0;
+#pragma unroll
+    for (int t = 0; t < P; t++) {
+        int32_t sc = v[t] ? s_sub[rb[t] * 8 + qb[t]] : 0;
+        if (wave_tile(sc, at_edge((int64_t)t * WAVE), xdrop, ps, pm)) return pm;
+    }
+    for (int64_t tile = (int64_t)P * WAVE;; tile += WAVE) {
+        uint8_t r1, q1;
+        bool v1 = cell(tile, r1, q1);
+        int32_t sc = v1 ? s_sub[r1 * 8 + q1] : 0;
+        if (wave_tile(sc, at_edge(tile), xdrop, ps, pm)) return pm;
+    }
+}
+
+// Exact gate for deferred hits (one wave per record). With arena_coords the
+// extensions run in arena coordinates: every block is fenced by separator
+// codes that force a drop under this matrix, so the maxima equal the
+// block-local ones and no block lookup is needed; otherwise block-local bounds
+// come from the hit's batch. The right extension preloads 2 tiles, the left 3
+// (a hit is deferred when its right side outlives 64 cells or its left 128).
 __global__ __launch_bounds__(256) void k_bgate_wave(const uint8_t *__restrict__ ref, uint32_t ref_len,
-                                                    const uint8_t *__restrict__ arena, BlockOf bo,
+                                                    const uint8_t *__restrict__ arena, uint32_t arena_len,
+                                                    int arena_coords, BlockOf bo,
                                                     const int32_t *__restrict__ sub, int32_t xdrop,
                                                     int32_t thresh, const SurvRec *__restrict__ slow,
                                                     const uint32_t *__restrict__ slow_n, uint32_t cap_slow,
@@ -1209,12 +1272,15 @@ __global__ __launch_bounds__(256) void k_bgate_wave(const uint8_t *__restrict__ 
     const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
     for (uint32_t i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; i < n; i += nwaves) {
         SurvRec rec = slow[i];
-        uint32_t b = bo.batch_of(rec.seed);
-        uint32_t blk = bo.batches[b].block;
-        uint32_t q0 = bo.blk_off[blk], qlen = bo.blk_len[blk];
-        int32_t mr, ml, pr, pl;
-        wave_extend<false>(ref, ref_len, arena + q0, qlen, s_sub, xdrop, rec.R, rec.Q - q0, false, &mr, &pr);
-        wave_extend<false>(ref, ref_len, arena + q0, qlen, s_sub, xdrop, rec.R, rec.Q - q0, true, &ml, &pl);
+        uint32_t q0 = 0, qlen = arena_len;
+        if (!arena_coords) {
+            uint32_t b = bo.batch_of(rec.seed);
+            uint32_t blk = bo.batches[b].block;
+            q0 = bo.blk_off[blk];
+            qlen = bo.blk_len[blk];
+        }
+        int32_t mr = wave_extend_pre<2>(ref, ref_len, arena + q0, qlen, s_sub, xdrop, rec.R, rec.Q - q0, false);
+        int32_t ml = wave_extend_pre<3>(ref, ref_len, arena + q0, qlen, s_sub, xdrop, rec.R, rec.Q - q0, true);
         bool pass = gate_keep((int32_t)((uint32_t)mr + (uint32_t)ml), thresh);
         if (pass && lane_id() == 0) {
             uint32_t at = atomicAdd(surv_n, 1u);
