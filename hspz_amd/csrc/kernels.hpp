@@ -747,22 +747,30 @@ __global__ __launch_bounds__(512) void k_gate_packed(
 
 // ----------------------------------------------------------------- wave extension
 
+// Inclusive scans over the 64 lanes of a wave in DPP moves: row_shr 1, 2, 4,
+// 8 within each row of 16 lanes, then row_bcast15 (lane 15 of each row into
+// rows 1 and 3) and row_bcast31 (lane 31 into rows 2 and 3). A lane whose DPP
+// source is absent (or whose row is masked off) reads the identity.
+constexpr int DPP_ROW_SHR = 0x110, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
 __device__ __forceinline__ int32_t wave_incl_scan_add_i32(int32_t v) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) {
-        int32_t up = __shfl_up(v, off, WAVE);
-        if (lane >= (uint32_t)off) v = (int32_t)((uint32_t)v + (uint32_t)up);
-    }
+    v = (int32_t)((uint32_t)v + (uint32_t)__builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 1, 0xF, 0xF, false));
+    v = (int32_t)((uint32_t)v + (uint32_t)__builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 2, 0xF, 0xF, false));
+    v = (int32_t)((uint32_t)v + (uint32_t)__builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 4, 0xF, 0xF, false));
+    v = (int32_t)((uint32_t)v + (uint32_t)__builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 8, 0xF, 0xF, false));
+    v = (int32_t)((uint32_t)v + (uint32_t)__builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST15, 0xA, 0xF, false));
+    v = (int32_t)((uint32_t)v + (uint32_t)__builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST31, 0xC, 0xF, false));
     return v;
 }
 
 __device__ __forceinline__ int32_t wave_incl_scan_max_i32(int32_t v) {
-#pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) {
-        int32_t up = __shfl_up(v, off, WAVE);  // lanes below off read their own value
-        v = up >= v ? up : v;
-    }
+    const int32_t lo = (int32_t)0x80000000;
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_SHR + 1, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_SHR + 2, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_SHR + 4, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_SHR + 8, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_BCAST15, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_BCAST31, 0xC, 0xF, false));
     return v;
 }
 
@@ -795,7 +803,7 @@ __device__ __forceinline__ void wave_extend(const uint8_t *__restrict__ ref, uin
         int32_t scanned = wave_incl_scan_max_i32(cand);
         uint64_t dropped = __ballot((int32_t)((uint32_t)scanned - (uint32_t)ps) > xdrop);
         uint32_t first = dropped ? (uint32_t)(__ffsll((unsigned long long)dropped) - 1) : WAVE;
-        int32_t mx = first == 0 ? prev_max : __shfl(scanned, first - 1, WAVE);
+        int32_t mx = first == 0 ? prev_max : __builtin_amdgcn_readlane(scanned, first - 1);
         int32_t mpos = prev_pos;
         if (WITH_POS && mx != prev_max) {
             uint64_t win = __ballot(ps == mx && lane < first);
@@ -809,7 +817,7 @@ __device__ __forceinline__ void wave_extend(const uint8_t *__restrict__ ref, uin
             *pos_out = mpos;
             return;
         }
-        prev_score = __shfl(ps, WAVE - 1, WAVE);
+        prev_score = __builtin_amdgcn_readlane(ps, WAVE - 1);
         prev_max = mx;
         prev_pos = mpos;
         tile += WAVE;

@@ -1205,9 +1205,9 @@ __device__ __forceinline__ bool wave_tile(int32_t sc, bool edge, int32_t xdrop, 
     int32_t scanned = wave_incl_scan_max_i32(cand);
     uint64_t dropped = __ballot((int32_t)((uint32_t)scanned - (uint32_t)ps) > xdrop);
     uint32_t first = dropped ? (uint32_t)(__ffsll((unsigned long long)dropped) - 1) : WAVE;
-    prev_max = first == 0 ? prev_max : __shfl(scanned, first - 1, WAVE);
+    prev_max = first == 0 ? prev_max : __builtin_amdgcn_readlane(scanned, first - 1);
     if (dropped || edge) return true;
-    prev_score = __shfl(ps, WAVE - 1, WAVE);
+    prev_score = __builtin_amdgcn_readlane(ps, WAVE - 1);
     return false;
 }
 
