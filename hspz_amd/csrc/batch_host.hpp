@@ -26,12 +26,12 @@ struct HostBatch {
 struct BatchBufs {
     DevBuf arena, plane, coarse, desc, poff, blk_off, blk_len, valid, ids, voff, seeds, counts, prefix,
         tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr, gcnt, sq, skm, qrec,
-        kvalo, ktmp;
+        kvalo, ktmp, exact;
 };
 
 // Device counters of one pass; CTR_WORK is the 64-bit seed-claim counter of
 // the batched gate (two slots, 8-byte aligned).
-enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_GROUPED = 5, CTR_WORK = 6,
+enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_EXACT = 5, CTR_WORK = 6,
        CTR_N = 8 };
 
 typedef void (*BGateFn)(const uint32_t *, const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, uint32_t,
@@ -76,7 +76,7 @@ BGateFn bgate_kernel(int step_cells, int threads) {
 // Everything a captured device pass depends on: if a call's key equals the
 // cached one, the instantiated graph is replayed instead of relaunching.
 struct GraphKey {
-    const void *ptrs[40];
+    const void *ptrs[48];
     uint64_t vals[24];
     uint32_t shape[33];
 };
@@ -306,6 +306,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         hspz_batch_state *bs = h->batch;
         if ((rc = reserve(h, B.surv, (size_t)bs->cap_surv * sizeof(SurvRec)))) return rc;
         if ((rc = reserve(h, B.slow, (size_t)bs->cap_slow * sizeof(SurvRec)))) return rc;
+        if ((rc = reserve(h, B.exact, (size_t)bs->cap_slow * sizeof(SurvRec)))) return rc;
         if ((rc = reserve(h, B.mhsp, (size_t)bs->cap_surv * 16))) return rc;
         if ((rc = reserve(h, B.mscope, (size_t)bs->cap_surv * 4))) return rc;
         if ((rc = reserve(h, B.keep, (size_t)bs->cap_surv + 16))) return rc;
@@ -437,9 +438,16 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK),
                                B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(),
                                ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
+        // deferred hits: past the fixed windows on the planes, lane per hit; the
+        // ones that meet an exception or a sequence end, exactly, wave per hit
+        hipLaunchKernelGGL(k_bgate_long, dim3(2 * h->num_cus), dim3(1024), 0, h->stream, h->ref2.as<uint32_t>(),
+                           h->refx.as<uint32_t>(), h->ref_len, B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(),
+                           arena_len, h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.slow.as<SurvRec>(),
+                           ctr + CTR_SLOW, bs->cap_slow, B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv,
+                           B.exact.as<SurvRec>(), ctr + CTR_EXACT, ctr + CTR_ERR);
         hipLaunchKernelGGL(k_bgate_wave, dim3(2048), dim3(256), 0, h->stream, r, h->ref_len, ar, arena_len,
                            h->sep_drops ? 1 : 0, bo,
-                           h->sub.as<int32_t>(), h->xdrop, h->hspthresh, B.slow.as<SurvRec>(), ctr + CTR_SLOW,
+                           h->sub.as<int32_t>(), h->xdrop, h->hspthresh, B.exact.as<SurvRec>(), ctr + CTR_EXACT,
                            bs->cap_slow, B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, ctr + CTR_ERR);
         };
         auto enqueue_post = [&]() {
@@ -469,13 +477,15 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                   B.nchunks.p, B.starts.p, B.bhits.p, B.surv.p, B.slow.p, B.mhsp.p, B.mscope.p,
                                   B.keep.p, B.kids.p, B.ohsp.p, B.oscope.p, B.ctr.p, h->tiles.p, h->ref2.p,
                                   h->refx.p, h->pairtab.p, h->index.p, h->pos.p, B.gcnt.p, B.sq.p, B.skm.p,
-                                  h->clean.p, B.qrec.p, h->rrec.p, B.kvalo.p};
+                                  h->clean.p, B.qrec.p, h->rrec.p, B.kvalo.p, B.exact.p};
+            static_assert(sizeof ptrs <= sizeof key.ptrs, "graph key pointer slots");
             memcpy(key.ptrs, ptrs, sizeof ptrs);
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow, cap_seeds, per, h->max_hits,
                                (uint64_t)grid | (uint64_t)h->bgate_claim << 32, (uint64_t)p->transitions, nt, kt,
                                cap_tiles, (uint64_t)h->bgate_step | (uint64_t)h->bgate_threads << 32,
                                (uint64_t)(uintptr_t)h->sub.p, s.kmer_size, (uint64_t)(uintptr_t)h->ref.p,
                                (uint64_t)kmer_order, ntk, (uint64_t)(uintptr_t)(const void *)kgate_fn};
+            static_assert(sizeof vals <= sizeof key.vals, "graph key value slots");
             memcpy(key.vals, vals, sizeof vals);
             memcpy(key.shape, s.pos, sizeof s.pos);
             key.shape[32] = s.size;

@@ -1194,6 +1194,86 @@ struct BlockOf {
     }
 };
 
+// Deferred hits extended one lane per hit past the main gate's fixed windows:
+// each side in 64-cell stretches read from the 2-bit planes through the same
+// two-cell maps (so the same upper bound of the exact score: every cell up to
+// the drop, plus the rest of its stretch), until it drops. A hit one of whose
+// stretches would touch an exception bit or come within a word of a sequence
+// end goes to the exact wave gate instead (list `exact`). Both sides restart
+// at the anchor.
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bgate_long(
+    const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ rcoarse, uint32_t ref_len,
+    const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse, uint32_t qlen,
+    const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh, const SurvRec *__restrict__ slow,
+    const uint32_t *__restrict__ slow_n, uint32_t cap_slow, SurvRec *__restrict__ surv,
+    uint32_t *__restrict__ surv_n, uint32_t cap_surv, SurvRec *__restrict__ exact,
+    uint32_t *__restrict__ exact_n, uint32_t *__restrict__ err) {
+    __shared__ uint32_t s_map[G2_MAP_WORDS];
+    load_map_table(s_map, pairtab, 1024);
+    const uint32_t n = min(*slow_n, cap_slow);
+    const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+        const uint32_t i = base + lane_id();
+        const bool have = i < n;
+        SurvRec rec = {0, 0, 0, 0};
+        if (have) rec = slow[i];
+        bool ex = false;
+        int32_t M[2] = {0, 0};
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const bool left = side == 1;
+            int32_t e = xdrop, c = 0;
+            bool live = have && !ex;
+            for (uint32_t k0 = 0; __ballot(live) != 0; k0 += 64) {
+                if (!live) continue;
+                // a stretch plus one word of alignment slack on either side stays inside both planes
+                bool inb = left ? (rec.R >= k0 + 80 && rec.Q >= k0 + 80)
+                                : ((uint64_t)rec.R + k0 + 80 <= ref_len && (uint64_t)rec.Q + k0 + 80 <= qlen);
+                if (!inb) {
+                    ex = true;
+                    live = false;
+                    continue;
+                }
+                const uint32_t rb = left ? rec.R - k0 : rec.R + k0, qb = left ? rec.Q - k0 : rec.Q + k0;
+                const uint32_t rlo = left ? rb - 64 : rb, qlo = left ? qb - 64 : qb;
+                const uint32_t *rc = rcoarse + (rlo >> 11), *qc = qcoarse + (qlo >> 11);
+                if (coarse_bits(rc[0], rc[1], rlo, rlo + 63) || coarse_bits(qc[0], qc[1], qlo, qlo + 63)) {
+                    ex = true;
+                    live = false;
+                    continue;
+                }
+                // five plane words around the stretch, aligned to its first cell
+                const uint32_t *rp = rplane + (rb >> 4) - (left ? 4 : 0), *qp = qplane + (qb >> 4) - (left ? 4 : 0);
+                uint32_t RW[5], QW[5];
+#pragma unroll
+                for (int k = 0; k < 5; k++) RW[k] = rp[k], QW[k] = qp[k];
+                const uint32_t sr = 2 * (rb & 15), sq = 2 * (qb & 15);
+                uint32_t A[4], B[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (left) {
+                        A[k] = nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(RW[4 - k], RW[3 - k], sr)));
+                        B[k] = nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(QW[4 - k], QW[3 - k], sq)));
+                    } else {
+                        A[k] = __builtin_amdgcn_alignbit(RW[k + 1], RW[k], sr);
+                        B[k] = __builtin_amdgcn_alignbit(QW[k + 1], QW[k], sq);
+                    }
+                }
+                int32_t emin = e;
+                map_stretch<4>((const uint8_t *)s_map, A, B, false, left ? off_rev : off_fwd, e, c, emin);
+                if (emin < 0) {
+                    M[side] = c + (xdrop - e);
+                    live = false;
+                }
+            }
+        }
+        const bool pass = have && !ex && gate_keep((int32_t)((uint32_t)M[0] + (uint32_t)M[1]), thresh);
+        SeedSource::append(pass, surv, surv_n, cap_surv, err, rec.R, rec.Q, rec.seed);
+        SeedSource::append(have && ex, exact, exact_n, cap_slow, err, rec.R, rec.Q, rec.seed);
+    }
+}
+
 // One 64-cell tile of wave_extend's recurrence (without positions): folds the
 // tile's cell scores into the running score and maximum; true when the
 // extension ends in this tile (a drop, or its last cell at a sequence edge),
