@@ -49,7 +49,8 @@ W_PACKED = 0.25 + 1.0 / 512
 
 
 def profiled_traffic(hits_per_launch, order_used):
-    """HBM bytes per launch of the batched gate (k_bgate_kmer + k_bgate_wave),
+    """HBM bytes per launch of the batched gate (the k-mer-major gate and the
+    deferred-hit gates),
     FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from the committed rocprofv3 counter
     summary of this same workload; None when this run's launch is not that one
     (another hit order or hit count)."""
@@ -204,6 +205,13 @@ def main():
     elapsed = time.perf_counter() - t_start
     g = {k: eng.stat(k) - g0[k] for k in g0}
     order_used = eng.get_flag("batch_order_used")
+    if not order_used:
+        gate_kernel = "k_bgate_packed"
+    elif (eng.get_flag("batch_step_cells") == 128 and eng.get_flag("batch_block_threads") == 512
+          and eng.get_flag("batch_lookahead") and eng.get_flag("ref_records") == 2):
+        gate_kernel = "k_bgate_ahead"
+    else:
+        gate_kernel = "k_bgate_kmer"
     value = args.steps * query_bp / 1e6 / elapsed
     hashes_b = [E.hash_pass(r) for r in res_b]
 
@@ -273,11 +281,10 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": "profiles/kmer_gate_1gbp_target.md (FETCH_SIZE x2.0 + WRITE_SIZE, "
                                   "same workload)" if traffic else None,
-                "binding_limit": "per-iteration load round trips with VALU at 68% of issue "
+                "binding_limit": "instruction issue: VALU at 69% of issue, HBM at 7.5 B/hit "
                                  "(profiles/kmer_gate_1gbp_target.md)"
                                  if order_used else "L2-miss bandwidth of random reference lines",
-                "kernel": "%s (+ k_bgate_wave for deferred hits)"
-                          % ("k_bgate_kmer" if order_used else "k_bgate_packed"),
+                "kernel": "%s (+ k_bgate_long, k_bgate_wave for deferred hits)" % gate_kernel,
                 "hit_order": "k-mer-major" if order_used else "seed-major",
                 "avg_launch_us": round(g["gate_ns"] / max(g["gate_launches"], 1) / 1e3, 1),
                 "hits_per_s": round(g["gate_hits"] / secs, 1),

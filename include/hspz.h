@@ -162,10 +162,13 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 #define HSPZ_FLAG_BATCH_ORDER_USED 6   /* read-only: the order of the last batched pass */
 #define HSPZ_FLAG_BATCH_DEVICE_DEDUP 7 /* 1 (default): dedup_and_order of the batched path on the device
                                           (stable radix sorts of all scopes at once); 0: on host threads */
-#define HSPZ_FLAG_REF_RECORDS 8        /* 1: the merged-step k-mer-major gate reads 48-byte reference
-                                          records in pos_table order, built on first use when device
-                                          memory allows (48 B per indexed position); 0 (default): it reads
-                                          the 2-bit plane. hspz_get_flag reports 2 once they are built */
+#define HSPZ_FLAG_REF_RECORDS 8        /* reference records: 48 B per indexed position in pos_table order
+                                          (the gate's reference words, aligned), built on first use when
+                                          device memory allows. The lookahead gate builds and reads them
+                                          whenever it runs; 1: the merged-step gate without lookahead wants
+                                          them too; 0 (default): only the lookahead gate does, and setting
+                                          0 releases built ones. hspz_get_flag reports 2 once they are built.
+                                          Any k-mer-major gate reads them when they exist */
 #define HSPZ_FLAG_BATCH_LOOKAHEAD 9    /* 1 (default): with 128-cell steps and 512-thread blocks, the
                                           k-mer-major gate issues each hit's and each item's loads one
                                           iteration ahead of their use; it reads the reference records
