@@ -49,23 +49,24 @@ KGateFn kgate_kernel(int step_cells, int threads) {
     if (threads == 1024)
         return step_cells == 16    ? k_bgate_kmer<1, 1024>
                : step_cells == 64  ? k_bgate_kmer<4, 1024>
-               : step_cells == 128 ? k_bgate_kmer<8, 1024>
+               : step_cells >= 96  ? k_bgate_kmer<8, 1024>
                                    : k_bgate_kmer<2, 1024>;
     return step_cells == 16    ? k_bgate_kmer<1, 512>
            : step_cells == 64  ? k_bgate_kmer<4, 512>
-           : step_cells == 128 ? k_bgate_kmer<8, 512>
+           : step_cells >= 96  ? k_bgate_kmer<8, 512>
                                : k_bgate_kmer<2, 512>;
 }
 
-// The lookahead gate (k_bgate_ahead) applies to merged 128-cell steps in
-// 512-thread blocks and needs the reference records.
+// The lookahead gate (k_bgate_ahead) applies to merged steps (96 or 128
+// cells) in 512-thread blocks and needs the reference records; the other
+// gates run a 96-cell step as 128.
 bool lookahead_wanted(const hspz_engine *h) {
-    return h->lookahead && h->bgate_step == 128 && h->bgate_threads == 512;
+    return h->lookahead && h->bgate_step >= 96 && h->bgate_threads == 512;
 }
 
 // The batched gate instantiation for a step width and block size.
 BGateFn bgate_kernel(int step_cells, int threads) {
-    if (step_cells == 128) step_cells = 64;  // the merged-phase variant exists in k-mer-major order only
+    if (step_cells >= 96) step_cells = 64;  // the merged-phase variants exist in k-mer-major order only
     if (threads == 1024)
         return step_cells == 16 ? k_bgate_packed<1, 1024> : step_cells == 64 ? k_bgate_packed<4, 1024> : k_bgate_packed<2, 1024>;
     return step_cells == 16 ? k_bgate_packed<1, 512> : step_cells == 64 ? k_bgate_packed<4, 512> : k_bgate_packed<2, 512>;
@@ -408,7 +409,8 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         // fused expansion + packed gate, then the exact gate for deferred hits
         BGateFn gate_fn = bgate_kernel(h->bgate_step, h->bgate_threads);
         KGateFn kgate_fn = kgate_kernel(h->bgate_step, h->bgate_threads);
-        if (kmer_order && lookahead_wanted(h) && h->rrec.p) kgate_fn = k_bgate_ahead;
+        if (kmer_order && lookahead_wanted(h) && h->rrec.p)
+            kgate_fn = h->bgate_step == 96 ? k_bgate_ahead<2> : k_bgate_ahead<4>;
         int grid = h->bgate_blocks;
         if (grid == 0) {  // resident grid: occupancy of this instantiation x CUs
             int per_cu = 0;

@@ -808,8 +808,8 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
         h->bgate_blocks = value;
         break;
     case HSPZ_FLAG_BATCH_STEP_CELLS:
-        if (value != 16 && value != 32 && value != 64 && value != 128)
-            return set_error(HSPZ_E_ARG, "step cells must be 16, 32, 64 or 128");
+        if (value != 16 && value != 32 && value != 64 && value != 96 && value != 128)
+            return set_error(HSPZ_E_ARG, "step cells must be 16, 32, 64, 96 or 128");
         h->bgate_step = value;
         break;
     case HSPZ_FLAG_BATCH_BLOCK_THREADS:

@@ -857,6 +857,40 @@ __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t
     }
 }
 
+// K words (16 K cells) of a stream of N words through the map table, as
+// map_stretch: words [0, K), or [K, 2K) when `hi` (N = 2K).
+template <int K, int N>
+__device__ __forceinline__ void map_words(const uint8_t *s_map, const uint32_t (&rw)[N], const uint32_t (&qw)[N],
+                                          bool hi, uint32_t off, int32_t &e, int32_t &c, int32_t &emin) {
+    static_assert(N == K || N == 2 * K, "a stretch is the whole stream or one of its halves");
+    constexpr uint32_t SEL[4] = {0x0C0C0400u, 0x0C0C0500u, 0x0C0C0600u, 0x0C0C0700u};
+#pragma unroll
+    for (int h = 0; h < K; h++) {
+        uint32_t r = rw[h], q = qw[h];
+        if constexpr (N == 2 * K) {
+            r = hi ? rw[K + h] : r;
+            q = hi ? qw[K + h] : q;
+        }
+        uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
+        uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
+        uint32_t p[8];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            p[2 * j] = map_entry(s_map, t1, off, SEL[j]);
+            p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            int32_t e1 = min(e + lo16(p[i]), hi16(p[i]));
+            e = min(e1 + lo16(p[i + 1]), hi16(p[i + 1]));
+            emin = min3_i32(emin, e1, e);
+        }
+        c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3]))) +
+             lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
+        asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
+    }
+}
+
 // The k-mer-major gate with merged phases: one iteration evaluates a fresh
 // hit's 64 right cells and its first 64 left cells; a hit whose left
 // extension outlives them takes a second iteration for left cells 64..127.
@@ -964,14 +998,21 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
 // of a finished hit are appended at the top of the next iteration. Hits,
 // their cells and decisions are those of kmer_gate_merged; only the order in
 // which lanes take items differs. Reads the reference records (k_ref_records).
+// RW = 4: right cells 0..63 in a hit's first iteration (the cells of
+// kmer_gate_merged). RW = 2: right cells 0..31, and a right side that outlives
+// them continues with cells 32..63 in the next iteration, while the left side
+// keeps its 64-cell stretches; fewer cells per hit (a right side mostly drops
+// within 32), the same upper-bound decisions.
+template <int RW>
 __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__restrict__ rrec,
                                                 const QRec *__restrict__ recs, const uint8_t *s_map, int32_t xdrop,
                                                 int32_t thresh) {
     const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
-    // current item and hit
-    bool cur = false, rok = false, has = false, cont = false, qok = false;
+    // current item and hit (per side: continuing, done, kept state, maximum)
+    bool cur = false, rok = false, has = false, qok = false;
+    bool cont = false, rcont = false, ldone = false, rdone = false;
     uint32_t R = 0, s_cur = 0, s_end = 0, Q = 0, tag = 0;
-    int32_t Mr = 0, el_keep = 0, cl_keep = 0;
+    int32_t Mr = 0, Ml = 0, el_keep = 0, cl_keep = 0, er_keep = 0, cr_keep = 0;
     uint32_t A0[G2_RIGHT_WORDS], LA0[G2_LEFT_WORDS], B[G2_RIGHT_WORDS], LB[G2_LEFT_WORDS];
     // next item, and the item whose loads are in flight
     bool nxt = false, nrok = false, fly = false;
@@ -1045,7 +1086,7 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
                 qok = v3.z != 0;
                 s_cur++;
                 has = true;
-                cont = false;
+                cont = rcont = rdone = ldone = false;
                 if (!(rok && qok)) defer = true;
             }
         }
@@ -1075,29 +1116,41 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
             P0 = rec[0], P1 = rec[1], P2 = rec[2], P3 = rec[3];
         }
 
-        // 5. the current hit: right cells 0..63 and left cells 0..63, or left
-        //    cells 64..127 for a continuing hit (as kmer_gate_merged)
+        // 5. the current hit: its next right stretch (RW words) and its next
+        //    left stretch (cells 0..63, or 64..127 for a continuing left side)
         bool done = false, pass = false;
         if (has && !defer) {
-            int32_t er = xdrop, cr = 0, eminr = xdrop;
-            map_stretch<4>(s_map, A0, B, false, off_fwd, er, cr, eminr);
+            int32_t er = rcont ? er_keep : xdrop, cr = rcont ? cr_keep : 0, eminr = er;
+            map_words<RW, G2_RIGHT_WORDS>(s_map, A0, B, rcont, off_fwd, er, cr, eminr);
             int32_t el = cont ? el_keep : xdrop, cl = cont ? cl_keep : 0, eminl = el;
-            map_stretch<8>(s_map, LA0, LB, cont, off_rev, el, cl, eminl);
-            if (!cont) {
-                if (eminr >= 0) defer = true;
-                else Mr = cr + (xdrop - er);
+            map_words<4, G2_LEFT_WORDS>(s_map, LA0, LB, cont, off_rev, el, cl, eminl);
+            if (!rdone) {
+                if (eminr < 0) {
+                    rdone = true;
+                    Mr = cr + (xdrop - er);
+                } else if (rcont || RW == (int)G2_RIGHT_WORDS) {
+                    defer = true;  // the right side outlived its 64 cells
+                } else {
+                    rcont = true;
+                    er_keep = er;
+                    cr_keep = cr;
+                }
             }
-            if (!defer) {
+            if (!defer && !ldone) {
                 if (eminl < 0) {
-                    done = true;
-                    pass = gate_keep(Mr + cl + (xdrop - el), thresh);
+                    ldone = true;
+                    Ml = cl + (xdrop - el);
                 } else if (cont) {
-                    defer = true;
+                    defer = true;  // the left side outlived its 128 cells
                 } else {
                     cont = true;
                     el_keep = el;
                     cl_keep = cl;
                 }
+            }
+            if (!defer && rdone && ldone) {
+                done = true;
+                pass = gate_keep(Mr + Ml, thresh);
             }
         }
         if (has && (done || defer)) {
@@ -1107,14 +1160,14 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
             eQ = Q;
             etag = tag;
             has = false;
-            cont = false;
         }
     }
 }
 
-// The lookahead k-mer-major gate (kmer_gate_ahead); same arguments as
+// The lookahead k-mer-major gate (kmer_gate_ahead<RW>); same arguments as
 // k_bgate_kmer, rrec required. Four waves per SIMD (512-thread blocks, two per
 // CU) leave it the 128 VGPRs its in-flight state needs.
+template <int RW>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_bgate_ahead(
     const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop,
     int32_t thresh, const QRec *__restrict__ recs, const uint32_t *__restrict__ skm,
@@ -1145,7 +1198,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     src.err = err;
     src.init();
     (void)rplane;
-    kmer_gate_ahead(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+    kmer_gate_ahead<RW>(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
 // The batched gate in k-mer-major order (same launch shape as k_bgate_packed).

@@ -90,7 +90,7 @@ extern "C" int hspz_autotune_batch(hspz_engine *h, const hspz_pass_params *p, co
     long bi = best_index();
     const int order = bi >= 0 ? knobs[bi].order : saved.order;
     // stage 1: step x block x claim at the resident grid
-    const int steps[] = {16, 32, 64, 128}, threads[] = {512, 1024}, claims[] = {16, 64, 256};
+    const int steps[] = {16, 32, 64, 96, 128}, threads[] = {512, 1024}, claims[] = {16, 64, 256};
     for (int st : steps)
         for (int th : threads)
             for (int cl : claims) measure({st, th, cl, 0, order});

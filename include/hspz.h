@@ -149,9 +149,11 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 #define HSPZ_FLAG_BATCH_GRAPH 0        /* 1 (default): capture/replay the batched pass as a hipGraph */
 #define HSPZ_FLAG_BATCH_GATE_BLOCKS 1  /* persistent grid of the batched gate in blocks; 0 (default) = as many
                                           as are resident at once */
-#define HSPZ_FLAG_BATCH_STEP_CELLS 2   /* cells per gate step: 16, 32, 64, or 128 (default): 64 right + 64
-                                          left cells of a hit in one step (k-mer-major order; seed-major
-                                          runs it as 64) */
+#define HSPZ_FLAG_BATCH_STEP_CELLS 2   /* cells per gate step: 16, 32, 64, 96 or 128 (default). 128: 64
+                                          right + 64 left cells of a hit in one step; 96: 32 right + 64
+                                          left, the right side continuing with its next 32 cells (lookahead
+                                          gate only; the others run it as 128). Seed-major order runs both
+                                          as 64 */
 #define HSPZ_FLAG_BATCH_BLOCK_THREADS 3 /* threads per gate block: 512 (default) or 1024 */
 #define HSPZ_FLAG_BATCH_CLAIM_SEEDS 4  /* seeds a wave claims from the work counter at a time (1..4096,
                                           default 256) */
@@ -287,7 +289,7 @@ typedef struct hspz_tune_cell {
 /* Autotuner of the batched gate (no counterpart in the reference). Runs
  * hspz_filter_batch over `chunks` for both hit orders at the engine's knobs,
  * then, in the faster order, for every cell of a grid over step width
- * {16, 32, 64} cells x block size {512, 1024} x wave claim {16, 64, 256} seeds at
+ * {16, 32, 64, 96, 128} cells x block size {512, 1024} x wave claim {16, 64, 256} seeds at
  * the resident grid, then over grid sizes {1/2, 3/4, 1, 3/2, 2} x resident
  * for the best of those, `reps` times each, and records every cell it measured
  * in cells[0..*n_cells) (up to cap; *n_cells counts all of them). The engine

@@ -96,6 +96,31 @@ def test_batch_noisy_and_soft_masked(gpu_available):
     _run(t, blocks, ambiguous="n")
 
 
+def test_batch_separator_that_does_not_force_a_drop(gpu_available):
+    _say("case batch: separator scored -900 (no forced drop): fenced windows go to the exact gate")
+    target = synth.random_bases(0xB00A, 300_000).tobytes()
+    blocks = _chunks_from(target, 10, 0x1A, lo=5_000, hi=40_000, div=(0.01, 0.1))
+    blocks.append(_block([("m1", target[1000:3000]), ("m2", target[2990:9000])]))  # '&' inside a homolog
+    sub = O.sub_mat("", 910).copy()
+    sub[56:64] = -900  # row of the separator code
+    sub[7::8] = -900   # its column
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    want = []
+    for f, r in blocks:
+        o = O.Oracle(target, f, r, sub, max_hits=16711680, index=(index, pos))
+        want.append(o.run_pass(threads=16)[0])
+    with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
+        chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
+        for order in (0, 1):
+            eng.set_flag("batch_order", order)
+            got, st = eng.filter_batch(chunks)
+            for i, (g, w) in enumerate(zip(got, want)):
+                assert same_intervals(g, w), "chunk %d differs in order %d" % (i, order)
+        eng.swap_query(*chunks[-1])
+        plain, _ = eng.run_pass()
+        assert same_intervals(plain, want[-1])
+
+
 def test_batch_repeat_fixture_digest(gpu_available):
     _say("case batch: the repeat and multi fixtures as two chunks of one call")
     from helpers import Pair
@@ -151,7 +176,8 @@ def test_batch_gate_configs_match(gpu_available):
                 (16, 512, 64, 5000, 0, 1), (64, 1024, 1, 0, 1, 1), (32, 512, 3, 2, 1, 1), (16, 1024, 4096, 0, 1, 1),
                 (64, 512, 64, 7, 1, 1), (32, 1024, 1, 5000, 1, 1), (128, 1024, 1, 0, 1, 1), (128, 512, 256, 3, 1, 0),
                 (128, 1024, 64, 0, 0, 1), (128, 512, 256, 0, 1, 1), (128, 512, 1, 5000, 1, 1),
-                (128, 512, 4096, 1, 1, 1), (128, 512, 16, 7, 1, 1)]
+                (128, 512, 4096, 1, 1, 1), (128, 512, 16, 7, 1, 1), (96, 512, 256, 0, 1, 1),
+                (96, 512, 1, 3, 1, 1), (96, 1024, 64, 0, 1, 1), (96, 512, 64, 0, 0, 1), (96, 512, 256, 0, 1, 0)]
     with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
         for step, threads, claim, blocks_n, order, ahead in variants:
             _say("  step %d threads %d claim %d grid %d order %d lookahead %d"
@@ -169,11 +195,11 @@ def test_batch_gate_configs_match(gpu_available):
                 assert eng.get_flag("batch_gate_blocks") == blocks_n
             else:
                 assert eng.get_flag("batch_gate_blocks") >= 256
-            if step == 128 and threads == 512 and order == 1 and ahead:
+            if step >= 96 and threads == 512 and order == 1 and ahead:
                 assert eng.get_flag("ref_records") == 2  # the lookahead gate's records were built
             for i, (g, w) in enumerate(zip(got, want)):
                 assert same_intervals(g, w), "chunk %d differs at step %d claim %d" % (i, step, claim)
-        for name, bad in [("batch_step_cells", 24), ("batch_block_threads", 256),
+        for name, bad in [("batch_step_cells", 24), ("batch_step_cells", 48), ("batch_block_threads", 256),
                           ("batch_claim_seeds", 0), ("batch_gate_blocks", -1), ("batch_order", 3),
                           ("batch_order_used", 1)]:
             with pytest.raises(E.HspzError) as ei:
@@ -187,13 +213,13 @@ def test_batch_autotune(gpu_available):
     with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
         before, st0 = eng.filter_batch(chunks)
         cells, best = eng.autotune(chunks, reps=1)
-        assert len(cells) == 30
+        assert len(cells) == 36
         assert all(c["status"] == 0 for c in cells)
         assert len({c["hits"] for c in cells}) == 1 and cells[0]["hits"] == st0["hits"]
         assert [c["order"] for c in cells[:2]] == [0, 1]
         assert len({c["order"] for c in cells[2:]}) == 1
-        assert {(c["step_cells"], c["block_threads"], c["claim_seeds"]) for c in cells[2:26]} == {
-            (s, t, c) for s in (16, 32, 64, 128) for t in (512, 1024) for c in (16, 64, 256)}
+        assert {(c["step_cells"], c["block_threads"], c["claim_seeds"]) for c in cells[2:32]} == {
+            (s, t, c) for s in (16, 32, 64, 96, 128) for t in (512, 1024) for c in (16, 64, 256)}
         fastest = min(c["gate_ms"] + c["after_ms"] for c in cells)
         assert best["gate_ms"] + best["after_ms"] == fastest
         assert eng.get_flag("batch_step_cells") == best["step_cells"]
