@@ -26,7 +26,7 @@ struct HostBatch {
 struct BatchBufs {
     DevBuf arena, plane, coarse, desc, poff, blk_off, blk_len, valid, ids, voff, seeds, counts, prefix,
         tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr, gcnt, sq, skm, qrec,
-        kvalo, ktmp, exact;
+        kvalo, ktmp, exact, kkey;
 };
 
 // Device counters of one pass; CTR_WORK is the 64-bit seed-claim counter of
@@ -324,18 +324,19 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         if ((rc = reserve(h, h->tiles, (size_t)std::max(std::max(kt, nt), ntk) * 4))) return rc;
         if (kmer_order) {
             if ((rc = reserve(h, B.gcnt, ((size_t)nkm + 1) * 4))) return rc;
-            if ((rc = reserve(h, B.sq, cap_seeds * 8 + 8))) return rc;  // sort keys and values (u32 each)
-            if ((rc = reserve(h, B.kvalo, cap_seeds * 4 + 4))) return rc;
+            if ((rc = reserve(h, B.kkey, cap_seeds * 4 + 4))) return rc;   // sort keys (u32)
+            if ((rc = reserve(h, B.sq, cap_seeds * 8 + 8))) return rc;     // sort values (u64), in
+            if ((rc = reserve(h, B.kvalo, cap_seeds * 8 + 8))) return rc;  // and out
             if ((rc = reserve(h, B.skm, cap_seeds * 4 + 4))) return rc;
             if ((rc = reserve(h, B.qrec, cap_seeds * sizeof(QRec) + sizeof(QRec)))) return rc;
             if ((rc = ensure_ref_records(h, h->use_rrec || lookahead_wanted(h)))) return rc;
         }
-        uint32_t *kkey = kmer_order ? B.sq.as<uint32_t>() : nullptr;
-        uint32_t *kval = kmer_order ? kkey + cap_seeds : nullptr;
+        uint32_t *kkey = kmer_order ? B.kkey.as<uint32_t>() : nullptr;
+        uint64_t *kval = kmer_order ? B.sq.as<uint64_t>() : nullptr;
         const int key_bits = 32 - __builtin_clz(nkm);  // keys are in [0, nk]
         if (kmer_order) {  // sort temp storage before any capture
             size_t bytes = 0;
-            HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint32_t>(),
+            HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint64_t>(),
                                               (uint32_t)cap_seeds, 0, key_bits, h->stream));
             if ((rc = reserve(h, B.ktmp, bytes + 256))) return rc;
         }
@@ -390,11 +391,11 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         auto enqueue_sort = [&]() -> int {
             if (!kmer_order) return 0;
             size_t bytes = 0;
-            HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint32_t>(),
+            HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint64_t>(),
                                               (uint32_t)cap_seeds, 0, key_bits, h->stream));
             if (int e = reserve(h, B.ktmp, bytes + 256)) return e;
             HIP_TRY(rocprim::radix_sort_pairs(B.ktmp.p, bytes, kkey, B.skm.as<uint32_t>(), kval,
-                                              B.kvalo.as<uint32_t>(), (uint32_t)cap_seeds, 0, key_bits, h->stream));
+                                              B.kvalo.as<uint64_t>(), (uint32_t)cap_seeds, 0, key_bits, h->stream));
             return 0;
         };
         auto enqueue_group = [&]() {
@@ -402,7 +403,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             hipLaunchKernelGGL(k_kbounds, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
                                h->stream, B.skm.as<uint32_t>(), (uint32_t)cap_seeds, nkm, B.gcnt.as<uint32_t>());
             hipLaunchKernelGGL(k_qrecords, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
-                               h->stream, B.kvalo.as<uint32_t>(), B.seeds.as<uint64_t>(), B.gcnt.as<uint32_t>() + nkm,
+                               h->stream, B.kvalo.as<uint64_t>(), B.gcnt.as<uint32_t>() + nkm,
                                B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(), arena_len, h->seed_size,
                                B.qrec.as<QRec>());
         };
@@ -479,7 +480,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                   B.nchunks.p, B.starts.p, B.bhits.p, B.surv.p, B.slow.p, B.mhsp.p, B.mscope.p,
                                   B.keep.p, B.kids.p, B.ohsp.p, B.oscope.p, B.ctr.p, h->tiles.p, h->ref2.p,
                                   h->refx.p, h->pairtab.p, h->index.p, h->pos.p, B.gcnt.p, B.sq.p, B.skm.p,
-                                  h->clean.p, B.qrec.p, h->rrec.p, B.kvalo.p, B.exact.p};
+                                  h->clean.p, B.qrec.p, h->rrec.p, B.kvalo.p, B.exact.p, B.kkey.p};
             static_assert(sizeof ptrs <= sizeof key.ptrs, "graph key pointer slots");
             memcpy(key.ptrs, ptrs, sizeof ptrs);
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow, cap_seeds, per, h->max_hits,

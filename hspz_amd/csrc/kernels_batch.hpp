@@ -460,14 +460,16 @@ static_assert(sizeof(QRec) == 64, "query records are four 16-byte loads");
 
 // Sort keys of the k-mer grouping: every seed slot below cap gets its k-mer
 // when the seed has hits and nk otherwise (so the seeds without hits sort
-// last), and its own index as the value.
+// last); the value is (slot index << 32) | arena position.
 __global__ void k_kkeys(const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ counts,
                         const uint32_t *__restrict__ nvalid, uint32_t per, uint32_t cap, uint32_t nk,
-                        uint32_t *__restrict__ key, uint32_t *__restrict__ val) {
+                        uint32_t *__restrict__ key, uint64_t *__restrict__ val) {
     const uint32_t n = *nvalid * per;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
-        key[i] = i < n && counts[i] ? (uint32_t)(seeds[i] >> 32) : nk;
-        val[i] = i;
+        const bool with_hits = i < n && counts[i];
+        const uint64_t sd = with_hits ? seeds[i] : 0;
+        key[i] = with_hits ? (uint32_t)(sd >> 32) : nk;
+        val[i] = ((uint64_t)i << 32) | (uint32_t)sd;
     }
 }
 
@@ -485,19 +487,18 @@ __global__ void k_kbounds(const uint32_t *__restrict__ key, uint32_t n, uint32_t
 }
 
 // One record per grouped seed, so that a hit's query side costs four loads
-// instead of a window load, alignment and exception test. order[p] is the
-// seed index at grouped slot p.
-__global__ void k_qrecords(const uint32_t *__restrict__ order, const uint64_t *__restrict__ seeds,
-                           const uint32_t *__restrict__ total_p,
+// instead of a window load, alignment and exception test. sorted[p] is the
+// (seed index << 32) | arena position of grouped slot p.
+__global__ void k_qrecords(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ total_p,
                            const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse,
                            uint32_t qlen, uint32_t seed_size, QRec *__restrict__ recs) {
     const uint32_t total = *total_p;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
-        uint32_t i = order[p];
-        uint32_t Q = (uint32_t)seeds[i] + seed_size;
+        const uint64_t f = sorted[p];
+        uint32_t Q = (uint32_t)f + seed_size;
         QRec r;
         r.Q = Q;
-        r.tag = i;
+        r.tag = (uint32_t)(f >> 32);
         r.pad = 0;
         bool ok = Q >= G2_MARGIN && (uint64_t)Q + G2_MARGIN <= qlen;
         uint32_t W[13];
@@ -987,6 +988,13 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
     }
 }
 
+// A 16-byte load marked non-temporal (data read once, streamed past the caches).
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 nt_load4(const uint4 *p) {
+    u32x4_t v = __builtin_nontemporal_load((const u32x4_t *)p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // kmer_gate_merged with every global load issued one iteration before its
 // data is used, so a wave computes one hit while the loads for its next hit
 // and next item are in flight. Each lane holds, besides its current item and
@@ -1102,8 +1110,11 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
         {
             uint32_t e = got ? ce : 0u;
             const uint4 *rr = rrec + 3 * (size_t)e;
-            T0 = rr[0], T1 = rr[1], T2 = rr[2];
-            tpos = src.pos[e];
+            // reference records and pos entries are read once per item: streamed past the caches
+            T0 = nt_load4(rr + 0);
+            T1 = nt_load4(rr + 1);
+            T2 = nt_load4(rr + 2);
+            tpos = __builtin_nontemporal_load(src.pos + e);
             tclean = src.clean[e >> 5];
             fsh = e & 31;
             fs0 = c0;
