@@ -88,7 +88,7 @@ def main():
     ap.add_argument("--tune-reps", type=int, default=1)
     ap.add_argument("--no-io", action="store_true", help="skip the host input/output timing")
     ap.add_argument("--gate", default="",
-                    help="step_cells,block_threads,claim_seeds,grid_blocks[,order] for the batched gate "
+                    help="step_cells,block_threads,claim_seeds,grid_blocks[,order[,lookahead]]] for the batched gate "
                          "(applied after the autotuner; use with --tune-mbp 0 to fix the configuration)")
     args = ap.parse_args()
 
@@ -186,10 +186,13 @@ def main():
         vals = [int(x) for x in args.gate.split(",")]
         step, thr, claim, grid = vals[:4]
         order = vals[4] if len(vals) > 4 else 2
+        ahead = vals[5] if len(vals) > 5 else eng.get_flag("batch_lookahead")
         for name, v in (("batch_step_cells", step), ("batch_block_threads", thr),
-                        ("batch_claim_seeds", claim), ("batch_gate_blocks", grid), ("batch_order", order)):
+                        ("batch_claim_seeds", claim), ("batch_gate_blocks", grid), ("batch_order", order),
+                        ("batch_lookahead", ahead)):
             eng.set_flag(name, v)
-        log("gate fixed: step %d threads %d claim %d grid %d order %d" % (step, thr, claim, grid, order))
+        log("gate fixed: step %d threads %d claim %d grid %d order %d lookahead %d"
+            % (step, thr, claim, grid, order, ahead))
 
     for i in range(args.warmup):
         res_b, st = batched()
@@ -342,7 +345,7 @@ def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rat
     import numpy as np
     eng2 = E.Engine(None, None, ref_enc, sub, max_hits=max_hits, device_index="12of19")
     for name in ("batch_step_cells", "batch_block_threads", "batch_claim_seeds", "batch_gate_blocks",
-                 "batch_order"):
+                 "batch_order", "batch_lookahead"):
         eng2.set_flag(name, eng.get_flag(name))  # the tuned gate configuration
     arms = []
     for k, rate in enumerate(rates):

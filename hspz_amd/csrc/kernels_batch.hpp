@@ -18,6 +18,8 @@
 namespace hspz {
 
 constexpr uint32_t QPAD = 256;       // code-7 bytes before every block and after the last
+constexpr uint32_t WQ_CAP = 64;      // records a wave queues in LDS before moving them to a global list
+constexpr uint32_t WQ_FLUSH = 32;    // queued records that trigger the move
 constexpr uint32_t MAX_CHUNKS = 64;  // max_hits chunks one batch may split into
 
 struct BatchDesc {
@@ -662,6 +664,44 @@ struct KmerSource {
     }
 };
 
+// A wave's LDS queue in front of a global record list (survivors, deferred
+// hits): a push ranks the wave's lanes that append by mbcnt and writes their
+// records to the queue; once WQ_FLUSH or more are queued (or before a push
+// that would not fit), one atomic reserves their slots in the list and the
+// wave copies them out with coalesced stores. One global atomic per flush
+// instead of one per pushing wave keeps the list's counter, which every wave
+// of the grid shares, out of the waves' critical path. The list's contents
+// are those of direct appends, in another order; overflow past `cap` sets
+// bit 8 of `err` as SeedSource::append does.
+struct WaveQueue {
+    uint4 *q;    // WQ_CAP slots of this wave
+    uint32_t n;  // queued records (wave-uniform)
+    SurvRec *list;
+    uint32_t *count, cap;
+
+    __device__ __forceinline__ void flush(uint32_t *err) {
+        if (n == 0) return;
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(count, n);
+        base = __builtin_amdgcn_readfirstlane(base);
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i = lane_id(); i < n; i += WAVE)
+            if (base + i < cap) *(uint4 *)(list + base + i) = q[i];
+        if (lane_id() == 0 && (uint64_t)base + n > cap) atomicOr(err, 8u);
+        __builtin_amdgcn_wave_barrier();
+        n = 0;
+    }
+    __device__ __forceinline__ void push(bool want, uint4 rec, uint32_t *err) {
+        uint64_t m = __ballot(want);
+        if (!m) return;
+        const uint32_t k = (uint32_t)__popcll(m);
+        if (n + k > WQ_CAP) flush(err);
+        if (want) q[n + mbcnt(m)] = rec;
+        n += k;
+        if (n >= WQ_FLUSH) flush(err);
+    }
+};
+
 // The packed gate over k-mer-major items: the same X-drop core and upper-bound
 // argument as packed_gate_loop; reference words are prepared once per item
 // (A0, LA0) and query words once per hit. At 64-cell steps no stream is
@@ -892,6 +932,73 @@ __device__ __forceinline__ void map_words(const uint8_t *s_map, const uint32_t (
     }
 }
 
+// A hit's right stretch (RW words of the 4-word right stream, words [RW, 2 RW)
+// when `rhi`) and left stretch (4 words of the 8-word left stream, the upper
+// half when `lhi`) through the map table, as two map_words calls, with the
+// two sides' words interleaved: each round issues the table reads of one
+// right word and one left word before either side's recurrence consumes them,
+// so one side's dependent adds and mins run while the other's reads are in
+// flight.
+template <int RW>
+__device__ __forceinline__ void map_words_lr(const uint8_t *s_map, const uint32_t (&ra)[G2_RIGHT_WORDS],
+                                             const uint32_t (&qa)[G2_RIGHT_WORDS], bool rhi, uint32_t off_r,
+                                             int32_t &er, int32_t &cr, int32_t &eminr,
+                                             const uint32_t (&rl)[G2_LEFT_WORDS], const uint32_t (&ql)[G2_LEFT_WORDS],
+                                             bool lhi, uint32_t off_l, int32_t &el, int32_t &cl, int32_t &eminl) {
+    static_assert(RW == 4 || RW == 2, "a right stretch is the whole right stream or one of its halves");
+    constexpr uint32_t SEL[4] = {0x0C0C0400u, 0x0C0C0500u, 0x0C0C0600u, 0x0C0C0700u};
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const bool right = h < RW;
+        uint32_t pr[8], pl[8];
+        if (right) {
+            uint32_t r = ra[h], q = qa[h];
+            if constexpr (RW == 2) {
+                r = rhi ? ra[2 + h] : r;
+                q = rhi ? qa[2 + h] : q;
+            }
+            uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
+            uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                pr[2 * j] = map_entry(s_map, t1, off_r, SEL[j]);
+                pr[2 * j + 1] = map_entry(s_map, t2, off_r, SEL[j]);
+            }
+        }
+        {
+            uint32_t r = lhi ? rl[4 + h] : rl[h], q = lhi ? ql[4 + h] : ql[h];
+            uint32_t t1 = (r & 0x0F0F0F0Fu) | ((q << 4) & 0xF0F0F0F0u);
+            uint32_t t2 = ((r >> 4) & 0x0F0F0F0Fu) | (q & 0xF0F0F0F0u);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                pl[2 * j] = map_entry(s_map, t1, off_l, SEL[j]);
+                pl[2 * j + 1] = map_entry(s_map, t2, off_l, SEL[j]);
+            }
+        }
+        if (right) {
+#pragma unroll
+            for (int i = 0; i < 8; i += 2) {
+                int32_t e1 = min(er + lo16(pr[i]), hi16(pr[i]));
+                er = min(e1 + lo16(pr[i + 1]), hi16(pr[i + 1]));
+                eminr = min3_i32(eminr, e1, er);
+            }
+            cr += lo16(pk_add16(pk_add16(pr[0], pr[1]), pk_add16(pr[2], pr[3]))) +
+                  lo16(pk_add16(pk_add16(pr[4], pr[5]), pk_add16(pr[6], pr[7])));
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            int32_t e1 = min(el + lo16(pl[i]), hi16(pl[i]));
+            el = min(e1 + lo16(pl[i + 1]), hi16(pl[i + 1]));
+            eminl = min3_i32(eminl, e1, el);
+        }
+        cl += lo16(pk_add16(pk_add16(pl[0], pl[1]), pk_add16(pl[2], pl[3]))) +
+              lo16(pk_add16(pk_add16(pl[4], pl[5]), pk_add16(pl[6], pl[7])));
+        // finish this round before the next one's reads, so at most two
+        // words' table entries are live
+        asm volatile("" : "+v"(cr), "+v"(er), "+v"(eminr), "+v"(cl), "+v"(el), "+v"(eminl));
+    }
+}
+
 // The k-mer-major gate with merged phases: one iteration evaluates a fresh
 // hit's 64 right cells and its first 64 left cells; a hit whose left
 // extension outlives them takes a second iteration for left cells 64..127.
@@ -1010,11 +1117,12 @@ __device__ __forceinline__ uint4 nt_load4(const uint4 *p) {
 // kmer_gate_merged). RW = 2: right cells 0..31, and a right side that outlives
 // them continues with cells 32..63 in the next iteration, while the left side
 // keeps its 64-cell stretches; fewer cells per hit (a right side mostly drops
-// within 32), the same upper-bound decisions.
+// within 32), the same upper-bound decisions. Survivors and deferred hits go
+// through the wave's LDS queues `sq` and `dq`.
 template <int RW>
 __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__restrict__ rrec,
                                                 const QRec *__restrict__ recs, const uint8_t *s_map, int32_t xdrop,
-                                                int32_t thresh) {
+                                                int32_t thresh, WaveQueue &sq, WaveQueue &dq) {
     const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
     // current item and hit (per side: continuing, done, kept state, maximum)
     bool cur = false, rok = false, has = false, qok = false;
@@ -1059,8 +1167,8 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
             nxt = true;
             fly = false;
         }
-        src.emit(true, e_pass, etag, eR, eQ);
-        src.defer(e_defer, etag, eR, eQ);
+        sq.push(e_pass, make_uint4(eR, eQ, etag, 0u), src.err);
+        dq.push(e_defer, make_uint4(eR, eQ, etag, 0u), src.err);
         e_pass = e_defer = false;
 
         // 2. a lane without a hit starts its item's next one, promoting the
@@ -1132,9 +1240,8 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
         bool done = false, pass = false;
         if (has && !defer) {
             int32_t er = rcont ? er_keep : xdrop, cr = rcont ? cr_keep : 0, eminr = er;
-            map_words<RW, G2_RIGHT_WORDS>(s_map, A0, B, rcont, off_fwd, er, cr, eminr);
             int32_t el = cont ? el_keep : xdrop, cl = cont ? cl_keep : 0, eminl = el;
-            map_words<4, G2_LEFT_WORDS>(s_map, LA0, LB, cont, off_rev, el, cl, eminl);
+            map_words_lr<RW>(s_map, A0, B, rcont, off_fwd, er, cr, eminr, LA0, LB, cont, off_rev, el, cl, eminl);
             if (!rdone) {
                 if (eminr < 0) {
                     rdone = true;
@@ -1173,6 +1280,8 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
             has = false;
         }
     }
+    sq.flush(src.err);
+    dq.flush(src.err);
 }
 
 // The lookahead k-mer-major gate (kmer_gate_ahead<RW>); same arguments as
@@ -1189,6 +1298,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     uint32_t *__restrict__ surv_n, uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n,
     uint32_t cap_slow, uint32_t *__restrict__ err) {
     __shared__ uint32_t s_map[G2_MAP_WORDS];
+    __shared__ uint4 s_q[2][512 / WAVE][WQ_CAP];
     load_map_table(s_map, pairtab, 512);
     KmerSource src;
     src.skm = skm;
@@ -1209,7 +1319,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     src.err = err;
     src.init();
     (void)rplane;
-    kmer_gate_ahead<RW>(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+    WaveQueue sq{s_q[0][threadIdx.x / WAVE], 0u, surv, surv_n, cap_surv};
+    WaveQueue dq{s_q[1][threadIdx.x / WAVE], 0u, slow, slow_n, cap_slow};
+    kmer_gate_ahead<RW>(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh, sq, dq);
 }
 
 // The batched gate in k-mer-major order (same launch shape as k_bgate_packed).
