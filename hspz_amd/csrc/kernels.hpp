@@ -40,6 +40,61 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// A wave's LDS queue in front of a global list that many waves append to
+// (deferred hits, survivors): a push ranks the wave's appending lanes by mbcnt
+// and writes their records to the queue; once WQ_FLUSH or more are queued (or
+// before a push that would not fit), one atomic reserves their slots in the
+// list and the wave copies them out with coalesced stores. One global atomic
+// per flush instead of one per appending wave keeps the list's counter, which
+// the whole grid shares, off the waves' critical path. The list holds what
+// direct appends would write, in another order. Slots past `cap` are dropped
+// and set bit 8 of `*err` when `err` is given. A null `q` appends directly.
+constexpr uint32_t WQ_CAP = 64;    // slots per wave
+constexpr uint32_t WQ_FLUSH = 32;  // queued records that trigger a flush
+
+template <class T>
+struct WaveQueue {
+    T *q;        // WQ_CAP LDS slots of this wave, or null
+    uint32_t n;  // queued records (wave-uniform)
+    T *list;
+    uint32_t *count, cap;
+
+    __device__ __forceinline__ void reserve_copy(uint32_t k, const T *src, uint32_t *err) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(count, k);
+        base = __builtin_amdgcn_readfirstlane(base);
+        for (uint32_t i = lane_id(); i < k; i += WAVE)
+            if (base + i < cap) list[base + i] = src[i];
+        if (err && lane_id() == 0 && (uint64_t)base + k > cap) atomicOr(err, 8u);
+    }
+    __device__ __forceinline__ void flush(uint32_t *err) {
+        if (n == 0) return;
+        __builtin_amdgcn_wave_barrier();
+        reserve_copy(n, q, err);
+        __builtin_amdgcn_wave_barrier();
+        n = 0;
+    }
+    __device__ __forceinline__ void push(bool want, T rec, uint32_t *err) {
+        uint64_t m = __ballot(want);
+        if (!m) return;
+        const uint32_t k = (uint32_t)__popcll(m);
+        if (!q) {  // direct: one atomic for this push
+            uint32_t leader = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+            uint32_t base = 0;
+            if (lane_id() == leader) base = atomicAdd(count, k);
+            base = __shfl(base, leader, WAVE);
+            uint32_t at = base + mbcnt(m);
+            if (want && at < cap) list[at] = rec;
+            if (err && lane_id() == leader && (uint64_t)base + k > cap) atomicOr(err, 8u);
+            return;
+        }
+        if (n + k > WQ_CAP) flush(err);
+        if (want) q[n + mbcnt(m)] = rec;
+        n += k;
+        if (n >= WQ_FLUSH) flush(err);
+    }
+};
+
 // Inclusive add-scan across the wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const uint32_t lane = lane_id();
@@ -670,16 +725,17 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
         src.defer(defer, tag, R, Q);
         if (done || defer) has = false;
     }
+    src.finish();
 }
 
 // Hit source over an anchor array (the plain path): per-wave queue of
 // GATE_CHUNK-hit ranges, flags written for every hit, deferred hit ids
-// appended to `slow`.
+// appended to `slow` through the wave's LDS queue.
 struct AnchorSource {
     const uint64_t *anchors;
     uint32_t n, nwaves, chunk, q_next, q_end;
     uint8_t *flags;
-    uint32_t *slow, *slow_n;
+    WaveQueue<uint32_t> dq;
 
     __device__ __forceinline__ void init() {
         nwaves = gridDim.x * (blockDim.x / WAVE);
@@ -715,15 +771,8 @@ struct AnchorSource {
     __device__ __forceinline__ void emit(bool done, bool pass, uint32_t tag, uint32_t, uint32_t) {
         if (done) flags[tag] = pass ? 1 : 0;
     }
-    __device__ __forceinline__ void defer(bool d, uint32_t tag, uint32_t, uint32_t) {
-        uint64_t dm = __ballot(d);
-        if (!dm) return;
-        uint32_t leader = (uint32_t)(__ffsll((unsigned long long)dm) - 1);
-        uint32_t base = 0;
-        if (lane_id() == leader) base = atomicAdd(slow_n, (uint32_t)__popcll(dm));
-        base = __shfl(base, leader, WAVE);
-        if (d) slow[base + mbcnt(dm)] = tag;
-    }
+    __device__ __forceinline__ void defer(bool d, uint32_t tag, uint32_t, uint32_t) { dq.push(d, tag, nullptr); }
+    __device__ __forceinline__ void finish() { dq.flush(nullptr); }
 };
 
 __global__ __launch_bounds__(512) void k_gate_packed(
@@ -733,13 +782,13 @@ __global__ __launch_bounds__(512) void k_gate_packed(
     const uint64_t *__restrict__ anchors, uint32_t n, uint8_t *__restrict__ flags,
     uint32_t *__restrict__ slow, uint32_t *__restrict__ slow_n) {
     __shared__ uint32_t s_map[G2_MAP_WORDS];
+    __shared__ uint32_t s_dq[512 / WAVE][WQ_CAP];
     load_map_table(s_map, pairtab, blockDim.x);
     AnchorSource src;
     src.anchors = anchors;
     src.n = n;
     src.flags = flags;
-    src.slow = slow;
-    src.slow_n = slow_n;
+    src.dq = {s_dq[threadIdx.x / WAVE], 0u, slow, slow_n, n};
     src.init();
     packed_gate_loop<2>(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, (const uint8_t *)s_map, xdrop,
                         thresh);

@@ -18,8 +18,6 @@
 namespace hspz {
 
 constexpr uint32_t QPAD = 256;       // code-7 bytes before every block and after the last
-constexpr uint32_t WQ_CAP = 64;      // records a wave queues in LDS before moving them to a global list
-constexpr uint32_t WQ_FLUSH = 32;    // queued records that trigger the move
 constexpr uint32_t MAX_CHUNKS = 64;  // max_hits chunks one batch may split into
 
 struct BatchDesc {
@@ -257,6 +255,7 @@ struct SeedSource {
     SurvRec *surv, *slow;
     uint32_t *surv_n, *slow_n, cap_surv, cap_slow;
     uint32_t *err;
+    WaveQueue<uint4> sq, dq;  // survivors, deferred hits
 
     // Claims the wave's next seed range [s_cur, s_end); false once every
     // seed has been claimed.
@@ -333,10 +332,20 @@ struct SeedSource {
         if (lane_id() == leader && (uint64_t)base + (uint64_t)__popcll(m) > cap) atomicOr(err, 8u);
     }
     __device__ __forceinline__ void emit(bool done, bool pass, uint32_t tag, uint32_t R, uint32_t Q) {
-        append(done && pass, surv, surv_n, cap_surv, err, R, Q, tag);
+        sq.push(done && pass, make_uint4(R, Q, tag, 0u), err);
     }
     __device__ __forceinline__ void defer(bool d, uint32_t tag, uint32_t R, uint32_t Q) {
-        append(d, slow, slow_n, cap_slow, err, R, Q, tag);
+        dq.push(d, make_uint4(R, Q, tag, 0u), err);
+    }
+    __device__ __forceinline__ void finish() {
+        sq.flush(err);
+        dq.flush(err);
+    }
+    // The lists and queues of a gate kernel: survivors appended directly (or
+    // queued when `sq_slots` is given), deferred hits queued in `dq_slots`.
+    __device__ __forceinline__ void set_lists(uint4 *sq_slots, uint4 *dq_slots) {
+        sq = {sq_slots, 0u, (uint4 *)surv, surv_n, cap_surv};
+        dq = {dq_slots, 0u, (uint4 *)slow, slow_n, cap_slow};
     }
 };
 
@@ -356,6 +365,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow,
     uint32_t *__restrict__ err) {
     __shared__ uint32_t s_map[G2_MAP_WORDS];
+    __shared__ uint4 s_dq[BT / WAVE][WQ_CAP];
     load_map_table(s_map, pairtab, BT);
     SeedSource src;
     src.seeds = seeds;
@@ -372,6 +382,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.slow_n = slow_n;
     src.cap_slow = cap_slow;
     src.err = err;
+    src.set_lists(nullptr, s_dq[threadIdx.x / WAVE]);
     src.init();
     packed_gate_loop<SW>(src, rplane, rcoarse, ref_len, qplane, qcoarse, qlen, (const uint8_t *)s_map, xdrop,
                          thresh);
@@ -550,6 +561,7 @@ struct KmerSource {
     SurvRec *surv, *slow;
     uint32_t *surv_n, *slow_n, cap_surv, cap_slow;
     uint32_t *err;
+    WaveQueue<uint4> sq, dq;  // survivors, deferred hits
 
     __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -657,48 +669,18 @@ struct KmerSource {
         return got;
     }
     __device__ __forceinline__ void emit(bool done, bool pass, uint32_t tag, uint32_t R, uint32_t Q) {
-        SeedSource::append(done && pass, surv, surv_n, cap_surv, err, R, Q, tag);
+        sq.push(done && pass, make_uint4(R, Q, tag, 0u), err);
     }
     __device__ __forceinline__ void defer(bool d, uint32_t tag, uint32_t R, uint32_t Q) {
-        SeedSource::append(d, slow, slow_n, cap_slow, err, R, Q, tag);
+        dq.push(d, make_uint4(R, Q, tag, 0u), err);
     }
-};
-
-// A wave's LDS queue in front of a global record list (survivors, deferred
-// hits): a push ranks the wave's lanes that append by mbcnt and writes their
-// records to the queue; once WQ_FLUSH or more are queued (or before a push
-// that would not fit), one atomic reserves their slots in the list and the
-// wave copies them out with coalesced stores. One global atomic per flush
-// instead of one per pushing wave keeps the list's counter, which every wave
-// of the grid shares, out of the waves' critical path. The list's contents
-// are those of direct appends, in another order; overflow past `cap` sets
-// bit 8 of `err` as SeedSource::append does.
-struct WaveQueue {
-    uint4 *q;    // WQ_CAP slots of this wave
-    uint32_t n;  // queued records (wave-uniform)
-    SurvRec *list;
-    uint32_t *count, cap;
-
-    __device__ __forceinline__ void flush(uint32_t *err) {
-        if (n == 0) return;
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(count, n);
-        base = __builtin_amdgcn_readfirstlane(base);
-        __builtin_amdgcn_wave_barrier();
-        for (uint32_t i = lane_id(); i < n; i += WAVE)
-            if (base + i < cap) *(uint4 *)(list + base + i) = q[i];
-        if (lane_id() == 0 && (uint64_t)base + n > cap) atomicOr(err, 8u);
-        __builtin_amdgcn_wave_barrier();
-        n = 0;
+    __device__ __forceinline__ void finish() {
+        sq.flush(err);
+        dq.flush(err);
     }
-    __device__ __forceinline__ void push(bool want, uint4 rec, uint32_t *err) {
-        uint64_t m = __ballot(want);
-        if (!m) return;
-        const uint32_t k = (uint32_t)__popcll(m);
-        if (n + k > WQ_CAP) flush(err);
-        if (want) q[n + mbcnt(m)] = rec;
-        n += k;
-        if (n >= WQ_FLUSH) flush(err);
+    __device__ __forceinline__ void set_lists(uint4 *sq_slots, uint4 *dq_slots) {
+        sq = {sq_slots, 0u, (uint4 *)surv, surv_n, cap_surv};
+        dq = {dq_slots, 0u, (uint4 *)slow, slow_n, cap_slow};
     }
 };
 
@@ -851,6 +833,7 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
         src.defer(defer, tag, R, Q);
         if (done || defer) has = false;
     }
+    src.finish();
 }
 
 // min(a, b, c) as one v_min3_i32 (the compiler otherwise builds min trees
@@ -1093,6 +1076,7 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
             cont = false;
         }
     }
+    src.finish();
 }
 
 // A 16-byte load marked non-temporal (data read once, streamed past the caches).
@@ -1117,12 +1101,11 @@ __device__ __forceinline__ uint4 nt_load4(const uint4 *p) {
 // kmer_gate_merged). RW = 2: right cells 0..31, and a right side that outlives
 // them continues with cells 32..63 in the next iteration, while the left side
 // keeps its 64-cell stretches; fewer cells per hit (a right side mostly drops
-// within 32), the same upper-bound decisions. Survivors and deferred hits go
-// through the wave's LDS queues `sq` and `dq`.
+// within 32), the same upper-bound decisions.
 template <int RW>
 __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__restrict__ rrec,
                                                 const QRec *__restrict__ recs, const uint8_t *s_map, int32_t xdrop,
-                                                int32_t thresh, WaveQueue &sq, WaveQueue &dq) {
+                                                int32_t thresh) {
     const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
     // current item and hit (per side: continuing, done, kept state, maximum)
     bool cur = false, rok = false, has = false, qok = false;
@@ -1167,8 +1150,8 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
             nxt = true;
             fly = false;
         }
-        sq.push(e_pass, make_uint4(eR, eQ, etag, 0u), src.err);
-        dq.push(e_defer, make_uint4(eR, eQ, etag, 0u), src.err);
+        src.emit(true, e_pass, etag, eR, eQ);
+        src.defer(e_defer, etag, eR, eQ);
         e_pass = e_defer = false;
 
         // 2. a lane without a hit starts its item's next one, promoting the
@@ -1280,8 +1263,7 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
             has = false;
         }
     }
-    sq.flush(src.err);
-    dq.flush(src.err);
+    src.finish();
 }
 
 // The lookahead k-mer-major gate (kmer_gate_ahead<RW>); same arguments as
@@ -1317,11 +1299,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     src.slow_n = slow_n;
     src.cap_slow = cap_slow;
     src.err = err;
+    src.set_lists(s_q[0][threadIdx.x / WAVE], s_q[1][threadIdx.x / WAVE]);
     src.init();
     (void)rplane;
-    WaveQueue sq{s_q[0][threadIdx.x / WAVE], 0u, surv, surv_n, cap_surv};
-    WaveQueue dq{s_q[1][threadIdx.x / WAVE], 0u, slow, slow_n, cap_slow};
-    kmer_gate_ahead<RW>(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh, sq, dq);
+    kmer_gate_ahead<RW>(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
 // The batched gate in k-mer-major order (same launch shape as k_bgate_packed).
@@ -1336,6 +1317,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     uint32_t *__restrict__ surv_n, uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n,
     uint32_t cap_slow, uint32_t *__restrict__ err) {
     __shared__ uint32_t s_map[G2_MAP_WORDS];
+    __shared__ uint4 s_dq[BT / WAVE][WQ_CAP];
     load_map_table(s_map, pairtab, BT);
     KmerSource src;
     src.skm = skm;
@@ -1354,6 +1336,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.slow_n = slow_n;
     src.cap_slow = cap_slow;
     src.err = err;
+    src.set_lists(nullptr, s_dq[threadIdx.x / WAVE]);
     src.init();
     if constexpr (SW == 8) kmer_gate_merged(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
     else kmer_gate_loop<SW>(src, rplane, recs, (const uint8_t *)s_map, xdrop, thresh);
