@@ -49,11 +49,13 @@ KGateFn kgate_kernel(int step_cells, int threads) {
     if (threads == 1024)
         return step_cells == 16    ? k_bgate_kmer<1, 1024>
                : step_cells == 64  ? k_bgate_kmer<4, 1024>
-               : step_cells >= 96  ? k_bgate_kmer<8, 1024>
+               : step_cells == 96  ? k_bgate_kmer<6, 1024>
+               : step_cells == 128 ? k_bgate_kmer<8, 1024>
                                    : k_bgate_kmer<2, 1024>;
     return step_cells == 16    ? k_bgate_kmer<1, 512>
            : step_cells == 64  ? k_bgate_kmer<4, 512>
-           : step_cells >= 96  ? k_bgate_kmer<8, 512>
+           : step_cells == 96  ? k_bgate_kmer<6, 512>
+           : step_cells == 128 ? k_bgate_kmer<8, 512>
                                : k_bgate_kmer<2, 512>;
 }
 

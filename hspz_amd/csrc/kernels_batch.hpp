@@ -983,20 +983,25 @@ __device__ __forceinline__ void map_words_lr(const uint8_t *s_map, const uint32_
 }
 
 // The k-mer-major gate with merged phases: one iteration evaluates a fresh
-// hit's 64 right cells and its first 64 left cells; a hit whose left
-// extension outlives them takes a second iteration for left cells 64..127.
-// It evaluates exactly the cells of kmer_gate_loop<4> and reaches the same
-// decisions, with about half the iterations per hit (the refill and
-// bookkeeping of an iteration are paid once per hit) and both table offsets
-// fixed per stretch.
-template <class Src>
+// hit's first right stretch and its first 64 left cells; a hit whose left
+// extension outlives them takes another iteration for left cells 64..127.
+// RW = 4: the right stretch is cells 0..63, so the iteration evaluates
+// exactly the cells of kmer_gate_loop<4> and reaches the same decisions,
+// with about half the iterations per hit (the refill and bookkeeping of an
+// iteration are paid once per hit) and both table offsets fixed per stretch.
+// RW = 2: right cells 0..31, and a right side that outlives them continues
+// with cells 32..63 in the next iteration (a random right extension mostly
+// drops within 32 cells, so most hits skip 32 cells of work), with the same
+// upper-bound decisions.
+template <int RW, class Src>
 __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__restrict__ rplane,
                                                  const uint4 *__restrict__ rrec, const QRec *__restrict__ recs,
                                                  const uint8_t *s_map, int32_t xdrop, int32_t thresh) {
-    bool has = false, item = false, rok = false, cont = false;
+    static_assert(RW == 4 || RW == 2, "a right stretch is the whole right stream or one of its halves");
+    bool has = false, item = false, rok = false, cont = false, rcont = false, rdone = false, ldone = false;
     uint32_t R = 0, Q = 0, tag = 0, s_cur = 0, s_end = 0;
     const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
-    int32_t Mr = 0, el_keep = 0, cl_keep = 0;
+    int32_t Mr = 0, Ml = 0, el_keep = 0, cl_keep = 0, er_keep = 0, cr_keep = 0;
     uint32_t A0[G2_RIGHT_WORDS], LA0[G2_LEFT_WORDS], B[G2_RIGHT_WORDS], LB[G2_LEFT_WORDS];
 #pragma unroll
     for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) A0[j] = B[j] = 0;
@@ -1041,25 +1046,33 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
             LB[4] = v2.x, LB[5] = v2.y, LB[6] = v2.z, LB[7] = v2.w;
             Q = v3.x;
             tag = v3.y;
-            cont = false;
+            cont = rcont = rdone = ldone = false;
             if (!(rok && v3.z != 0)) defer = true;
         }
 
         if (has && !defer) {
-            // right cells 0..63 (used by fresh hits only)
-            int32_t er = xdrop, cr = 0, eminr = xdrop;
-            map_stretch<4>(s_map, A0, B, false, off_fwd, er, cr, eminr);
-            // left cells 0..63, or 64..127 for a continuing hit
+            // the right stretch: cells 0..16 RW-1, or the next ones for a continuing right side
+            int32_t er = rcont ? er_keep : xdrop, cr = rcont ? cr_keep : 0, eminr = er;
+            map_words<RW, G2_RIGHT_WORDS>(s_map, A0, B, rcont, off_fwd, er, cr, eminr);
+            // left cells 0..63, or 64..127 for a continuing left side
             int32_t el = cont ? el_keep : xdrop, cl = cont ? cl_keep : 0, eminl = el;
             map_stretch<8>(s_map, LA0, LB, cont, off_rev, el, cl, eminl);
-            if (!cont) {
-                if (eminr >= 0) defer = true;  // the right extension outlived its 64 cells
-                else Mr = cr + (xdrop - er);
+            if (!rdone) {
+                if (eminr < 0) {
+                    rdone = true;
+                    Mr = cr + (xdrop - er);
+                } else if (rcont || RW == (int)G2_RIGHT_WORDS) {
+                    defer = true;  // the right extension outlived its 64 cells
+                } else {
+                    rcont = true;
+                    er_keep = er;
+                    cr_keep = cr;
+                }
             }
-            if (!defer) {
+            if (!defer && !ldone) {
                 if (eminl < 0) {
-                    done = true;
-                    pass = gate_keep(Mr + cl + (xdrop - el), thresh);
+                    ldone = true;
+                    Ml = cl + (xdrop - el);
                 } else if (cont) {
                     defer = true;  // the left extension outlived its 128 cells
                 } else {
@@ -1068,12 +1081,16 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
                     cl_keep = cl;
                 }
             }
+            if (!defer && rdone && ldone) {
+                done = true;
+                pass = gate_keep(Mr + Ml, thresh);
+            }
         }
         src.emit(done, pass, tag, R, Q);
         src.defer(defer, tag, R, Q);
         if (done || defer) {
             has = false;
-            cont = false;
+            cont = rcont = false;
         }
     }
     src.finish();
@@ -1305,7 +1322,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     kmer_gate_ahead<RW>(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
-// The batched gate in k-mer-major order (same launch shape as k_bgate_packed).
+// The batched gate in k-mer-major order (same launch shape as k_bgate_packed):
+// the merged gate for SW = 8 (128-cell steps) and SW = 6 (96-cell steps, 32
+// right cells at a time), kmer_gate_loop<SW> for the others.
 template <int SW, int BT>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_kmer(
     const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop,
@@ -1338,7 +1357,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.err = err;
     src.set_lists(nullptr, s_dq[threadIdx.x / WAVE]);
     src.init();
-    if constexpr (SW == 8) kmer_gate_merged(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+    if constexpr (SW == 8) kmer_gate_merged<4>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+    else if constexpr (SW == 6) kmer_gate_merged<2>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
     else kmer_gate_loop<SW>(src, rplane, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 

@@ -151,9 +151,8 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           as are resident at once */
 #define HSPZ_FLAG_BATCH_STEP_CELLS 2   /* cells per gate step: 16, 32, 64, 96 or 128 (default). 128: 64
                                           right + 64 left cells of a hit in one step; 96: 32 right + 64
-                                          left, the right side continuing with its next 32 cells (lookahead
-                                          gate only; the others run it as 128). Seed-major order runs both
-                                          as 64 */
+                                          left, the right side continuing with its next 32 cells.
+                                          Seed-major order runs both as 64 */
 #define HSPZ_FLAG_BATCH_BLOCK_THREADS 3 /* threads per gate block: 512 (default) or 1024 */
 #define HSPZ_FLAG_BATCH_CLAIM_SEEDS 4  /* seeds a wave claims from the work counter at a time (1..4096,
                                           default 256) */

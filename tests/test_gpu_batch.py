@@ -177,7 +177,8 @@ def test_batch_gate_configs_match(gpu_available):
                 (64, 512, 64, 7, 1, 1), (32, 1024, 1, 5000, 1, 1), (128, 1024, 1, 0, 1, 1), (128, 512, 256, 3, 1, 0),
                 (128, 1024, 64, 0, 0, 1), (128, 512, 256, 0, 1, 1), (128, 512, 1, 5000, 1, 1),
                 (128, 512, 4096, 1, 1, 1), (128, 512, 16, 7, 1, 1), (96, 512, 256, 0, 1, 1),
-                (96, 512, 1, 3, 1, 1), (96, 1024, 64, 0, 1, 1), (96, 512, 64, 0, 0, 1), (96, 512, 256, 0, 1, 0)]
+                (96, 512, 1, 3, 1, 1), (96, 1024, 64, 0, 1, 1), (96, 512, 64, 0, 0, 1), (96, 512, 256, 0, 1, 0),
+                (96, 1024, 1, 5000, 1, 1), (96, 1024, 4096, 3, 1, 1)]
     with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
         for step, threads, claim, blocks_n, order, ahead in variants:
             _say("  step %d threads %d claim %d grid %d order %d lookahead %d"
