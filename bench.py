@@ -210,7 +210,7 @@ def main():
     order_used = eng.get_flag("batch_order_used")
     if not order_used:
         gate_kernel = "k_bgate_packed"
-    elif (eng.get_flag("batch_step_cells") == 128 and eng.get_flag("batch_block_threads") == 512
+    elif (eng.get_flag("batch_step_cells") >= 96 and eng.get_flag("batch_block_threads") == 512
           and eng.get_flag("batch_lookahead") and eng.get_flag("ref_records") == 2):
         gate_kernel = "k_bgate_ahead"
     else:

@@ -100,7 +100,7 @@ struct hspz_batch_state {
 namespace {
 
 // Builds the engine's reference records (k_ref_records) on first use, when a
-// gate wants them (HSPZ_FLAG_REF_RECORDS, or the lookahead gate) and device
+// gate wants them (HSPZ_FLAG_REF_RECORDS, or a merged-step gate) and device
 // memory leaves room for them and a margin; otherwise the k-mer-major gate
 // reads its windows from the plane.
 int ensure_ref_records(hspz_engine *h, bool want) {
@@ -331,7 +331,8 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             if ((rc = reserve(h, B.kvalo, cap_seeds * 8 + 8))) return rc;  // and out
             if ((rc = reserve(h, B.skm, cap_seeds * 4 + 4))) return rc;
             if ((rc = reserve(h, B.qrec, cap_seeds * sizeof(QRec) + sizeof(QRec)))) return rc;
-            if ((rc = ensure_ref_records(h, h->use_rrec || lookahead_wanted(h)))) return rc;
+            if ((rc = ensure_ref_records(h, h->rrec_mode > 0 || (h->rrec_mode == 0 && h->bgate_step >= 96))))
+                return rc;
         }
         uint32_t *kkey = kmer_order ? B.kkey.as<uint32_t>() : nullptr;
         uint64_t *kval = kmer_order ? B.sq.as<uint64_t>() : nullptr;

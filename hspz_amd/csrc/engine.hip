@@ -95,7 +95,8 @@ struct hspz_engine {
     int batch_order_used = 0;               // order of the last batched pass
     DevBuf clean;                           // pos_table-aligned clean-window bits (k-mer-major gate)
     DevBuf rrec;                            // pos_table-ordered reference records (k_ref_records)
-    bool use_rrec = false, rrec_declined = false;
+    int rrec_mode = 0;  // HSPZ_FLAG_REF_RECORDS: -1 never, 0 merged steps, 1 every k-mer-major gate
+    bool rrec_declined = false;
     bool lookahead = true;                  // k_bgate_ahead for merged steps (needs rrec)
     bool sep_drops = false;                 // code 7 scores <= -(xdrop+1) against every code
     int bgate_grid_used = 0;                // grid of the last batched launch
@@ -827,8 +828,9 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
     case HSPZ_FLAG_BATCH_DEVICE_DEDUP: h->device_dedup = value != 0; break;
     case HSPZ_FLAG_BATCH_LOOKAHEAD: h->lookahead = value != 0; break;
     case HSPZ_FLAG_REF_RECORDS:
-        h->use_rrec = value != 0;
-        if (!h->use_rrec && h->rrec.p) {
+        if (value < -1 || value > 1) return set_error(HSPZ_E_ARG, "ref_records must be -1, 0 or 1");
+        h->rrec_mode = value;
+        if (value <= 0 && h->rrec.p) {
             (void)hipStreamSynchronize(h->stream);
             std::lock_guard<std::mutex> lk(capture_mutex());
             release(h->rrec);
@@ -852,7 +854,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_ORDER: *value = h->batch_order; break;
     case HSPZ_FLAG_BATCH_ORDER_USED: *value = h->batch_order_used; break;
     case HSPZ_FLAG_BATCH_DEVICE_DEDUP: *value = h->device_dedup ? 1 : 0; break;
-    case HSPZ_FLAG_REF_RECORDS: *value = h->rrec.p ? 2 : h->use_rrec ? 1 : 0; break;
+    case HSPZ_FLAG_REF_RECORDS: *value = h->rrec.p ? 2 : h->rrec_mode; break;
     case HSPZ_FLAG_BATCH_LOOKAHEAD: *value = h->lookahead ? 1 : 0; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }

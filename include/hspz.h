@@ -165,11 +165,12 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           (stable radix sorts of all scopes at once); 0: on host threads */
 #define HSPZ_FLAG_REF_RECORDS 8        /* reference records: 48 B per indexed position in pos_table order
                                           (the gate's reference words, aligned), built on first use when
-                                          device memory allows. The lookahead gate builds and reads them
-                                          whenever it runs; 1: the merged-step gate without lookahead wants
-                                          them too; 0 (default): only the lookahead gate does, and setting
-                                          0 releases built ones. hspz_get_flag reports 2 once they are built.
-                                          Any k-mer-major gate reads them when they exist */
+                                          device memory allows. The merged-step gates (96- and 128-cell
+                                          steps) build and read them whenever they run; 1: the narrower
+                                          steps want them too; 0 (default): only the merged steps do;
+                                          -1: none does (the gates read windows from the plane). Setting 0
+                                          or -1 releases built ones. hspz_get_flag reports 2 once they are
+                                          built. Any k-mer-major gate reads them when they exist */
 #define HSPZ_FLAG_BATCH_LOOKAHEAD 9    /* 1 (default): with 128-cell steps and 512-thread blocks, the
                                           k-mer-major gate issues each hit's and each item's loads one
                                           iteration ahead of their use; it reads the reference records

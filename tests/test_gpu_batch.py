@@ -196,13 +196,28 @@ def test_batch_gate_configs_match(gpu_available):
                 assert eng.get_flag("batch_gate_blocks") == blocks_n
             else:
                 assert eng.get_flag("batch_gate_blocks") >= 256
-            if step >= 96 and threads == 512 and order == 1 and ahead:
-                assert eng.get_flag("ref_records") == 2  # the lookahead gate's records were built
+            if step >= 96 and order == 1:
+                assert eng.get_flag("ref_records") == 2  # a merged-step gate builds the records
             for i, (g, w) in enumerate(zip(got, want)):
                 assert same_intervals(g, w), "chunk %d differs at step %d claim %d" % (i, step, claim)
+        # merged steps with their reference windows read from the plane (no records)
+        eng.set_flag("ref_records", -1)
+        assert eng.get_flag("ref_records") == -1
+        for step, threads, claim in [(128, 1024, 64), (96, 1024, 64), (96, 512, 256), (128, 512, 1)]:
+            _say("  step %d threads %d claim %d without reference records" % (step, threads, claim))
+            eng.set_flag("batch_order", 1)
+            eng.set_flag("batch_step_cells", step)
+            eng.set_flag("batch_block_threads", threads)
+            eng.set_flag("batch_claim_seeds", claim)
+            eng.set_flag("batch_gate_blocks", 0)
+            got, _ = eng.filter_batch(chunks)
+            assert eng.get_flag("ref_records") == -1
+            for i, (g, w) in enumerate(zip(got, want)):
+                assert same_intervals(g, w), "chunk %d differs at step %d (plane windows)" % (i, step)
+        eng.set_flag("ref_records", 0)
         for name, bad in [("batch_step_cells", 24), ("batch_step_cells", 48), ("batch_block_threads", 256),
                           ("batch_claim_seeds", 0), ("batch_gate_blocks", -1), ("batch_order", 3),
-                          ("batch_order_used", 1)]:
+                          ("batch_order_used", 1), ("ref_records", 2), ("ref_records", -2)]:
             with pytest.raises(E.HspzError) as ei:
                 eng.set_flag(name, bad)
             assert ei.value.status == 2
