@@ -284,8 +284,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": "profiles/kmer_gate_1gbp_target.md (FETCH_SIZE x2.0 + WRITE_SIZE, "
                                   "same workload)" if traffic else None,
-                "binding_limit": "vector-memory pipeline (reference-record misses through L1), VALU at "
-                                 "69% of issue, HBM at 7.5 B/hit (profiles/kmer_gate_1gbp_target.md)"
+                "binding_limit": "VALU issue: the vector ALU is saturated by the X-drop cell pairs; HBM at "
+                                 "7.8 B/hit (profiles/kmer_gate_1gbp_target.md)"
                                  if order_used else "L2-miss bandwidth of random reference lines",
                 "kernel": "%s (+ k_bgate_long, k_bgate_wave for deferred hits)" % gate_kernel,
                 "hit_order": "k-mer-major" if order_used else "seed-major",
