@@ -501,41 +501,56 @@ __global__ void k_kbounds(const uint32_t *__restrict__ key, uint32_t n, uint32_t
 
 // One record per grouped seed, so that a hit's query side costs four loads
 // instead of a window load, alignment and exception test. sorted[p] is the
-// (seed index << 32) | arena position of grouped slot p.
-__global__ void k_qrecords(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ total_p,
-                           const uint32_t *__restrict__ qplane, const uint32_t *__restrict__ qcoarse,
-                           uint32_t qlen, uint32_t seed_size, QRec *__restrict__ recs) {
+// (seed index << 32) | arena position of grouped slot p. A block builds 256
+// consecutive records in LDS (80-byte rows, so neither the row writes nor the
+// column reads meet more than two-way bank conflicts) and stores them as
+// one contiguous 16 KiB run.
+__global__ __launch_bounds__(256) void k_qrecords(const uint64_t *__restrict__ sorted,
+                                                  const uint32_t *__restrict__ total_p,
+                                                  const uint32_t *__restrict__ qplane,
+                                                  const uint32_t *__restrict__ qcoarse, uint32_t qlen,
+                                                  uint32_t seed_size, QRec *__restrict__ recs) {
+    constexpr uint32_t ROW = 5;  // uint4 slots per staged record
+    __shared__ uint4 s_rec[256 * ROW];
     const uint32_t total = *total_p;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
-        const uint64_t f = sorted[p];
-        uint32_t Q = (uint32_t)f + seed_size;
-        QRec r;
-        r.Q = Q;
-        r.tag = (uint32_t)(f >> 32);
-        r.pad = 0;
-        bool ok = Q >= G2_MARGIN && (uint64_t)Q + G2_MARGIN <= qlen;
-        uint32_t W[13];
+    for (uint32_t base = blockIdx.x * 256; base < total; base += gridDim.x * 256) {
+        const uint32_t p = base + threadIdx.x;
+        if (p < total) {
+            const uint64_t f = sorted[p];
+            uint32_t Q = (uint32_t)f + seed_size;
+            QRec r;
+            r.Q = Q;
+            r.tag = (uint32_t)(f >> 32);
+            r.pad = 0;
+            bool ok = Q >= G2_MARGIN && (uint64_t)Q + G2_MARGIN <= qlen;
+            uint32_t W[13];
 #pragma unroll
-        for (int k = 0; k < 13; k++) W[k] = 0;
-        if (ok) {
-            const uint32_t qlo = Q - 16 * G2_LEFT_WORDS, qhi = Q + 16 * G2_RIGHT_WORDS - 1;
-            const uint32_t *qc = qcoarse + (qlo >> 11);
-            ok = !coarse_bits(qc[0], qc[1], qlo, qhi);
-            const uint32_t *qp = qplane + (Q >> 4) - G2_LEFT_WORDS;
+            for (int k = 0; k < 13; k++) W[k] = 0;
+            if (ok) {
+                const uint32_t qlo = Q - 16 * G2_LEFT_WORDS, qhi = Q + 16 * G2_RIGHT_WORDS - 1;
+                const uint32_t *qc = qcoarse + (qlo >> 11);
+                ok = !coarse_bits(qc[0], qc[1], qlo, qhi);
+                const uint32_t *qp = qplane + (Q >> 4) - G2_LEFT_WORDS;
 #pragma unroll
-            for (int k = 0; k < 13; k++) W[k] = qp[k];  // word 8 holds base Q
+                for (int k = 0; k < 13; k++) W[k] = qp[k];  // word 8 holds base Q
+            }
+            uint32_t sq2 = 2 * (Q & 15);
+#pragma unroll
+            for (int k = 0; k < (int)G2_RIGHT_WORDS; k++)
+                r.right[k] = __builtin_amdgcn_alignbit(W[9 + k], W[8 + k], sq2);
+#pragma unroll
+            for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
+                r.left[k] = nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sq2)));
+            r.ok = ok ? 1u : 0u;
+            const uint4 *s = (const uint4 *)&r;
+#pragma unroll
+            for (int k = 0; k < 4; k++) s_rec[threadIdx.x * ROW + k] = s[k];
         }
-        uint32_t sq2 = 2 * (Q & 15);
-#pragma unroll
-        for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) r.right[k] = __builtin_amdgcn_alignbit(W[9 + k], W[8 + k], sq2);
-#pragma unroll
-        for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
-            r.left[k] = nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sq2)));
-        r.ok = ok ? 1u : 0u;
-        uint4 *o = (uint4 *)(recs + p);
-        const uint4 *s = (const uint4 *)&r;
-#pragma unroll
-        for (int k = 0; k < 4; k++) o[k] = s[k];
+        __syncthreads();
+        const uint32_t n = min(256u, total - base);
+        uint4 *o = (uint4 *)(recs + base);
+        for (uint32_t j = threadIdx.x; j < 4 * n; j += 256) o[j] = s_rec[(j >> 2) * ROW + (j & 3)];
+        __syncthreads();
     }
 }
 
