@@ -163,6 +163,49 @@ def _engine_and_chunks(seed, n_chunks, target_len=300_000):
     return target, blocks, sub, index, pos, chunks
 
 
+def test_batch_low_complexity_mass_deferral(gpu_available):
+    """Poly-A and dinucleotide runs in both sequences: whole k-mer groups of
+    hits outlive the gate's windows, so waves defer up to 64 hits at once and
+    their LDS queues flush before and after pushes; every gate configuration
+    gives the oracle's lists."""
+    _say("case batch: low-complexity runs (mass deferral) vs the oracle")
+    rng = np.random.default_rng(0xB00D)
+    t = bytearray(synth.random_bases(0xB00C, 120_000).tobytes())
+    for at, unit, n in ((10_000, b"A", 400), (40_000, b"AC", 200), (70_000, b"AAT", 100), (95_000, b"T", 350)):
+        run = unit * n
+        t[at:at + len(run)] = run
+    target = bytes(t)
+    blocks = []
+    for i, (a, b) in enumerate(((9_000, 12_000), (39_500, 41_000), (69_000, 72_000), (94_000, 97_000))):
+        q = bytearray(target[a:b])
+        for j in rng.integers(0, len(q), len(q) // 30):  # 3% substitutions outside the runs too
+            q[j] = int(rng.choice(list(b"ACGT")))
+        blocks.append(_block([("lc%d" % i, bytes(q))]))
+    blocks.append(_block([("polyA", b"A" * 600), ("AC", b"AC" * 300)]))
+    sub = O.sub_mat("", 910)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    want = []
+    for f, r in blocks:
+        o = O.Oracle(target, f, r, sub, max_hits=16711680, index=(index, pos))
+        want.append(o.run_pass(threads=16)[0])
+    chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
+    with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
+        for step, threads, claim, order, ahead in [(96, 1024, 64, 1, 1), (128, 1024, 64, 1, 1), (128, 512, 256, 1, 1),
+                                                   (96, 512, 256, 1, 1), (64, 1024, 16, 1, 1), (128, 512, 256, 0, 1)]:
+            _say("  step %d threads %d claim %d order %d lookahead %d" % (step, threads, claim, order, ahead))
+            for name, v in (("batch_step_cells", step), ("batch_block_threads", threads), ("batch_claim_seeds", claim),
+                            ("batch_order", order), ("batch_lookahead", ahead), ("batch_gate_blocks", 0)):
+                eng.set_flag(name, v)
+            got, st = eng.filter_batch(chunks)
+            assert eng.stat("gate_deferred") > 0
+            for i, (g, w) in enumerate(zip(got, want)):
+                assert same_intervals(g, w), "chunk %d differs at step %d threads %d" % (i, step, threads)
+        for i in range(2):  # the plain path on the same blocks
+            eng.swap_query(*chunks[i])
+            g, _ = eng.run_pass()
+            assert same_intervals(g, want[i]), "plain chunk %d differs" % i
+
+
 def test_batch_gate_configs_match(gpu_available):
     _say("case batch: gate step/block/claim/grid variants vs the oracle")
     target, blocks, sub, index, pos, chunks = _engine_and_chunks(0xB005, 12)
