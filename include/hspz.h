@@ -149,13 +149,13 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 #define HSPZ_FLAG_BATCH_GRAPH 0        /* 1 (default): capture/replay the batched pass as a hipGraph */
 #define HSPZ_FLAG_BATCH_GATE_BLOCKS 1  /* persistent grid of the batched gate in blocks; 0 (default) = as many
                                           as are resident at once */
-#define HSPZ_FLAG_BATCH_STEP_CELLS 2   /* cells per gate step: 16, 32, 64, 96 or 128 (default). 128: 64
+#define HSPZ_FLAG_BATCH_STEP_CELLS 2   /* cells per gate step: 16, 32, 64, 96 (default) or 128. 128: 64
                                           right + 64 left cells of a hit in one step; 96: 32 right + 64
                                           left, the right side continuing with its next 32 cells.
                                           Seed-major order runs both as 64 */
-#define HSPZ_FLAG_BATCH_BLOCK_THREADS 3 /* threads per gate block: 512 (default) or 1024 */
+#define HSPZ_FLAG_BATCH_BLOCK_THREADS 3 /* threads per gate block: 512 or 1024 (default) */
 #define HSPZ_FLAG_BATCH_CLAIM_SEEDS 4  /* seeds a wave claims from the work counter at a time (1..4096,
-                                          default 256) */
+                                          default 64) */
 #define HSPZ_FLAG_BATCH_ORDER 5        /* hit order of the batched gate: 0 seed-major, 1 k-mer-major (a
                                           reference window is reused by every query seed of its k-mer),
                                           2 (default) k-mer-major when the batch holds at least one seed
@@ -171,7 +171,7 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           -1: none does (the gates read windows from the plane). Setting 0
                                           or -1 releases built ones. hspz_get_flag reports 2 once they are
                                           built. Any k-mer-major gate reads them when they exist */
-#define HSPZ_FLAG_BATCH_LOOKAHEAD 9    /* 1 (default): with 128-cell steps and 512-thread blocks, the
+#define HSPZ_FLAG_BATCH_LOOKAHEAD 9    /* 1 (default): with 96- or 128-cell steps and 512-thread blocks, the
                                           k-mer-major gate issues each hit's and each item's loads one
                                           iteration ahead of their use; it reads the reference records
                                           (built on first use when device memory allows, as
