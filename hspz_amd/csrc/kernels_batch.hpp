@@ -859,9 +859,27 @@ __device__ __forceinline__ int32_t min3_i32(int32_t a, int32_t b, int32_t c) {
     return d;
 }
 
+// The 8 cell pairs of one key word pair through their map entries p: steps e
+// and folds e after every second pair into emin. Testing every second pair
+// (every 4 cells) instead of every pair is a further relaxation of the upper
+// bound: a drop whose e recovers within the next pair is seen at a later test
+// or not at all (the hit then continues or is deferred), and the maximum
+// c + xdrop - e of the stretch does not depend on the tests.
+__device__ __forceinline__ void map_pairs8(const uint32_t (&p)[8], int32_t &e, int32_t &emin) {
+    int32_t t[4];
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+        int32_t e1 = min(e + lo16(p[i]), hi16(p[i]));
+        e = min(e1 + lo16(p[i + 1]), hi16(p[i + 1]));
+        t[i / 2] = e;
+    }
+    emin = min3_i32(emin, t[0], t[1]);
+    emin = min3_i32(emin, t[2], t[3]);
+}
+
 // One 64-cell stretch (4 word pairs) of a stream through the map table at
 // lane offset `off`: updates e, the score sum c, and the minimum of e over
-// the cell pairs.
+// every second cell pair (map_pairs8).
 // With N = 8 words, `hi` picks words 4..7 instead of 0..3 (per lane).
 template <int N>
 __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t (&rw)[N], const uint32_t (&qw)[N],
@@ -882,12 +900,7 @@ __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t
             p[2 * j] = map_entry(s_map, t1, off, SEL[j]);
             p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
         }
-#pragma unroll
-        for (int i = 0; i < 8; i += 2) {
-            int32_t e1 = min(e + lo16(p[i]), hi16(p[i]));
-            e = min(e1 + lo16(p[i + 1]), hi16(p[i + 1]));
-            emin = min3_i32(emin, e1, e);
-        }
+        map_pairs8(p, e, emin);
         c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3]))) +
              lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
         // finish this word before the next one's reads: deferring the sums
@@ -918,12 +931,7 @@ __device__ __forceinline__ void map_words(const uint8_t *s_map, const uint32_t (
             p[2 * j] = map_entry(s_map, t1, off, SEL[j]);
             p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
         }
-#pragma unroll
-        for (int i = 0; i < 8; i += 2) {
-            int32_t e1 = min(e + lo16(p[i]), hi16(p[i]));
-            e = min(e1 + lo16(p[i + 1]), hi16(p[i + 1]));
-            emin = min3_i32(emin, e1, e);
-        }
+        map_pairs8(p, e, emin);
         c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3]))) +
              lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
         asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
@@ -974,21 +982,11 @@ __device__ __forceinline__ void map_words_lr(const uint8_t *s_map, const uint32_
             }
         }
         if (right) {
-#pragma unroll
-            for (int i = 0; i < 8; i += 2) {
-                int32_t e1 = min(er + lo16(pr[i]), hi16(pr[i]));
-                er = min(e1 + lo16(pr[i + 1]), hi16(pr[i + 1]));
-                eminr = min3_i32(eminr, e1, er);
-            }
+            map_pairs8(pr, er, eminr);
             cr += lo16(pk_add16(pk_add16(pr[0], pr[1]), pk_add16(pr[2], pr[3]))) +
                   lo16(pk_add16(pk_add16(pr[4], pr[5]), pk_add16(pr[6], pr[7])));
         }
-#pragma unroll
-        for (int i = 0; i < 8; i += 2) {
-            int32_t e1 = min(el + lo16(pl[i]), hi16(pl[i]));
-            el = min(e1 + lo16(pl[i + 1]), hi16(pl[i + 1]));
-            eminl = min3_i32(eminl, e1, el);
-        }
+        map_pairs8(pl, el, eminl);
         cl += lo16(pk_add16(pk_add16(pl[0], pl[1]), pk_add16(pl[2], pl[3]))) +
               lo16(pk_add16(pk_add16(pl[4], pl[5]), pk_add16(pl[6], pl[7])));
         // finish this round before the next one's reads, so at most two
