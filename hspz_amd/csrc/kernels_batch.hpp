@@ -699,6 +699,13 @@ struct KmerSource {
     }
 };
 
+// The sum of the a halves of 8 map entries: one tree of packed int16 adds,
+// exact since 16 cells clamped at -(xdrop + 1) stay within int16.
+__device__ __forceinline__ int32_t sum_a8(const uint32_t (&p)[8]) {
+    return lo16(pk_add16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3])),
+                         pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7]))));
+}
+
 // The packed gate over k-mer-major items: the same X-drop core and upper-bound
 // argument as packed_gate_loop; reference words are prepared once per item
 // (A0, LA0) and query words once per hit. At 64-cell steps no stream is
@@ -809,8 +816,7 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
                     e = min(e1 + lo16(p[i + 1]), hi16(p[i + 1]));
                     emin = min(emin, min(e1, e));
                 }
-                c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3]))) +
-                     lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
+                c += sum_a8(p);
             }
             if (emin >= 0) {
                 grp++;
@@ -901,8 +907,7 @@ __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t
             p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
         }
         map_pairs8(p, e, emin);
-        c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3]))) +
-             lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
+        c += sum_a8(p);
         // finish this word before the next one's reads: deferring the sums
         // keeps every table entry of the stretch live
         asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
@@ -932,8 +937,7 @@ __device__ __forceinline__ void map_words(const uint8_t *s_map, const uint32_t (
             p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
         }
         map_pairs8(p, e, emin);
-        c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3]))) +
-             lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
+        c += sum_a8(p);
         asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
     }
 }
@@ -983,12 +987,10 @@ __device__ __forceinline__ void map_words_lr(const uint8_t *s_map, const uint32_
         }
         if (right) {
             map_pairs8(pr, er, eminr);
-            cr += lo16(pk_add16(pk_add16(pr[0], pr[1]), pk_add16(pr[2], pr[3]))) +
-                  lo16(pk_add16(pk_add16(pr[4], pr[5]), pk_add16(pr[6], pr[7])));
+            cr += sum_a8(pr);
         }
         map_pairs8(pl, el, eminl);
-        cl += lo16(pk_add16(pk_add16(pl[0], pl[1]), pk_add16(pl[2], pl[3]))) +
-              lo16(pk_add16(pk_add16(pl[4], pl[5]), pk_add16(pl[6], pl[7])));
+        cl += sum_a8(pl);
         // finish this round before the next one's reads, so at most two
         // words' table entries are live
         asm volatile("" : "+v"(cr), "+v"(er), "+v"(eminr), "+v"(cl), "+v"(el), "+v"(eminl));
