@@ -699,11 +699,12 @@ struct KmerSource {
     }
 };
 
-// The sum of the a halves of 8 map entries: one tree of packed int16 adds,
-// exact since 16 cells clamped at -(xdrop + 1) stay within int16.
-__device__ __forceinline__ int32_t sum_a8(const uint32_t (&p)[8]) {
-    return lo16(pk_add16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3])),
-                         pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7]))));
+// Adds the a halves of 8 map entries to c: two trees of packed int16 adds
+// over 4 entries each (8 cells, exact in int16 for xdrop and scores within
+// the packed gate's limits), each folded into c by one sign-extending add.
+__device__ __forceinline__ void add_a8(int32_t &c, const uint32_t (&p)[8]) {
+    c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3])));
+    c += lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
 }
 
 // The packed gate over k-mer-major items: the same X-drop core and upper-bound
@@ -816,7 +817,7 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
                     e = min(e1 + lo16(p[i + 1]), hi16(p[i + 1]));
                     emin = min(emin, min(e1, e));
                 }
-                c += sum_a8(p);
+                add_a8(c, p);
             }
             if (emin >= 0) {
                 grp++;
@@ -907,7 +908,7 @@ __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t
             p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
         }
         map_pairs8(p, e, emin);
-        c += sum_a8(p);
+        add_a8(c, p);
         // finish this word before the next one's reads: deferring the sums
         // keeps every table entry of the stretch live
         asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
@@ -937,7 +938,7 @@ __device__ __forceinline__ void map_words(const uint8_t *s_map, const uint32_t (
             p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
         }
         map_pairs8(p, e, emin);
-        c += sum_a8(p);
+        add_a8(c, p);
         asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
     }
 }
@@ -987,10 +988,10 @@ __device__ __forceinline__ void map_words_lr(const uint8_t *s_map, const uint32_
         }
         if (right) {
             map_pairs8(pr, er, eminr);
-            cr += sum_a8(pr);
+            add_a8(cr, pr);
         }
         map_pairs8(pl, el, eminl);
-        cl += sum_a8(pl);
+        add_a8(cl, pl);
         // finish this round before the next one's reads, so at most two
         // words' table entries are live
         asm volatile("" : "+v"(cr), "+v"(er), "+v"(eminr), "+v"(cl), "+v"(el), "+v"(eminl));

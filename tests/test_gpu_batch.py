@@ -206,6 +206,42 @@ def test_batch_low_complexity_mass_deferral(gpu_available):
             assert same_intervals(g, want[i]), "plain chunk %d differs" % i
 
 
+def test_batch_packed_gate_limits(gpu_available):
+    """xdrop 4000 and ACGT scores up to 4000 (HOXD70 x 40), the largest the
+    packed gates take: a word's 16 cells then reach +-64 000, so each of the
+    gate's int16 partial sums must cover at most 8 cells. Batched (both hit
+    orders, merged and narrow steps) and plain lists equal the oracle's."""
+    _say("case batch: packed-gate limits (xdrop 4000, scores x40)")
+    hoxd70 = [91, -114, -31, -123, -114, 100, -125, -31, -31, -125, 100, -114, -123, -31, -114, 91]
+    acgt = [40 * v for v in hoxd70]
+    xdrop, thresh = 4000, 120000
+    target = synth.random_bases(0xB00E, 150_000).tobytes()
+    blocks = _chunks_from(target, 10, 17, lo=300, hi=12_000, div=(0.0, 0.15))
+    blocks.append(_block([("same", target[20_000:21_000]), ("near", target[60_000:60_400])]))
+    sub = O.sub_mat("", xdrop, acgt)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    want, hsps = [], 0
+    for f, r in blocks:
+        o = O.Oracle(target, f, r, sub, xdrop=xdrop, hspthresh=thresh, max_hits=16711680, index=(index, pos))
+        res, st, _ = o.run_pass(threads=16)
+        want.append(res)
+        hsps += st["hsps"]
+    assert hsps > 100
+    chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
+    with E.Engine(index, pos, S.encode(target), sub, xdrop=xdrop, hspthresh=thresh, max_hits=16711680) as eng:
+        for step, threads, order in [(96, 1024, 1), (128, 512, 1), (64, 1024, 1), (128, 512, 0), (32, 512, 1)]:
+            _say("  step %d threads %d order %d" % (step, threads, order))
+            for name, v in (("batch_step_cells", step), ("batch_block_threads", threads), ("batch_order", order)):
+                eng.set_flag(name, v)
+            got, _ = eng.filter_batch(chunks)
+            for i, (g, w) in enumerate(zip(got, want)):
+                assert same_intervals(g, w), "chunk %d differs at step %d threads %d order %d" % (i, step, threads, order)
+        for i in range(3):
+            eng.swap_query(*chunks[i])
+            g, _ = eng.run_pass()
+            assert same_intervals(g, want[i]), "plain chunk %d differs" % i
+
+
 def test_batch_gate_configs_match(gpu_available):
     _say("case batch: gate step/block/claim/grid variants vs the oracle")
     target, blocks, sub, index, pos, chunks = _engine_and_chunks(0xB005, 12)
