@@ -140,6 +140,12 @@ __device__ __forceinline__ bool gate_keep(int32_t total, int32_t thresh) {
     return trunc_sat((double)(float)total) >= thresh;
 }
 
+// gate_keep for the packed gates' totals: a total is the maximum over at most
+// 64 right and 128 left cells of ACGT scores <= G2_MAX_SCORE, so it lies in
+// [0, 2^24), where the f32 round trip of gate_keep is exact and the test is
+// the plain comparison.
+__device__ __forceinline__ bool packed_keep(int32_t total, int32_t thresh) { return total >= thresh; }
+
 // Little-endian 8 bytes starting at an arbitrary byte address, from three
 // dword-aligned loads and v_alignbyte. The sequences carry PAD bytes on both
 // sides, so the aligned dwords around any in-range address are readable.
@@ -718,7 +724,7 @@ __device__ __forceinline__ void packed_gate_loop(Src &src, const uint32_t *__res
             } else {
                 int32_t total = Mr + c + (xdrop - e);
                 done = true;
-                pass = gate_keep(total, thresh);
+                pass = packed_keep(total, thresh);
             }
         }
         src.emit(done, pass, tag, R, Q);

@@ -848,7 +848,7 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
             } else {
                 int32_t total = Mr + c + (xdrop - e);
                 done = true;
-                pass = gate_keep(total, thresh);
+                pass = packed_keep(total, thresh);
             }
         }
         src.emit(done, pass, tag, R, Q);
@@ -1099,7 +1099,7 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
             }
             if (!defer && rdone && ldone) {
                 done = true;
-                pass = gate_keep(Mr + Ml, thresh);
+                pass = packed_keep(Mr + Ml, thresh);
             }
         }
         src.emit(done, pass, tag, R, Q);
@@ -1284,7 +1284,7 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
             }
             if (!defer && rdone && ldone) {
                 done = true;
-                pass = gate_keep(Mr + Ml, thresh);
+                pass = packed_keep(Mr + Ml, thresh);
             }
         }
         if (has && (done || defer)) {
