@@ -1073,34 +1073,27 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
             // left cells 0..63, or 64..127 for a continuing left side
             int32_t el = cont ? el_keep : xdrop, cl = cont ? cl_keep : 0, eminl = el;
             map_stretch<8>(s_map, LA0, LB, cont, off_rev, el, cl, eminl);
-            if (!rdone) {
-                if (eminr < 0) {
-                    rdone = true;
-                    Mr = cr + (xdrop - er);
-                } else if (rcont || RW == (int)G2_RIGHT_WORDS) {
-                    defer = true;  // the right extension outlived its 64 cells
-                } else {
-                    rcont = true;
-                    er_keep = er;
-                    cr_keep = cr;
-                }
-            }
-            if (!defer && !ldone) {
-                if (eminl < 0) {
-                    ldone = true;
-                    Ml = cl + (xdrop - el);
-                } else if (cont) {
-                    defer = true;  // the left extension outlived its 128 cells
-                } else {
-                    cont = true;
-                    el_keep = el;
-                    cl_keep = cl;
-                }
-            }
-            if (!defer && rdone && ldone) {
-                done = true;
-                pass = packed_keep(Mr + Ml, thresh);
-            }
+            // the same decisions as nested branches would take, as selects:
+            // a side that drops records its maximum; one that outlives its
+            // last stretch (right: 64 cells, left: 128) defers the hit; one
+            // that outlives an earlier stretch continues from its state
+            const bool r_drop = !rdone && eminr < 0, l_drop = !ldone && eminl < 0;
+            const bool r_open = !rdone && !r_drop, l_open = !ldone && !l_drop;
+            const bool r_last = rcont || RW == (int)G2_RIGHT_WORDS;
+            Mr = r_drop ? cr + (xdrop - er) : Mr;
+            Ml = l_drop ? cl + (xdrop - el) : Ml;
+            defer = (r_open && r_last) || (l_open && cont);  // the right side past 64 cells, the left past 128
+            const bool r_next = r_open && !r_last, l_next = l_open && !cont;
+            er_keep = r_next ? er : er_keep;
+            cr_keep = r_next ? cr : cr_keep;
+            el_keep = l_next ? el : el_keep;
+            cl_keep = l_next ? cl : cl_keep;
+            rdone = rdone || r_drop;
+            ldone = ldone || l_drop;
+            rcont = rcont || r_next;
+            cont = cont || l_next;
+            done = !defer && rdone && ldone;
+            pass = done && packed_keep(Mr + Ml, thresh);
         }
         src.emit(done, pass, tag, R, Q);
         src.defer(defer, tag, R, Q);
