@@ -998,6 +998,44 @@ __device__ __forceinline__ void map_words_lr(const uint8_t *s_map, const uint32_
     }
 }
 
+// Per-lane state of a hit's two sides across the merged gates' iterations:
+// whether each side has dropped (its maximum recorded) or continues from a
+// kept X-drop state and score sum.
+struct HitSides {
+    bool rdone = false, ldone = false, rcont = false, cont = false;
+    int32_t Mr = 0, Ml = 0, er_keep = 0, cr_keep = 0, el_keep = 0, cl_keep = 0;
+
+    __device__ __forceinline__ void start() { rdone = ldone = rcont = cont = false; }
+    // The decisions after an iteration's stretches, as selects (nested
+    // branches around them cost more issue than the selects): a side that
+    // drops records its maximum; one that outlives its last stretch (right:
+    // 64 cells, left: 128) defers the hit; one that outlives an earlier
+    // stretch keeps its state for the next. `done` once both sides dropped,
+    // `pass` if their maxima reach the threshold.
+    template <int RW>
+    __device__ __forceinline__ void decide(int32_t er, int32_t cr, int32_t eminr, int32_t el, int32_t cl,
+                                           int32_t eminl, int32_t xdrop, int32_t thresh, bool &defer, bool &done,
+                                           bool &pass) {
+        const bool r_drop = !rdone && eminr < 0, l_drop = !ldone && eminl < 0;
+        const bool r_open = !rdone && !r_drop, l_open = !ldone && !l_drop;
+        const bool r_last = rcont || RW == (int)G2_RIGHT_WORDS;
+        Mr = r_drop ? cr + (xdrop - er) : Mr;
+        Ml = l_drop ? cl + (xdrop - el) : Ml;
+        defer = (r_open && r_last) || (l_open && cont);
+        const bool r_next = r_open && !r_last, l_next = l_open && !cont;
+        er_keep = r_next ? er : er_keep;
+        cr_keep = r_next ? cr : cr_keep;
+        el_keep = l_next ? el : el_keep;
+        cl_keep = l_next ? cl : cl_keep;
+        rdone = rdone || r_drop;
+        ldone = ldone || l_drop;
+        rcont = rcont || r_next;
+        cont = cont || l_next;
+        done = !defer && rdone && ldone;
+        pass = done && packed_keep(Mr + Ml, thresh);
+    }
+};
+
 // The k-mer-major gate with merged phases: one iteration evaluates a fresh
 // hit's first right stretch and its first 64 left cells; a hit whose left
 // extension outlives them takes another iteration for left cells 64..127.
@@ -1014,10 +1052,10 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
                                                  const uint4 *__restrict__ rrec, const QRec *__restrict__ recs,
                                                  const uint8_t *s_map, int32_t xdrop, int32_t thresh) {
     static_assert(RW == 4 || RW == 2, "a right stretch is the whole right stream or one of its halves");
-    bool has = false, item = false, rok = false, cont = false, rcont = false, rdone = false, ldone = false;
+    bool has = false, item = false, rok = false;
+    HitSides st;
     uint32_t R = 0, Q = 0, tag = 0, s_cur = 0, s_end = 0;
     const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
-    int32_t Mr = 0, Ml = 0, el_keep = 0, cl_keep = 0, er_keep = 0, cr_keep = 0;
     uint32_t A0[G2_RIGHT_WORDS], LA0[G2_LEFT_WORDS], B[G2_RIGHT_WORDS], LB[G2_LEFT_WORDS];
 #pragma unroll
     for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) A0[j] = B[j] = 0;
@@ -1062,44 +1100,24 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
             LB[4] = v2.x, LB[5] = v2.y, LB[6] = v2.z, LB[7] = v2.w;
             Q = v3.x;
             tag = v3.y;
-            cont = rcont = rdone = ldone = false;
+            st.start();
             if (!(rok && v3.z != 0)) defer = true;
         }
 
         if (has && !defer) {
-            // the right stretch: cells 0..16 RW-1, or the next ones for a continuing right side
-            int32_t er = rcont ? er_keep : xdrop, cr = rcont ? cr_keep : 0, eminr = er;
-            map_words<RW, G2_RIGHT_WORDS>(s_map, A0, B, rcont, off_fwd, er, cr, eminr);
+            // the right stretch: cells 0 .. 16 RW - 1, or the next ones for a continuing right side
+            int32_t er = st.rcont ? st.er_keep : xdrop, cr = st.rcont ? st.cr_keep : 0, eminr = er;
+            map_words<RW, G2_RIGHT_WORDS>(s_map, A0, B, st.rcont, off_fwd, er, cr, eminr);
             // left cells 0..63, or 64..127 for a continuing left side
-            int32_t el = cont ? el_keep : xdrop, cl = cont ? cl_keep : 0, eminl = el;
-            map_stretch<8>(s_map, LA0, LB, cont, off_rev, el, cl, eminl);
-            // the same decisions as nested branches would take, as selects:
-            // a side that drops records its maximum; one that outlives its
-            // last stretch (right: 64 cells, left: 128) defers the hit; one
-            // that outlives an earlier stretch continues from its state
-            const bool r_drop = !rdone && eminr < 0, l_drop = !ldone && eminl < 0;
-            const bool r_open = !rdone && !r_drop, l_open = !ldone && !l_drop;
-            const bool r_last = rcont || RW == (int)G2_RIGHT_WORDS;
-            Mr = r_drop ? cr + (xdrop - er) : Mr;
-            Ml = l_drop ? cl + (xdrop - el) : Ml;
-            defer = (r_open && r_last) || (l_open && cont);  // the right side past 64 cells, the left past 128
-            const bool r_next = r_open && !r_last, l_next = l_open && !cont;
-            er_keep = r_next ? er : er_keep;
-            cr_keep = r_next ? cr : cr_keep;
-            el_keep = l_next ? el : el_keep;
-            cl_keep = l_next ? cl : cl_keep;
-            rdone = rdone || r_drop;
-            ldone = ldone || l_drop;
-            rcont = rcont || r_next;
-            cont = cont || l_next;
-            done = !defer && rdone && ldone;
-            pass = done && packed_keep(Mr + Ml, thresh);
+            int32_t el = st.cont ? st.el_keep : xdrop, cl = st.cont ? st.cl_keep : 0, eminl = el;
+            map_stretch<8>(s_map, LA0, LB, st.cont, off_rev, el, cl, eminl);
+            st.decide<RW>(er, cr, eminr, el, cl, eminl, xdrop, thresh, defer, done, pass);
         }
         src.emit(done, pass, tag, R, Q);
         src.defer(defer, tag, R, Q);
         if (done || defer) {
             has = false;
-            cont = rcont = false;
+            st.cont = st.rcont = false;
         }
     }
     src.finish();
@@ -1135,9 +1153,8 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
     const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
     // current item and hit (per side: continuing, done, kept state, maximum)
     bool cur = false, rok = false, has = false, qok = false;
-    bool cont = false, rcont = false, ldone = false, rdone = false;
+    HitSides st;
     uint32_t R = 0, s_cur = 0, s_end = 0, Q = 0, tag = 0;
-    int32_t Mr = 0, Ml = 0, el_keep = 0, cl_keep = 0, er_keep = 0, cr_keep = 0;
     uint32_t A0[G2_RIGHT_WORDS], LA0[G2_LEFT_WORDS], B[G2_RIGHT_WORDS], LB[G2_LEFT_WORDS];
     // next item, and the item whose loads are in flight
     bool nxt = false, nrok = false, fly = false;
@@ -1211,7 +1228,7 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
                 qok = v3.z != 0;
                 s_cur++;
                 has = true;
-                cont = rcont = rdone = ldone = false;
+                st.start();
                 if (!(rok && qok)) defer = true;
             }
         }
@@ -1248,37 +1265,11 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
         //    left stretch (cells 0..63, or 64..127 for a continuing left side)
         bool done = false, pass = false;
         if (has && !defer) {
-            int32_t er = rcont ? er_keep : xdrop, cr = rcont ? cr_keep : 0, eminr = er;
-            int32_t el = cont ? el_keep : xdrop, cl = cont ? cl_keep : 0, eminl = el;
-            map_words_lr<RW>(s_map, A0, B, rcont, off_fwd, er, cr, eminr, LA0, LB, cont, off_rev, el, cl, eminl);
-            if (!rdone) {
-                if (eminr < 0) {
-                    rdone = true;
-                    Mr = cr + (xdrop - er);
-                } else if (rcont || RW == (int)G2_RIGHT_WORDS) {
-                    defer = true;  // the right side outlived its 64 cells
-                } else {
-                    rcont = true;
-                    er_keep = er;
-                    cr_keep = cr;
-                }
-            }
-            if (!defer && !ldone) {
-                if (eminl < 0) {
-                    ldone = true;
-                    Ml = cl + (xdrop - el);
-                } else if (cont) {
-                    defer = true;  // the left side outlived its 128 cells
-                } else {
-                    cont = true;
-                    el_keep = el;
-                    cl_keep = cl;
-                }
-            }
-            if (!defer && rdone && ldone) {
-                done = true;
-                pass = packed_keep(Mr + Ml, thresh);
-            }
+            int32_t er = st.rcont ? st.er_keep : xdrop, cr = st.rcont ? st.cr_keep : 0, eminr = er;
+            int32_t el = st.cont ? st.el_keep : xdrop, cl = st.cont ? st.cl_keep : 0, eminl = el;
+            map_words_lr<RW>(s_map, A0, B, st.rcont, off_fwd, er, cr, eminr, LA0, LB, st.cont, off_rev, el, cl,
+                             eminl);
+            st.decide<RW>(er, cr, eminr, el, cl, eminl, xdrop, thresh, defer, done, pass);
         }
         if (has && (done || defer)) {
             e_pass = done && pass;
