@@ -44,18 +44,19 @@ typedef void (*KGateFn)(const uint32_t *, const uint32_t *, int32_t, int32_t, co
                         const uint4 *, uint32_t, uint32_t, unsigned long long *, SurvRec *, uint32_t *, uint32_t,
                         SurvRec *, uint32_t *, uint32_t, uint32_t *);
 
-// The k-mer-major gate instantiation for a step width and block size.
-KGateFn kgate_kernel(int step_cells, int threads) {
+// The k-mer-major gate instantiation for a step width and block size; the
+// merged steps take the narrow score sums when the engine's matrix allows.
+KGateFn kgate_kernel(int step_cells, int threads, bool narrow) {
     if (threads == 1024)
         return step_cells == 16    ? k_bgate_kmer<1, 1024>
                : step_cells == 64  ? k_bgate_kmer<4, 1024>
-               : step_cells == 96  ? k_bgate_kmer<6, 1024>
-               : step_cells == 128 ? k_bgate_kmer<8, 1024>
+               : step_cells == 96  ? (narrow ? k_bgate_kmer<6, 1024, true> : k_bgate_kmer<6, 1024>)
+               : step_cells == 128 ? (narrow ? k_bgate_kmer<8, 1024, true> : k_bgate_kmer<8, 1024>)
                                    : k_bgate_kmer<2, 1024>;
     return step_cells == 16    ? k_bgate_kmer<1, 512>
            : step_cells == 64  ? k_bgate_kmer<4, 512>
-           : step_cells == 96  ? k_bgate_kmer<6, 512>
-           : step_cells == 128 ? k_bgate_kmer<8, 512>
+           : step_cells == 96  ? (narrow ? k_bgate_kmer<6, 512, true> : k_bgate_kmer<6, 512>)
+           : step_cells == 128 ? (narrow ? k_bgate_kmer<8, 512, true> : k_bgate_kmer<8, 512>)
                                : k_bgate_kmer<2, 512>;
 }
 
@@ -412,7 +413,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         };
         // fused expansion + packed gate, then the exact gate for deferred hits
         BGateFn gate_fn = bgate_kernel(h->bgate_step, h->bgate_threads);
-        KGateFn kgate_fn = kgate_kernel(h->bgate_step, h->bgate_threads);
+        KGateFn kgate_fn = kgate_kernel(h->bgate_step, h->bgate_threads, h->narrow_scores);
         if (kmer_order && lookahead_wanted(h) && h->rrec.p)
             kgate_fn = h->bgate_step == 96 ? k_bgate_ahead<2> : k_bgate_ahead<4>;
         int grid = h->bgate_blocks;

@@ -69,6 +69,7 @@ struct hspz_engine {
     // packed planes + coarse exception bitmaps for the fast gate (k_gate_packed)
     DevBuf ref2, refx, qf2, qfx, qr2, qrx, pairtab, slow, slow_n;
     bool fast_gate = false;
+    bool narrow_scores = false;  // xdrop and ACGT scores <= G2_NARROW_SCORE: 16-cell score sums
     uint32_t unsafe_codes = 0xFFFFFFFFu;  // perm table: byte k = 0xFF when code 4 + k needs the exact gate
     // seed slots (gpu/mod.rs:933-945)
     DevBuf seeds[2];
@@ -552,6 +553,10 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
         for (int i = 0; i < 4 && ok; i++)
             for (int j = 0; j < 4; j++) ok = ok && cfg->sub_mat[i * 8 + j] <= G2_MAX_SCORE;
         h->fast_gate = ok;
+        bool narrow = ok && cfg->xdrop <= G2_NARROW_SCORE;
+        for (int i = 0; i < 4 && narrow; i++)
+            for (int j = 0; j < 4; j++) narrow = narrow && cfg->sub_mat[i * 8 + j] <= G2_NARROW_SCORE;
+        h->narrow_scores = narrow;
         if (ok) {
             uint32_t tab[512];
             const int32_t X = cfg->xdrop;

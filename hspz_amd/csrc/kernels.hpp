@@ -508,6 +508,7 @@ constexpr uint32_t G2_RIGHT_WORDS = 4;  // 64 right cells in registers
 constexpr uint32_t G2_LEFT_WORDS = 8;   // 128 left cells in registers
 constexpr int32_t G2_MAX_XDROP = 4000;  // packed int16 partial sums of 8 cells stay exact
 constexpr int32_t G2_MAX_SCORE = 4095;  // largest ACGT score the packed sums take
+constexpr int32_t G2_NARROW_SCORE = 2047;  // xdrop and ACGT scores up to this take 16-cell int16 sums
 constexpr uint32_t G2_MARGIN = 128;     // fast path needs this many bases either side
 // LDS map table: per 8-bit key a 256-byte row of 32 forward then 32 reverse
 // copies of the key's 4-byte entry (64 KiB), so that one v_perm_b32 forms

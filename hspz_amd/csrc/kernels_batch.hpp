@@ -699,12 +699,20 @@ struct KmerSource {
     }
 };
 
-// Adds the a halves of 8 map entries to c: two trees of packed int16 adds
+// Adds the a halves of 8 map entries to c. Two trees of packed int16 adds
 // over 4 entries each (8 cells, exact in int16 for xdrop and scores within
-// the packed gate's limits), each folded into c by one sign-extending add.
+// the packed gate's limits), each folded into c; NARROW: one tree over all 8
+// (16 cells, exact in int16 when xdrop + 1 and every ACGT score are at most
+// G2_NARROW_SCORE + 1 and G2_NARROW_SCORE), folded once.
+template <bool NARROW = false>
 __device__ __forceinline__ void add_a8(int32_t &c, const uint32_t (&p)[8]) {
-    c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3])));
-    c += lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
+    if constexpr (NARROW) {
+        c += lo16(pk_add16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3])),
+                           pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7]))));
+    } else {
+        c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3])));
+        c += lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
+    }
 }
 
 // The packed gate over k-mer-major items: the same X-drop core and upper-bound
@@ -888,7 +896,7 @@ __device__ __forceinline__ void map_pairs8(const uint32_t (&p)[8], int32_t &e, i
 // lane offset `off`: updates e, the score sum c, and the minimum of e over
 // every second cell pair (map_pairs8).
 // With N = 8 words, `hi` picks words 4..7 instead of 0..3 (per lane).
-template <int N>
+template <int N, bool NARROW = false>
 __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t (&rw)[N], const uint32_t (&qw)[N],
                                             bool hi, uint32_t off, int32_t &e, int32_t &c, int32_t &emin) {
     constexpr uint32_t SEL[4] = {0x0C0C0400u, 0x0C0C0500u, 0x0C0C0600u, 0x0C0C0700u};
@@ -908,7 +916,7 @@ __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t
             p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
         }
         map_pairs8(p, e, emin);
-        add_a8(c, p);
+        add_a8<NARROW>(c, p);
         // finish this word before the next one's reads: deferring the sums
         // keeps every table entry of the stretch live
         asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
@@ -917,7 +925,7 @@ __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t
 
 // K words (16 K cells) of a stream of N words through the map table, as
 // map_stretch: words [0, K), or [K, 2K) when `hi` (N = 2K).
-template <int K, int N>
+template <int K, int N, bool NARROW = false>
 __device__ __forceinline__ void map_words(const uint8_t *s_map, const uint32_t (&rw)[N], const uint32_t (&qw)[N],
                                           bool hi, uint32_t off, int32_t &e, int32_t &c, int32_t &emin) {
     static_assert(N == K || N == 2 * K, "a stretch is the whole stream or one of its halves");
@@ -938,7 +946,7 @@ __device__ __forceinline__ void map_words(const uint8_t *s_map, const uint32_t (
             p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
         }
         map_pairs8(p, e, emin);
-        add_a8(c, p);
+        add_a8<NARROW>(c, p);
         asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
     }
 }
@@ -1047,7 +1055,7 @@ struct HitSides {
 // with cells 32..63 in the next iteration (a random right extension mostly
 // drops within 32 cells, so most hits skip 32 cells of work), with the same
 // upper-bound decisions.
-template <int RW, class Src>
+template <int RW, bool NARROW, class Src>
 __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__restrict__ rplane,
                                                  const uint4 *__restrict__ rrec, const QRec *__restrict__ recs,
                                                  const uint8_t *s_map, int32_t xdrop, int32_t thresh) {
@@ -1107,10 +1115,10 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
         if (has && !defer) {
             // the right stretch: cells 0 .. 16 RW - 1, or the next ones for a continuing right side
             int32_t er = st.rcont ? st.er_keep : xdrop, cr = st.rcont ? st.cr_keep : 0, eminr = er;
-            map_words<RW, G2_RIGHT_WORDS>(s_map, A0, B, st.rcont, off_fwd, er, cr, eminr);
+            map_words<RW, G2_RIGHT_WORDS, NARROW>(s_map, A0, B, st.rcont, off_fwd, er, cr, eminr);
             // left cells 0..63, or 64..127 for a continuing left side
             int32_t el = st.cont ? st.el_keep : xdrop, cl = st.cont ? st.cl_keep : 0, eminl = el;
-            map_stretch<8>(s_map, LA0, LB, st.cont, off_rev, el, cl, eminl);
+            map_stretch<8, NARROW>(s_map, LA0, LB, st.cont, off_rev, el, cl, eminl);
             st.decide<RW>(er, cr, eminr, el, cl, eminl, xdrop, thresh, defer, done, pass);
         }
         src.emit(done, pass, tag, R, Q);
@@ -1324,8 +1332,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 
 // The batched gate in k-mer-major order (same launch shape as k_bgate_packed):
 // the merged gate for SW = 8 (128-cell steps) and SW = 6 (96-cell steps, 32
-// right cells at a time), kmer_gate_loop<SW> for the others.
-template <int SW, int BT>
+// right cells at a time), kmer_gate_loop<SW> for the others. NARROW: the
+// merged gate's score sums for narrow matrices (add_a8).
+template <int SW, int BT, bool NARROW = false>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_kmer(
     const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop,
     int32_t thresh, const QRec *__restrict__ recs, const uint32_t *__restrict__ skm,
@@ -1357,8 +1366,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.err = err;
     src.set_lists(nullptr, s_dq[threadIdx.x / WAVE]);
     src.init();
-    if constexpr (SW == 8) kmer_gate_merged<4>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
-    else if constexpr (SW == 6) kmer_gate_merged<2>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+    if constexpr (SW == 8)
+        kmer_gate_merged<4, NARROW>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+    else if constexpr (SW == 6)
+        kmer_gate_merged<2, NARROW>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
     else kmer_gate_loop<SW>(src, rplane, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
