@@ -115,10 +115,21 @@ def test_error_paths(gpu_available):
 
 
 def test_c1_full_size(gpu_available):
-    _say("case c1 full size: 4 Mbp x 4 Mbp")
+    _say("case c1 full size: 4 Mbp x 4 Mbp, plain and batched")
     t, q = synth.c1_pair(4_000_000, seed=0xC0FFEE01)
-    got, st = _check(Pair.raw(t, q))
+    pair = Pair.raw(t, q)
+    got, st = _check(pair)
     assert st["hsps"] > 0
+    # the batched path on the same query as one block: every hit order and merged step
+    with pair.engine() as eng:
+        block = [(S.encode(pair.q_raw), S.encode(pair.q_rc_raw))]
+        for step, threads, order in [(96, 1024, 2), (128, 1024, 1), (96, 512, 1), (64, 512, 0)]:
+            _say("  batched step %d threads %d order %d" % (step, threads, order))
+            for name, v in (("batch_step_cells", step), ("batch_block_threads", threads), ("batch_order", order)):
+                eng.set_flag(name, v)
+            bgot, bst = eng.filter_batch(block)
+            assert bst["hits"] == st["hits"] and bst["raw_hsps"] == st["raw_hsps"], (bst, st)
+            assert same_intervals(bgot[0], got)
 
 
 @pytest.mark.parametrize("case", ["random", "softmasked_n", "step3", "tiny"])
