@@ -28,11 +28,11 @@ def _block(records):
 
 
 def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy=False,
-         wga_chunk=250000, interval_size=10000000):
+         wga_chunk=250000, interval_size=10000000, hspthresh=3000):
     sub = O.sub_mat(ambiguous, 910)
     index, pos = O.seed_table(target, O.shape("12of19"), 1)
     ref_enc = S.encode(target)
-    with E.Engine(index, pos, ref_enc, sub, max_hits=max_hits, noentropy=noentropy) as eng:
+    with E.Engine(index, pos, ref_enc, sub, max_hits=max_hits, noentropy=noentropy, hspthresh=hspthresh) as eng:
         chunks = [(S.encode(f), S.encode(r)) for f, r in query_blocks]
         got, st = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
         for order in (0, 1):  # both hit orders of the batched gate give the same lists
@@ -52,7 +52,8 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
             eng.swap_query(f, r)
             plain.append(eng.run_pass(wga_chunk=wga_chunk, interval_size=interval_size)[0])
     for i, (f, r) in enumerate(query_blocks):
-        o = O.Oracle(target, f, r, sub, max_hits=max_hits, noentropy=noentropy, index=(index, pos))
+        o = O.Oracle(target, f, r, sub, max_hits=max_hits, noentropy=noentropy, hspthresh=hspthresh,
+                     index=(index, pos))
         want, wst, wh = o.run_pass(threads=16, wga_chunk=wga_chunk, interval_size=interval_size)
         assert same_intervals(got[i], want), "chunk %d differs from the oracle" % i
         assert same_intervals(plain[i], want), "plain chunk %d differs from the oracle" % i
@@ -204,6 +205,45 @@ def test_batch_low_complexity_mass_deferral(gpu_available):
             eng.swap_query(*chunks[i])
             g, _ = eng.run_pass()
             assert same_intervals(g, want[i]), "plain chunk %d differs" % i
+
+
+def _entropy_band_case():
+    """Short low-complexity runs between unrelated flanks, so the HSPs are the
+    runs and their shifted-diagonal overlaps, with totals of roughly 1,500 to
+    6,000: at a threshold of 1500 most land inside [T, 3T], where the
+    materializer rescales them by the entropy of their matched bases."""
+    rng = np.random.default_rng(0xE7)
+    runs = [b"A" * 24, b"A" * 40, b"AC" * 14, b"AC" * 30, b"ACGT" * 8, b"ACGT" * 12, b"ACACG" * 9,
+            b"AAT" * 15, b"TTTG" * 10, b"GC" * 20]
+    t = bytearray(synth.random_bases(0xE8, 40_000).tobytes())
+    blocks = []
+    for i, run in enumerate(runs):
+        at = 1_000 + 3_500 * i
+        t[at:at + len(run)] = run
+        q = bytearray(synth.random_bases(0xE9 + i, 600).tobytes())
+        copy = bytearray(run)
+        if i % 3 == 2:  # a few substitutions inside some copies
+            for j in rng.integers(2, len(copy) - 2, 2):
+                copy[j] = int(rng.choice(list(b"ACGT")))
+        q[280:280 + len(copy)] = copy
+        blocks.append(_block([("run%d" % i, bytes(q))]))
+    return bytes(t), blocks
+
+
+@pytest.mark.parametrize("noentropy", [False, True])
+def test_batch_entropy_band(gpu_available, noentropy):
+    """The entropy branch where it decides: batched (both hit orders, device
+    and host dedup) and plain paths give the oracle's lists with and without
+    the correction, and the correction changes those lists."""
+    _say("case batch: low-complexity runs in the entropy band, noentropy=%s" % noentropy)
+    target, blocks = _entropy_band_case()
+    got, _ = _run(target, blocks, noentropy=noentropy, hspthresh=1500)
+    sub = O.sub_mat("", 910)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    other = [O.Oracle(target, f, r, sub, noentropy=not noentropy, hspthresh=1500,
+                      index=(index, pos)).run_pass(threads=16)[0] for f, r in blocks]
+    assert sum(len(fw) + len(rc) for w in got for fw, rc in w) > 20
+    assert sum(not same_intervals(g, o) for g, o in zip(got, other)) >= 5
 
 
 def test_batch_packed_gate_limits(gpu_available):

@@ -297,6 +297,54 @@ def test_extension_closed_forms():
     assert re == 49 and rs == int(diag[100:150].sum())
 
 
+def _materialize(ref, qry, thresh, noentropy=False, r0=None):
+    r, q = np.array(ref, np.uint8), np.array(qry, np.uint8)
+    r0 = len(r) // 2 if r0 is None else r0
+    return O.materialize(r, q, O.sub_mat("", 910), 910, thresh, noentropy, r0, r0)
+
+
+def test_entropy_closed_forms():
+    """The materializer's entropy branch (kernels.rs:1556-1702) on self-hits
+    whose base composition is fixed by construction. The score is rescaled by
+    the Shannon entropy of the matched-base composition over the extent's
+    positions, in units of ln 4, only for totals inside [T, 3T] with at least
+    20 matched bases; the divisor is float32(ln 4) widened (gpu/mod.rs:2349-2352)."""
+    A, Cc, G = 0, 1, 2
+    # one letter: p = 1, entropy 0, so an in-band HSP is dropped; 30 x A/A = 2730
+    assert _materialize([A] * 30, [A] * 30, 910) is None          # 2730 = 3T: in band
+    assert _materialize([A] * 30, [A] * 30, 909) == (0, 0, 29, 2730)  # above 3T: untouched
+    assert _materialize([A] * 30, [A] * 30, 910, noentropy=True) == (0, 0, 29, 2730)
+    # the matched-base count must reach 20 for the entropy to apply
+    assert _materialize([A] * 19, [A] * 19, 800) == (0, 0, 18, 1729)
+    assert _materialize([A] * 20, [A] * 20, 800) is None
+    # two letters, half each: ln(1/2) = -ln 2 exactly, so entropy = ln 2 / float32(ln 4),
+    # a hair below 1/2: the even total 16 x (91 + 100) = 3056 scores 1527, not 1528,
+    # and a threshold of 1528 drops it (a double-precision ln 4 would keep it)
+    ac = [A, Cc] * 16
+    assert _materialize(ac, ac, 1100) == (0, 0, 31, 1527)
+    assert _materialize(ac, ac, 1527) == (0, 0, 31, 1527)
+    assert _materialize(ac, ac, 1528) is None
+    # four letters, a quarter each: ln(1/4) = -2 ln 2 exactly, entropy = 2 ln 2 /
+    # float32(ln 4) = 1 - 2.7e-9 since float32(ln 4) rounds up: 3056 scores 3055
+    acgt = [0, 1, 2, 3] * 8
+    total = 8 * (91 + 100 + 100 + 91)
+    assert _materialize(acgt, acgt, 1100) == (0, 0, 31, total - 1)
+    assert _materialize(acgt, acgt, 1100, noentropy=True) == (0, 0, 31, total)
+    # mismatched positions count in the denominator (extent + 1) but not in the
+    # counts: 18 A, 18 C and 8 G/A transitions over 44 positions, hit at 20
+    ref = [A, Cc, A, Cc, G] * 8 + [A, Cc, A, Cc]
+    qry = [A, Cc, A, Cc, A] * 8 + [A, Cc, A, Cc]
+    total = 18 * 91 + 18 * 100 - 8 * 31
+    p = 18 / 44
+    h = -2 * p * math.log(p) / float(np.float32(math.log(4)))
+    assert abs(total * h - round(total * h)) > 1e-3  # far from a rounding edge
+    got = _materialize(ref, qry, 1200, r0=20)
+    assert got == (0, 0, 43, int(total * h)) == (0, 0, 43, 1682)
+    for wrong in (36, 43):  # matched count or extent as the denominator
+        pw = 18 / wrong
+        assert int(total * (-2 * pw * math.log(pw) / math.log(4))) != got[3]
+
+
 # ------------------------------------------------------------- end to end
 
 def test_frozen_digest_repeat():
