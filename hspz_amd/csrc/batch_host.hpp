@@ -303,8 +303,12 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         // likely to need (~4% and ~20% of its seed capacity against a 1 Gbp
         // target), so that large batches rarely pay an overflow rerun
         hspz_batch_state *bs = h->batch;
-        bs->cap_surv = (uint32_t)std::max<uint64_t>(bs->cap_surv, std::min<uint64_t>(cap_seeds / 16, 1u << 28));
-        bs->cap_slow = (uint32_t)std::max<uint64_t>(bs->cap_slow, std::min<uint64_t>(cap_seeds / 4, 1u << 28));
+        if (h->list_cap) {
+            bs->cap_surv = bs->cap_slow = h->list_cap;
+        } else {
+            bs->cap_surv = (uint32_t)std::max<uint64_t>(bs->cap_surv, std::min<uint64_t>(cap_seeds / 16, 1u << 28));
+            bs->cap_slow = (uint32_t)std::max<uint64_t>(bs->cap_slow, std::min<uint64_t>(cap_seeds / 4, 1u << 28));
+        }
     }
     for (;;) {  // one attempt per survivor/slow capacity
         hspz_batch_state *bs = h->batch;
@@ -564,6 +568,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         if (hc[CTR_ERR] & 8u) {  // survivor or slow list overflowed: grow and rerun
             bs->cap_surv = std::max(bs->cap_surv, hc[CTR_SURV]) * 2;
             bs->cap_slow = std::max(bs->cap_slow, hc[CTR_SLOW]) * 2;
+            h->batch_reruns++;
             continue;
         }
         if (hc[CTR_ERR] & 1u) return set_error(HSPZ_E_CHUNK, "a single seed exceeds max_hits %u", h->max_hits);

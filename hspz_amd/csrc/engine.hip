@@ -88,6 +88,8 @@ struct hspz_engine {
     // wall time of hspz_filter_batch: host layout and uploads, device pass
     // (launch to drained stream), read-back, host dedup and assembly
     uint64_t batch_prep_ns = 0, batch_device_ns = 0, batch_read_ns = 0, batch_post_ns = 0;
+    uint64_t batch_reruns = 0;  // batched passes rerun after a list overflow
+    uint32_t list_cap = 0;      // HSPZ_FLAG_BATCH_LIST_CAP; 0 = sized from the batch
     int gate_blocks = 2048;
     // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
     int bgate_blocks = 0;  // persistent grid; 0 = as many blocks as are resident at once
@@ -832,6 +834,11 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
         break;
     case HSPZ_FLAG_BATCH_DEVICE_DEDUP: h->device_dedup = value != 0; break;
     case HSPZ_FLAG_BATCH_LOOKAHEAD: h->lookahead = value != 0; break;
+    case HSPZ_FLAG_BATCH_LIST_CAP:
+        if (value != 0 && (value < 64 || value > (1 << 28)))
+            return set_error(HSPZ_E_ARG, "list capacity must be 0 or in [64, 2^28]");
+        h->list_cap = (uint32_t)value;
+        break;
     case HSPZ_FLAG_REF_RECORDS:
         if (value < -1 || value > 1) return set_error(HSPZ_E_ARG, "ref_records must be -1, 0 or 1");
         h->rrec_mode = value;
@@ -861,6 +868,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_DEVICE_DEDUP: *value = h->device_dedup ? 1 : 0; break;
     case HSPZ_FLAG_REF_RECORDS: *value = h->rrec.p ? 2 : h->rrec_mode; break;
     case HSPZ_FLAG_BATCH_LOOKAHEAD: *value = h->lookahead ? 1 : 0; break;
+    case HSPZ_FLAG_BATCH_LIST_CAP: *value = (int)h->list_cap; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;
@@ -889,6 +897,7 @@ extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
     case HSPZ_STAT_INDEX_BUILD_NS: *value = h->index_build_ns; break;
     case HSPZ_STAT_POS_LEN: *value = h->pos_len; break;
     case HSPZ_STAT_INDEX_LEN: *value = h->index_len; break;
+    case HSPZ_STAT_BATCH_RERUNS: *value = h->batch_reruns; break;
     default: return set_error(HSPZ_E_ARG, "unknown stat %d", which);
     }
     return 0;

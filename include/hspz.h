@@ -142,6 +142,8 @@ void hspz_filter_output_free(hspz_filter_output *out);
 #define HSPZ_STAT_INDEX_BUILD_NS 16  /* device index build of hspz_engine_new_device_index */
 #define HSPZ_STAT_POS_LEN 17         /* indexed positions held by the engine */
 #define HSPZ_STAT_INDEX_LEN 18       /* index_table entries (4^k) held by the engine */
+#define HSPZ_STAT_BATCH_RERUNS 19    /* batched passes run again because a survivor or deferred-hit
+                                        list overflowed (each time both lists are doubled) */
 int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 
 /* Engine switches (the reference's pub flags, gpu/mod.rs:836-949, have no
@@ -176,6 +178,12 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           iteration ahead of their use; it reads the reference records
                                           (built on first use when device memory allows, as
                                           HSPZ_FLAG_REF_RECORDS builds them). 0: the plain merged gate */
+#define HSPZ_FLAG_BATCH_LIST_CAP 10    /* starting size, in records, of the batched pass's survivor and
+                                          deferred-hit lists. 0 (default): sized from the batch, and kept at
+                                          what earlier passes grew them to. 64..2^28: every pass starts at
+                                          this size. A list that overflows is doubled and the pass run again
+                                          (HSPZ_STAT_BATCH_RERUNS), so the lists only change how often that
+                                          happens, never the results */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

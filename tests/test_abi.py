@@ -35,3 +35,12 @@ def test_binding_export_list_matches_header():
 
 def test_device_count_is_callable_without_gpu():
     assert E.device_count() >= 0
+
+
+def test_flag_and_stat_tables_match_header():
+    text = open(os.path.join(ROOT, "include", "hspz.h")).read()
+    found = re.findall(r"#define HSPZ_(FLAG|STAT)_([A-Z0-9_]+)\s+(\d+)", text)
+    flags = {name.lower(): int(v) for kind, name, v in found if kind == "FLAG"}
+    stats = {name.lower(): int(v) for kind, name, v in found if kind == "STAT"}
+    assert flags == E.FLAG
+    assert stats == E.STAT

@@ -164,12 +164,9 @@ def _engine_and_chunks(seed, n_chunks, target_len=300_000):
     return target, blocks, sub, index, pos, chunks
 
 
-def test_batch_low_complexity_mass_deferral(gpu_available):
-    """Poly-A and dinucleotide runs in both sequences: whole k-mer groups of
-    hits outlive the gate's windows, so waves defer up to 64 hits at once and
-    their LDS queues flush before and after pushes; every gate configuration
-    gives the oracle's lists."""
-    _say("case batch: low-complexity runs (mass deferral) vs the oracle")
+def _low_complexity_case():
+    """Poly-A, dinucleotide and trinucleotide runs in a 120 kbp target, and
+    query blocks around them with 3% substitutions, plus pure runs."""
     rng = np.random.default_rng(0xB00D)
     t = bytearray(synth.random_bases(0xB00C, 120_000).tobytes())
     for at, unit, n in ((10_000, b"A", 400), (40_000, b"AC", 200), (70_000, b"AAT", 100), (95_000, b"T", 350)):
@@ -183,6 +180,16 @@ def test_batch_low_complexity_mass_deferral(gpu_available):
             q[j] = int(rng.choice(list(b"ACGT")))
         blocks.append(_block([("lc%d" % i, bytes(q))]))
     blocks.append(_block([("polyA", b"A" * 600), ("AC", b"AC" * 300)]))
+    return target, blocks
+
+
+def test_batch_low_complexity_mass_deferral(gpu_available):
+    """Poly-A and dinucleotide runs in both sequences: whole k-mer groups of
+    hits outlive the gate's windows, so waves defer up to 64 hits at once and
+    their LDS queues flush before and after pushes; every gate configuration
+    gives the oracle's lists."""
+    _say("case batch: low-complexity runs (mass deferral) vs the oracle")
+    target, blocks = _low_complexity_case()
     sub = O.sub_mat("", 910)
     index, pos = O.seed_table(target, O.shape("12of19"), 1)
     want = []
@@ -244,6 +251,41 @@ def test_batch_entropy_band(gpu_available, noentropy):
                       index=(index, pos)).run_pass(threads=16)[0] for f, r in blocks]
     assert sum(len(fw) + len(rc) for w in got for fw, rc in w) > 20
     assert sum(not same_intervals(g, o) for g, o in zip(got, other)) >= 5
+
+
+def test_batch_list_overflow_reruns(gpu_available):
+    """Survivor and deferred-hit lists started at 64 records overflow (writes
+    past the end are dropped and flagged); the pass doubles them and runs
+    again, and every gate gives the oracle's lists."""
+    _say("case batch: lists started at 64 records overflow and the pass reruns")
+    target, blocks = _low_complexity_case()
+    sub = O.sub_mat("", 910)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    want = [O.Oracle(target, f, r, sub, max_hits=16711680, index=(index, pos)).run_pass(threads=16)[0]
+            for f, r in blocks]
+    chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
+    with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
+        eng.set_flag("batch_list_cap", 64)
+        assert eng.get_flag("batch_list_cap") == 64
+        for step, threads, claim, order, ahead in [(96, 1024, 64, 1, 0), (128, 512, 256, 1, 1), (64, 512, 16, 0, 0)]:
+            _say("  step %d threads %d claim %d order %d lookahead %d" % (step, threads, claim, order, ahead))
+            for name, v in (("batch_step_cells", step), ("batch_block_threads", threads), ("batch_claim_seeds", claim),
+                            ("batch_order", order), ("batch_lookahead", ahead)):
+                eng.set_flag(name, v)
+            r0 = eng.stat("batch_reruns")
+            got, _ = eng.filter_batch(chunks)
+            assert eng.stat("batch_reruns") > r0
+            assert eng.stat("gate_deferred") > 64
+            for i, (g, w) in enumerate(zip(got, want)):
+                assert same_intervals(g, w), "chunk %d differs after reruns at step %d" % (i, step)
+        eng.set_flag("batch_list_cap", 0)
+        r0 = eng.stat("batch_reruns")
+        got, _ = eng.filter_batch(chunks)
+        assert eng.stat("batch_reruns") == r0
+        for g, w in zip(got, want):
+            assert same_intervals(g, w)
+        with pytest.raises(E.HspzError):
+            eng.set_flag("batch_list_cap", 63)
 
 
 def test_batch_packed_gate_limits(gpu_available):
