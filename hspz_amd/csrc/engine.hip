@@ -913,7 +913,9 @@ extern "C" int hspz_run_pass(hspz_engine *h, const hspz_pass_params *p, hspz_seg
     *hsps = nullptr;
     *counts = nullptr;
     *n_intervals = 0;
-    if (h->query_len == 0) return set_error(HSPZ_E_NO_QUERY, "Engine has no query");
+    // no swap_query yet is an error; a swapped empty query has no intervals
+    // and gives an empty pass, as seed_and_filter_all does (run.rs:278-300)
+    if (h->query_swaps == 0) return set_error(HSPZ_E_NO_QUERY, "Engine has no query");
     Shape s;
     if (int rc = parse_shape(p->pattern ? p->pattern : "12of19", &s)) return rc;
     if (s.size != h->seed_size) return set_error(HSPZ_E_SHAPE, "seed shape span %u != engine seed size %u", s.size, h->seed_size);

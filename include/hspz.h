@@ -23,7 +23,8 @@ extern "C" {
 #endif
 
 #define HSPZ_OK 0
-#define HSPZ_E_NO_QUERY 1      /* seed_and_filter before swap_query (gpu/mod.rs:1515-1519) */
+#define HSPZ_E_NO_QUERY 1      /* seed_and_filter on an empty or absent query, run_pass before any
+                                  swap_query (gpu/mod.rs:1515-1519) */
 #define HSPZ_E_ARG 2           /* bad argument or size */
 #define HSPZ_E_SHAPE 3         /* seed shape mismatch (gpu/mod.rs:1320-1341) */
 #define HSPZ_E_RANGE 4         /* seed range beyond the query (gpu/mod.rs:1326-1331) */

@@ -288,6 +288,30 @@ def test_batch_list_overflow_reruns(gpu_available):
             eng.set_flag("batch_list_cap", 63)
 
 
+def test_batch_empty_and_hitless_chunks(gpu_available):
+    """Chunks that give no batches or no seeds beside ordinary ones: an empty
+    block, one shorter than the seed, all N, all soft-masked, and an unrelated
+    random block; an empty call returns no chunks."""
+    _say("case batch: empty, tiny, all-N, all-masked and unrelated chunks beside homologs")
+    target = synth.random_bases(0xB0E0, 200_000).tobytes()
+    blocks = _chunks_from(target, 3, 0xE0, lo=5_000, hi=30_000)
+    blocks += [_block([("empty", b"")]), _block([("tiny", b"ACGTAC")]), _block([("n", b"N" * 5_000)]),
+               _block([("masked", target[50_000:53_000].lower())]),
+               _block([("unrelated", synth.random_bases(0xB0E1, 30_000).tobytes())])]
+    blocks.append(_chunks_from(target, 1, 0xE1, lo=5_000, hi=30_000)[0])
+    got, st = _run(target, blocks)
+    assert all(len(w) == 0 for w in got[3:5])  # no intervals: no batches
+    assert all(sum(len(fw) + len(rc) for fw, rc in w) == 0 for w in got[5:8])
+    assert sum(len(fw) + len(rc) for fw, rc in got[0]) > 0
+    sub = O.sub_mat("", 910)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
+        res, _ = eng.filter_batch([])
+        assert res == []
+        res, _ = eng.filter_batch([(S.encode(f), S.encode(r)) for f, r in blocks[3:6]])
+        assert [len(w) for w in res] == [0, 0, 1]
+
+
 def test_batch_packed_gate_limits(gpu_available):
     """xdrop 4000 and ACGT scores up to 4000 (HOXD70 x 40), the largest the
     packed gates take: a word's 16 cells then reach +-64 000, so each of the

@@ -103,6 +103,15 @@ def test_error_paths(gpu_available):
     with pytest.raises(E.HspzError) as ei:
         eng.seed_and_filter(0, False)
     assert ei.value.status == 1
+    with pytest.raises(E.HspzError) as ei:  # a pass before any swap_query
+        eng.run_pass()
+    assert ei.value.status == 1
+    empty = np.zeros(0, dtype=np.uint8)
+    eng.swap_query(empty, empty)  # an empty query: seed_and_filter refuses it, a pass is empty
+    with pytest.raises(E.HspzError) as ei:
+        eng.seed_and_filter(0, False)
+    assert ei.value.status == 1
+    assert eng.run_pass()[0] == []
     eng.swap_query(o.q_enc, o.q_rc_enc)
     with pytest.raises(E.HspzError) as ei:
         eng.generate_seeds(0, False, 0, len(o.q_enc))
