@@ -123,6 +123,34 @@ def test_error_paths(gpu_available):
     eng.close()
 
 
+def _big_score_pair(thresh):
+    """Six identical segments of 170..333 bp between unrelated flanks, under
+    matches of about 100,000 and mismatches of -90,001: HSP totals of 17 M to
+    33 M, above 2^24, where float32 spacing is 2 or 4."""
+    m = [100003, 99997, 99991, 100007]
+    acgt = [m[i] if i == j else -90001 for i in range(4) for j in range(4)]
+    ref = bytearray(synth.random_bases(0x2425, 8000).tobytes())
+    qry = bytearray(synth.random_bases(0x2426, 8000).tobytes())
+    for k, ln in enumerate((170, 171, 190, 230, 260, 333)):
+        a, b = 500 + 1200 * k, 700 + 1200 * k
+        qry[b:b + ln] = ref[a:a + ln]
+    return Pair.raw(bytes(ref), bytes(qry), acgt=acgt, xdrop=1_000_000, hspthresh=thresh, noentropy=True)
+
+
+def test_scores_above_2_24_keep_the_float32_rounding(gpu_available):
+    """kernels.rs:2036-2041: the gate and the materializer compare the total
+    through float32. Above 2^24 that rounds: 23,120,075 rounds up to the
+    threshold 23,120,076 and is kept; 17,330,177 rounds down below a threshold
+    of its own value and is dropped. The matrix is beyond the packed gate's
+    limits, so the plain path runs the byte-code gate."""
+    _say("case scores above 2^24 (byte-code gate, float32 rounding)")
+    for thresh, want_scores in ((23_120_076, [23120075, 26010094, 33330021]),
+                                (17_330_177, [19130053, 23120075, 26010094, 33330021])):
+        p = _big_score_pair(thresh)
+        got, _ = _check(p)
+        assert sorted(int(x) for fw, rc in got for a in (fw, rc) for x in a["score"]) == want_scores
+
+
 def test_c1_full_size(gpu_available):
     _say("case c1 full size: 4 Mbp x 4 Mbp, plain and batched")
     t, q = synth.c1_pair(4_000_000, seed=0xC0FFEE01)
