@@ -52,10 +52,12 @@ def test_repeat_with_small_max_hits(gpu_available):
     _check(Pair.fixture("repeat", max_hits=1000))
 
 
-def test_c1_shape_small_multi_interval(gpu_available):
-    _say("case c1-shape 400 kbp, 3 intervals, 100 kbp chunks")
+@pytest.mark.parametrize("xdrop", [910, 5000])
+def test_c1_shape_small_multi_interval(gpu_available, xdrop):
+    # X = 5000 is beyond the packed gate: the plain path runs the byte-code gate
+    _say("case c1-shape 400 kbp, 3 intervals, 100 kbp chunks, X=%d" % xdrop)
     t, q = synth.c1_pair(400_000, seed=0xC0FFEE01)
-    _check(Pair.raw(t, q), wga_chunk=100_000, interval_size=150_000)
+    _check(Pair.raw(t, q, xdrop=xdrop), wga_chunk=100_000, interval_size=150_000)
 
 
 @pytest.mark.parametrize("ambiguous,noentropy", [("", False), ("n", False), ("iupac,5,7", True)])
