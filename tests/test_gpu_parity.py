@@ -153,6 +153,31 @@ def test_scores_above_2_24_keep_the_float32_rounding(gpu_available):
         assert sorted(int(x) for fw, rc in got for a in (fw, rc) for x in a["score"]) == want_scores
 
 
+@pytest.mark.parametrize("seed,transitions,plus,minus", [
+    ("12of19", False, True, True), ("12of19", True, True, False), ("12of19", True, False, True),
+    ("1110110111", True, True, True), ("14of22", True, True, True)])
+def test_seed_shapes_strands_and_transitions(gpu_available, seed, transitions, plus, minus):
+    """The pass options of seed_and_filter_all (run.rs:274-300) and the seed
+    shapes of Shape::parse (seed.rs:47-84), on the plain and the batched path."""
+    _say("case seed %s transitions=%s plus=%s minus=%s" % (seed, transitions, plus, minus))
+    t, q = synth.c1_pair(200_000, seed=0xC0FFEE07)
+    p = Pair.raw(t, q, seed=seed)
+    kw = dict(transitions=transitions, plus=plus, minus=minus, wga_chunk=60_000, interval_size=90_000)
+    want, wst, wh = p.oracle().run_pass(threads=16, **kw)
+    assert wst["hsps"] > 0
+    with p.engine() as eng:
+        got, gst = eng.run_pass(pattern=seed, **kw)
+        assert gst["hits"] == wst["hits"] and gst["raw_hsps"] == wst["raw_hsps"], (gst, wst)
+        assert same_intervals(got, want)
+        assert E.hash_pass(got) == wh
+        o = p.oracle()
+        for order in (0, 1):  # seed-major and k-mer-major hit order
+            eng.set_flag("batch_order", order)
+            batched, _ = eng.filter_batch([(o.q_enc, o.q_rc_enc)], pattern=seed, **kw)
+            assert eng.get_flag("batch_order_used") == order
+            assert same_intervals(batched[0], want), "batched differs in order %d" % order
+
+
 def test_c1_full_size(gpu_available):
     _say("case c1 full size: 4 Mbp x 4 Mbp, plain and batched")
     t, q = synth.c1_pair(4_000_000, seed=0xC0FFEE01)
