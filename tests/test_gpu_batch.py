@@ -85,8 +85,9 @@ def test_batch_multi_interval_and_small_max_hits(gpu_available):
     _run(target, blocks, max_hits=2000, wga_chunk=20_000, interval_size=45_000)
 
 
-def test_batch_noisy_and_soft_masked(gpu_available):
-    _say("case batch: soft-masked repeats, N runs, IUPAC codes")
+@pytest.mark.parametrize("ambiguous", ["n", "iupac,5,7"])
+def test_batch_noisy_and_soft_masked(gpu_available, ambiguous):
+    _say("case batch: soft-masked repeats, N runs, IUPAC codes, ambiguous=%r" % ambiguous)
     t, q = synth.c2_pair(400_000, seed=0xB003)
     rng = np.random.default_rng(3)
     q = bytearray(q)
@@ -94,7 +95,7 @@ def test_batch_noisy_and_soft_masked(gpu_available):
         q[i] = rng.choice(list(b"NnRYX"))
     q = bytes(q)
     blocks = [_block([("a", q[:150_000])]), _block([("b", q[150_000:260_000]), ("c", q[260_000:])])]
-    _run(t, blocks, ambiguous="n")
+    _run(t, blocks, ambiguous=ambiguous)
 
 
 def test_batch_separator_that_does_not_force_a_drop(gpu_available):
