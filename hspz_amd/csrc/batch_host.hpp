@@ -341,7 +341,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         }
         uint32_t *kkey = kmer_order ? B.kkey.as<uint32_t>() : nullptr;
         uint64_t *kval = kmer_order ? B.sq.as<uint64_t>() : nullptr;
-        const int key_bits = 32 - __builtin_clz(nkm);  // keys are in [0, nk]
+        const int key_bits = 32 - __builtin_clz(nkm - 1);  // keys are in [0, nk - 1]
         if (kmer_order) {  // sort temp storage before any capture
             size_t bytes = 0;
             HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint64_t>(),
@@ -388,10 +388,10 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         hipLaunchKernelGGL(k_bchunks, dim3(ceil_div(nb, 64)), dim3(64), 0, h->stream, B.prefix.as<uint64_t>(),
                            B.voff.as<uint32_t>(), nb, per, h->max_hits, B.nchunks.as<uint32_t>(),
                            B.starts.as<uint32_t>(), B.bhits.as<uint64_t>(), ctr + CTR_ERR);
-        if (kmer_order)  // keys of the grouping sort: k-mer for seeds with hits, nk for the rest
+        if (kmer_order)  // keys of the grouping sort: k-mer for the valid seeds, nk - 1 for unused slots
             hipLaunchKernelGGL(k_kkeys, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
-                               h->stream, B.seeds.as<uint64_t>(), B.counts.as<uint32_t>(), ctr + CTR_NVALID, per,
-                               (uint32_t)cap_seeds, nkm, kkey, kval);
+                               h->stream, B.seeds.as<uint64_t>(), ctr + CTR_NVALID, per, (uint32_t)cap_seeds, nkm,
+                               kkey, kval);
         };
         // k-mer grouping: a stable radix sort of all seed slots by key (run
         // eagerly between the graphs: rocPRIM issues its own fills), then the
@@ -409,7 +409,8 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         auto enqueue_group = [&]() {
             if (!kmer_order) return;
             hipLaunchKernelGGL(k_kbounds, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
-                               h->stream, B.skm.as<uint32_t>(), (uint32_t)cap_seeds, nkm, B.gcnt.as<uint32_t>());
+                               h->stream, B.skm.as<uint32_t>(), ctr + CTR_NVALID, per, (uint32_t)cap_seeds, nkm,
+                               B.gcnt.as<uint32_t>());
             hipLaunchKernelGGL(k_qrecords, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
                                h->stream, B.kvalo.as<uint64_t>(), B.gcnt.as<uint32_t>() + nkm,
                                B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(), arena_len, h->seed_size,

@@ -471,26 +471,35 @@ struct QRec {
 };
 static_assert(sizeof(QRec) == 64, "query records are four 16-byte loads");
 
-// Sort keys of the k-mer grouping: every seed slot below cap gets its k-mer
-// when the seed has hits and nk otherwise (so the seeds without hits sort
-// last); the value is (slot index << 32) | arena position.
-__global__ void k_kkeys(const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ counts,
-                        const uint32_t *__restrict__ nvalid, uint32_t per, uint32_t cap, uint32_t nk,
-                        uint32_t *__restrict__ key, uint64_t *__restrict__ val) {
-    const uint32_t n = *nvalid * per;
+// Sort keys of the k-mer grouping: the n valid seed slots get their k-mer
+// (a seed without hits lands in a group whose bucket is empty, which the gate
+// skips), the unused slots [n, cap) get nk - 1: a stable sort puts them after
+// every valid slot, so keys stay below nk (2k bits) and the grouped seeds are
+// exactly the sorted slots [0, n). The value is (slot index << 32) | arena
+// position.
+__global__ void k_kkeys(const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ nvalid, uint32_t per,
+                        uint32_t cap, uint32_t nk, uint32_t *__restrict__ key, uint64_t *__restrict__ val) {
+    const uint32_t n = min(*nvalid * per, cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
-        const bool with_hits = i < n && counts[i];
-        const uint64_t sd = with_hits ? seeds[i] : 0;
-        key[i] = with_hits ? (uint32_t)(sd >> 32) : nk;
+        const uint64_t sd = i < n ? seeds[i] : 0;
+        key[i] = i < n ? (uint32_t)(sd >> 32) : nk - 1;
         val[i] = ((uint64_t)i << 32) | (uint32_t)sd;
     }
 }
 
-// Group starts from the sorted keys: goff[k] = the first slot whose key is at
-// least k, for k in [0, nk] (goff[nk] is the number of grouped seeds). Slot j
-// writes the k-mers in (key[j - 1], key[j]], and the last slot those after it.
-__global__ void k_kbounds(const uint32_t *__restrict__ key, uint32_t n, uint32_t nk, uint32_t *__restrict__ goff) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+// Group starts from the sorted keys of the n valid slots: goff[k] = the first
+// slot whose key is at least k, for k in [0, nk] (goff[nk] = n, the number of
+// grouped seeds). Slot j writes the k-mers in (key[j - 1], key[j]], and the
+// last slot those after it.
+__global__ void k_kbounds(const uint32_t *__restrict__ key, const uint32_t *__restrict__ nvalid, uint32_t per,
+                          uint32_t cap, uint32_t nk, uint32_t *__restrict__ goff) {
+    const uint32_t n = min(*nvalid * per, cap);
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    if (n == 0) {
+        for (uint32_t k = t0; k <= nk; k += stride) goff[k] = 0;
+        return;
+    }
+    for (uint32_t j = t0; j < n; j += stride) {
         uint32_t kj = key[j];
         uint32_t lo = j ? key[j - 1] + 1 : 0;
         for (uint32_t k = lo; k <= kj; k++) goff[k] = j;
