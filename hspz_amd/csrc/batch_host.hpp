@@ -26,13 +26,13 @@ struct HostBatch {
 struct BatchBufs {
     DevBuf arena, plane, coarse, desc, poff, blk_off, blk_len, valid, ids, voff, seeds, counts, prefix,
         tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr, gcnt, sq, skm, qrec,
-        kvalo, ktmp, exact, kkey;
+        kvalo, ktmp, exact, kkey, kgaps;
 };
 
 // Device counters of one pass; CTR_WORK is the 64-bit seed-claim counter of
 // the batched gate (two slots, 8-byte aligned).
 enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_EXACT = 5, CTR_WORK = 6,
-       CTR_N = 8 };
+       CTR_KGAPS = 8, CTR_N = 10 };
 
 typedef void (*BGateFn)(const uint32_t *, const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, uint32_t,
                         const uint32_t *, int32_t, int32_t, const uint64_t *, const uint32_t *, uint32_t,
@@ -331,6 +331,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         if ((rc = reserve(h, h->tiles, (size_t)std::max(std::max(kt, nt), ntk) * 4))) return rc;
         if (kmer_order) {
             if ((rc = reserve(h, B.gcnt, ((size_t)nkm + 1) * 4))) return rc;
+            if ((rc = reserve(h, B.kgaps, (size_t)kgap_capacity(nkm) * sizeof(uint4)))) return rc;
             if ((rc = reserve(h, B.kkey, cap_seeds * 4 + 4))) return rc;   // sort keys (u32)
             if ((rc = reserve(h, B.sq, cap_seeds * 8 + 8))) return rc;     // sort values (u64), in
             if ((rc = reserve(h, B.kvalo, cap_seeds * 8 + 8))) return rc;  // and out
@@ -410,7 +411,9 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             if (!kmer_order) return;
             hipLaunchKernelGGL(k_kbounds, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
                                h->stream, B.skm.as<uint32_t>(), ctr + CTR_NVALID, per, (uint32_t)cap_seeds, nkm,
-                               B.gcnt.as<uint32_t>());
+                               B.gcnt.as<uint32_t>(), B.kgaps.as<uint4>(), ctr + CTR_KGAPS, kgap_capacity(nkm));
+            hipLaunchKernelGGL(k_kgaps, dim3(1024), dim3(256), 0, h->stream, B.kgaps.as<uint4>(), ctr + CTR_KGAPS,
+                               kgap_capacity(nkm), B.gcnt.as<uint32_t>());
             hipLaunchKernelGGL(k_qrecords, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
                                h->stream, B.kvalo.as<uint64_t>(), B.gcnt.as<uint32_t>() + nkm,
                                B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(), arena_len, h->seed_size,
@@ -489,7 +492,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                   B.nchunks.p, B.starts.p, B.bhits.p, B.surv.p, B.slow.p, B.mhsp.p, B.mscope.p,
                                   B.keep.p, B.kids.p, B.ohsp.p, B.oscope.p, B.ctr.p, h->tiles.p, h->ref2.p,
                                   h->refx.p, h->pairtab.p, h->index.p, h->pos.p, B.gcnt.p, B.sq.p, B.skm.p,
-                                  h->clean.p, B.qrec.p, h->rrec.p, B.kvalo.p, B.exact.p, B.kkey.p};
+                                  h->clean.p, B.qrec.p, h->rrec.p, B.kvalo.p, B.exact.p, B.kkey.p, B.kgaps.p};
             static_assert(sizeof ptrs <= sizeof key.ptrs, "graph key pointer slots");
             memcpy(key.ptrs, ptrs, sizeof ptrs);
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow, cap_seeds, per, h->max_hits,

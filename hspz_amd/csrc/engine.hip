@@ -648,7 +648,7 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
         DevBuf *bb[] = {&B.arena, &B.plane, &B.coarse, &B.desc, &B.poff, &B.blk_off, &B.blk_len, &B.valid, &B.ids,
                         &B.voff, &B.seeds, &B.counts, &B.prefix, &B.tiles, &B.nchunks, &B.starts, &B.bhits, &B.surv,
                         &B.slow, &B.mhsp, &B.mscope, &B.keep, &B.kids, &B.ohsp, &B.oscope, &B.ctr, &B.gcnt,
-                        &B.sq, &B.skm, &B.qrec, &B.kvalo, &B.ktmp, &B.exact, &B.kkey};
+                        &B.sq, &B.skm, &B.qrec, &B.kvalo, &B.ktmp, &B.exact, &B.kkey, &B.kgaps};
         for (DevBuf *b : bb) release(*b);
         hspz_batch_state *bs = h->batch;
         DevBuf *dd[] = {&bs->dd_key_a, &bs->dd_key_b, &bs->dd_id_a, &bs->dd_id_b, &bs->dd_keep, &bs->dd_kids,

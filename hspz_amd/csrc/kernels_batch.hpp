@@ -487,12 +487,35 @@ __global__ void k_kkeys(const uint64_t *__restrict__ seeds, const uint32_t *__re
     }
 }
 
+// Key ranges longer than this are written by k_kgaps, a block per range,
+// instead of by the slot that owns them.
+constexpr uint32_t KGAP_SERIAL = 64;
+
+// Capacity of the long-range queue for nk k-mers: the ranges are disjoint
+// and each holds at least KGAP_SERIAL of the nk + 1 entries.
+__host__ __device__ constexpr uint32_t kgap_capacity(uint32_t nk) { return (nk + 1) / KGAP_SERIAL + 2; }
+
+// goff[a..b] = v: short ranges in place, long ones queued for k_kgaps.
+__device__ __forceinline__ void kbounds_range(uint32_t a, uint32_t b, uint32_t v, uint32_t *__restrict__ goff,
+                                              uint4 *__restrict__ gaps, uint32_t *__restrict__ ngaps,
+                                              uint32_t cap_gaps) {
+    if (a > b) return;
+    if (b - a < KGAP_SERIAL) {
+        for (uint32_t k = a; k <= b; k++) goff[k] = v;
+        return;
+    }
+    const uint32_t at = atomicAdd(ngaps, 1u);
+    if (at < cap_gaps) gaps[at] = make_uint4(a, b, v, 0u);
+}
+
 // Group starts from the sorted keys of the n valid slots: goff[k] = the first
 // slot whose key is at least k, for k in [0, nk] (goff[nk] = n, the number of
-// grouped seeds). Slot j writes the k-mers in (key[j - 1], key[j]], and the
-// last slot those after it.
+// grouped seeds). Slot j owns the k-mers (key[j - 1], key[j]], and the last
+// slot also those after it. A batch with few distinct k-mers leaves long
+// ranges; those go to k_kgaps, so that no thread writes millions of entries.
 __global__ void k_kbounds(const uint32_t *__restrict__ key, const uint32_t *__restrict__ nvalid, uint32_t per,
-                          uint32_t cap, uint32_t nk, uint32_t *__restrict__ goff) {
+                          uint32_t cap, uint32_t nk, uint32_t *__restrict__ goff, uint4 *__restrict__ gaps,
+                          uint32_t *__restrict__ ngaps, uint32_t cap_gaps) {
     const uint32_t n = min(*nvalid * per, cap);
     const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
     if (n == 0) {
@@ -500,11 +523,19 @@ __global__ void k_kbounds(const uint32_t *__restrict__ key, const uint32_t *__re
         return;
     }
     for (uint32_t j = t0; j < n; j += stride) {
-        uint32_t kj = key[j];
-        uint32_t lo = j ? key[j - 1] + 1 : 0;
-        for (uint32_t k = lo; k <= kj; k++) goff[k] = j;
-        if (j + 1 == n)
-            for (uint32_t k = kj + 1; k <= nk; k++) goff[k] = n;
+        const uint32_t kj = key[j];
+        kbounds_range(j ? key[j - 1] + 1 : 0, kj, j, goff, gaps, ngaps, cap_gaps);
+        if (j + 1 == n) kbounds_range(kj + 1, nk, n, goff, gaps, ngaps, cap_gaps);
+    }
+}
+
+// The long ranges k_kbounds queued: a block per range, a thread per entry.
+__global__ void k_kgaps(const uint4 *__restrict__ gaps, const uint32_t *__restrict__ ngaps, uint32_t cap_gaps,
+                        uint32_t *__restrict__ goff) {
+    const uint32_t ng = min(*ngaps, cap_gaps);
+    for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x) {
+        const uint4 r = gaps[g];
+        for (uint32_t k = r.x + threadIdx.x; k <= r.y; k += blockDim.x) goff[k] = r.z;
     }
 }
 
