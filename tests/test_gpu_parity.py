@@ -178,6 +178,35 @@ def test_seed_shapes_strands_and_transitions(gpu_available, seed, transitions, p
             assert same_intervals(batched[0], want), "batched differs in order %d" % order
 
 
+def test_max_hits_underflow_and_oversize_seed(gpu_available):
+    """chunk_limits (gpu/mod.rs:2397-2420): a first seed that alone holds more
+    than max_hits hits is the reference's underflow error, on the plain and
+    the batched path; a later such seed only makes an oversize chunk, and the
+    lists equal the oracle's."""
+    _say("case max_hits=60: first seed over the limit (error), later seed over it (oversize chunk)")
+    t = bytearray(synth.random_bases(0x51, 50_000).tobytes())
+    t[1000:1100] = b"A" * 100
+    t = bytes(t)
+    # the poly-A k-mer has 82 positions; a batch stays within the 64 chunks the batched path takes
+    first = Pair.raw(t, b"A" * 40 + synth.random_bases(0x52, 1000).tobytes(), max_hits=60)
+    with pytest.raises(RuntimeError):
+        first.oracle().run_pass(threads=4)
+    o = first.oracle()
+    with first.engine() as eng:
+        with pytest.raises(E.HspzError) as ei:
+            eng.run_pass()
+        assert ei.value.status == 5
+        with pytest.raises(E.HspzError) as ei:
+            eng.filter_batch([(o.q_enc, o.q_rc_enc)])
+        assert ei.value.status == 5 and "single seed exceeds max_hits" in str(ei.value)
+    later = Pair.raw(t, synth.random_bases(0x53, 1000).tobytes() + b"A" * 40 + t[20000:21000], max_hits=60)
+    want, _ = _check(later)
+    o = later.oracle()
+    with later.engine() as eng:
+        got, _ = eng.filter_batch([(o.q_enc, o.q_rc_enc)])
+        assert same_intervals(got[0], want)
+
+
 def test_c1_full_size(gpu_available):
     _say("case c1 full size: 4 Mbp x 4 Mbp, plain and batched")
     t, q = synth.c1_pair(4_000_000, seed=0xC0FFEE01)
