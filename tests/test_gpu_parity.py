@@ -225,13 +225,14 @@ def test_c1_full_size(gpu_available):
             assert same_intervals(bgot[0], got)
 
 
-@pytest.mark.parametrize("case", ["random", "softmasked_n", "step3", "tiny"])
+@pytest.mark.parametrize("case", ["random", "softmasked_n", "step3", "tiny", "1110110111", "14of22"])
 def test_device_index_equals_host(gpu_available, case):
     """hspz_engine_new_device_index builds the same index as the host
     SeedTable::build (seed.rs:126-373) and the oracle, and the engine it makes
     gives the same HSP lists as one built from the host index."""
     print("case device index: %s" % case, flush=True)
     step = 3 if case == "step3" else 1
+    shape = case if case in ("1110110111", "14of22") else "12of19"
     if case == "tiny":
         target = b"AAAAACGTAAAAACGTTTTTTTTTTTTTTTTTTTTTTTGGGGA"
     elif case == "softmasked_n":
@@ -242,13 +243,13 @@ def test_device_index_equals_host(gpu_available, case):
             t[i:i + rng.integers(1, 40)] = b"N" * len(t[i:i + rng.integers(1, 40)])
         target = bytes(t)
     else:
-        target = synth.random_bases(0xD2, 500_000).tobytes()
+        target = synth.random_bases(0xD2, 500_000 if shape == "12of19" else 200_000).tobytes()
     sub = O.sub_mat("", 910)
-    h_index, h_pos = E.seed_table_build(target, "12of19", step, 8)
-    o_index, o_pos = O.seed_table(target, O.shape("12of19"), step)
+    h_index, h_pos = E.seed_table_build(target, shape, step, 8)
+    o_index, o_pos = O.seed_table(target, O.shape(shape), step)
     assert np.array_equal(h_index, o_index) and np.array_equal(h_pos, o_pos)
-    with E.Engine(None, None, S.encode(target), sub, max_hits=16711680, device_index="12of19",
-                  index_step=step) as eng:
+    with E.Engine(None, None, S.encode(target), sub, seed_size=E.shape_parse(shape)[0], max_hits=16711680,
+                  device_index=shape, index_step=step) as eng:
         d_index, d_pos = eng.index()
         assert eng.stat("pos_len") == len(h_pos)
         assert np.array_equal(d_index, h_index), "index_table differs"
