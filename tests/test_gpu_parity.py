@@ -207,6 +207,24 @@ def test_max_hits_underflow_and_oversize_seed(gpu_available):
         assert same_intervals(got[0], want)
 
 
+@pytest.mark.parametrize("xdrop,thresh", [(0, 100), (1, 300), (4000, 0), (4000, 1)])
+def test_extreme_xdrop_and_threshold(gpu_available, xdrop, thresh):
+    """X-drop at 0 and 1 (a side stops at its first loss) and thresholds of 0
+    and 1 (every hit that scores at all survives the gate), plain and batched."""
+    _say("case xdrop=%d thresh=%d" % (xdrop, thresh))
+    t = synth.random_bases(0xE5, 60_000).tobytes()
+    q = synth.mutate(np.frombuffer(t[10_000:40_000], dtype=np.uint8), 0xE6, 0.12, 0.004).tobytes()
+    p = Pair.raw(t, q, xdrop=xdrop, hspthresh=thresh)
+    want, st = _check(p)
+    assert st["hsps"] > 0
+    o = p.oracle()
+    with p.engine() as eng:
+        for order in (0, 1):
+            eng.set_flag("batch_order", order)
+            got, _ = eng.filter_batch([(o.q_enc, o.q_rc_enc)])
+            assert same_intervals(got[0], want), "batched differs in order %d" % order
+
+
 def test_c1_full_size(gpu_available):
     _say("case c1 full size: 4 Mbp x 4 Mbp, plain and batched")
     t, q = synth.c1_pair(4_000_000, seed=0xC0FFEE01)
