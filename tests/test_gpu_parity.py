@@ -225,6 +225,25 @@ def test_extreme_xdrop_and_threshold(gpu_available, xdrop, thresh):
             assert same_intervals(got[0], want), "batched differs in order %d" % order
 
 
+def test_long_homolog_extensions(gpu_available):
+    """A 20 kbp homolog at 0.5% substitutions: most hits extend thousands of
+    cells past the gate's windows, through the deferred-hit gates' stretch
+    loop to the sequence ends, on the plain and the batched path."""
+    _say("case 20 kbp homolog, 0.5% substitutions (long extensions)")
+    t = synth.random_bases(0xE7, 80_000).tobytes()
+    q = synth.mutate(np.frombuffer(t[30_000:50_000], dtype=np.uint8), 0xE8, 0.005, 0.0).tobytes()
+    p = Pair.raw(t, q)
+    want, st = _check(p)
+    assert max(int(a["len"].max()) for fw, rc in want for a in (fw, rc) if len(a)) > 5_000
+    o = p.oracle()
+    with p.engine() as eng:
+        for order in (0, 1):
+            eng.set_flag("batch_order", order)
+            got, _ = eng.filter_batch([(o.q_enc, o.q_rc_enc)])
+            assert eng.stat("gate_deferred") > 0
+            assert same_intervals(got[0], want), "batched differs in order %d" % order
+
+
 def test_c1_full_size(gpu_available):
     _say("case c1 full size: 4 Mbp x 4 Mbp, plain and batched")
     t, q = synth.c1_pair(4_000_000, seed=0xC0FFEE01)
