@@ -244,6 +244,25 @@ def test_long_homolog_extensions(gpu_available):
             assert same_intervals(got[0], want), "batched differs in order %d" % order
 
 
+def test_homologs_at_sequence_edges(gpu_available):
+    """Homologs that touch the first and last bases of both sequences: their
+    hits sit within the gate's margins of an end and take the exact paths."""
+    _say("case homologs at the sequence edges")
+    t = synth.random_bases(0xE9, 30_000).tobytes()
+    mid = synth.random_bases(0xEA, 2_000).tobytes()
+    q = t[-600:] + mid + t[:600]
+    for qq in (q, t[:700] + mid + t[-700:]):
+        p = Pair.raw(t, qq)
+        want, st = _check(p)
+        assert st["hsps"] >= 2
+        o = p.oracle()
+        with p.engine() as eng:
+            for order in (0, 1):
+                eng.set_flag("batch_order", order)
+                got, _ = eng.filter_batch([(o.q_enc, o.q_rc_enc)])
+                assert same_intervals(got[0], want), "batched differs in order %d" % order
+
+
 def test_c1_full_size(gpu_available):
     _say("case c1 full size: 4 Mbp x 4 Mbp, plain and batched")
     t, q = synth.c1_pair(4_000_000, seed=0xC0FFEE01)
