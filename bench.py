@@ -25,7 +25,8 @@ cell; every measured cell is reported under `autotune`.
 
 Prints one JSON line (N = 1). `roofline` covers the fused expansion + score
 gate kernel of the batched path (HIP events around its launches);
-`cpu_baseline` is the C oracle on a bounded sample of the same chunks. `io`
+`cpu_baseline` is the C oracle on a bounded sample of the same chunks, one
+block per worker thread (all-core), with a 1-thread figure beside it. `io`
 times the host paths around the engine outside the timed region: writing the
 step's lists as .segments (directory, -D partitions, .tar.gz) and reading a
 FASTA / gzipped FASTA slice of the target.
@@ -74,15 +75,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--target-mbp", type=float, default=1000.0)
     ap.add_argument("--query-mbp", type=float, default=4.0)
-    ap.add_argument("--cpu-sample-kbp", type=int, default=40)
+    ap.add_argument("--cpu-blocks", type=int, default=32,
+                    help="20 kbp blocks of the query the CPU baseline runs, one block per worker thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-only", action="store_true",
                     help="time only the plain per-chunk path (profiling)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--serve-requests", type=int, default=150,
+    ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests per arrival-rate arm of the serving measurement (0 = skip)")
     ap.add_argument("--serve-rates", default="50,200,800")
+    ap.add_argument("--serve-check", type=int, default=50,
+                    help="requests per arm, drawn at random, whose served lists are compared with the "
+                         "same request processed alone")
     ap.add_argument("--tune-mbp", type=float, default=1.0,
                     help="query Mbp the gate autotuner measures on before the timed region (0 = skip)")
     ap.add_argument("--tune-reps", type=int, default=1)
@@ -90,7 +95,20 @@ def main():
     ap.add_argument("--gate", default="",
                     help="step_cells,block_threads,claim_seeds,grid_blocks[,order[,lookahead]]] for the batched gate "
                          "(applied after the autotuner; use with --tune-mbp 0 to fix the configuration)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device once and exit: the plain copy of known size "
+                         "that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied %d bytes" % (args.copy_probe << 20))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the seed-and-filter path has no cross-GPU step "
@@ -234,34 +252,8 @@ def main():
 
     cpu, cells_per_hit, parity_sample = None, None, None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_ffi as O
-        # bounded sample: the leading part of each of the first chunks
-        budget, sample_bp, o_time, o_hits, o_cells, ok = args.cpu_sample_kbp * 1000, 0, 0.0, 0, 0, True
-        for c in pieces:
-            if sample_bp >= budget:
-                break
-            part = c[:min(len(c), budget - sample_bp, 20_000)]
-            if len(part) <= 19:
-                continue
-            buf, chrs, bl = S.pack([("s", part)])
-            rcb, _ = S.reverse_complement(buf, chrs, bl)
-            orc = O.Oracle(traw, buf[:bl], rcb, sub, max_hits=max_hits, index=(index, pos))
-            t1 = time.perf_counter()
-            o_res, o_st, o_hash = orc.run_pass(threads=args.threads)
-            o_time += time.perf_counter() - t1
-            o_hits += o_st["hits"]
-            o_cells += o_st["cells"]
-            sample_bp += len(part)
-            g_res, _ = eng.filter_batch([(buf[:bl], None)])
-            ok = ok and E.hash_pass(g_res[0]) == o_hash
-        cells_per_hit = o_cells / max(o_hits, 1)
-        parity_sample = ok
-        cpu = {"value": round(sample_bp / 1e6 / o_time, 5), "unit": "query Mbp/s",
-               "cores": args.threads, "kind": "port",
-               "sample": "%d bp: the leading 20 kbp of the first chunks, each its own block, both "
-                         "strands, against the full target (%d hits, %.1f s)" % (sample_bp, o_hits, o_time)}
-        log("cpu baseline %.5f Mbp/s on %d threads; sample parity %s" % (cpu["value"], args.threads, ok))
+        cpu, cells_per_hit, parity_sample = cpu_baseline(E, S, eng, pieces, traw, ref_enc, index, pos, sub,
+                                                         max_hits, args.cpu_blocks, args.threads)
 
     io = None
     if not args.no_io:
@@ -270,7 +262,8 @@ def main():
     serving = None
     if args.serve_requests > 0:
         serving = serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target,
-                             [float(x) for x in args.serve_rates.split(",")], args.serve_requests)
+                             [float(x) for x in args.serve_rates.split(",")], args.serve_requests,
+                             args.serve_check)
 
     roof = None
     if g["gate_ns"] > 0 and cells_per_hit is not None:
@@ -334,13 +327,14 @@ def main():
     eng.close()
 
 
-def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rates, n):
+def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rates, n, n_check):
     """Serving on the resident 1 Gbp target: per arrival rate, n requests with
     clipped lognormal sizes (median 100 kbp, sigma 1.535, 1 kbp..10 Mbp),
     divergence U(2%, 30%), half reverse-complemented, arriving as a Poisson
     process; two engines (same reference) serve them by continuous batching.
-    Latency = completion - arrival. Every arm also checks that the first
-    requests' lists equal the same requests processed alone."""
+    Latency = completion - arrival. Every arm also checks, on n_check requests
+    drawn at random, that their served lists equal the same request processed
+    alone on the plain path."""
     import threading
     import numpy as np
     eng2 = E.Engine(None, None, ref_enc, sub, max_hits=max_hits, device_index="12of19")
@@ -372,19 +366,20 @@ def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rat
         t_start = time.perf_counter()
         th.start()
         th.join()
-        lat, last_done, first_sub, results = [], 0.0, None, []
+        check = set(np.random.default_rng(2000 + k).choice(n, size=min(n_check, n), replace=False).tolist())
+        lat, last_done, first_sub, results = [], 0.0, None, {}
         for i in range(n):
             res, tm = srv.wait(ids[i])
             lat.append(tm["t_done"] - tm["t_submit"])
             last_done = max(last_done, tm["t_done"])
             first_sub = tm["t_submit"] if first_sub is None else min(first_sub, tm["t_submit"])
-            if i < 4:
-                results.append(res)
+            if i in check:
+                results[i] = res
         wall = time.perf_counter() - t_start
         st = srv.stats()
         srv.close()
         same = 0
-        for i, res in enumerate(results):
+        for i, res in sorted(results.items()):
             eng.swap_query_raw(reqs[i])
             alone, _ = eng.run_pass()
             same += E.hash_pass(alone) == E.hash_pass(res)
@@ -405,6 +400,78 @@ def serve_arms(E, S, synth, eng, index, pos, ref_enc, sub, max_hits, target, rat
     return {"engines": 2, "arms": arms,
             "note": "offered load above the service rate saturates the loop; latency then "
                     "includes queueing"}
+
+
+def cpu_info():
+    """CPU model and logical CPUs of this host (the GPU box's host when run there)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
+def cpu_baseline(E, S, eng, pieces, traw, ref_enc, index, pos, sub, max_hits, n_blocks, threads):
+    """The C oracle (the reference's semantics restated; the reference itself
+    cannot be built here) on a bounded sample of the same workload: the leading
+    20 kbp of n_blocks query chunks, each its own block (both strands, against
+    the whole resident target). All-core: one block per worker thread, `threads`
+    workers busy at once (the oracle call releases the interpreter lock).
+    1-thread: the first two blocks run one after the other on one thread. Every
+    block's lists are checked equal to the GPU's (one filter_batch over the
+    sample)."""
+    import concurrent.futures as F
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+    blocks = []
+    for c in pieces:
+        if len(blocks) >= n_blocks:
+            break
+        part = c[:20_000]
+        if len(part) <= 19:
+            continue
+        buf, chrs, bl = S.pack([("s", part)])
+        rcb, _ = S.reverse_complement(buf, chrs, bl)
+        blocks.append((buf[:bl], rcb))
+    oracles = [O.Oracle(traw, f, r, sub, max_hits=max_hits, index=(index, pos), ref_enc=ref_enc)
+               for f, r in blocks]
+    sample_bp = sum(len(f) for f, _ in blocks)
+    # 1 thread
+    t1 = time.perf_counter()
+    one_bp = 0
+    for o, (f, _) in list(zip(oracles, blocks))[:2]:
+        o.run_pass(threads=1)
+        one_bp += len(f)
+    one_s = time.perf_counter() - t1
+    # all cores: one block per worker
+    t1 = time.perf_counter()
+    with F.ThreadPoolExecutor(max_workers=threads) as ex:
+        outs = list(ex.map(lambda o: o.run_pass(threads=1), oracles))
+    all_s = time.perf_counter() - t1
+    hits = sum(st["hits"] for _, st, _ in outs)
+    cells = sum(st["cells"] for _, st, _ in outs)
+    g_res, _ = eng.filter_batch([(f, None) for f, _ in blocks])
+    ok = all(E.hash_pass(g) == h for g, (_, _, h) in zip(g_res, outs))
+    model, ncpu, usable = cpu_info()
+    busy = min(threads, len(blocks))
+    cpu = {"value": round(sample_bp / 1e6 / all_s, 5), "unit": "query Mbp/s", "cores": busy, "kind": "port",
+           "one_thread_mbp_s": round(one_bp / 1e6 / one_s, 6),
+           "threads_busy": busy, "cpu_model": model, "nproc": ncpu, "usable_cpus": usable,
+           "sample": "%d bp: the leading 20 kbp of %d query chunks, each its own block (both strands) "
+                     "against the full target, one block per worker thread on %d threads (%d hits, %.1f s); "
+                     "1-thread figure from %d bp run serially (%.1f s)"
+                     % (sample_bp, len(blocks), busy, hits, all_s, one_bp, one_s)}
+    log("cpu baseline: %.5f Mbp/s on %d threads, %.6f Mbp/s on 1 thread (%s, nproc %d); sample parity %s"
+        % (cpu["value"], busy, cpu["one_thread_mbp_s"], model, ncpu, ok))
+    return cpu, cells / max(hits, 1), ok
 
 
 def io_measure(results, pieces, traw):

@@ -29,6 +29,30 @@ struct BatchBufs {
         kvalo, ktmp, exact, kkey, kgaps;
 };
 
+// Largest survivor or deferred-hit list of one pass, in records (16 B each).
+constexpr uint32_t LIST_CAP_MAX = 1u << 30;
+
+// Capacity after an overflow: at least twice the larger of the old capacity
+// and the count the pass reached, computed in 64 bits and clamped to
+// LIST_CAP_MAX. False when the count itself is beyond LIST_CAP_MAX (growing
+// could never hold it).
+bool grow_list_cap(uint32_t cap, uint32_t count, uint32_t *out) {
+    if (count > LIST_CAP_MAX) return false;
+    uint64_t want = 2 * (uint64_t)std::max(cap, count);
+    *out = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), LIST_CAP_MAX);
+    return *out >= count;
+}
+
+}  // namespace
+
+// Test hook (not part of the ABI in include/hspz.h): grow_list_cap for the
+// CPU unit test of the growth arithmetic; 1 when the grown list holds count.
+extern "C" int hspz_test_grow_list_cap(uint32_t cap, uint32_t count, uint32_t *out) {
+    return grow_list_cap(cap, count, out) ? 1 : 0;
+}
+
+namespace {
+
 // Device counters of one pass; CTR_WORK is the 64-bit seed-claim counter of
 // the batched gate (two slots, 8-byte aligned).
 enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_EXACT = 5, CTR_WORK = 6,
@@ -250,6 +274,9 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             }
         }
     }
+    // scope keys are b * MAX_CHUNKS + chunk in 32 bits
+    if (desc.size() >= 0xFFFFFFFFull / MAX_CHUNKS)
+        return set_error(HSPZ_E_ARG, "batch too large: %zu seed-and-filter batches", desc.size());
     uint32_t nb = (uint32_t)desc.size();
     poff.push_back((uint32_t)npos);
     if (npos >= 0xFFFFFFFFull / 16) return set_error(HSPZ_E_ARG, "batch too large: %llu positions", (unsigned long long)npos);
@@ -569,15 +596,21 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         h->last_post_ms = post_ms;
         h->last_surv = hc[CTR_SURV];
         h->last_deferred = hc[CTR_SLOW];
-        if (hc[CTR_ERR] & 8u) {  // survivor or slow list overflowed: grow and rerun
-            bs->cap_surv = std::max(bs->cap_surv, hc[CTR_SURV]) * 2;
-            bs->cap_slow = std::max(bs->cap_slow, hc[CTR_SLOW]) * 2;
-            h->batch_reruns++;
-            continue;
-        }
+        // fatal conditions first: the gate still fills the lists on a failing
+        // batch, and an overflow rerun would only repeat the failure
         if (hc[CTR_ERR] & 1u) return set_error(HSPZ_E_CHUNK, "a single seed exceeds max_hits %u", h->max_hits);
         if (hc[CTR_ERR] & 2u) return set_error(HSPZ_E_CHUNK, "a batch needs more than %u max_hits chunks", MAX_CHUNKS);
         if (hc[CTR_ERR] & 4u) return set_error(HSPZ_E_ARG, "a batch has more than 2^32 hits");
+        if (hc[CTR_ERR] & 8u) {  // survivor or slow list overflowed: grow and rerun
+            uint32_t ns = 0, nd = 0;
+            if (!grow_list_cap(bs->cap_surv, hc[CTR_SURV], &ns) || !grow_list_cap(bs->cap_slow, hc[CTR_SLOW], &nd))
+                return set_error(HSPZ_E_OOM, "batch needs more than %u survivor or deferred-hit records (%u, %u)",
+                                 LIST_CAP_MAX, hc[CTR_SURV], hc[CTR_SLOW]);
+            bs->cap_surv = ns;
+            bs->cap_slow = nd;
+            h->batch_reruns++;
+            continue;
+        }
         std::vector<uint64_t> bh(nb);
         HIP_TRY(hipMemcpyAsync(bh.data(), B.bhits.p, (size_t)nb * 8, hipMemcpyDeviceToHost, h->stream));
         uint64_t hits = 0;

@@ -18,6 +18,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 
 namespace {
 
@@ -51,6 +52,7 @@ struct hspz_server {
     std::condition_variable cv_work, cv_done;
     std::deque<ServerReq> queue;
     std::unordered_map<uint64_t, ServerDone> done;
+    std::unordered_set<uint64_t> taken;  // ids whose result a wait has already collected
     uint64_t next_id = 1;
     uint32_t next_batch = 0;
     bool stopping = false;
@@ -220,10 +222,16 @@ extern "C" int hspz_server_wait(hspz_server *s, uint64_t id, hspz_request_result
     memset(out, 0, sizeof *out);
     std::unique_lock<std::mutex> lk(s->mu);
     if (id == 0 || id >= s->next_id) return set_error(HSPZ_E_ARG, "unknown request %llu", (unsigned long long)id);
-    s->cv_done.wait(lk, [&] { return s->done.count(id) != 0; });
+    // a result is handed out once: a later (or concurrent second) wait on the
+    // same id gets an error instead of blocking forever
+    s->cv_done.wait(lk, [&] { return s->done.count(id) != 0 || s->taken.count(id) != 0; });
+    if (s->taken.count(id))
+        return set_error(HSPZ_E_ARG, "request %llu was already collected", (unsigned long long)id);
     ServerDone d = std::move(s->done[id]);
     s->done.erase(id);
+    s->taken.insert(id);
     lk.unlock();
+    s->cv_done.notify_all();
     out->status = d.status;
     out->t_submit = d.t_submit;
     out->t_start = d.t_start;

@@ -347,7 +347,9 @@ int hspz_server_new(hspz_engine **engines, size_t n_engines, const hspz_server_c
  * request whose estimated footprint exceeds the whole budget. */
 int hspz_server_submit(hspz_server *s, const uint8_t *fwd, const uint8_t *rc, size_t len,
                        uint64_t *request_id);
-/* Blocks until the request is done; release with hspz_request_result_free. */
+/* Blocks until the request is done; release with hspz_request_result_free.
+ * A result is handed out once: waiting again on a collected id (or a second
+ * concurrent wait on it) returns HSPZ_E_ARG. */
 int hspz_server_wait(hspz_server *s, uint64_t request_id, hspz_request_result *out);
 void hspz_request_result_free(hspz_request_result *r);
 int hspz_server_get_stats(hspz_server *s, hspz_server_stats *out);

@@ -195,11 +195,14 @@ class Oracle:
 
     def __init__(self, ref_raw: bytes, q_raw: bytes, q_rc_raw: bytes, sub: np.ndarray,
                  seed: str = "12of19", step: int = 1, xdrop: int = 910, hspthresh: int = 3000,
-                 noentropy: bool = False, max_hits: int = 16711680, index=None):
+                 noentropy: bool = False, max_hits: int = 16711680, index=None, ref_enc=None):
+        """index: (index_table, pos_table) when already built; ref_enc: the
+        encoded reference when already encoded (shared, not copied)."""
         from hspz_amd import sequence as S
         self.shape = shape(seed)
         self.ref_raw, self.q_raw, self.q_rc_raw = ref_raw, q_raw, q_rc_raw
-        self.ref_enc = S.encode(ref_raw).copy()
+        self.ref_enc = (np.ascontiguousarray(ref_enc, dtype=np.uint8) if ref_enc is not None
+                        else S.encode(ref_raw).copy())
         self.q_enc = S.encode(q_raw).copy()
         self.q_rc_enc = S.encode(q_rc_raw).copy()
         self.sub = np.ascontiguousarray(sub, dtype=np.int32)

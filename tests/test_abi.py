@@ -44,3 +44,29 @@ def test_flag_and_stat_tables_match_header():
     stats = {name.lower(): int(v) for kind, name, v in found if kind == "STAT"}
     assert flags == E.FLAG
     assert stats == E.STAT
+
+
+def test_list_capacity_growth_is_bounded():
+    """The batched pass's overflow rerun grows its survivor and deferred-hit
+    lists in 64-bit arithmetic: at least double the larger of the old capacity
+    and the count reached, clamped to 2^30 records; a count beyond 2^30 can
+    never be held and is refused (the pass then fails with HSPZ_E_OOM instead
+    of rerunning forever)."""
+    import ctypes as C
+    f = E.lib().hspz_test_grow_list_cap
+    f.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
+    out = C.c_uint32()
+    cases = [  # cap, count, holds, new cap
+        (64, 100, 1, 200),
+        (1 << 22, 5, 1, 1 << 23),
+        (1 << 29, (1 << 29) + 1, 1, 1 << 30),
+        ((1 << 30) - 1, 1 << 30, 1, 1 << 30),  # 2^31 would wrap in 32 bits
+        (1 << 31, (1 << 31) + 1, 0, None),
+        (1 << 30, (1 << 30) + 1, 0, None),
+        (0xFFFFFFFF, 0xFFFFFFFF, 0, None),
+    ]
+    for cap, count, holds, want in cases:
+        got = f(cap, count, C.byref(out))
+        assert got == holds, (cap, count)
+        if holds:
+            assert out.value == want and out.value >= count, (cap, count, out.value)
