@@ -50,6 +50,10 @@ struct Batch {
 
 struct hspz_batch_state;  // batch_host.hpp
 
+namespace {
+constexpr size_t H_SMALL_BYTES = 4096;  // pinned host words for counters and chunk limits
+}
+
 struct hspz_engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -76,7 +80,7 @@ struct hspz_engine {
     uint32_t seed_len[2] = {0, 0};
     // work buffers, grow-only (gpu/mod.rs:1539-1565)
     DevBuf hit_num, tiles, total, anchors, flags, surv, hsp, done, ids2, reduced, valid;
-    uint32_t *h_small = nullptr;  // pinned scratch for counts
+    uint32_t *h_small = nullptr;  // pinned scratch for counts (H_SMALL_BYTES)
     // parameters
     uint32_t seed_size = 19;
     int32_t xdrop = 910, hspthresh = 3000, noentropy = 0;
@@ -109,6 +113,10 @@ struct hspz_engine {
     hspz_batch_state *batch = nullptr;
     bool batch_graph = true;  // capture the batched device pass in a hipGraph and replay it
     bool device_dedup = true; // dedup_and_order of the batched path on the device
+    // hspz_seed_and_filter_noalloc: this device's address of the scratch array
+    // (looked up once per device at engine creation) and the test cap on its tile
+    uint8_t *scratch = nullptr;
+    uint32_t noalloc_tile_hits = 0;
 };
 
 namespace {
@@ -369,6 +377,7 @@ int encode_on_device(hspz_engine *h, const uint8_t *raw, size_t n, const std::ve
 
 }  // namespace
 
+#include "noalloc.hpp"
 #include "batch_host.hpp"
 #include "server.hpp"
 #include "tune_host.hpp"
@@ -501,11 +510,12 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
     if (hipEventCreate(&h->ev_gate0) != hipSuccess || hipEventCreate(&h->ev_gate1) != hipSuccess ||
         hipEventCreate(&h->ev_post1) != hipSuccess)
         return fail(set_error(-1, "events"));
-    if (hipHostMalloc((void **)&h->h_small, 64, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&h->h_small, H_SMALL_BYTES, hipHostMallocDefault) != hipSuccess)
         return fail(set_error(HSPZ_E_OOM, "pinned scratch"));
     if (hipDeviceGetAttribute(&h->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         h->num_cus < 1)
         return fail(set_error(-1, "device attributes"));
+    if (int rc = scratch_ptr(device, &h->scratch)) return fail(rc);
     {
         uint8_t tabs[512];
         const uint8_t *enc = encoding_table(), *comp = complement_table();
@@ -839,6 +849,10 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
             return set_error(HSPZ_E_ARG, "list capacity must be 0 or in [64, 2^28]");
         h->list_cap = (uint32_t)value;
         break;
+    case HSPZ_FLAG_NOALLOC_TILE_HITS:
+        if (value != 0 && value < 64) return set_error(HSPZ_E_ARG, "noalloc tile hits must be 0 or at least 64");
+        h->noalloc_tile_hits = (uint32_t)value;
+        break;
     case HSPZ_FLAG_REF_RECORDS:
         if (value < -1 || value > 1) return set_error(HSPZ_E_ARG, "ref_records must be -1, 0 or 1");
         h->rrec_mode = value;
@@ -869,6 +883,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_REF_RECORDS: *value = h->rrec.p ? 2 : h->rrec_mode; break;
     case HSPZ_FLAG_BATCH_LOOKAHEAD: *value = h->lookahead ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_LIST_CAP: *value = (int)h->list_cap; break;
+    case HSPZ_FLAG_NOALLOC_TILE_HITS: *value = (int)h->noalloc_tile_hits; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;

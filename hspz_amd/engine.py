@@ -28,7 +28,7 @@ STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 
         "batch_reruns": 19}
 FLAG = {"batch_graph": 0, "batch_gate_blocks": 1, "batch_step_cells": 2, "batch_block_threads": 3,
         "batch_claim_seeds": 4, "batch_order": 5, "batch_order_used": 6, "batch_device_dedup": 7,
-        "ref_records": 8, "batch_lookahead": 9, "batch_list_cap": 10}
+        "ref_records": 8, "batch_lookahead": 9, "batch_list_cap": 10, "noalloc_tile_hits": 11}
 
 # Every symbol include/hspz.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -40,7 +40,7 @@ EXPORTS = [
     "hspz_batch_output_free", "hspz_server_new", "hspz_server_submit", "hspz_server_wait",
     "hspz_request_result_free", "hspz_server_get_stats", "hspz_server_free", "hspz_set_flag",
     "hspz_get_flag", "hspz_autotune_batch", "hspz_engine_new_device_index", "hspz_engine_index",
-    "hspz_swap_query_raw", "hspz_engine_query",
+    "hspz_swap_query_raw", "hspz_engine_query", "hspz_seed_and_filter_noalloc",
 ]
 
 
@@ -143,6 +143,8 @@ def lib():
         L.hspz_generate_seeds.argtypes = [vp, C.c_int, C.c_int, u32, u32, vp, u32, C.c_int,
                                           C.POINTER(u32)]
         L.hspz_seed_and_filter.argtypes = [vp, C.c_int, C.c_int, C.POINTER(FilterOutput)]
+        L.hspz_seed_and_filter_noalloc.argtypes = [vp, C.c_int, C.c_int, vp, sz, C.POINTER(sz),
+                                                   C.POINTER(u32), C.POINTER(u32)]
         L.hspz_filter_output_free.argtypes = [C.POINTER(FilterOutput)]
         L.hspz_filter_output_free.restype = None
         L.hspz_get_stat.argtypes = [vp, C.c_int, C.POINTER(C.c_uint64)]
@@ -353,6 +355,15 @@ class Engine:
             C.memmove(_ptr(res), out.hsps, out.n_hsps * 16)
         lib().hspz_filter_output_free(C.byref(out))
         return res, int(out.num_hits), int(out.raw_hsps)
+
+    def seed_and_filter_noalloc(self, slot: int, rev: bool, cap: int):
+        """hspz_seed_and_filter_noalloc into a caller array of cap records.
+        Returns (hsps structured array, num_hits, raw_hsps)."""
+        out = np.zeros(max(cap, 1), dtype=SP_DTYPE)
+        n, nh, raw = C.c_size_t(), C.c_uint32(), C.c_uint32()
+        _check(lib().hspz_seed_and_filter_noalloc(self._h, slot, int(rev), _ptr(out), cap, C.byref(n),
+                                                  C.byref(nh), C.byref(raw)))
+        return out[:n.value].copy(), int(nh.value), int(raw.value)
 
     def set_flag(self, name: str, value: int):
         _check(lib().hspz_set_flag(self._h, FLAG[name], int(value)))

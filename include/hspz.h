@@ -122,6 +122,26 @@ int hspz_generate_seeds(hspz_engine *h, int slot, int rev, uint32_t start, uint3
 int hspz_seed_and_filter(hspz_engine *h, int slot, int rev, hspz_filter_output *out);
 void hspz_filter_output_free(hspz_filter_output *out);
 
+/* hspz_seed_and_filter without any allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same lists as hspz_seed_and_filter on the same slot and strand, written to
+ * the caller's out[0..cap); *n_out receives the kept count, *num_hits and
+ * *raw_hsps (either may be NULL) what FilterOutput carries. All device work
+ * runs in a file-scope device array of 256 MiB per device (held from module
+ * load to process exit), never in memory the call allocates: it calls no
+ * hipMalloc, hipFree or allocating library routine. Limits, refused with
+ * HSPZ_E_OOM and a message: the seeds of the slot must leave at least 64 MiB of
+ * the array (up to about 47 M seeds), a max_hits chunk may have 1.3 M gate
+ * survivors (fewer, down to 0.3 M, as the slot holds more seeds), a call up to
+ * 256 max_hits chunks, and out must hold each
+ * chunk's pre-dedup list after the lists kept before it. Any number of hits:
+ * they are expanded and gated in tiles. Every call on a device shares that
+ * device's array, so calls that overlap on one device race: serialise them
+ * (one thread, or one stream's worth of calls at a time). */
+int hspz_seed_and_filter_noalloc(hspz_engine *h, int slot, int rev, hspz_segment_pair *out, size_t cap,
+                                 size_t *n_out, uint32_t *num_hits, uint32_t *raw_hsps);
+
 /* Accessors (gpu/mod.rs:1201-1213, 2292-2306). */
 #define HSPZ_STAT_MAX_HITS 0
 #define HSPZ_STAT_LAUNCHES 1
@@ -185,6 +205,9 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           this size. A list that overflows is doubled and the pass run again
                                           (HSPZ_STAT_BATCH_RERUNS), so the lists only change how often that
                                           happens, never the results */
+#define HSPZ_FLAG_NOALLOC_TILE_HITS 11 /* hits hspz_seed_and_filter_noalloc expands and gates at a time: 0
+                                          (default) = as many as its scratch array holds; 64 or more caps it
+                                          (smaller tiles only add launches; results do not change) */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */
