@@ -17,9 +17,13 @@
 
 namespace {
 
+// One seed-and-filter batch of a call (run.rs:278-299): query positions
+// [lo, hi) of chunk `chunk`'s forward (rev = 0) or reverse-complement block,
+// in interval `interval` of that chunk.
 struct HostBatch {
     uint32_t chunk, interval;
     int rev;
+    uint32_t lo, hi;
 };
 
 // Device buffers of the batched path, grow-only, owned by the engine.
@@ -64,7 +68,7 @@ typedef void (*BGateFn)(const uint32_t *, const uint32_t *, uint32_t, const uint
                         uint32_t *, uint32_t, SurvRec *, uint32_t *, uint32_t, uint32_t *);
 
 typedef void (*KGateFn)(const uint32_t *, const uint32_t *, int32_t, int32_t, const QRec *, const uint32_t *,
-                        const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, const uint32_t *,
+                        const uint32_t *, const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, const uint32_t *,
                         const uint4 *, uint32_t, uint32_t, unsigned long long *, SurvRec *, uint32_t *, uint32_t,
                         SurvRec *, uint32_t *, uint32_t, uint32_t *);
 
@@ -211,10 +215,14 @@ int device_dedup(hspz_engine *h, BatchBufs &B, uint32_t n, uint32_t *kept) {
     return 0;
 }
 
+// One device pass over the batches hb (chunk indices into chunks[0..n_chunks),
+// in output order): kept HSPs with their scope keys (local batch index *
+// MAX_CHUNKS + max_hits chunk), grouped by ascending scope and deduplicated
+// when *deduped comes back true.
 int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
-               const hspz_query_chunk *chunks, size_t n_chunks, std::vector<HostBatch> &hb,
+               const hspz_query_chunk *chunks, size_t n_chunks, const std::vector<HostBatch> &hb,
                std::vector<hspz_segment_pair> &hsps, std::vector<uint32_t> &scopes,
-               hspz_pass_stats *st, bool *deduped) {
+               hspz_pass_stats *st, bool *deduped, bool use_graph) {
     *deduped = false;
     BatchBufs &B = h->batch->b;
     const auto t_begin = std::chrono::steady_clock::now();
@@ -247,39 +255,23 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             rawcat.insert(rawcat.end(), chunks[c].fwd, chunks[c].fwd + chunks[c].len);
         }
     }
-    // ---- batches in the reference's order (run.rs:278-299), per chunk
+    // ---- the pass's batches, in the order given
     std::vector<BatchDesc> desc;
     std::vector<uint32_t> poff;
-    hb.clear();
+    desc.reserve(hb.size());
+    poff.reserve(hb.size() + 1);
     uint64_t npos = 0;
-    for (size_t c = 0; c < n_chunks; c++) {
-        uint32_t bl = (uint32_t)chunks[c].len;
-        if (bl <= s.size) continue;
-        uint32_t end_pos = bl - s.size, qbl = bl - s.size;
-        uint32_t ni = 0;
-        for (uint64_t x = 0; x < end_pos; x += p->interval_size) ni++;
-        for (uint32_t i = 0; i < ni; i++) {
-            uint32_t a = (uint32_t)((uint64_t)i * p->interval_size);
-            uint32_t e = (uint32_t)std::min<uint64_t>(end_pos, (uint64_t)a + p->interval_size);
-            for (int rev = 0; rev < 2; rev++) {
-                if ((rev == 0 && !p->plus) || (rev == 1 && !p->minus)) continue;
-                uint32_t lo = rev ? qbl - e : a, hi = rev ? qbl - a : e;
-                for (uint64_t x = lo; x < hi; x += p->wga_chunk) {
-                    uint32_t y = (uint32_t)std::min<uint64_t>(x + p->wga_chunk, (uint64_t)hi + 1);
-                    desc.push_back({(uint32_t)(2 * c + rev), (uint32_t)x, y});
-                    hb.push_back({(uint32_t)c, i, rev});
-                    poff.push_back((uint32_t)npos);
-                    npos += y - x;
-                }
-            }
-        }
+    for (const HostBatch &b : hb) {
+        desc.push_back({2 * b.chunk + (uint32_t)b.rev, b.lo, b.hi});
+        poff.push_back((uint32_t)std::min<uint64_t>(npos, 0xFFFFFFFFull));
+        npos += b.hi - b.lo;
     }
     // scope keys are b * MAX_CHUNKS + chunk in 32 bits
     if (desc.size() >= 0xFFFFFFFFull / MAX_CHUNKS)
         return set_error(HSPZ_E_ARG, "batch too large: %zu seed-and-filter batches", desc.size());
     uint32_t nb = (uint32_t)desc.size();
     poff.push_back((uint32_t)npos);
-    if (npos >= 0xFFFFFFFFull / 16) return set_error(HSPZ_E_ARG, "batch too large: %llu positions", (unsigned long long)npos);
+    if (npos >= GV_MAX_POSITIONS) return set_error(HSPZ_E_ARG, "batch pass too large: %llu positions", (unsigned long long)npos);
     if (nb == 0) return 0;
     const uint32_t per = p->transitions ? 1 + s.kmer_size : 1;
     const uint64_t cap_seeds = npos * per;
@@ -360,19 +352,19 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             if ((rc = reserve(h, B.gcnt, ((size_t)nkm + 1) * 4))) return rc;
             if ((rc = reserve(h, B.kgaps, (size_t)kgap_capacity(nkm) * sizeof(uint4)))) return rc;
             if ((rc = reserve(h, B.kkey, cap_seeds * 4 + 4))) return rc;   // sort keys (u32)
-            if ((rc = reserve(h, B.sq, cap_seeds * 8 + 8))) return rc;     // sort values (u64), in
-            if ((rc = reserve(h, B.kvalo, cap_seeds * 8 + 8))) return rc;  // and out
+            if ((rc = reserve(h, B.sq, cap_seeds * 4 + 4))) return rc;     // sort values (u32), in
+            if ((rc = reserve(h, B.kvalo, cap_seeds * 4 + 4))) return rc;  // and out
             if ((rc = reserve(h, B.skm, cap_seeds * 4 + 4))) return rc;
-            if ((rc = reserve(h, B.qrec, cap_seeds * sizeof(QRec) + sizeof(QRec)))) return rc;
+            if ((rc = reserve(h, B.qrec, npos * sizeof(QRec) + sizeof(QRec)))) return rc;  // one per position
             if ((rc = ensure_ref_records(h, h->rrec_mode > 0 || (h->rrec_mode == 0 && h->bgate_step >= 96))))
                 return rc;
         }
         uint32_t *kkey = kmer_order ? B.kkey.as<uint32_t>() : nullptr;
-        uint64_t *kval = kmer_order ? B.sq.as<uint64_t>() : nullptr;
+        uint32_t *kval = kmer_order ? B.sq.as<uint32_t>() : nullptr;
         const int key_bits = 32 - __builtin_clz(nkm - 1);  // keys are in [0, nk - 1]
         if (kmer_order) {  // sort temp storage before any capture
             size_t bytes = 0;
-            HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint64_t>(),
+            HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint32_t>(),
                                               (uint32_t)cap_seeds, 0, key_bits, h->stream));
             if ((rc = reserve(h, B.ktmp, bytes + 256))) return rc;
         }
@@ -427,11 +419,11 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         auto enqueue_sort = [&]() -> int {
             if (!kmer_order) return 0;
             size_t bytes = 0;
-            HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint64_t>(),
+            HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kkey, B.skm.as<uint32_t>(), kval, B.kvalo.as<uint32_t>(),
                                               (uint32_t)cap_seeds, 0, key_bits, h->stream));
             if (int e = reserve(h, B.ktmp, bytes + 256)) return e;
             HIP_TRY(rocprim::radix_sort_pairs(B.ktmp.p, bytes, kkey, B.skm.as<uint32_t>(), kval,
-                                              B.kvalo.as<uint64_t>(), (uint32_t)cap_seeds, 0, key_bits, h->stream));
+                                              B.kvalo.as<uint32_t>(), (uint32_t)cap_seeds, 0, key_bits, h->stream));
             return 0;
         };
         auto enqueue_group = [&]() {
@@ -441,8 +433,8 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                B.gcnt.as<uint32_t>(), B.kgaps.as<uint4>(), ctr + CTR_KGAPS, kgap_capacity(nkm));
             hipLaunchKernelGGL(k_kgaps, dim3(1024), dim3(256), 0, h->stream, B.kgaps.as<uint4>(), ctr + CTR_KGAPS,
                                kgap_capacity(nkm), B.gcnt.as<uint32_t>());
-            hipLaunchKernelGGL(k_qrecords, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
-                               h->stream, B.kvalo.as<uint64_t>(), B.gcnt.as<uint32_t>() + nkm,
+            hipLaunchKernelGGL(k_qrecords, dim3(std::min<uint32_t>(ceil_div(npos, 256), 16384)), dim3(256), 0,
+                               h->stream, B.seeds.as<uint64_t>(), ctr + CTR_NVALID, per,
                                B.plane.as<uint32_t>(), B.coarse.as<uint32_t>(), arena_len, h->seed_size,
                                B.qrec.as<QRec>());
         };
@@ -466,7 +458,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         if (kmer_order)
             hipLaunchKernelGGL(kgate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
                                h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
-                               B.skm.as<uint32_t>(), B.gcnt.as<uint32_t>(), nkm,
+                               B.skm.as<uint32_t>(), B.kvalo.as<uint32_t>(), B.gcnt.as<uint32_t>(), nkm,
                                h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->clean.as<uint32_t>(),
                                h->rrec.p ? h->rrec.as<uint4>() : nullptr, h->seed_size, (uint32_t)h->bgate_claim,
                                (unsigned long long *)(ctr + CTR_WORK),
@@ -511,7 +503,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                            B.oscope.as<uint32_t>());
         };
         const auto t_launch = std::chrono::steady_clock::now();
-        if (h->batch_graph) {
+        if (h->batch_graph && use_graph) {
             GraphKey key;
             memset(&key, 0, sizeof key);
             const void *ptrs[] = {B.arena.p, B.plane.p, B.coarse.p, B.desc.p, B.poff.p, B.blk_off.p, B.blk_len.p,
@@ -651,6 +643,34 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
     }
 }
 
+// Device bytes one query position of a batch pass needs (k-mer-major order,
+// up to per = 16 seed slots): valid flag and rank 5; per slot the seed 8, its
+// count 4 and prefix 8, the grouping keys and values in and out 16, the sort's
+// temporary storage ~8, the survivor and deferred-hit lists' share ~12; the
+// query record 64; the arena and its planes ~3.
+constexpr uint64_t PASS_BYTES_PER_POSITION = 5 + 16 * 56 + 64 + 3;
+
+// Query positions per device pass: the engine's HSPZ_FLAG_BATCH_MAX_POSITIONS,
+// or (0) as many as half of the device memory that is free or already held by
+// this engine's pass buffers allows, within [2^20, 2^26].
+uint64_t pass_positions(hspz_engine *h) {
+    if (h->batch_max_positions) return h->batch_max_positions;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1ull << 24;
+    }
+    uint64_t held = 0;
+    const BatchBufs &B = h->batch->b;
+    for (const DevBuf *b : {&B.arena, &B.plane, &B.coarse, &B.valid, &B.ids, &B.seeds, &B.counts, &B.prefix,
+                            &B.surv, &B.slow, &B.exact, &B.mhsp, &B.ohsp, &B.sq, &B.skm, &B.qrec, &B.kvalo,
+                            &B.ktmp, &B.kkey})
+        held += b->cap;
+    const uint64_t avail = (uint64_t)f + held;
+    const uint64_t want = avail > (4ull << 30) ? (avail - (4ull << 30)) / 2 / PASS_BYTES_PER_POSITION : 0;
+    return std::min<uint64_t>(std::max<uint64_t>(want, 1ull << 20), 1ull << 26);
+}
+
 }  // namespace
 
 extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, const hspz_query_chunk *chunks,
@@ -663,69 +683,116 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
     if (int rc = parse_shape(p->pattern ? p->pattern : "12of19", &s)) return rc;
     if (s.size != h->seed_size) return set_error(HSPZ_E_SHAPE, "seed shape span %u != engine seed size %u", s.size, h->seed_size);
     if (p->wga_chunk == 0 || p->interval_size == 0) return set_error(HSPZ_E_ARG, "zero chunk or interval size");
-    for (size_t c = 0; c < n_chunks; c++)
+    for (size_t c = 0; c < n_chunks; c++) {
         if (chunks[c].len && !chunks[c].fwd) return set_error(HSPZ_E_ARG, "chunk %zu has null data", c);
+        if (chunks[c].len >= 0x7FFFFFFFull - 2 * QPAD) return set_error(HSPZ_E_ARG, "chunk %zu is longer than 2^31 bases", c);
+    }
     HIP_TRY(hipSetDevice(h->device));
     if (!h->batch) h->batch = new hspz_batch_state();
-    hspz_pass_stats st = {0, 0, 0, 0, 0};
-    std::vector<HostBatch> hb;
-    std::vector<hspz_segment_pair> raw;
-    std::vector<uint32_t> scopes;
-    bool deduped = false;
-    if (int rc = batch_pass(h, s, p, chunks, n_chunks, hb, raw, scopes, &st, &deduped)) return rc;
-    const auto t_post = std::chrono::steady_clock::now();
 
-    // group by scope (batch * MAX_CHUNKS + chunk): ascending scope = output order
-    std::vector<uint32_t> order(raw.size());
-    std::vector<hspz_segment_pair> grouped;
-    if (deduped) {  // already grouped by ascending scope and ordered within each
-        for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
-        grouped.swap(raw);
-    } else {  // counting sort by scope, stable
-        uint32_t smax = 0;
-        for (uint32_t sc : scopes) smax = std::max(smax, sc);
-        std::vector<uint32_t> cnt(raw.empty() ? 1 : (size_t)smax + 2, 0);
-        for (uint32_t sc : scopes) cnt[sc + 1]++;
-        for (size_t i = 1; i < cnt.size(); i++) cnt[i] += cnt[i - 1];
-        for (uint32_t i = 0; i < order.size(); i++) order[cnt[scopes[i]]++] = i;
-        grouped.resize(raw.size());
-        for (size_t i = 0; i < order.size(); i++) grouped[i] = raw[order[i]];
-    }
-    std::vector<size_t> gstart;
-    for (size_t i = 0; i < order.size(); i++)
-        if (i == 0 || scopes[order[i]] != scopes[order[i - 1]]) gstart.push_back(i);
-    gstart.push_back(order.size());
-    size_t ng = gstart.size() - 1;
-    std::vector<size_t> kept(ng);
-    if (deduped) {
-        for (size_t g = 0; g < ng; g++) kept[g] = gstart[g + 1] - gstart[g];
-    } else {
-        unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        std::vector<std::thread> th;
-        for (unsigned t = 0; t < nth; t++)
-            th.emplace_back([&, t] {
-                for (size_t g = t; g < ng; g += nth)
-                    kept[g] = dedup_and_order(grouped.data() + gstart[g], gstart[g + 1] - gstart[g]);
-            });
-        for (auto &x : th) x.join();
-    }
-    // assemble per chunk -> interval -> fw, rc (scopes ascending = batch order)
+    // ---- every batch of every chunk in the reference's order (run.rs:278-299)
+    std::vector<HostBatch> all;
     std::vector<size_t> civ(n_chunks, 0);
     for (size_t c = 0; c < n_chunks; c++) {
         uint32_t bl = (uint32_t)chunks[c].len;
-        if (bl > s.size)
-            for (uint64_t x = 0; x < bl - s.size; x += p->interval_size) civ[c]++;
+        if (bl <= s.size) continue;
+        uint32_t end_pos = bl - s.size, qbl = bl - s.size;
+        for (uint64_t x = 0; x < end_pos; x += p->interval_size) civ[c]++;
+        for (uint32_t i = 0; i < civ[c]; i++) {
+            uint32_t a = (uint32_t)((uint64_t)i * p->interval_size);
+            uint32_t e = (uint32_t)std::min<uint64_t>(end_pos, (uint64_t)a + p->interval_size);
+            for (int rev = 0; rev < 2; rev++) {
+                if ((rev == 0 && !p->plus) || (rev == 1 && !p->minus)) continue;
+                uint32_t lo = rev ? qbl - e : a, hi = rev ? qbl - a : e;
+                for (uint64_t x = lo; x < hi; x += p->wga_chunk)
+                    all.push_back({(uint32_t)c, i, rev, (uint32_t)x,
+                                   (uint32_t)std::min<uint64_t>(x + p->wga_chunk, (uint64_t)hi + 1)});
+            }
+        }
     }
     std::vector<size_t> ibase(n_chunks + 1, 0);
     for (size_t c = 0; c < n_chunks; c++) ibase[c + 1] = ibase[c] + civ[c];
-    size_t nlists = 2 * ibase[n_chunks];
+    const size_t nlists = 2 * ibase[n_chunks];
     std::vector<std::vector<hspz_segment_pair>> lists(nlists);
-    for (size_t g = 0; g < ng; g++) {
-        uint32_t b = scopes[order[gstart[g]]] / MAX_CHUNKS;
-        const HostBatch &x = hb[b];
-        auto &dst = lists[2 * (ibase[x.chunk] + x.interval) + x.rev];
-        dst.insert(dst.end(), grouped.begin() + gstart[g], grouped.begin() + gstart[g] + kept[g]);
+
+    // ---- device passes over consecutive runs of batches: each holds at most
+    // pass_positions() query positions (a batch larger than that runs alone)
+    // and 2^20 batches. Every batch is its own dedup scope, so splitting the
+    // call between batches changes no list.
+    const uint64_t pmax = pass_positions(h);
+    std::vector<std::pair<size_t, size_t>> passes;
+    for (size_t b0 = 0; b0 < all.size();) {
+        size_t b1 = b0;
+        uint64_t np = 0;
+        while (b1 < all.size() && b1 - b0 < (1u << 20)) {
+            uint64_t w = all[b1].hi - all[b1].lo;
+            if (b1 > b0 && np + w > pmax) break;
+            np += w;
+            b1++;
+        }
+        passes.push_back({b0, b1});
+        b0 = b1;
     }
+    h->batch_passes_last = passes.size();
+    hspz_pass_stats st = {0, 0, 0, 0, 0};
+    double post_ns = 0;
+    for (const auto &ps : passes) {
+        const size_t cf = all[ps.first].chunk, cl = all[ps.second - 1].chunk;
+        std::vector<HostBatch> hb(all.begin() + ps.first, all.begin() + ps.second);
+        for (HostBatch &b : hb) b.chunk -= (uint32_t)cf;
+        std::vector<hspz_segment_pair> raw;
+        std::vector<uint32_t> scopes;
+        bool deduped = false;
+        // one pass per call replays a captured graph; several run eagerly (their shapes differ)
+        if (int rc = batch_pass(h, s, p, chunks + cf, cl - cf + 1, hb, raw, scopes, &st, &deduped,
+                                passes.size() == 1))
+            return rc;
+        const auto t_post = std::chrono::steady_clock::now();
+        // group by scope (local batch * MAX_CHUNKS + chunk): ascending scope = output order
+        std::vector<uint32_t> order(raw.size());
+        std::vector<hspz_segment_pair> grouped;
+        if (deduped) {  // already grouped by ascending scope and ordered within each
+            for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+            grouped.swap(raw);
+        } else {  // counting sort by scope, stable
+            uint32_t smax = 0;
+            for (uint32_t sc : scopes) smax = std::max(smax, sc);
+            std::vector<uint32_t> cnt(raw.empty() ? 1 : (size_t)smax + 2, 0);
+            for (uint32_t sc : scopes) cnt[sc + 1]++;
+            for (size_t i = 1; i < cnt.size(); i++) cnt[i] += cnt[i - 1];
+            for (uint32_t i = 0; i < order.size(); i++) order[cnt[scopes[i]]++] = i;
+            grouped.resize(raw.size());
+            for (size_t i = 0; i < order.size(); i++) grouped[i] = raw[order[i]];
+        }
+        std::vector<size_t> gstart;
+        for (size_t i = 0; i < order.size(); i++)
+            if (i == 0 || scopes[order[i]] != scopes[order[i - 1]]) gstart.push_back(i);
+        gstart.push_back(order.size());
+        size_t ng = gstart.size() - 1;
+        std::vector<size_t> kept(ng);
+        if (deduped) {
+            for (size_t g = 0; g < ng; g++) kept[g] = gstart[g + 1] - gstart[g];
+        } else {
+            unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < nth; t++)
+                th.emplace_back([&, t] {
+                    for (size_t g = t; g < ng; g += nth)
+                        kept[g] = dedup_and_order(grouped.data() + gstart[g], gstart[g + 1] - gstart[g]);
+                });
+            for (auto &x : th) x.join();
+        }
+        // append per chunk -> interval -> fw, rc (scopes ascending = batch order)
+        for (size_t g = 0; g < ng; g++) {
+            const HostBatch &x = all[ps.first + scopes[order[gstart[g]]] / MAX_CHUNKS];
+            auto &dst = lists[2 * (ibase[x.chunk] + x.interval) + x.rev];
+            dst.insert(dst.end(), grouped.begin() + gstart[g], grouped.begin() + gstart[g] + kept[g]);
+        }
+        post_ns += (double)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                       std::chrono::steady_clock::now() - t_post)
+                       .count();
+    }
+    const auto t_out = std::chrono::steady_clock::now();
     size_t total = 0;
     for (auto &l : lists) total += l.size();
     st.hsps = total;
@@ -745,9 +812,9 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
     for (size_t c = 0; c < n_chunks; c++) out->chunk_intervals[c] = civ[c];
     out->n_hsps = total;
     out->n_chunks = n_chunks;
-    h->batch_post_ns += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
-                            std::chrono::steady_clock::now() - t_post)
-                            .count();
+    h->batch_post_ns += (uint64_t)post_ns + (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                                std::chrono::steady_clock::now() - t_out)
+                                                .count();
     if (stats) *stats = st;
     return 0;
 }

@@ -94,6 +94,8 @@ struct hspz_engine {
     uint64_t batch_prep_ns = 0, batch_device_ns = 0, batch_read_ns = 0, batch_post_ns = 0;
     uint64_t batch_reruns = 0;  // batched passes rerun after a list overflow
     uint32_t list_cap = 0;      // HSPZ_FLAG_BATCH_LIST_CAP; 0 = sized from the batch
+    uint32_t batch_max_positions = 0;  // HSPZ_FLAG_BATCH_MAX_POSITIONS; 0 = from free device memory
+    uint64_t batch_passes_last = 0;    // device passes of the last hspz_filter_batch call
     int gate_blocks = 2048;
     // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
     int bgate_blocks = 0;  // persistent grid; 0 = as many blocks as are resident at once
@@ -849,6 +851,10 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
             return set_error(HSPZ_E_ARG, "list capacity must be 0 or in [64, 2^28]");
         h->list_cap = (uint32_t)value;
         break;
+    case HSPZ_FLAG_BATCH_MAX_POSITIONS:
+        if (value < 0) return set_error(HSPZ_E_ARG, "max positions must be 0 (automatic) or positive");
+        h->batch_max_positions = (uint32_t)value;
+        break;
     case HSPZ_FLAG_NOALLOC_TILE_HITS:
         if (value != 0 && value < 64) return set_error(HSPZ_E_ARG, "noalloc tile hits must be 0 or at least 64");
         h->noalloc_tile_hits = (uint32_t)value;
@@ -884,6 +890,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_LOOKAHEAD: *value = h->lookahead ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_LIST_CAP: *value = (int)h->list_cap; break;
     case HSPZ_FLAG_NOALLOC_TILE_HITS: *value = (int)h->noalloc_tile_hits; break;
+    case HSPZ_FLAG_BATCH_MAX_POSITIONS: *value = (int)h->batch_max_positions; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;
@@ -913,6 +920,7 @@ extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
     case HSPZ_STAT_POS_LEN: *value = h->pos_len; break;
     case HSPZ_STAT_INDEX_LEN: *value = h->index_len; break;
     case HSPZ_STAT_BATCH_RERUNS: *value = h->batch_reruns; break;
+    case HSPZ_STAT_BATCH_PASSES: *value = h->batch_passes_last; break;
     default: return set_error(HSPZ_E_ARG, "unknown stat %d", which);
     }
     return 0;

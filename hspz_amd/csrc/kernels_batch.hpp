@@ -460,30 +460,37 @@ __global__ void k_ref_records(const uint32_t *__restrict__ pos, uint32_t n, cons
     }
 }
 
-// Query record of one grouped seed: its right and left query streams as the
-// k-mer-major gate consumes them (aligned to Q, left words byte- and
-// nibble-swapped),
-// the anchor Q, the seed index and whether the query window is clean.
+// Query record of one seeded query position (shared by the exact k-mer and
+// every transition variant of that position): its right and left query
+// streams as the k-mer-major gate consumes them (aligned to Q, left words
+// byte- and nibble-swapped), the anchor Q, the seed slot of the position's
+// exact k-mer (slot of variant t = tag0 + t) and whether the query window is
+// clean.
 struct QRec {
     uint32_t right[G2_RIGHT_WORDS];
     uint32_t left[G2_LEFT_WORDS];
-    uint32_t Q, tag, ok, pad;
+    uint32_t Q, tag0, ok, pad;
 };
 static_assert(sizeof(QRec) == 64, "query records are four 16-byte loads");
+
+// A grouped seed's sort value: its position rank r (the query record index)
+// and its variant t (0 = exact k-mer), as (r << GV_SHIFT) | t.
+constexpr uint32_t GV_SHIFT = 4;  // per = 1 + kmer_size <= 16 variants
+constexpr uint32_t GV_MAX_POSITIONS = 1u << (32 - GV_SHIFT);
 
 // Sort keys of the k-mer grouping: the n valid seed slots get their k-mer
 // (a seed without hits lands in a group whose bucket is empty, which the gate
 // skips), the unused slots [n, cap) get nk - 1: a stable sort puts them after
 // every valid slot, so keys stay below nk (2k bits) and the grouped seeds are
-// exactly the sorted slots [0, n). The value is (slot index << 32) | arena
-// position.
+// exactly the sorted slots [0, n). Slot i = r * per + t (position rank r,
+// variant t) gets the value (r << GV_SHIFT) | t.
 __global__ void k_kkeys(const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ nvalid, uint32_t per,
-                        uint32_t cap, uint32_t nk, uint32_t *__restrict__ key, uint64_t *__restrict__ val) {
+                        uint32_t cap, uint32_t nk, uint32_t *__restrict__ key, uint32_t *__restrict__ val) {
     const uint32_t n = min(*nvalid * per, cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
-        const uint64_t sd = i < n ? seeds[i] : 0;
-        key[i] = i < n ? (uint32_t)(sd >> 32) : nk - 1;
-        val[i] = ((uint64_t)i << 32) | (uint32_t)sd;
+        key[i] = i < n ? (uint32_t)(seeds[i] >> 32) : nk - 1;
+        const uint32_t r = i / per;
+        val[i] = (r << GV_SHIFT) | (i - r * per);
     }
 }
 
@@ -539,28 +546,28 @@ __global__ void k_kgaps(const uint4 *__restrict__ gaps, const uint32_t *__restri
     }
 }
 
-// One record per grouped seed, so that a hit's query side costs four loads
-// instead of a window load, alignment and exception test. sorted[p] is the
-// (seed index << 32) | arena position of grouped slot p. A block builds 256
-// consecutive records in LDS (80-byte rows, so neither the row writes nor the
-// column reads meet more than two-way bank conflicts) and stores them as
-// one contiguous 16 KiB run.
-__global__ __launch_bounds__(256) void k_qrecords(const uint64_t *__restrict__ sorted,
-                                                  const uint32_t *__restrict__ total_p,
+// One record per seeded query position (rank r of the valid positions), so
+// that a hit's query side costs four loads instead of a window load, alignment
+// and exception test; the position's k-mer variants share it. The arena
+// position of rank r is the low word of its exact seed, seeds[r * per]. A
+// block builds 256 consecutive records in LDS (80-byte rows, so neither the
+// row writes nor the column reads meet more than two-way bank conflicts) and
+// stores them as one contiguous 16 KiB run.
+__global__ __launch_bounds__(256) void k_qrecords(const uint64_t *__restrict__ seeds,
+                                                  const uint32_t *__restrict__ nvalid, uint32_t per,
                                                   const uint32_t *__restrict__ qplane,
                                                   const uint32_t *__restrict__ qcoarse, uint32_t qlen,
                                                   uint32_t seed_size, QRec *__restrict__ recs) {
     constexpr uint32_t ROW = 5;  // uint4 slots per staged record
     __shared__ uint4 s_rec[256 * ROW];
-    const uint32_t total = *total_p;
+    const uint32_t total = *nvalid;
     for (uint32_t base = blockIdx.x * 256; base < total; base += gridDim.x * 256) {
         const uint32_t p = base + threadIdx.x;
         if (p < total) {
-            const uint64_t f = sorted[p];
-            uint32_t Q = (uint32_t)f + seed_size;
+            uint32_t Q = (uint32_t)seeds[(uint64_t)p * per] + seed_size;
             QRec r;
             r.Q = Q;
-            r.tag = (uint32_t)(f >> 32);
+            r.tag0 = p * per;
             r.pad = 0;
             bool ok = Q >= G2_MARGIN && (uint64_t)Q + G2_MARGIN <= qlen;
             uint32_t W[13];
@@ -608,7 +615,7 @@ __device__ __forceinline__ uint32_t pick_word(const uint32_t (&w)[N], uint32_t g
 // entries go to idle lanes as items (one reference position each), and a
 // lane walks its item's group of query records one hit at a time.
 struct KmerSource {
-    const uint32_t *skm, *goff, *index, *pos, *clean;
+    const uint32_t *skm, *sval, *goff, *index, *pos, *clean;
     unsigned long long *work;
     uint32_t total, claim, seed_size;
     uint32_t p_cur, p_end, g0, g1, b0, n, j;
@@ -619,6 +626,13 @@ struct KmerSource {
     WaveQueue<uint4> sq, dq;  // survivors, deferred hits
 
     __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+    // The query record of grouped seed p and that seed's slot (its tag).
+    __device__ __forceinline__ const uint4 *record(const QRec *recs, uint32_t p, uint32_t &var) const {
+        const uint32_t gv = sval[p];
+        var = gv & ((1u << GV_SHIFT) - 1);
+        return (const uint4 *)(recs + (gv >> GV_SHIFT));
+    }
 
     // Claims the next range of grouped seeds; p_cur = first group start in it.
     __device__ __forceinline__ bool take() {
@@ -810,14 +824,15 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
                     __builtin_amdgcn_alignbit(LA0[k], k + 1 < (int)G2_LEFT_WORDS ? LA0[k + 1] : lend, sr)));
         }
         if (fresh_q) {
-            const uint4 *rec = (const uint4 *)(recs + s_cur);
+            uint32_t var;
+            const uint4 *rec = src.record(recs, s_cur, var);
             s_cur++;
             uint4 v0 = rec[0], v1 = rec[1], v2 = rec[2], v3 = rec[3];
             B[0] = v0.x, B[1] = v0.y, B[2] = v0.z, B[3] = v0.w;
             LB[0] = v1.x, LB[1] = v1.y, LB[2] = v1.z, LB[3] = v1.w;
             LB[4] = v2.x, LB[5] = v2.y, LB[6] = v2.z, LB[7] = v2.w;
             Q = v3.x;
-            tag = v3.y;
+            tag = v3.y + var;
             bool ok = rok && v3.z != 0;
             if (!ok) {
                 defer = true;
@@ -1140,14 +1155,15 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
                     __builtin_amdgcn_alignbit(LA0[k], k + 1 < (int)G2_LEFT_WORDS ? LA0[k + 1] : lend, sr)));
         }
         if (fresh_q) {
-            const uint4 *rec = (const uint4 *)(recs + s_cur);
+            uint32_t var;
+            const uint4 *rec = src.record(recs, s_cur, var);
             s_cur++;
             uint4 v0 = rec[0], v1 = rec[1], v2 = rec[2], v3 = rec[3];
             B[0] = v0.x, B[1] = v0.y, B[2] = v0.z, B[3] = v0.w;
             LB[0] = v1.x, LB[1] = v1.y, LB[2] = v1.z, LB[3] = v1.w;
             LB[4] = v2.x, LB[5] = v2.y, LB[6] = v2.z, LB[7] = v2.w;
             Q = v3.x;
-            tag = v3.y;
+            tag = v3.y + var;
             st.start();
             if (!(rok && v3.z != 0)) defer = true;
         }
@@ -1212,7 +1228,7 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
     uint32_t tpos = 0, tclean = 0;
     // prefetched query record of the next fresh hit
     bool pv = false;
-    uint32_t pidx = 0;
+    uint32_t pidx = 0, pvar = 0;
     uint4 P0 = T0, P1 = T0, P2 = T0, P3 = T0;
     // the hit finished in the previous iteration, appended at the top of this one
     bool e_pass = false, e_defer = false;
@@ -1264,15 +1280,16 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
             }
             if (cur) {
                 uint4 v0 = P0, v1 = P1, v2 = P2, v3 = P3;
+                uint32_t var = pvar;
                 if (!(pv && pidx == s_cur)) {  // not prefetched (kept for safety; the schedule prefetches it)
-                    const uint4 *rec = (const uint4 *)(recs + s_cur);
+                    const uint4 *rec = src.record(recs, s_cur, var);
                     v0 = rec[0], v1 = rec[1], v2 = rec[2], v3 = rec[3];
                 }
                 B[0] = v0.x, B[1] = v0.y, B[2] = v0.z, B[3] = v0.w;
                 LB[0] = v1.x, LB[1] = v1.y, LB[2] = v1.z, LB[3] = v1.w;
                 LB[4] = v2.x, LB[5] = v2.y, LB[6] = v2.z, LB[7] = v2.w;
                 Q = v3.x;
-                tag = v3.y;
+                tag = v3.y + var;
                 qok = v3.z != 0;
                 s_cur++;
                 has = true;
@@ -1305,7 +1322,7 @@ __device__ __forceinline__ void kmer_gate_ahead(KmerSource &src, const uint4 *__
             uint32_t idx = pending ? s_cur : nxt ? ns0 : c0;
             pv = pending || nxt || got;
             pidx = idx;
-            const uint4 *rec = (const uint4 *)(recs + (pv ? idx : 0u));
+            const uint4 *rec = src.record(recs, pv ? idx : 0u, pvar);
             P0 = rec[0], P1 = rec[1], P2 = rec[2], P3 = rec[3];
         }
 
@@ -1338,7 +1355,7 @@ template <int RW>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_bgate_ahead(
     const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop,
     int32_t thresh, const QRec *__restrict__ recs, const uint32_t *__restrict__ skm,
-    const uint32_t *__restrict__ goff, uint32_t nk,
+    const uint32_t *__restrict__ sval, const uint32_t *__restrict__ goff, uint32_t nk,
     const uint32_t *__restrict__ index, const uint32_t *__restrict__ pos, const uint32_t *__restrict__ clean,
     const uint4 *__restrict__ rrec, uint32_t seed_size, uint32_t claim, unsigned long long *__restrict__ work,
     SurvRec *__restrict__ surv,
@@ -1349,6 +1366,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     load_map_table(s_map, pairtab, 512);
     KmerSource src;
     src.skm = skm;
+    src.sval = sval;
     src.goff = goff;
     src.index = index;
     src.pos = pos;
@@ -1378,7 +1396,7 @@ template <int SW, int BT, bool NARROW = false>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_kmer(
     const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop,
     int32_t thresh, const QRec *__restrict__ recs, const uint32_t *__restrict__ skm,
-    const uint32_t *__restrict__ goff, uint32_t nk,
+    const uint32_t *__restrict__ sval, const uint32_t *__restrict__ goff, uint32_t nk,
     const uint32_t *__restrict__ index, const uint32_t *__restrict__ pos, const uint32_t *__restrict__ clean,
     const uint4 *__restrict__ rrec, uint32_t seed_size, uint32_t claim, unsigned long long *__restrict__ work,
     SurvRec *__restrict__ surv,
@@ -1389,6 +1407,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     load_map_table(s_map, pairtab, BT);
     KmerSource src;
     src.skm = skm;
+    src.sval = sval;
     src.goff = goff;
     src.index = index;
     src.pos = pos;

@@ -165,6 +165,8 @@ int hspz_seed_and_filter_noalloc(hspz_engine *h, int slot, int rev, hspz_segment
 #define HSPZ_STAT_INDEX_LEN 18       /* index_table entries (4^k) held by the engine */
 #define HSPZ_STAT_BATCH_RERUNS 19    /* batched passes run again because a survivor or deferred-hit
                                         list overflowed (each time both lists are doubled) */
+#define HSPZ_STAT_BATCH_PASSES 20     /* device passes the last hspz_filter_batch call ran (see
+                                        HSPZ_FLAG_BATCH_MAX_POSITIONS) */
 int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 
 /* Engine switches (the reference's pub flags, gpu/mod.rs:836-949, have no
@@ -208,6 +210,11 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 #define HSPZ_FLAG_NOALLOC_TILE_HITS 11 /* hits hspz_seed_and_filter_noalloc expands and gates at a time: 0
                                           (default) = as many as its scratch array holds; 64 or more caps it
                                           (smaller tiles only add launches; results do not change) */
+#define HSPZ_FLAG_BATCH_MAX_POSITIONS 12 /* query positions (both strands) one device pass of
+                                          hspz_filter_batch holds: a call runs its batches, in order, in
+                                          as many passes as this needs (a batch is never split; each is its
+                                          own dedup scope, so the lists do not change). 0 (default): half of
+                                          the free device memory's worth, between 2^20 and 2^26 */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

@@ -42,6 +42,21 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
             for i, (a, g) in enumerate(zip(again, got)):
                 assert same_intervals(a, g), "chunk %d differs in hit order %d" % (i, order)
         eng.set_flag("batch_order", 2)
+        # the call split into device passes between batches gives the same lists:
+        # every batch alone (a batch is never split, so 1 position per pass runs each
+        # batch in a pass of its own), then runs of batches (a third of the positions)
+        npos = sum(2 * max(len(f) - 19, 0) for f, _ in chunks)
+        for pmax in (1, max(npos // 3, 1)):
+            eng.set_flag("batch_max_positions", pmax)
+            again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
+            if pmax == 1:
+                assert eng.stat("batch_passes") == st["batches"]
+            elif st["batches"] > 3:
+                assert eng.stat("batch_passes") > 1
+            for i, (a, g) in enumerate(zip(again, got)):
+                assert same_intervals(a, g), "chunk %d differs with %d positions per device pass" % (i, pmax)
+        eng.set_flag("batch_max_positions", 0)
+        assert eng.stat("batch_passes") >= 1
         eng.set_flag("batch_device_dedup", 0)  # host dedup_and_order gives the same lists
         again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
         for i, (a, g) in enumerate(zip(again, got)):
