@@ -36,13 +36,15 @@ struct BatchBufs {
 // Largest survivor or deferred-hit list of one pass, in records (16 B each).
 constexpr uint32_t LIST_CAP_MAX = 1u << 30;
 
-// Capacity after an overflow: at least twice the larger of the old capacity
-// and the count the pass reached, computed in 64 bits and clamped to
-// LIST_CAP_MAX. False when the count itself is beyond LIST_CAP_MAX (growing
-// could never hold it).
+// Capacity after an overflow. The pass's list counter keeps counting past the
+// capacity, so `count` is what the list needed, and a rerun of the same pass
+// needs exactly that (the gate's decisions do not depend on the capacity):
+// the larger of count + 1/4 and 1.5 x the old capacity, computed in 64 bits
+// and clamped to LIST_CAP_MAX. False when the count itself is beyond
+// LIST_CAP_MAX (growing could never hold it).
 bool grow_list_cap(uint32_t cap, uint32_t count, uint32_t *out) {
     if (count > LIST_CAP_MAX) return false;
-    uint64_t want = 2 * (uint64_t)std::max(cap, count);
+    uint64_t want = std::max<uint64_t>((uint64_t)count + count / 4, (uint64_t)cap + cap / 2);
     *out = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), LIST_CAP_MAX);
     return *out >= count;
 }
@@ -128,6 +130,13 @@ struct hspz_batch_state {
 
 namespace {
 
+// Expected hits per query position and strand against the engine's index:
+// per seeds, each meeting the mean bucket (indexed positions / 4^k; repeats
+// only add to it).
+double hits_per_position(const hspz_engine *h, uint32_t per) {
+    return (double)per * (double)h->pos_len / (double)std::max<size_t>(h->index_len, 1);
+}
+
 // Builds the engine's reference records (k_ref_records) on first use, when a
 // gate wants them (HSPZ_FLAG_REF_RECORDS, or a merged-step gate) and device
 // memory leaves room for them and a margin; otherwise the k-mer-major gate
@@ -141,7 +150,7 @@ int ensure_ref_records(hspz_engine *h, bool want) {
         h->rrec_declined = true;
         return 0;
     }
-    if (int rc = reserve(h, h->rrec, need)) return rc;
+    if (int rc = reserve(h, h->rrec, need, true)) return rc;
     hipLaunchKernelGGL(k_ref_records, dim3(std::min<uint32_t>(ceil_div(h->pos_len, 256), 65536)), dim3(256), 0,
                        h->stream, h->pos.as<uint32_t>(), (uint32_t)h->pos_len, h->clean.as<uint32_t>(),
                        h->ref2.as<uint32_t>(), h->seed_size, h->rrec.as<uint4>());
@@ -318,15 +327,19 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
     sa.kmer_size = s.kmer_size;
 
     {
-        // start the survivor and deferred-hit lists at a size the batch is
-        // likely to need (~4% and ~20% of its seed capacity against a 1 Gbp
-        // target), so that large batches rarely pay an overflow rerun
+        // start the survivor and deferred-hit lists at a size the pass is
+        // likely to need (a sixteenth and a quarter of its seed slots, or
+        // 1/4096 and 1/256 of its expected hits if more), so that large passes
+        // rarely pay an overflow rerun
         hspz_batch_state *bs = h->batch;
         if (h->list_cap) {
             bs->cap_surv = bs->cap_slow = h->list_cap;
         } else {
-            bs->cap_surv = (uint32_t)std::max<uint64_t>(bs->cap_surv, std::min<uint64_t>(cap_seeds / 16, 1u << 28));
-            bs->cap_slow = (uint32_t)std::max<uint64_t>(bs->cap_slow, std::min<uint64_t>(cap_seeds / 4, 1u << 28));
+            const double hits = (double)npos * hits_per_position(h, per);
+            const uint64_t surv = std::max<uint64_t>(cap_seeds / 16, (uint64_t)(hits / 4096));
+            const uint64_t slow = std::max<uint64_t>(cap_seeds / 4, (uint64_t)(hits / 256));
+            bs->cap_surv = (uint32_t)std::max<uint64_t>(bs->cap_surv, std::min<uint64_t>(surv, 1u << 28));
+            bs->cap_slow = (uint32_t)std::max<uint64_t>(bs->cap_slow, std::min<uint64_t>(slow, 1u << 28));
         }
     }
     for (;;) {  // one attempt per survivor/slow capacity
@@ -598,6 +611,8 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             if (!grow_list_cap(bs->cap_surv, hc[CTR_SURV], &ns) || !grow_list_cap(bs->cap_slow, hc[CTR_SLOW], &nd))
                 return set_error(HSPZ_E_OOM, "batch needs more than %u survivor or deferred-hit records (%u, %u)",
                                  LIST_CAP_MAX, hc[CTR_SURV], hc[CTR_SLOW]);
+            h->last_surv = hc[CTR_SURV];
+            h->last_deferred = hc[CTR_SLOW];
             bs->cap_surv = ns;
             bs->cap_slow = nd;
             h->batch_reruns++;
@@ -650,10 +665,16 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
 // query record 64; the arena and its planes ~3.
 constexpr uint64_t PASS_BYTES_PER_POSITION = 5 + 16 * 56 + 64 + 3;
 
+// Hits one device pass is sized for: its survivor and deferred-hit lists grow
+// with the hits (about 0.1-0.5% of them are deferred), so a pass against a
+// dense index holds fewer positions.
+constexpr double PASS_HITS = 32e9;
+
 // Query positions per device pass: the engine's HSPZ_FLAG_BATCH_MAX_POSITIONS,
 // or (0) as many as half of the device memory that is free or already held by
-// this engine's pass buffers allows, within [2^20, 2^26].
-uint64_t pass_positions(hspz_engine *h) {
+// this engine's pass buffers allows, within [2^20, 2^26], and at most
+// PASS_HITS expected hits.
+uint64_t pass_positions(hspz_engine *h, uint32_t per) {
     if (h->batch_max_positions) return h->batch_max_positions;
     size_t f = 0, t = 0;
     if (hipMemGetInfo(&f, &t) != hipSuccess) {
@@ -667,7 +688,8 @@ uint64_t pass_positions(hspz_engine *h) {
                             &B.ktmp, &B.kkey})
         held += b->cap;
     const uint64_t avail = (uint64_t)f + held;
-    const uint64_t want = avail > (4ull << 30) ? (avail - (4ull << 30)) / 2 / PASS_BYTES_PER_POSITION : 0;
+    uint64_t want = avail > (4ull << 30) ? (avail - (4ull << 30)) / 2 / PASS_BYTES_PER_POSITION : 0;
+    want = std::min<uint64_t>(want, (uint64_t)(PASS_HITS / std::max(hits_per_position(h, per), 1.0)));
     return std::min<uint64_t>(std::max<uint64_t>(want, 1ull << 20), 1ull << 26);
 }
 
@@ -719,7 +741,7 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
     // pass_positions() query positions (a batch larger than that runs alone)
     // and 2^20 batches. Every batch is its own dedup scope, so splitting the
     // call between batches changes no list.
-    const uint64_t pmax = pass_positions(h);
+    const uint64_t pmax = pass_positions(h, p->transitions ? 1 + s.kmer_size : 1);
     std::vector<std::pair<size_t, size_t>> passes;
     for (size_t b0 = 0; b0 < all.size();) {
         size_t b1 = b0;

@@ -131,10 +131,13 @@ std::mutex &capture_mutex() {
     return mu;
 }
 
-int reserve(hspz_engine *h, DevBuf &b, size_t bytes) {
+// Grow-only device buffer: at least `bytes`, with 1/8 slack so that a
+// slightly larger next call reuses it (exact = no slack, for buffers sized
+// once, such as the resident reference records).
+int reserve(hspz_engine *h, DevBuf &b, size_t bytes, bool exact = false) {
     if (bytes <= b.cap && b.p) return 0;
     std::lock_guard<std::mutex> lk(capture_mutex());
-    size_t want = std::max<size_t>(bytes + bytes / 8, 256);
+    size_t want = std::max<size_t>(exact ? bytes : bytes + bytes / 8, 256);
     if (b.p) {
         HIP_TRY(hipStreamSynchronize(h->stream));
         HIP_TRY(hipFree(b.p));
@@ -146,7 +149,8 @@ int reserve(hspz_engine *h, DevBuf &b, size_t bytes) {
     if (e != hipSuccess) {
         (void)hipGetLastError();
         b.p = nullptr;
-        return set_error(HSPZ_E_OOM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+        return set_error(HSPZ_E_OOM, "hipMalloc(%zu) failed: %s (the engine holds %llu bytes)", want,
+                         hipGetErrorString(e), (unsigned long long)h->dev_bytes);
     }
     b.cap = want;
     h->dev_bytes += want;
@@ -158,6 +162,12 @@ void release(DevBuf &b) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+}
+
+// release() with the engine's device-byte count kept.
+void release(hspz_engine *h, DevBuf &b) {
+    h->dev_bytes -= b.cap;
+    release(b);
 }
 
 // Uploads a code-7-padded copy of an encoded sequence: PAD bytes either side.
@@ -402,15 +412,15 @@ int device_index_build(hspz_engine *h, const Shape &s, uint32_t step) {
     DevBuf keys, sorted, tmp;
     auto done = [&](int code) {
         (void)hipStreamSynchronize(h->stream);
-        release(keys);
-        release(sorted);
-        release(tmp);
+        release(h, keys);
+        release(h, sorted);
+        release(h, tmp);
         return code;
     };
     hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, h->stream, h->index.as<uint32_t>(), nk, 0u);
     uint32_t nvalid = 0;
     if (nsteps) {
-        if ((rc = reserve(h, keys, nsteps * 8)) || (rc = reserve(h, sorted, nsteps * 8))) return done(rc);
+        if ((rc = reserve(h, keys, nsteps * 8, true)) || (rc = reserve(h, sorted, nsteps * 8, true))) return done(rc);
         ShapeArg sa;
         memset(&sa, 0, sizeof sa);
         memcpy(sa.pos, s.pos, s.kmer_size * 4);
@@ -423,7 +433,7 @@ int device_index_build(hspz_engine *h, const Shape &s, uint32_t step) {
         hipError_t e = rocprim::radix_sort_keys(nullptr, bytes, keys.as<uint64_t>(), sorted.as<uint64_t>(),
                                                 (size_t)nsteps, 32, end_bit, h->stream);
         if (e != hipSuccess) return done(set_error(-(int)e, "index sort: %s", hipGetErrorString(e)));
-        if ((rc = reserve(h, tmp, bytes + 256))) return done(rc);
+        if ((rc = reserve(h, tmp, bytes + 256, true))) return done(rc);
         e = rocprim::radix_sort_keys(tmp.p, bytes, keys.as<uint64_t>(), sorted.as<uint64_t>(), (size_t)nsteps, 32,
                                      end_bit, h->stream);
         if (e != hipSuccess) return done(set_error(-(int)e, "index sort: %s", hipGetErrorString(e)));
@@ -434,7 +444,7 @@ int device_index_build(hspz_engine *h, const Shape &s, uint32_t step) {
         HIP_TRY(hipStreamSynchronize(h->stream));
         nvalid = h->h_small[0];
     }
-    if ((rc = reserve(h, h->pos, std::max<size_t>(nvalid, 1) * 4))) return done(rc);
+    if ((rc = reserve(h, h->pos, std::max<size_t>(nvalid, 1) * 4, true))) return done(rc);
     if (nvalid) {
         const dim3 grid((uint32_t)std::min<uint64_t>((nvalid + 255) / 256, 65536));
         hipLaunchKernelGGL(k_index_pos, grid, dim3(256), 0, h->stream, sorted.as<uint64_t>(), (uint64_t)nvalid,
@@ -550,7 +560,7 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
         if ((rc = device_index_build(h, *build, build_step))) return fail(rc);
     } else {
         if ((rc = reserve(h, h->index, cfg->index_len * 4))) return fail(rc);
-        if ((rc = reserve(h, h->pos, std::max<size_t>(cfg->pos_len, 1) * 4))) return fail(rc);
+        if ((rc = reserve(h, h->pos, std::max<size_t>(cfg->pos_len, 1) * 4, true))) return fail(rc);
         if (hipMemcpy(h->index.p, cfg->index_table, cfg->index_len * 4, hipMemcpyHostToDevice) != hipSuccess ||
             (cfg->pos_len &&
              hipMemcpy(h->pos.p, cfg->pos_table, cfg->pos_len * 4, hipMemcpyHostToDevice) != hipSuccess))
@@ -865,7 +875,7 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
         if (value <= 0 && h->rrec.p) {
             (void)hipStreamSynchronize(h->stream);
             std::lock_guard<std::mutex> lk(capture_mutex());
-            release(h->rrec);
+            release(h, h->rrec);
         }
         h->rrec_declined = false;
         break;
@@ -921,6 +931,7 @@ extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
     case HSPZ_STAT_INDEX_LEN: *value = h->index_len; break;
     case HSPZ_STAT_BATCH_RERUNS: *value = h->batch_reruns; break;
     case HSPZ_STAT_BATCH_PASSES: *value = h->batch_passes_last; break;
+    case HSPZ_STAT_DEVICE_BYTES: *value = h->dev_bytes; break;
     default: return set_error(HSPZ_E_ARG, "unknown stat %d", which);
     }
     return 0;

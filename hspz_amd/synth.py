@@ -101,6 +101,49 @@ def c2_pair(target_len: int = 100_000_000, seed: int = 0xC0FFEE02):
     return t.tobytes(), q.tobytes()
 
 
+C4_RECORDS, C4_RECORD_LEN = 20, 150_000_000
+
+
+def c4_target(seed: int = 0xC0FFEE04, records: int = C4_RECORDS, record_len: int = C4_RECORD_LEN):
+    """C4 shape, mammal-scale target: `records` x `record_len` bases (20 x
+    150 Mbp = 3 Gbp), 41% GC, 45% repeat-derived (500 families of 300 bp
+    consensus, copies 10-20% divergent), all uppercase so that every window
+    seeds (about 3e9 indexed positions). Returns one array of records *
+    record_len bases; record r is [r * record_len, (r + 1) * record_len)."""
+    t = random_bases(seed, records * record_len)
+    return add_repeats(t, seed ^ 0xAB, 500, 300, 0.10, 0.20, 0.45, False)
+
+
+def c4_query(target: np.ndarray, seed: int, length: int = 50_000_000, record_len: int = C4_RECORD_LEN):
+    """One C4 query record: `length` bases sampled from one target record at
+    12% divergence (substitutions 12%, indels 0.8%); then segmental
+    duplications (10-100 kbp copies of other target regions at divergence
+    U(1%, 5%), written over the query until they make 10% of it) and
+    reverse-complemented segments (10-100 kbp, reversed in place until 10% of
+    the query). Returns raw bytes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nrec = len(target) // record_len
+    r = int(rng.integers(0, nrec))
+    start = r * record_len + int(rng.integers(0, record_len - length + 1))
+    q = mutate(target[start:start + length], seed ^ 0x5EED, 0.12, 0.008)
+    covered = 0
+    while covered < 0.10 * len(q):
+        n = int(rng.integers(10_000, 100_001))
+        src = int(rng.integers(0, len(target) - n))
+        d = float(rng.uniform(0.01, 0.05))
+        seg = mutate(target[src:src + n], int(rng.integers(0, 1 << 62)), d, d / 15.0)
+        at = int(rng.integers(0, len(q) - len(seg)))
+        q[at:at + len(seg)] = seg
+        covered += len(seg)
+    covered = 0
+    while covered < 0.10 * len(q):
+        n = int(rng.integers(10_000, 100_001))
+        at = int(rng.integers(0, len(q) - n))
+        q[at:at + n] = revcomp(q[at:at + n])
+        covered += n
+    return q.tobytes()
+
+
 def sample_chunks(target: np.ndarray, seed: int, count: int, median: float, sigma: float,
                   lo: float, hi: float, div_lo: float, div_hi: float):
     """C3/C5 query chunks: clipped lognormal lengths, uniform start,

@@ -167,6 +167,8 @@ int hspz_seed_and_filter_noalloc(hspz_engine *h, int slot, int rev, hspz_segment
                                         list overflowed (each time both lists are doubled) */
 #define HSPZ_STAT_BATCH_PASSES 20     /* device passes the last hspz_filter_batch call ran (see
                                         HSPZ_FLAG_BATCH_MAX_POSITIONS) */
+#define HSPZ_STAT_DEVICE_BYTES 21     /* device bytes the engine holds now (resident reference side and
+                                        grow-only work buffers; PEAK_DEVICE_BYTES is the most it has held) */
 int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 
 /* Engine switches (the reference's pub flags, gpu/mod.rs:836-949, have no

@@ -48,8 +48,8 @@ def test_flag_and_stat_tables_match_header():
 
 def test_list_capacity_growth_is_bounded():
     """The batched pass's overflow rerun grows its survivor and deferred-hit
-    lists in 64-bit arithmetic: at least double the larger of the old capacity
-    and the count reached, clamped to 2^30 records; a count beyond 2^30 can
+    lists in 64-bit arithmetic: to the larger of the count reached + 1/4 and
+    1.5 x the old capacity, clamped to 2^30 records; a count beyond 2^30 can
     never be held and is refused (the pass then fails with HSPZ_E_OOM instead
     of rerunning forever)."""
     import ctypes as C
@@ -57,10 +57,12 @@ def test_list_capacity_growth_is_bounded():
     f.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
     out = C.c_uint32()
     cases = [  # cap, count, holds, new cap
-        (64, 100, 1, 200),
-        (1 << 22, 5, 1, 1 << 23),
-        (1 << 29, (1 << 29) + 1, 1, 1 << 30),
-        ((1 << 30) - 1, 1 << 30, 1, 1 << 30),  # 2^31 would wrap in 32 bits
+        (64, 100, 1, 125),
+        (1 << 22, 5, 1, 3 << 21),
+        (1 << 28, 300_000_000, 1, 3 << 27),
+        (1 << 20, 300_000_000, 1, 375_000_000),
+        (1 << 29, (1 << 29) + 1, 1, 3 << 28),
+        ((1 << 30) - 1, 1 << 30, 1, 1 << 30),  # 1.25 x 2^30 would pass the clamp
         (1 << 31, (1 << 31) + 1, 0, None),
         (1 << 30, (1 << 30) + 1, 0, None),
         (0xFFFFFFFF, 0xFFFFFFFF, 0, None),
