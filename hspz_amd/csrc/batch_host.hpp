@@ -75,19 +75,24 @@ typedef void (*KGateFn)(const uint32_t *, const uint32_t *, int32_t, int32_t, co
                         SurvRec *, uint32_t *, uint32_t, uint32_t *);
 
 // The k-mer-major gate instantiation for a step width and block size; the
-// merged steps take the narrow score sums when the engine's matrix allows.
-KGateFn kgate_kernel(int step_cells, int threads, bool narrow) {
-    if (threads == 1024)
-        return step_cells == 16    ? k_bgate_kmer<1, 1024>
-               : step_cells == 64  ? k_bgate_kmer<4, 1024>
-               : step_cells == 96  ? (narrow ? k_bgate_kmer<6, 1024, true> : k_bgate_kmer<6, 1024>)
-               : step_cells == 128 ? (narrow ? k_bgate_kmer<8, 1024, true> : k_bgate_kmer<8, 1024>)
-                                   : k_bgate_kmer<2, 1024>;
-    return step_cells == 16    ? k_bgate_kmer<1, 512>
-           : step_cells == 64  ? k_bgate_kmer<4, 512>
-           : step_cells == 96  ? (narrow ? k_bgate_kmer<6, 512, true> : k_bgate_kmer<6, 512>)
-           : step_cells == 128 ? (narrow ? k_bgate_kmer<8, 512, true> : k_bgate_kmer<8, 512>)
-                               : k_bgate_kmer<2, 512>;
+// merged steps take the cheapest score-sum form the engine's matrix allows.
+template <int BT>
+KGateFn kgate_kernel_bt(int step_cells, int sums) {
+    switch (step_cells) {
+    case 16: return k_bgate_kmer<1, BT>;
+    case 64: return k_bgate_kmer<4, BT>;
+    case 96:
+        return sums == SUM_RAW ? k_bgate_kmer<6, BT, SUM_RAW>
+               : sums == SUM_TREE16 ? k_bgate_kmer<6, BT, SUM_TREE16> : k_bgate_kmer<6, BT, SUM_TREE8>;
+    case 128:
+        return sums == SUM_RAW ? k_bgate_kmer<8, BT, SUM_RAW>
+               : sums == SUM_TREE16 ? k_bgate_kmer<8, BT, SUM_TREE16> : k_bgate_kmer<8, BT, SUM_TREE8>;
+    default: return k_bgate_kmer<2, BT>;
+    }
+}
+
+KGateFn kgate_kernel(int step_cells, int threads, int sums) {
+    return threads == 1024 ? kgate_kernel_bt<1024>(step_cells, sums) : kgate_kernel_bt<512>(step_cells, sums);
 }
 
 // The lookahead gate (k_bgate_ahead) applies to merged steps (96 or 128
@@ -453,7 +458,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         };
         // fused expansion + packed gate, then the exact gate for deferred hits
         BGateFn gate_fn = bgate_kernel(h->bgate_step, h->bgate_threads);
-        KGateFn kgate_fn = kgate_kernel(h->bgate_step, h->bgate_threads, h->narrow_scores);
+        KGateFn kgate_fn = kgate_kernel(h->bgate_step, h->bgate_threads, h->sum_mode);
         if (kmer_order && lookahead_wanted(h) && h->rrec.p)
             kgate_fn = h->bgate_step == 96 ? k_bgate_ahead<2> : k_bgate_ahead<4>;
         int grid = h->bgate_blocks;

@@ -73,7 +73,8 @@ struct hspz_engine {
     // packed planes + coarse exception bitmaps for the fast gate (k_gate_packed)
     DevBuf ref2, refx, qf2, qfx, qr2, qrx, pairtab, slow, slow_n;
     bool fast_gate = false;
-    bool narrow_scores = false;  // xdrop and ACGT scores <= G2_NARROW_SCORE: 16-cell score sums
+    int sum_mode = 0;  // score-sum form of the merged gates (SUM_*): raw when every ACGT score is within
+                       // +-G2_RAW_SCORE, 16-cell trees when xdrop and the scores are <= G2_NARROW_SCORE
     uint32_t unsafe_codes = 0xFFFFFFFFu;  // perm table: byte k = 0xFF when code 4 + k needs the exact gate
     // seed slots (gpu/mod.rs:933-945)
     DevBuf seeds[2];
@@ -580,7 +581,11 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
         bool narrow = ok && cfg->xdrop <= G2_NARROW_SCORE;
         for (int i = 0; i < 4 && narrow; i++)
             for (int j = 0; j < 4; j++) narrow = narrow && cfg->sub_mat[i * 8 + j] <= G2_NARROW_SCORE;
-        h->narrow_scores = narrow;
+        bool raw = ok;
+        for (int i = 0; i < 4 && raw; i++)
+            for (int j = 0; j < 4; j++)
+                raw = raw && std::abs(std::max(cfg->sub_mat[i * 8 + j], -(cfg->xdrop + 1))) <= G2_RAW_SCORE;
+        h->sum_mode = raw ? SUM_RAW : narrow ? SUM_TREE16 : SUM_TREE8;
         if (ok) {
             uint32_t tab[512];
             const int32_t X = cfg->xdrop;

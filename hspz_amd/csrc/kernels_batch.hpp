@@ -753,20 +753,37 @@ struct KmerSource {
     }
 };
 
-// Adds the a halves of 8 map entries to c. Two trees of packed int16 adds
-// over 4 entries each (8 cells, exact in int16 for xdrop and scores within
-// the packed gate's limits), each folded into c; NARROW: one tree over all 8
-// (16 cells, exact in int16 when xdrop + 1 and every ACGT score are at most
-// G2_NARROW_SCORE + 1 and G2_NARROW_SCORE), folded once.
-template <bool NARROW = false>
+// Score-sum forms of the merged gates (the a halves of the map entries):
+// SUM_TREE8: two trees of packed int16 adds over 4 entries each (8 cells,
+// exact in int16 for xdrop and scores within the packed gate's limits), each
+// folded into c; SUM_TREE16: one tree over all 8 (16 cells, exact in int16
+// when xdrop + 1 and every ACGT score are at most G2_NARROW_SCORE + 1 and
+// G2_NARROW_SCORE), folded once; SUM_RAW: c is the plain 32-bit sum of every
+// entry of the side so far (three v_add3 per word), whose low 16 bits are the
+// score sum mod 2^16 -- exact as an int16 while every ACGT score is within
+// +-G2_RAW_SCORE, since a side spans at most 128 cells (side_sum reads it).
+enum { SUM_TREE8 = 0, SUM_TREE16 = 1, SUM_RAW = 2 };
+constexpr int32_t G2_RAW_SCORE = 255;
+
+template <int SUMS = SUM_TREE8>
 __device__ __forceinline__ void add_a8(int32_t &c, const uint32_t (&p)[8]) {
-    if constexpr (NARROW) {
+    if constexpr (SUMS == SUM_RAW) {
+        const uint32_t x = p[0] + p[1] + p[2], y = p[3] + p[4] + p[5], z = p[6] + p[7] + (uint32_t)c;
+        c = (int32_t)(x + y + z);
+    } else if constexpr (SUMS == SUM_TREE16) {
         c += lo16(pk_add16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3])),
                            pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7]))));
     } else {
         c += lo16(pk_add16(pk_add16(p[0], p[1]), pk_add16(p[2], p[3])));
         c += lo16(pk_add16(pk_add16(p[4], p[5]), pk_add16(p[6], p[7])));
     }
+}
+
+// A side's score sum from its add_a8 accumulator.
+template <int SUMS>
+__device__ __forceinline__ int32_t side_sum(int32_t c) {
+    if constexpr (SUMS == SUM_RAW) return (int32_t)(int16_t)(uint16_t)(uint32_t)c;
+    return c;
 }
 
 // The packed gate over k-mer-major items: the same X-drop core and upper-bound
@@ -951,7 +968,7 @@ __device__ __forceinline__ void map_pairs8(const uint32_t (&p)[8], int32_t &e, i
 // lane offset `off`: updates e, the score sum c, and the minimum of e over
 // every second cell pair (map_pairs8).
 // With N = 8 words, `hi` picks words 4..7 instead of 0..3 (per lane).
-template <int N, bool NARROW = false>
+template <int N, int SUMS = SUM_TREE8>
 __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t (&rw)[N], const uint32_t (&qw)[N],
                                             bool hi, uint32_t off, int32_t &e, int32_t &c, int32_t &emin) {
     constexpr uint32_t SEL[4] = {0x0C0C0400u, 0x0C0C0500u, 0x0C0C0600u, 0x0C0C0700u};
@@ -971,7 +988,7 @@ __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t
             p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
         }
         map_pairs8(p, e, emin);
-        add_a8<NARROW>(c, p);
+        add_a8<SUMS>(c, p);
         // finish this word before the next one's reads: deferring the sums
         // keeps every table entry of the stretch live
         asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
@@ -980,7 +997,7 @@ __device__ __forceinline__ void map_stretch(const uint8_t *s_map, const uint32_t
 
 // K words (16 K cells) of a stream of N words through the map table, as
 // map_stretch: words [0, K), or [K, 2K) when `hi` (N = 2K).
-template <int K, int N, bool NARROW = false>
+template <int K, int N, int SUMS = SUM_TREE8>
 __device__ __forceinline__ void map_words(const uint8_t *s_map, const uint32_t (&rw)[N], const uint32_t (&qw)[N],
                                           bool hi, uint32_t off, int32_t &e, int32_t &c, int32_t &emin) {
     static_assert(N == K || N == 2 * K, "a stretch is the whole stream or one of its halves");
@@ -1001,7 +1018,7 @@ __device__ __forceinline__ void map_words(const uint8_t *s_map, const uint32_t (
             p[2 * j + 1] = map_entry(s_map, t2, off, SEL[j]);
         }
         map_pairs8(p, e, emin);
-        add_a8<NARROW>(c, p);
+        add_a8<SUMS>(c, p);
         asm volatile("" : "+v"(c), "+v"(e), "+v"(emin));
     }
 }
@@ -1110,7 +1127,7 @@ struct HitSides {
 // with cells 32..63 in the next iteration (a random right extension mostly
 // drops within 32 cells, so most hits skip 32 cells of work), with the same
 // upper-bound decisions.
-template <int RW, bool NARROW, class Src>
+template <int RW, int SUMS, class Src>
 __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__restrict__ rplane,
                                                  const uint4 *__restrict__ rrec, const QRec *__restrict__ recs,
                                                  const uint8_t *s_map, int32_t xdrop, int32_t thresh) {
@@ -1171,11 +1188,12 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
         if (has && !defer) {
             // the right stretch: cells 0 .. 16 RW - 1, or the next ones for a continuing right side
             int32_t er = st.rcont ? st.er_keep : xdrop, cr = st.rcont ? st.cr_keep : 0, eminr = er;
-            map_words<RW, G2_RIGHT_WORDS, NARROW>(s_map, A0, B, st.rcont, off_fwd, er, cr, eminr);
+            map_words<RW, G2_RIGHT_WORDS, SUMS>(s_map, A0, B, st.rcont, off_fwd, er, cr, eminr);
             // left cells 0..63, or 64..127 for a continuing left side
             int32_t el = st.cont ? st.el_keep : xdrop, cl = st.cont ? st.cl_keep : 0, eminl = el;
-            map_stretch<8, NARROW>(s_map, LA0, LB, st.cont, off_rev, el, cl, eminl);
-            st.decide<RW>(er, cr, eminr, el, cl, eminl, xdrop, thresh, defer, done, pass);
+            map_stretch<8, SUMS>(s_map, LA0, LB, st.cont, off_rev, el, cl, eminl);
+            st.decide<RW>(er, side_sum<SUMS>(cr), eminr, el, side_sum<SUMS>(cl), eminl, xdrop, thresh, defer, done,
+                          pass);
         }
         src.emit(done, pass, tag, R, Q);
         src.defer(defer, tag, R, Q);
@@ -1390,9 +1408,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 
 // The batched gate in k-mer-major order (same launch shape as k_bgate_packed):
 // the merged gate for SW = 8 (128-cell steps) and SW = 6 (96-cell steps, 32
-// right cells at a time), kmer_gate_loop<SW> for the others. NARROW: the
-// merged gate's score sums for narrow matrices (add_a8).
-template <int SW, int BT, bool NARROW = false>
+// right cells at a time), kmer_gate_loop<SW> for the others. SUMS: the
+// merged gate's score-sum form (add_a8), the cheapest the engine's matrix allows.
+template <int SW, int BT, int SUMS = SUM_TREE8>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_kmer(
     const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop,
     int32_t thresh, const QRec *__restrict__ recs, const uint32_t *__restrict__ skm,
@@ -1426,9 +1444,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.set_lists(nullptr, s_dq[threadIdx.x / WAVE]);
     src.init();
     if constexpr (SW == 8)
-        kmer_gate_merged<4, NARROW>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+        kmer_gate_merged<4, SUMS>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
     else if constexpr (SW == 6)
-        kmer_gate_merged<2, NARROW>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+        kmer_gate_merged<2, SUMS>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
     else kmer_gate_loop<SW>(src, rplane, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 

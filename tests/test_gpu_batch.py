@@ -448,3 +448,38 @@ def test_batch_autotune(gpu_available):
         for g, w in zip(after, before):
             assert same_intervals(g, w)
         assert st1["hits"] == st0["hits"]
+
+
+@pytest.mark.parametrize("scale,xdrop", [(1, 910), (2, 240), (4, 1800), (16, 3000)])
+def test_batch_score_sum_forms(gpu_available, scale, xdrop):
+    """The merged gates' score sums in each form the matrix selects, against
+    the oracle: HOXD70 (scores within +-255: raw 32-bit sums of the table
+    entries, read as int16 at the decision); x2 at xdrop 240 (scores to -250
+    clamped at -241, so raw sums again); x4 at xdrop 1800 (scores within 2047:
+    16-cell int16 trees); x16 at xdrop 3000 (beyond 2047: 8-cell trees). At
+    96- and 128-cell steps in both block sizes, on homologs long enough to
+    take the continuation iterations (right past 32 cells, left past 64)."""
+    _say("case batch: score-sum form, HOXD70 x%d, xdrop %d" % (scale, xdrop))
+    hoxd70 = [91, -114, -31, -123, -114, 100, -125, -31, -31, -125, 100, -114, -123, -31, -114, 91]
+    acgt = [scale * v for v in hoxd70]
+    thresh = 3000 * scale
+    target = synth.random_bases(0xB5C0 + scale, 200_000).tobytes()
+    blocks = _chunks_from(target, 12, 23 + scale, lo=500, hi=15_000, div=(0.0, 0.2))
+    sub = O.sub_mat("", xdrop, acgt)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    want, hsps = [], 0
+    for f, r in blocks:
+        o = O.Oracle(target, f, r, sub, xdrop=xdrop, hspthresh=thresh, max_hits=16711680, index=(index, pos))
+        res, st, _ = o.run_pass(threads=16)
+        want.append(res)
+        hsps += st["hsps"]
+    assert hsps > 10
+    chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
+    with E.Engine(index, pos, S.encode(target), sub, xdrop=xdrop, hspthresh=thresh, max_hits=16711680) as eng:
+        for step, threads in [(96, 1024), (128, 512), (96, 512), (128, 1024)]:
+            for name, v in (("batch_step_cells", step), ("batch_block_threads", threads), ("batch_order", 1),
+                            ("batch_lookahead", 0)):
+                eng.set_flag(name, v)
+            got, _ = eng.filter_batch(chunks)
+            for i, (g, w) in enumerate(zip(got, want)):
+                assert same_intervals(g, w), "chunk %d differs at step %d threads %d" % (i, step, threads)
