@@ -30,7 +30,7 @@ struct HostBatch {
 struct BatchBufs {
     DevBuf arena, plane, coarse, desc, poff, blk_off, blk_len, valid, ids, voff, seeds, counts, prefix,
         tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr, gcnt, sq, skm, qrec,
-        kvalo, ktmp, exact, kkey, kgaps;
+        kvalo, ktmp, exact, kkey, kgaps, cand;
 };
 
 // Largest survivor or deferred-hit list of one pass, in records (16 B each).
@@ -61,8 +61,10 @@ namespace {
 
 // Device counters of one pass; CTR_WORK is the 64-bit seed-claim counter of
 // the batched gate (two slots, 8-byte aligned).
+// CTR_CAND counts the coarse gate's candidates and CTR_WORK2 (64-bit, slots
+// 10-11) is the candidate-claim counter of k_bgate_exact.
 enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_EXACT = 5, CTR_WORK = 6,
-       CTR_KGAPS = 8, CTR_N = 10 };
+       CTR_KGAPS = 8, CTR_CAND = 9, CTR_WORK2 = 10, CTR_N = 12 };
 
 typedef void (*BGateFn)(const uint32_t *, const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, uint32_t,
                         const uint32_t *, int32_t, int32_t, const uint64_t *, const uint32_t *, uint32_t,
@@ -122,7 +124,7 @@ struct GraphKey {
 
 struct hspz_batch_state {
     BatchBufs b;
-    uint32_t cap_surv = 1u << 22, cap_slow = 1u << 22;
+    uint32_t cap_surv = 1u << 22, cap_slow = 1u << 22, cap_cand = 1u << 22;
     bool have_graph = false;
     GraphKey key;
     // before the grouping sort, after it, the gate, after the gate
@@ -338,13 +340,16 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         // rarely pay an overflow rerun
         hspz_batch_state *bs = h->batch;
         if (h->list_cap) {
-            bs->cap_surv = bs->cap_slow = h->list_cap;
+            bs->cap_surv = bs->cap_slow = bs->cap_cand = h->list_cap;
         } else {
+            // the coarse gate's candidates: ~1.5% of random hits and every homologous one
             const double hits = (double)npos * hits_per_position(h, per);
             const uint64_t surv = std::max<uint64_t>(cap_seeds / 16, (uint64_t)(hits / 4096));
             const uint64_t slow = std::max<uint64_t>(cap_seeds / 4, (uint64_t)(hits / 256));
+            const uint64_t cand = std::max<uint64_t>(cap_seeds / 4, (uint64_t)(hits / 32));
             bs->cap_surv = (uint32_t)std::max<uint64_t>(bs->cap_surv, std::min<uint64_t>(surv, 1u << 28));
             bs->cap_slow = (uint32_t)std::max<uint64_t>(bs->cap_slow, std::min<uint64_t>(slow, 1u << 28));
+            bs->cap_cand = (uint32_t)std::max<uint64_t>(bs->cap_cand, std::min<uint64_t>(cand, 1u << 29));
         }
     }
     for (;;) {  // one attempt per survivor/slow capacity
@@ -368,15 +373,19 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         if ((rc = reserve(h, h->tiles, (size_t)std::max(std::max(kt, nt), ntk) * 4))) return rc;
         if (kmer_order) {
             if ((rc = reserve(h, B.gcnt, ((size_t)nkm + 1) * 4))) return rc;
+            if (h->coarse_gate && (rc = reserve(h, B.cand, (size_t)bs->cap_cand * 16))) return rc;
             if ((rc = reserve(h, B.kgaps, (size_t)kgap_capacity(nkm) * sizeof(uint4)))) return rc;
             if ((rc = reserve(h, B.kkey, cap_seeds * 4 + 4))) return rc;   // sort keys (u32)
             if ((rc = reserve(h, B.sq, cap_seeds * 4 + 4))) return rc;     // sort values (u32), in
             if ((rc = reserve(h, B.kvalo, cap_seeds * 4 + 4))) return rc;  // and out
             if ((rc = reserve(h, B.skm, cap_seeds * 4 + 4))) return rc;
             if ((rc = reserve(h, B.qrec, npos * sizeof(QRec) + sizeof(QRec)))) return rc;  // one per position
-            if ((rc = ensure_ref_records(h, h->rrec_mode > 0 || (h->rrec_mode == 0 && h->bgate_step >= 96))))
+            if ((rc = ensure_ref_records(h, h->rrec_mode > 0 ||
+                                                (h->rrec_mode == 0 && (h->bgate_step >= 96 || h->coarse_gate)))))
                 return rc;
         }
+        // the coarse gate runs on the reference records, in k-mer-major order
+        const bool coarse = kmer_order && h->coarse_gate && h->rrec.p;
         uint32_t *kkey = kmer_order ? B.kkey.as<uint32_t>() : nullptr;
         uint32_t *kval = kmer_order ? B.sq.as<uint32_t>() : nullptr;
         const int key_bits = 32 - __builtin_clz(nkm - 1);  // keys are in [0, nk - 1]
@@ -461,19 +470,58 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         KGateFn kgate_fn = kgate_kernel(h->bgate_step, h->bgate_threads, h->sum_mode);
         if (kmer_order && lookahead_wanted(h) && h->rrec.p)
             kgate_fn = h->bgate_step == 96 ? k_bgate_ahead<2> : k_bgate_ahead<4>;
-        int grid = h->bgate_blocks;
+        typedef void (*CGateFn)(const QRec *, const uint32_t *, const uint32_t *, const uint32_t *, uint32_t,
+                                const uint32_t *, const uint32_t *, const uint32_t *, const uint4 *, uint32_t,
+                                uint32_t, unsigned long long *, CoarseCoef, int32_t, int32_t, uint4 *, uint32_t *,
+                                uint32_t, SurvRec *, uint32_t *, uint32_t, uint32_t *);
+        typedef void (*XGateFn)(const uint32_t *, const uint32_t *, int32_t, int32_t, const QRec *,
+                                const uint32_t *, const uint4 *, const uint4 *, const uint32_t *, uint32_t, uint32_t,
+                                unsigned long long *, SurvRec *, uint32_t *, uint32_t, SurvRec *, uint32_t *,
+                                uint32_t, uint32_t *);
+        const CGateFn cgate_fn = h->bgate_threads == 1024 ? k_bgate_coarse<1024> : k_bgate_coarse<512>;
+        XGateFn xgate_fn;
+        if (h->bgate_threads == 1024)
+            xgate_fn = h->sum_mode == SUM_RAW      ? k_bgate_exact<1024, SUM_RAW>
+                       : h->sum_mode == SUM_TREE16 ? k_bgate_exact<1024, SUM_TREE16>
+                                                   : k_bgate_exact<1024, SUM_TREE8>;
+        else
+            xgate_fn = h->sum_mode == SUM_RAW      ? k_bgate_exact<512, SUM_RAW>
+                       : h->sum_mode == SUM_TREE16 ? k_bgate_exact<512, SUM_TREE16>
+                                                   : k_bgate_exact<512, SUM_TREE8>;
+        int grid = h->bgate_blocks, xgrid = h->bgate_blocks;
         if (grid == 0) {  // resident grid: occupancy of this instantiation x CUs
-            int per_cu = 0;
-            if (kmer_order)
+            int per_cu = 0, xper_cu = 0;
+            if (coarse) {
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cgate_fn, h->bgate_threads, 0));
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&xper_cu, xgate_fn, h->bgate_threads, 0));
+            } else if (kmer_order) {
                 HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kgate_fn, h->bgate_threads, 0));
-            else
+            } else {
                 HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gate_fn, h->bgate_threads, 0));
+            }
             grid = std::max(per_cu, 1) * h->num_cus;
+            xgrid = std::max(xper_cu, 1) * h->num_cus;
         }
         h->bgate_grid_used = grid;
         h->batch_order_used = kmer_order ? 1 : 0;
+        h->coarse_used = coarse;
         auto enqueue_gate = [&]() {
-        if (kmer_order)
+        if (coarse) {
+            // coarse bound over every hit, then the two-cell maps over its candidates
+            hipLaunchKernelGGL(cgate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, B.qrec.as<QRec>(),
+                               B.skm.as<uint32_t>(), B.kvalo.as<uint32_t>(), B.gcnt.as<uint32_t>(), nkm,
+                               h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->clean.as<uint32_t>(),
+                               h->rrec.as<uint4>(), h->seed_size, (uint32_t)h->bgate_claim,
+                               (unsigned long long *)(ctr + CTR_WORK), h->coarse_cc, h->xdrop, h->hspthresh,
+                               B.cand.as<uint4>(), ctr + CTR_CAND, bs->cap_cand, B.slow.as<SurvRec>(),
+                               ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
+            hipLaunchKernelGGL(xgate_fn, dim3(xgrid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
+                               h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
+                               B.kvalo.as<uint32_t>(), h->rrec.as<uint4>(), B.cand.as<uint4>(), ctr + CTR_CAND,
+                               bs->cap_cand, (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK2),
+                               B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(),
+                               ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
+        } else if (kmer_order)
             hipLaunchKernelGGL(kgate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
                                h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
                                B.skm.as<uint32_t>(), B.kvalo.as<uint32_t>(), B.gcnt.as<uint32_t>(), nkm,
@@ -529,10 +577,12 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                   B.nchunks.p, B.starts.p, B.bhits.p, B.surv.p, B.slow.p, B.mhsp.p, B.mscope.p,
                                   B.keep.p, B.kids.p, B.ohsp.p, B.oscope.p, B.ctr.p, h->tiles.p, h->ref2.p,
                                   h->refx.p, h->pairtab.p, h->index.p, h->pos.p, B.gcnt.p, B.sq.p, B.skm.p,
-                                  h->clean.p, B.qrec.p, h->rrec.p, B.kvalo.p, B.exact.p, B.kkey.p, B.kgaps.p};
+                                  h->clean.p, B.qrec.p, h->rrec.p, B.kvalo.p, B.exact.p, B.kkey.p, B.kgaps.p,
+                                  B.cand.p};
             static_assert(sizeof ptrs <= sizeof key.ptrs, "graph key pointer slots");
             memcpy(key.ptrs, ptrs, sizeof ptrs);
-            uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow, cap_seeds, per, h->max_hits,
+            uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow | (uint64_t)bs->cap_cand << 32,
+                               cap_seeds | (uint64_t)coarse << 40 | (uint64_t)xgrid << 41, per, h->max_hits,
                                (uint64_t)grid | (uint64_t)h->bgate_claim << 32, (uint64_t)p->transitions, nt, kt,
                                cap_tiles, (uint64_t)h->bgate_step | (uint64_t)h->bgate_threads << 32,
                                (uint64_t)(uintptr_t)h->sub.p, s.kmer_size, (uint64_t)(uintptr_t)h->ref.p,
@@ -606,20 +656,22 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         h->last_post_ms = post_ms;
         h->last_surv = hc[CTR_SURV];
         h->last_deferred = hc[CTR_SLOW];
+        h->last_cand = coarse ? hc[CTR_CAND] : 0;
         // fatal conditions first: the gate still fills the lists on a failing
         // batch, and an overflow rerun would only repeat the failure
         if (hc[CTR_ERR] & 1u) return set_error(HSPZ_E_CHUNK, "a single seed exceeds max_hits %u", h->max_hits);
         if (hc[CTR_ERR] & 2u) return set_error(HSPZ_E_CHUNK, "a batch needs more than %u max_hits chunks", MAX_CHUNKS);
         if (hc[CTR_ERR] & 4u) return set_error(HSPZ_E_ARG, "a batch has more than 2^32 hits");
         if (hc[CTR_ERR] & 8u) {  // survivor or slow list overflowed: grow and rerun
-            uint32_t ns = 0, nd = 0;
-            if (!grow_list_cap(bs->cap_surv, hc[CTR_SURV], &ns) || !grow_list_cap(bs->cap_slow, hc[CTR_SLOW], &nd))
-                return set_error(HSPZ_E_OOM, "batch needs more than %u survivor or deferred-hit records (%u, %u)",
-                                 LIST_CAP_MAX, hc[CTR_SURV], hc[CTR_SLOW]);
-            h->last_surv = hc[CTR_SURV];
-            h->last_deferred = hc[CTR_SLOW];
+            uint32_t ns = 0, nd = 0, nc = bs->cap_cand;
+            if (!grow_list_cap(bs->cap_surv, hc[CTR_SURV], &ns) || !grow_list_cap(bs->cap_slow, hc[CTR_SLOW], &nd) ||
+                (coarse && !grow_list_cap(bs->cap_cand, hc[CTR_CAND], &nc)))
+                return set_error(HSPZ_E_OOM,
+                                 "batch needs more than %u survivor, deferred-hit or candidate records (%u, %u, %u)",
+                                 LIST_CAP_MAX, hc[CTR_SURV], hc[CTR_SLOW], hc[CTR_CAND]);
             bs->cap_surv = ns;
             bs->cap_slow = nd;
+            bs->cap_cand = nc;
             h->batch_reruns++;
             continue;
         }

@@ -97,6 +97,10 @@ struct hspz_engine {
     uint32_t list_cap = 0;      // HSPZ_FLAG_BATCH_LIST_CAP; 0 = sized from the batch
     uint32_t batch_max_positions = 0;  // HSPZ_FLAG_BATCH_MAX_POSITIONS; 0 = from free device memory
     uint64_t batch_passes_last = 0;    // device passes of the last hspz_filter_batch call
+    bool coarse_gate = true;           // HSPZ_FLAG_BATCH_COARSE: class-score bound ahead of the two-cell maps
+    bool coarse_used = false;          // the last batched pass ran it
+    CoarseCoef coarse_cc = {0, 0, 0, 0, 0, 0};
+    uint64_t last_cand = 0;            // candidates of the last batched pass
     int gate_blocks = 2048;
     // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
     int bgate_blocks = 0;  // persistent grid; 0 = as many blocks as are resident at once
@@ -586,6 +590,20 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
             for (int j = 0; j < 4; j++)
                 raw = raw && std::abs(std::max(cfg->sub_mat[i * 8 + j], -(cfg->xdrop + 1))) <= G2_RAW_SCORE;
         h->sum_mode = raw ? SUM_RAW : narrow ? SUM_TREE16 : SUM_TREE8;
+        {
+            // the coarse bound's class scores: the best ACGT score of each class
+            // (match; transition, codes xor 2; transversion, xor odd)
+            int32_t um = INT32_MIN, ut = INT32_MIN, uv = INT32_MIN;
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    const int32_t v = cfg->sub_mat[i * 8 + j];
+                    int32_t &u = (i ^ j) == 0 ? um : (i ^ j) == 2 ? ut : uv;
+                    u = std::max(u, v);
+                }
+            const int32_t pm = std::max(um, 0), pt = std::max(ut, 0), pv = std::max(uv, 0);
+            // nw mismatches, nv of them transversions: nm = 16 - nw, nt = nw - nv
+            h->coarse_cc = {16 * um, ut - um, uv - ut, 16 * pm, pt - pm, pv - pt};
+        }
         if (ok) {
             uint32_t tab[512];
             const int32_t X = cfg->xdrop;
@@ -675,7 +693,7 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
         DevBuf *bb[] = {&B.arena, &B.plane, &B.coarse, &B.desc, &B.poff, &B.blk_off, &B.blk_len, &B.valid, &B.ids,
                         &B.voff, &B.seeds, &B.counts, &B.prefix, &B.tiles, &B.nchunks, &B.starts, &B.bhits, &B.surv,
                         &B.slow, &B.mhsp, &B.mscope, &B.keep, &B.kids, &B.ohsp, &B.oscope, &B.ctr, &B.gcnt,
-                        &B.sq, &B.skm, &B.qrec, &B.kvalo, &B.ktmp, &B.exact, &B.kkey, &B.kgaps};
+                        &B.sq, &B.skm, &B.qrec, &B.kvalo, &B.ktmp, &B.exact, &B.kkey, &B.kgaps, &B.cand};
         for (DevBuf *b : bb) release(*b);
         hspz_batch_state *bs = h->batch;
         DevBuf *dd[] = {&bs->dd_key_a, &bs->dd_key_b, &bs->dd_id_a, &bs->dd_id_b, &bs->dd_keep, &bs->dd_kids,
@@ -866,6 +884,7 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
             return set_error(HSPZ_E_ARG, "list capacity must be 0 or in [64, 2^28]");
         h->list_cap = (uint32_t)value;
         break;
+    case HSPZ_FLAG_BATCH_COARSE: h->coarse_gate = value != 0; break;
     case HSPZ_FLAG_BATCH_MAX_POSITIONS:
         if (value < 0) return set_error(HSPZ_E_ARG, "max positions must be 0 (automatic) or positive");
         h->batch_max_positions = (uint32_t)value;
@@ -906,6 +925,8 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_LIST_CAP: *value = (int)h->list_cap; break;
     case HSPZ_FLAG_NOALLOC_TILE_HITS: *value = (int)h->noalloc_tile_hits; break;
     case HSPZ_FLAG_BATCH_MAX_POSITIONS: *value = (int)h->batch_max_positions; break;
+    case HSPZ_FLAG_BATCH_COARSE: *value = h->coarse_gate ? 1 : 0; break;
+    case HSPZ_FLAG_BATCH_COARSE_USED: *value = h->coarse_used ? 1 : 0; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;
@@ -937,6 +958,7 @@ extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
     case HSPZ_STAT_BATCH_RERUNS: *value = h->batch_reruns; break;
     case HSPZ_STAT_BATCH_PASSES: *value = h->batch_passes_last; break;
     case HSPZ_STAT_DEVICE_BYTES: *value = h->dev_bytes; break;
+    case HSPZ_STAT_BATCH_CANDIDATES: *value = h->last_cand; break;
     default: return set_error(HSPZ_E_ARG, "unknown stat %d", which);
     }
     return 0;

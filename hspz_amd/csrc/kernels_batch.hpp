@@ -614,16 +614,36 @@ __device__ __forceinline__ uint32_t pick_word(const uint32_t (&w)[N], uint32_t g
 // time and own the groups that start inside the claim; a group's bucket
 // entries go to idle lanes as items (one reference position each), and a
 // lane walks its item's group of query records one hit at a time.
-struct KmerSource {
+// The lists a k-mer-major gate appends to: survivors (or, for the coarse
+// gate, candidates) through `sq`, deferred hits through `dq`.
+struct GateOut {
+    SurvRec *surv, *slow;
+    uint32_t *surv_n, *slow_n, cap_surv, cap_slow;
+    uint32_t *err;
+    WaveQueue<uint4> sq, dq;  // survivors, deferred hits
+
+    __device__ __forceinline__ void emit(bool done, bool pass, uint32_t tag, uint32_t R, uint32_t Q) {
+        sq.push(done && pass, make_uint4(R, Q, tag, 0u), err);
+    }
+    __device__ __forceinline__ void defer(bool d, uint32_t tag, uint32_t R, uint32_t Q) {
+        dq.push(d, make_uint4(R, Q, tag, 0u), err);
+    }
+    __device__ __forceinline__ void finish() {
+        sq.flush(err);
+        dq.flush(err);
+    }
+    __device__ __forceinline__ void set_lists(uint4 *sq_slots, uint4 *dq_slots) {
+        sq = {sq_slots, 0u, (uint4 *)surv, surv_n, cap_surv};
+        dq = {dq_slots, 0u, (uint4 *)slow, slow_n, cap_slow};
+    }
+};
+
+struct KmerSource : GateOut {
     const uint32_t *skm, *sval, *goff, *index, *pos, *clean;
     unsigned long long *work;
     uint32_t total, claim, seed_size;
     uint32_t p_cur, p_end, g0, g1, b0, n, j;
     bool dry;
-    SurvRec *surv, *slow;
-    uint32_t *surv_n, *slow_n, cap_surv, cap_slow;
-    uint32_t *err;
-    WaveQueue<uint4> sq, dq;  // survivors, deferred hits
 
     __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -736,20 +756,6 @@ struct KmerSource {
             need = __ballot(want && !got);
         }
         return got;
-    }
-    __device__ __forceinline__ void emit(bool done, bool pass, uint32_t tag, uint32_t R, uint32_t Q) {
-        sq.push(done && pass, make_uint4(R, Q, tag, 0u), err);
-    }
-    __device__ __forceinline__ void defer(bool d, uint32_t tag, uint32_t R, uint32_t Q) {
-        dq.push(d, make_uint4(R, Q, tag, 0u), err);
-    }
-    __device__ __forceinline__ void finish() {
-        sq.flush(err);
-        dq.flush(err);
-    }
-    __device__ __forceinline__ void set_lists(uint4 *sq_slots, uint4 *dq_slots) {
-        sq = {sq_slots, 0u, (uint4 *)surv, surv_n, cap_surv};
-        dq = {dq_slots, 0u, (uint4 *)slow, slow_n, cap_slow};
     }
 };
 
@@ -1448,6 +1454,206 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     else if constexpr (SW == 6)
         kmer_gate_merged<2, SUMS>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
     else kmer_gate_loop<SW>(src, rplane, recs, (const uint8_t *)s_map, xdrop, thresh);
+}
+
+// ---------------------------------------------------------------- coarse bound
+//
+// A cheaper upper bound of a hit's X-drop total that rejects almost every
+// random hit before the two-cell maps run. Each cell is scored by the best
+// score of its class under the engine's matrix: match, transition (codes
+// xor 2) or transversion (xor odd), so the class score u >= the cell's score.
+// Over a side, U_k = sum of u (prefix) >= P_k (the true prefix), and for j <=
+// k, U_j - U_k <= P_j - P_k; so wherever the u-process has fallen more than X
+// below its running maximum, the true process already has, and the true
+// extension ends no later. Checking the fall only at 16-cell word boundaries
+// (whose U are exact) declares a drop only where the u-process has dropped.
+// The true maximum is then at most the largest U_start(word) + (positive
+// class scores of the word) over the words up to that boundary: B. A hit
+// whose sides both drop within the windows and whose B_right + B_left fails
+// the gate's keep test (monotone in the total) fails the exact test too. Per
+// word: the classes come from r ^ q as two masks and two popcounts.
+
+// A 16-cell word with nw mismatches, nv of them transversions: its class-score
+// sum is s0 + sw * nw + sv * nv, the sum of its positive class scores
+// p0 + pw * nw + pv * nv.
+struct CoarseCoef {
+    int32_t s0, sw, sv, p0, pw, pv;
+};
+
+constexpr int CW_RIGHT = 4, CW_LEFT = 6;  // coarse windows: 64 right cells, 96 left cells
+
+__device__ __forceinline__ void coarse_word(uint32_t r, uint32_t q, const CoarseCoef &cc, int32_t xdrop,
+                                            int32_t &U, int32_t &mx, int32_t &B, bool &dropped) {
+    const uint32_t x = r ^ q;
+    const uint32_t tv = x & 0x55555555u;              // odd xor: transversion
+    const uint32_t ts = (x >> 1) & ~x & 0x55555555u;  // xor 2: transition
+    const int32_t nv = __popc(tv), nw = __popc(ts) + nv;
+    const int32_t bnd = U + cc.p0 + cc.pw * nw + cc.pv * nv;
+    B = dropped ? B : max(B, bnd);
+    U += cc.s0 + cc.sw * nw + cc.sv * nv;
+    mx = max(mx, U);
+    dropped = dropped || mx - U > xdrop;
+}
+
+// The coarse gate over the k-mer-major walk (KmerSource, reference records
+// required): every hit with clean windows is bounded over CW_RIGHT right and
+// CW_LEFT left words; a hit not rejected by the bound (or whose u-process
+// outlives a window) becomes a candidate (R, reference-record index, grouped
+// seed) for k_bgate_exact; a hit whose windows meet an exception is deferred
+// as in k_bgate_kmer. One iteration per hit.
+template <int BT>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_coarse(
+    const QRec *__restrict__ recs, const uint32_t *__restrict__ skm, const uint32_t *__restrict__ sval,
+    const uint32_t *__restrict__ goff, uint32_t nk, const uint32_t *__restrict__ index,
+    const uint32_t *__restrict__ pos, const uint32_t *__restrict__ clean, const uint4 *__restrict__ rrec,
+    uint32_t seed_size, uint32_t claim, unsigned long long *__restrict__ work, CoarseCoef cc, int32_t xdrop,
+    int32_t thresh, uint4 *__restrict__ cand, uint32_t *__restrict__ cand_n, uint32_t cap_cand,
+    SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow, uint32_t *__restrict__ err) {
+    __shared__ uint4 s_q[2][BT / WAVE][WQ_CAP];
+    KmerSource src;
+    src.skm = skm;
+    src.sval = sval;
+    src.goff = goff;
+    src.index = index;
+    src.pos = pos;
+    src.clean = clean;
+    src.work = work;
+    src.total = __builtin_amdgcn_readfirstlane(goff[nk]);
+    src.claim = claim;
+    src.seed_size = seed_size;
+    src.surv = (SurvRec *)cand;
+    src.surv_n = cand_n;
+    src.cap_surv = cap_cand;
+    src.slow = slow;
+    src.slow_n = slow_n;
+    src.cap_slow = cap_slow;
+    src.err = err;
+    src.set_lists(s_q[0][threadIdx.x / WAVE], s_q[1][threadIdx.x / WAVE]);
+    src.init();
+    bool has = false, item = false, rok = false;
+    uint32_t R = 0, s_cur = 0, s_end = 0, iidx = 0;
+    uint32_t A0[CW_RIGHT], LA0[CW_LEFT];
+#pragma unroll
+    for (int k = 0; k < CW_RIGHT; k++) A0[k] = 0;
+#pragma unroll
+    for (int k = 0; k < CW_LEFT; k++) LA0[k] = 0;
+    for (;;) {
+        bool fresh_r = false, fresh_q = false;
+        uint32_t ridx = 0;
+        src.refill(has, item, fresh_r, fresh_q, R, s_cur, s_end, rok, ridx);
+        if (__ballot(has) == 0) break;
+        if (fresh_r && rok) {
+            const uint4 *rr = rrec + 3 * (size_t)ridx;
+            const uint4 w0 = rr[0], w1 = rr[1], w2 = rr[2];
+            A0[0] = w0.x, A0[1] = w0.y, A0[2] = w0.z, A0[3] = w0.w;
+            LA0[0] = w1.x, LA0[1] = w1.y, LA0[2] = w1.z, LA0[3] = w1.w;
+            LA0[4] = w2.x, LA0[5] = w2.y;
+        }
+        iidx = fresh_r ? ridx : iidx;
+        // every lane with a hit has a fresh one: a hit takes one iteration
+        uint32_t var;
+        const uint32_t p = s_cur;
+        const uint4 *rec = src.record(recs, has ? p : 0u, var);
+        const uint4 v0 = rec[0], v1 = rec[1], v2 = rec[2], v3 = rec[3];
+        s_cur += has ? 1u : 0u;
+        const uint32_t B[CW_RIGHT] = {v0.x, v0.y, v0.z, v0.w};
+        const uint32_t LB[CW_LEFT] = {v1.x, v1.y, v1.z, v1.w, v2.x, v2.y};
+        int32_t Ur = 0, mxr = 0, Br = 0, Ul = 0, mxl = 0, Bl = 0;
+        bool dr = false, dl = false;
+#pragma unroll
+        for (int k = 0; k < CW_LEFT; k++) {
+            if (k < CW_RIGHT) coarse_word(A0[k], B[k], cc, xdrop, Ur, mxr, Br, dr);
+            coarse_word(LA0[k], LB[k], cc, xdrop, Ul, mxl, Bl, dl);
+        }
+        const bool ok = rok && v3.z != 0;
+        const bool defer = has && !ok;
+        const bool pass = has && ok && (!(dr && dl) || packed_keep(max(Br, 0) + max(Bl, 0), thresh));
+        src.sq.push(pass, make_uint4(R, iidx, p, 0u), err);
+        src.defer(defer, v3.y + var, R, v3.x);
+        has = false;
+    }
+    src.finish();
+}
+
+// Hit source over the coarse gate's candidates (R, reference-record index,
+// grouped seed): waves claim `claim` candidates at a time from a grid-wide
+// counter and hand them to idle lanes, each one hit on its own item.
+struct CandSource : GateOut {
+    const uint4 *list;
+    const uint32_t *sval;
+    unsigned long long *work;
+    uint32_t n, claim, c_cur, c_end;
+
+    __device__ __forceinline__ bool take() {
+        uint32_t leader = (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1);
+        unsigned long long v = 0;
+        if (lane_id() == leader) v = atomicAdd(work, (unsigned long long)claim);
+        const uint64_t start = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+                               (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+        if (start >= (uint64_t)n) return false;
+        c_cur = (uint32_t)start;
+        c_end = start + claim < (uint64_t)n ? (uint32_t)(start + claim) : n;
+        return true;
+    }
+    __device__ __forceinline__ void init() { c_cur = c_end = 0; }
+    __device__ __forceinline__ const uint4 *record(const QRec *recs, uint32_t p, uint32_t &var) const {
+        const uint32_t gv = sval[p];
+        var = gv & ((1u << GV_SHIFT) - 1);
+        return (const uint4 *)(recs + (gv >> GV_SHIFT));
+    }
+    __device__ __forceinline__ void refill(bool &has, bool &item, bool &fresh_r, bool &fresh_q, uint32_t &R,
+                                           uint32_t &s_cur, uint32_t &s_end, bool &rok, uint32_t &ridx) {
+        uint64_t need = __ballot(!has);
+        while (need) {
+            if (c_cur >= c_end) {
+                if (!take()) break;
+                continue;
+            }
+            const uint32_t avail = c_end - c_cur, rank = mbcnt(need);
+            if (!has && rank < avail) {
+                const uint4 e = list[c_cur + rank];
+                R = e.x;
+                ridx = e.y;
+                s_cur = e.z;
+                s_end = e.z + 1;
+                rok = true;
+                has = item = fresh_r = fresh_q = true;
+            }
+            c_cur += min((uint32_t)__popcll(need), avail);
+            need = __ballot(!has);
+        }
+    }
+};
+
+// The merged two-cell-map gate (kmer_gate_merged, 96-cell steps) over the
+// coarse gate's candidates: survivors and deferred hits as k_bgate_kmer's.
+template <int BT, int SUMS>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_exact(
+    const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh,
+    const QRec *__restrict__ recs, const uint32_t *__restrict__ sval, const uint4 *__restrict__ rrec,
+    const uint4 *__restrict__ cand, const uint32_t *__restrict__ cand_n, uint32_t cap_cand, uint32_t claim,
+    unsigned long long *__restrict__ work, SurvRec *__restrict__ surv, uint32_t *__restrict__ surv_n,
+    uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow,
+    uint32_t *__restrict__ err) {
+    __shared__ uint32_t s_map[G2_MAP_WORDS];
+    __shared__ uint4 s_dq[BT / WAVE][WQ_CAP];
+    load_map_table(s_map, pairtab, BT);
+    CandSource src;
+    src.list = cand;
+    src.sval = sval;
+    src.work = work;
+    src.n = __builtin_amdgcn_readfirstlane(min(*cand_n, cap_cand));
+    src.claim = claim;
+    src.surv = surv;
+    src.surv_n = surv_n;
+    src.cap_surv = cap_surv;
+    src.slow = slow;
+    src.slow_n = slow_n;
+    src.cap_slow = cap_slow;
+    src.err = err;
+    src.set_lists(nullptr, s_dq[threadIdx.x / WAVE]);
+    src.init();
+    kmer_gate_merged<2, SUMS>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
 // Block bounds of a seed's batch: arena offset and length of its block.

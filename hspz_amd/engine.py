@@ -25,11 +25,12 @@ STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 
         "gate_launches": 8, "gate_deferred": 9, "graph_captures": 10, "graph_replays": 11,
         "batch_prep_ns": 12, "batch_device_ns": 13, "batch_read_ns": 14, "batch_post_ns": 15,
         "index_build_ns": 16, "pos_len": 17, "index_len": 18,
-        "batch_reruns": 19, "batch_passes": 20, "device_bytes": 21}
+        "batch_reruns": 19, "batch_passes": 20, "device_bytes": 21,
+        "batch_candidates": 22}
 FLAG = {"batch_graph": 0, "batch_gate_blocks": 1, "batch_step_cells": 2, "batch_block_threads": 3,
         "batch_claim_seeds": 4, "batch_order": 5, "batch_order_used": 6, "batch_device_dedup": 7,
         "ref_records": 8, "batch_lookahead": 9, "batch_list_cap": 10, "noalloc_tile_hits": 11,
-        "batch_max_positions": 12}
+        "batch_max_positions": 12, "batch_coarse": 13, "batch_coarse_used": 14}
 
 # Every symbol include/hspz.h declares (checked by tests/test_abi.py).
 EXPORTS = [

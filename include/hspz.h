@@ -169,6 +169,7 @@ int hspz_seed_and_filter_noalloc(hspz_engine *h, int slot, int rev, hspz_segment
                                         HSPZ_FLAG_BATCH_MAX_POSITIONS) */
 #define HSPZ_STAT_DEVICE_BYTES 21     /* device bytes the engine holds now (resident reference side and
                                         grow-only work buffers; PEAK_DEVICE_BYTES is the most it has held) */
+#define HSPZ_STAT_BATCH_CANDIDATES 22 /* hits of the last batched pass the coarse bound passed on */
 int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 
 /* Engine switches (the reference's pub flags, gpu/mod.rs:836-949, have no
@@ -217,6 +218,13 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           as many passes as this needs (a batch is never split; each is its
                                           own dedup scope, so the lists do not change). 0 (default): half of
                                           the free device memory's worth, between 2^20 and 2^26 */
+#define HSPZ_FLAG_BATCH_COARSE 13      /* 1 (default): in k-mer-major order with reference records, every hit
+                                          first meets a cheaper upper bound of its X-drop total (each cell
+                                          scored by the best score of its class: match, transition,
+                                          transversion; counted per 16-cell word) and only hits the bound
+                                          does not reject run the two-cell maps. 0: every hit runs them.
+                                          The lists do not change */
+#define HSPZ_FLAG_BATCH_COARSE_USED 14 /* read-only: the last batched pass ran the coarse bound */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

@@ -41,6 +41,14 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
             assert eng.get_flag("batch_order_used") == order
             for i, (a, g) in enumerate(zip(again, got)):
                 assert same_intervals(a, g), "chunk %d differs in hit order %d" % (i, order)
+        # k-mer-major order with and without the coarse bound ahead of the two-cell maps
+        eng.set_flag("batch_order", 1)
+        for coarse in (0, 1):
+            eng.set_flag("batch_coarse", coarse)
+            again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
+            assert eng.get_flag("batch_coarse_used") == coarse
+            for i, (a, g) in enumerate(zip(again, got)):
+                assert same_intervals(a, g), "chunk %d differs with the coarse bound %s" % (i, ("off", "on")[coarse])
         eng.set_flag("batch_order", 2)
         # the call split into device passes between batches gives the same lists:
         # every batch alone (a batch is never split, so 1 position per pass runs each
