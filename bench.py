@@ -231,6 +231,7 @@ def main():
         res_b, st_b = batched()
         log("batched step %d done" % i)
     elapsed = time.perf_counter() - t_start
+    cand_step = eng.stat("batch_candidates")  # of the last timed step
     g = {k: eng.stat(k) - g0[k] for k in g0}
     order_used = eng.get_flag("batch_order_used")
     if not order_used:
@@ -292,7 +293,11 @@ def main():
                 "avg_launch_us": round(g["gate_ns"] / max(g["gate_launches"], 1) / 1e3, 1),
                 "hits_per_s": round(g["gate_hits"] / secs, 1),
                 "deferred_frac": round(g["gate_deferred"] / max(g["gate_hits"], 1), 5),
-                "cells_per_hit": round(cells_per_hit, 3), "bytes_per_hit": round(per_hit, 2)}
+                "cells_per_hit": round(cells_per_hit, 3), "bytes_per_hit": round(per_hit, 2),
+                "coarse_bound": {"form": ("off", "one hit per lane", "two hits per lane")[
+                                     eng.get_flag("batch_coarse_used")],
+                                 "candidates_per_step": cand_step,
+                                 "candidate_frac": round(cand_step / max(st_b["hits"], 1), 5)}}
 
     out = {
         "metric": "query Mbp/s vs 1 Gbp target (batched; plain per-chunk beside it)",

@@ -478,7 +478,12 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                 const uint32_t *, const uint4 *, const uint4 *, const uint32_t *, uint32_t, uint32_t,
                                 unsigned long long *, SurvRec *, uint32_t *, uint32_t, SurvRec *, uint32_t *,
                                 uint32_t, uint32_t *);
-        const CGateFn cgate_fn = h->bgate_threads == 1024 ? k_bgate_coarse<1024> : k_bgate_coarse<512>;
+        // the paired form (two hits per lane, packed 16-bit sums) where the class scores fit
+        const bool pair = h->coarse_gate == 1 && h->coarse_fits16;
+        const CGateFn cgate_fn =
+            !pair ? (h->bgate_threads == 1024 ? k_bgate_coarse<1024> : k_bgate_coarse<512>)
+            : h->coarse_pv ? (h->bgate_threads == 1024 ? k_bgate_coarse2<1024, true> : k_bgate_coarse2<512, true>)
+                           : (h->bgate_threads == 1024 ? k_bgate_coarse2<1024, false> : k_bgate_coarse2<512, false>);
         XGateFn xgate_fn;
         if (h->bgate_threads == 1024)
             xgate_fn = h->sum_mode == SUM_RAW      ? k_bgate_exact<1024, SUM_RAW>
@@ -504,7 +509,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         }
         h->bgate_grid_used = grid;
         h->batch_order_used = kmer_order ? 1 : 0;
-        h->coarse_used = coarse;
+        h->coarse_used = !coarse ? 0 : pair ? 2 : 1;
         auto enqueue_gate = [&]() {
         if (coarse) {
             // coarse bound over every hit, then the two-cell maps over its candidates

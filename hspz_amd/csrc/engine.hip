@@ -97,8 +97,11 @@ struct hspz_engine {
     uint32_t list_cap = 0;      // HSPZ_FLAG_BATCH_LIST_CAP; 0 = sized from the batch
     uint32_t batch_max_positions = 0;  // HSPZ_FLAG_BATCH_MAX_POSITIONS; 0 = from free device memory
     uint64_t batch_passes_last = 0;    // device passes of the last hspz_filter_batch call
-    bool coarse_gate = true;           // HSPZ_FLAG_BATCH_COARSE: class-score bound ahead of the two-cell maps
-    bool coarse_used = false;          // the last batched pass ran it
+    int coarse_gate = 1;               // HSPZ_FLAG_BATCH_COARSE: class-score bound ahead of the two-cell maps
+                                       // (0 off, 1 paired form where the class scores fit, 2 single-hit form)
+    int coarse_used = 0;               // the form the last batched pass ran (0: none)
+    bool coarse_fits16 = false;        // every |class score| <= COARSE_PAIR_MAX_CLASS
+    bool coarse_pv = false;            // the transversion class adds to the positive part
     CoarseCoef coarse_cc = {0, 0, 0, 0, 0, 0};
     uint64_t last_cand = 0;            // candidates of the last batched pass
     int gate_blocks = 2048;
@@ -603,6 +606,9 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
             const int32_t pm = std::max(um, 0), pt = std::max(ut, 0), pv = std::max(uv, 0);
             // nw mismatches, nv of them transversions: nm = 16 - nw, nt = nw - nv
             h->coarse_cc = {16 * um, ut - um, uv - ut, 16 * pm, pt - pm, pv - pt};
+            const int32_t a = std::max({std::abs(um), std::abs(ut), std::abs(uv)});
+            h->coarse_fits16 = a <= COARSE_PAIR_MAX_CLASS;
+            h->coarse_pv = pv - pt != 0;
         }
         if (ok) {
             uint32_t tab[512];
@@ -884,7 +890,10 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
             return set_error(HSPZ_E_ARG, "list capacity must be 0 or in [64, 2^28]");
         h->list_cap = (uint32_t)value;
         break;
-    case HSPZ_FLAG_BATCH_COARSE: h->coarse_gate = value != 0; break;
+    case HSPZ_FLAG_BATCH_COARSE:
+        if (value < 0 || value > 2) return set_error(HSPZ_E_ARG, "coarse mode must be 0, 1 or 2");
+        h->coarse_gate = value;
+        break;
     case HSPZ_FLAG_BATCH_MAX_POSITIONS:
         if (value < 0) return set_error(HSPZ_E_ARG, "max positions must be 0 (automatic) or positive");
         h->batch_max_positions = (uint32_t)value;
@@ -925,8 +934,8 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_LIST_CAP: *value = (int)h->list_cap; break;
     case HSPZ_FLAG_NOALLOC_TILE_HITS: *value = (int)h->noalloc_tile_hits; break;
     case HSPZ_FLAG_BATCH_MAX_POSITIONS: *value = (int)h->batch_max_positions; break;
-    case HSPZ_FLAG_BATCH_COARSE: *value = h->coarse_gate ? 1 : 0; break;
-    case HSPZ_FLAG_BATCH_COARSE_USED: *value = h->coarse_used ? 1 : 0; break;
+    case HSPZ_FLAG_BATCH_COARSE: *value = h->coarse_gate; break;
+    case HSPZ_FLAG_BATCH_COARSE_USED: *value = h->coarse_used; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
     return 0;

@@ -1480,27 +1480,33 @@ struct CoarseCoef {
     int32_t s0, sw, sv, p0, pw, pv;
 };
 
-constexpr int CW_RIGHT = 4, CW_LEFT = 6;  // coarse windows: 64 right cells, 96 left cells
+constexpr int CW_RIGHT = 4, CW_LEFT = 8;  // coarse windows: 64 right cells, 128 left cells
 
+// The counts are at most 16 and the class scores small, so the products are
+// 24-bit multiply-adds (full rate; a 32-bit multiply is quarter rate).
 __device__ __forceinline__ void coarse_word(uint32_t r, uint32_t q, const CoarseCoef &cc, int32_t xdrop,
                                             int32_t &U, int32_t &mx, int32_t &B, bool &dropped) {
     const uint32_t x = r ^ q;
     const uint32_t tv = x & 0x55555555u;              // odd xor: transversion
     const uint32_t ts = (x >> 1) & ~x & 0x55555555u;  // xor 2: transition
     const int32_t nv = __popc(tv), nw = __popc(ts) + nv;
-    const int32_t bnd = U + cc.p0 + cc.pw * nw + cc.pv * nv;
-    B = dropped ? B : max(B, bnd);
-    U += cc.s0 + cc.sw * nw + cc.sv * nv;
+    const int32_t bnd = U + cc.p0 + __mul24(cc.pw, nw) + __mul24(cc.pv, nv);
+    B = max(B, dropped ? (int32_t)0x80000000 : bnd);
+    U += cc.s0 + __mul24(cc.sw, nw) + __mul24(cc.sv, nv);
     mx = max(mx, U);
     dropped = dropped || mx - U > xdrop;
 }
 
+// Candidate record: (R, reference-record index, query-record index |
+// variant << CAND_VAR_SHIFT, 0), everything k_bgate_exact loads directly.
+constexpr uint32_t CAND_VAR_SHIFT = 32 - GV_SHIFT;
+
 // The coarse gate over the k-mer-major walk (KmerSource, reference records
 // required): every hit with clean windows is bounded over CW_RIGHT right and
 // CW_LEFT left words; a hit not rejected by the bound (or whose u-process
-// outlives a window) becomes a candidate (R, reference-record index, grouped
-// seed) for k_bgate_exact; a hit whose windows meet an exception is deferred
-// as in k_bgate_kmer. One iteration per hit.
+// outlives a window) becomes a candidate for k_bgate_exact; a hit whose
+// windows meet an exception is deferred as in k_bgate_kmer. One iteration per
+// hit.
 template <int BT>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_coarse(
     const QRec *__restrict__ recs, const uint32_t *__restrict__ skm, const uint32_t *__restrict__ sval,
@@ -1547,17 +1553,17 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
             const uint4 w0 = rr[0], w1 = rr[1], w2 = rr[2];
             A0[0] = w0.x, A0[1] = w0.y, A0[2] = w0.z, A0[3] = w0.w;
             LA0[0] = w1.x, LA0[1] = w1.y, LA0[2] = w1.z, LA0[3] = w1.w;
-            LA0[4] = w2.x, LA0[5] = w2.y;
+            LA0[4] = w2.x, LA0[5] = w2.y, LA0[6] = w2.z, LA0[7] = w2.w;
         }
         iidx = fresh_r ? ridx : iidx;
         // every lane with a hit has a fresh one: a hit takes one iteration
-        uint32_t var;
-        const uint32_t p = s_cur;
-        const uint4 *rec = src.record(recs, has ? p : 0u, var);
+        const uint32_t gv = src.sval[has ? s_cur : 0u];
+        const uint32_t var = gv & ((1u << GV_SHIFT) - 1), qi = gv >> GV_SHIFT;
+        const uint4 *rec = (const uint4 *)(recs + qi);
         const uint4 v0 = rec[0], v1 = rec[1], v2 = rec[2], v3 = rec[3];
         s_cur += has ? 1u : 0u;
         const uint32_t B[CW_RIGHT] = {v0.x, v0.y, v0.z, v0.w};
-        const uint32_t LB[CW_LEFT] = {v1.x, v1.y, v1.z, v1.w, v2.x, v2.y};
+        const uint32_t LB[CW_LEFT] = {v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
         int32_t Ur = 0, mxr = 0, Br = 0, Ul = 0, mxl = 0, Bl = 0;
         bool dr = false, dl = false;
 #pragma unroll
@@ -1568,19 +1574,151 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
         const bool ok = rok && v3.z != 0;
         const bool defer = has && !ok;
         const bool pass = has && ok && (!(dr && dl) || packed_keep(max(Br, 0) + max(Bl, 0), thresh));
-        src.sq.push(pass, make_uint4(R, iidx, p, 0u), err);
+        src.sq.push(pass, make_uint4(R, iidx, qi | var << CAND_VAR_SHIFT, 0u), err);
         src.defer(defer, v3.y + var, R, v3.x);
         has = false;
     }
     src.finish();
 }
 
-// Hit source over the coarse gate's candidates (R, reference-record index,
-// grouped seed): waves claim `claim` candidates at a time from a grid-wide
-// counter and hand them to idle lanes, each one hit on its own item.
+// ------------------------------------------------- coarse bound, paired hits
+//
+// The same bound for two hits of one item at once, in packed 16-bit
+// arithmetic: a lane takes its item's next two query records (hit a in the
+// low halves, hit b in the high halves; the reference words are the item's,
+// shared), so every word's sums, maximum and drop test serve both hits in
+// one instruction and the per-iteration bookkeeping is paid once per two
+// hits. The bound B is taken over every window word, drop or not (a larger
+// value than the maximum up to the drop, so still an upper bound of the true
+// maximum). Values in range: with every class score within +-a, a side's
+// prefix sums over 8 words lie within +-128a and U - max within [-256a, 16a],
+// so a <= 127 keeps every compared value within 16 bits (the host checks it);
+// sums and products wrap like the 16-bit ring they are, exact wherever the
+// result is in range.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ s16x2 splat16(int32_t v) { return as_s16x2(((uint32_t)v & 0xffffu) * 0x10001u); }
+
+constexpr int32_t COARSE_PAIR_MAX_CLASS = 127;  // largest |class score| the packed form holds
+
+struct CoarsePair {
+    s16x2 s0, sw, sv, pw, pv;
+};
+
+// One 16-cell word of two chains (query words qa and qb against reference
+// word r); U, mx, mg (minimum of U - running maximum) and B as in
+// coarse_word, per half. PV: the transversion class adds to the positive part.
+template <bool PV>
+__device__ __forceinline__ void coarse_pair_word(uint32_t r, uint32_t qa, uint32_t qb, const CoarsePair &cp,
+                                                 s16x2 &U, s16x2 &mx, s16x2 &mg, s16x2 &B) {
+    constexpr uint32_t M = 0x55555555u;
+    const uint32_t xa = r ^ qa, xb = r ^ qb;
+    const uint32_t nvp = (uint32_t)__popc(xa & M) | (uint32_t)__popc(xb & M) << 16;
+    const uint32_t tsa = (xa >> 1) & ~xa & M, tsb = (xb >> 1) & ~xb & M;
+    const uint32_t nwp = (uint32_t)__popc(tsa) + ((uint32_t)__popc(tsb) << 16) + nvp;
+    const s16x2 nv = as_s16x2(nvp), nw = as_s16x2(nwp);
+    s16x2 bnd = U + cp.pw * nw;
+    if constexpr (PV) bnd = bnd + cp.pv * nv;
+    B = __builtin_elementwise_max(B, bnd);
+    const s16x2 Un = U + cp.s0 + cp.sw * nw + cp.sv * nv;
+    mg = __builtin_elementwise_min(mg, Un - mx);
+    mx = __builtin_elementwise_max(mx, Un);
+    U = Un;
+}
+
+// k_bgate_coarse with two hits per lane and iteration (the second half idle
+// when the item has one record left); same arguments, lists and decisions.
+template <int BT, bool PV>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_coarse2(
+    const QRec *__restrict__ recs, const uint32_t *__restrict__ skm, const uint32_t *__restrict__ sval,
+    const uint32_t *__restrict__ goff, uint32_t nk, const uint32_t *__restrict__ index,
+    const uint32_t *__restrict__ pos, const uint32_t *__restrict__ clean, const uint4 *__restrict__ rrec,
+    uint32_t seed_size, uint32_t claim, unsigned long long *__restrict__ work, CoarseCoef cc, int32_t xdrop,
+    int32_t thresh, uint4 *__restrict__ cand, uint32_t *__restrict__ cand_n, uint32_t cap_cand,
+    SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow, uint32_t *__restrict__ err) {
+    __shared__ uint4 s_q[2][BT / WAVE][WQ_CAP];
+    KmerSource src;
+    src.skm = skm;
+    src.sval = sval;
+    src.goff = goff;
+    src.index = index;
+    src.pos = pos;
+    src.clean = clean;
+    src.work = work;
+    src.total = __builtin_amdgcn_readfirstlane(goff[nk]);
+    src.claim = claim;
+    src.seed_size = seed_size;
+    src.surv = (SurvRec *)cand;
+    src.surv_n = cand_n;
+    src.cap_surv = cap_cand;
+    src.slow = slow;
+    src.slow_n = slow_n;
+    src.cap_slow = cap_slow;
+    src.err = err;
+    src.set_lists(s_q[0][threadIdx.x / WAVE], s_q[1][threadIdx.x / WAVE]);
+    src.init();
+    const CoarsePair cp = {splat16(cc.s0), splat16(cc.sw), splat16(cc.sv), splat16(cc.pw), splat16(cc.pv)};
+    const s16x2 zero = splat16(0), lowest = splat16(-32768);
+    bool has = false, item = false, rok = false;
+    uint32_t R = 0, s_cur = 0, s_end = 0, iidx = 0;
+    uint32_t A0[CW_RIGHT], LA0[CW_LEFT];
+#pragma unroll
+    for (int k = 0; k < CW_RIGHT; k++) A0[k] = 0;
+#pragma unroll
+    for (int k = 0; k < CW_LEFT; k++) LA0[k] = 0;
+    for (;;) {
+        bool fresh_r = false, fresh_q = false;
+        uint32_t ridx = 0;
+        src.refill(has, item, fresh_r, fresh_q, R, s_cur, s_end, rok, ridx);
+        if (__ballot(has) == 0) break;
+        if (fresh_r && rok) {
+            const uint4 *rr = rrec + 3 * (size_t)ridx;
+            const uint4 w0 = rr[0], w1 = rr[1], w2 = rr[2];
+            A0[0] = w0.x, A0[1] = w0.y, A0[2] = w0.z, A0[3] = w0.w;
+            LA0[0] = w1.x, LA0[1] = w1.y, LA0[2] = w1.z, LA0[3] = w1.w;
+            LA0[4] = w2.x, LA0[5] = w2.y, LA0[6] = w2.z, LA0[7] = w2.w;
+        }
+        iidx = fresh_r ? ridx : iidx;
+        // a lane with a hit has a fresh one; hit b is its item's next record, if any
+        const bool two = has && s_cur + 1 < s_end;
+        const uint32_t ga = src.sval[has ? s_cur : 0u], gb = src.sval[two ? s_cur + 1 : 0u];
+        s_cur += two ? 2u : has ? 1u : 0u;
+        const uint32_t va = ga & ((1u << GV_SHIFT) - 1), qa = ga >> GV_SHIFT;
+        const uint32_t vb = gb & ((1u << GV_SHIFT) - 1), qb = gb >> GV_SHIFT;
+        const uint4 *ra = (const uint4 *)(recs + qa), *rb = (const uint4 *)(recs + qb);
+        const uint4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3];
+        const uint4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3];
+        const uint32_t QA[CW_RIGHT] = {a0.x, a0.y, a0.z, a0.w}, QB[CW_RIGHT] = {b0.x, b0.y, b0.z, b0.w};
+        const uint32_t LQA[CW_LEFT] = {a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+        const uint32_t LQB[CW_LEFT] = {b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+        s16x2 Ur = zero, mxr = zero, mgr = zero, Br = lowest, Ul = zero, mxl = zero, mgl = zero, Bl = lowest;
+#pragma unroll
+        for (int k = 0; k < CW_LEFT; k++) {
+            if (k < CW_RIGHT) coarse_pair_word<PV>(A0[k], QA[k], QB[k], cp, Ur, mxr, mgr, Br);
+            coarse_pair_word<PV>(LA0[k], LQA[k], LQB[k], cp, Ul, mxl, mgl, Bl);
+        }
+        const int32_t X = xdrop;
+        const bool da = mgr.x < -X && mgl.x < -X, db = mgr.y < -X && mgl.y < -X;
+        const int32_t ta = max(Br.x + cc.p0, 0) + max(Bl.x + cc.p0, 0);
+        const int32_t tb = max(Br.y + cc.p0, 0) + max(Bl.y + cc.p0, 0);
+        const bool oka = rok && a3.z != 0, okb = rok && b3.z != 0;
+        const bool pa = has && oka && (!da || packed_keep(ta, thresh));
+        const bool pb = two && okb && (!db || packed_keep(tb, thresh));
+        src.sq.push(pa, make_uint4(R, iidx, qa | va << CAND_VAR_SHIFT, 0u), err);
+        src.sq.push(pb, make_uint4(R, iidx, qb | vb << CAND_VAR_SHIFT, 0u), err);
+        src.defer(has && !oka, a3.y + va, R, a3.x);
+        src.defer(two && !okb, b3.y + vb, R, b3.x);
+        has = false;
+    }
+    src.finish();
+}
+
+// Hit source over the coarse gate's candidates: waves claim `claim`
+// candidates at a time from a grid-wide counter and hand them to idle lanes,
+// each one hit on its own item. A lane's s_cur is the candidate's query-record
+// index with the variant in its top bits.
 struct CandSource : GateOut {
     const uint4 *list;
-    const uint32_t *sval;
     unsigned long long *work;
     uint32_t n, claim, c_cur, c_end;
 
@@ -1597,9 +1735,8 @@ struct CandSource : GateOut {
     }
     __device__ __forceinline__ void init() { c_cur = c_end = 0; }
     __device__ __forceinline__ const uint4 *record(const QRec *recs, uint32_t p, uint32_t &var) const {
-        const uint32_t gv = sval[p];
-        var = gv & ((1u << GV_SHIFT) - 1);
-        return (const uint4 *)(recs + (gv >> GV_SHIFT));
+        var = p >> CAND_VAR_SHIFT;
+        return (const uint4 *)(recs + (p & ((1u << CAND_VAR_SHIFT) - 1)));
     }
     __device__ __forceinline__ void refill(bool &has, bool &item, bool &fresh_r, bool &fresh_q, uint32_t &R,
                                            uint32_t &s_cur, uint32_t &s_end, bool &rok, uint32_t &ridx) {
@@ -1640,8 +1777,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     load_map_table(s_map, pairtab, BT);
     CandSource src;
     src.list = cand;
-    src.sval = sval;
     src.work = work;
+    (void)sval;
     src.n = __builtin_amdgcn_readfirstlane(min(*cand_n, cap_cand));
     src.claim = claim;
     src.surv = surv;

@@ -222,9 +222,12 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           first meets a cheaper upper bound of its X-drop total (each cell
                                           scored by the best score of its class: match, transition,
                                           transversion; counted per 16-cell word) and only hits the bound
-                                          does not reject run the two-cell maps. 0: every hit runs them.
-                                          The lists do not change */
-#define HSPZ_FLAG_BATCH_COARSE_USED 14 /* read-only: the last batched pass ran the coarse bound */
+                                          does not reject run the two-cell maps; two hits per lane in packed
+                                          16-bit sums when every class score is within +-127, else one hit
+                                          per lane in 32-bit sums. 2: always the one-hit form. 0: every hit
+                                          runs the two-cell maps. The lists do not change */
+#define HSPZ_FLAG_BATCH_COARSE_USED 14 /* read-only: the coarse bound form the last batched pass ran:
+                                          0 none, 1 one hit per lane, 2 two hits per lane */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

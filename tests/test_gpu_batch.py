@@ -41,14 +41,16 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
             assert eng.get_flag("batch_order_used") == order
             for i, (a, g) in enumerate(zip(again, got)):
                 assert same_intervals(a, g), "chunk %d differs in hit order %d" % (i, order)
-        # k-mer-major order with and without the coarse bound ahead of the two-cell maps
+        # k-mer-major order without the coarse bound ahead of the two-cell maps, with
+        # it in its paired form (HOXD70's class scores fit 16 bits) and single-hit form
         eng.set_flag("batch_order", 1)
-        for coarse in (0, 1):
+        for coarse, used in ((0, 0), (1, 2), (2, 1)):
             eng.set_flag("batch_coarse", coarse)
             again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
-            assert eng.get_flag("batch_coarse_used") == coarse
+            assert eng.get_flag("batch_coarse_used") == used
             for i, (a, g) in enumerate(zip(again, got)):
-                assert same_intervals(a, g), "chunk %d differs with the coarse bound %s" % (i, ("off", "on")[coarse])
+                assert same_intervals(a, g), "chunk %d differs with coarse mode %d" % (i, coarse)
+        eng.set_flag("batch_coarse", 1)
         eng.set_flag("batch_order", 2)
         # the call split into device passes between batches gives the same lists:
         # every batch alone (a batch is never split, so 1 position per pass runs each
@@ -491,3 +493,42 @@ def test_batch_score_sum_forms(gpu_available, scale, xdrop):
             got, _ = eng.filter_batch(chunks)
             for i, (g, w) in enumerate(zip(got, want)):
                 assert same_intervals(g, w), "chunk %d differs at step %d threads %d" % (i, step, threads)
+
+
+@pytest.mark.parametrize("name,acgt,xdrop,used", [
+    ("hoxd70", [91, -114, -31, -123, -114, 100, -125, -31, -31, -125, 100, -114, -123, -31, -114, 91], 910, 2),
+    # a transversion pair scoring +20: the transversion class adds to the positive part
+    ("positive transversion", [91, 20, -31, -123, 20, 100, -125, -31, -31, -125, 100, -114, -123, -31, -114, 91],
+     910, 2),
+    # x2: class scores beyond 127, so the single-hit form even when the paired one is asked for
+    ("hoxd70 x2", [2 * v for v in [91, -114, -31, -123, -114, 100, -125, -31, -31, -125, 100, -114, -123, -31,
+                                   -114, 91]], 1820, 1),
+])
+def test_batch_coarse_forms(gpu_available, name, acgt, xdrop, used):
+    """The coarse bound's two forms (two hits per lane in packed 16-bit sums;
+    one hit per lane in 32-bit sums) and no bound, against the oracle, on
+    matrices that select each form and both positive-part variants."""
+    _say("case batch: coarse bound forms, %s" % name)
+    thresh = 3000 * (2 if "x2" in name else 1)
+    target = synth.random_bases(0xC0A5 + len(name), 200_000).tobytes()
+    blocks = _chunks_from(target, 10, 0xC0 + len(name), lo=500, hi=15_000, div=(0.0, 0.2))
+    sub = O.sub_mat("", xdrop, acgt)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    want, hsps = [], 0
+    for f, r in blocks:
+        o = O.Oracle(target, f, r, sub, xdrop=xdrop, hspthresh=thresh, max_hits=16711680, index=(index, pos))
+        res, st, _ = o.run_pass(threads=16)
+        want.append(res)
+        hsps += st["hsps"]
+    assert hsps > 10
+    chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
+    with E.Engine(index, pos, S.encode(target), sub, xdrop=xdrop, hspthresh=thresh, max_hits=16711680) as eng:
+        eng.set_flag("batch_order", 1)
+        for threads in (1024, 512):
+            eng.set_flag("batch_block_threads", threads)
+            for coarse, u in ((1, used), (2, 1), (0, 0)):
+                eng.set_flag("batch_coarse", coarse)
+                got, _ = eng.filter_batch(chunks)
+                assert eng.get_flag("batch_coarse_used") == u
+                for i, (g, w) in enumerate(zip(got, want)):
+                    assert same_intervals(g, w), "chunk %d differs in coarse mode %d, %d threads" % (i, coarse, threads)
