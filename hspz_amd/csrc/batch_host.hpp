@@ -485,7 +485,14 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             : h->coarse_pv ? (h->bgate_threads == 1024 ? k_bgate_coarse2<1024, true> : k_bgate_coarse2<512, true>)
                            : (h->bgate_threads == 1024 ? k_bgate_coarse2<1024, false> : k_bgate_coarse2<512, false>);
         XGateFn xgate_fn;
-        if (h->bgate_threads == 1024)
+        // the candidates' gate with its loads issued ahead (512-thread blocks) unless the lookahead is off
+        const bool xahead = h->lookahead;
+        const int xthreads = xahead ? 512 : h->bgate_threads;
+        if (xahead)
+            xgate_fn = h->sum_mode == SUM_RAW      ? k_bgate_exact_ahead<SUM_RAW>
+                       : h->sum_mode == SUM_TREE16 ? k_bgate_exact_ahead<SUM_TREE16>
+                                                   : k_bgate_exact_ahead<SUM_TREE8>;
+        else if (h->bgate_threads == 1024)
             xgate_fn = h->sum_mode == SUM_RAW      ? k_bgate_exact<1024, SUM_RAW>
                        : h->sum_mode == SUM_TREE16 ? k_bgate_exact<1024, SUM_TREE16>
                                                    : k_bgate_exact<1024, SUM_TREE8>;
@@ -494,18 +501,23 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                        : h->sum_mode == SUM_TREE16 ? k_bgate_exact<512, SUM_TREE16>
                                                    : k_bgate_exact<512, SUM_TREE8>;
         int grid = h->bgate_blocks, xgrid = h->bgate_blocks;
+        if (coarse && xahead) {  // the candidates' gate keeps its resident grid
+            int xper_cu = 0;
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&xper_cu, xgate_fn, xthreads, 0));
+            xgrid = std::max(xper_cu, 1) * h->num_cus;
+        }
         if (grid == 0) {  // resident grid: occupancy of this instantiation x CUs
             int per_cu = 0, xper_cu = 0;
             if (coarse) {
                 HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cgate_fn, h->bgate_threads, 0));
-                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&xper_cu, xgate_fn, h->bgate_threads, 0));
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&xper_cu, xgate_fn, xthreads, 0));
             } else if (kmer_order) {
                 HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kgate_fn, h->bgate_threads, 0));
             } else {
                 HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gate_fn, h->bgate_threads, 0));
             }
             grid = std::max(per_cu, 1) * h->num_cus;
-            xgrid = std::max(xper_cu, 1) * h->num_cus;
+            if (!(coarse && xahead)) xgrid = std::max(xper_cu, 1) * h->num_cus;
         }
         h->bgate_grid_used = grid;
         h->batch_order_used = kmer_order ? 1 : 0;
@@ -520,7 +532,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                (unsigned long long *)(ctr + CTR_WORK), h->coarse_cc, h->xdrop, h->hspthresh,
                                B.cand.as<uint4>(), ctr + CTR_CAND, bs->cap_cand, B.slow.as<SurvRec>(),
                                ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
-            hipLaunchKernelGGL(xgate_fn, dim3(xgrid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
+            hipLaunchKernelGGL(xgate_fn, dim3(xgrid), dim3(xthreads), 0, h->stream, h->ref2.as<uint32_t>(),
                                h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
                                B.kvalo.as<uint32_t>(), h->rrec.as<uint4>(), B.cand.as<uint4>(), ctr + CTR_CAND,
                                bs->cap_cand, (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK2),
@@ -591,7 +603,9 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                (uint64_t)grid | (uint64_t)h->bgate_claim << 32, (uint64_t)p->transitions, nt, kt,
                                cap_tiles, (uint64_t)h->bgate_step | (uint64_t)h->bgate_threads << 32,
                                (uint64_t)(uintptr_t)h->sub.p, s.kmer_size, (uint64_t)(uintptr_t)h->ref.p,
-                               (uint64_t)kmer_order, ntk, (uint64_t)(uintptr_t)(const void *)kgate_fn};
+                               (uint64_t)kmer_order, ntk, (uint64_t)(uintptr_t)(const void *)kgate_fn,
+                               (uint64_t)(uintptr_t)(const void *)cgate_fn,
+                               (uint64_t)(uintptr_t)(const void *)xgate_fn | (uint64_t)xthreads << 48};
             static_assert(sizeof vals <= sizeof key.vals, "graph key value slots");
             memcpy(key.vals, vals, sizeof vals);
             memcpy(key.shape, s.pos, sizeof s.pos);

@@ -107,7 +107,7 @@ struct hspz_engine {
     int gate_blocks = 2048;
     // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
     int bgate_blocks = 0;  // persistent grid; 0 = as many blocks as are resident at once
-    int bgate_step = 96, bgate_threads = 1024, bgate_claim = 64;  // the autotuner's usual pick
+    int bgate_step = 96, bgate_threads = 1024, bgate_claim = 256;  // the autotuner's usual pick
     int batch_order = 2;                    // 0 seed-major, 1 k-mer-major, 2 by batch size
     int batch_order_used = 0;               // order of the last batched pass
     DevBuf clean;                           // pos_table-aligned clean-window bits (k-mer-major gate)

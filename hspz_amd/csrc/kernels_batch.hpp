@@ -1480,7 +1480,8 @@ struct CoarseCoef {
     int32_t s0, sw, sv, p0, pw, pv;
 };
 
-constexpr int CW_RIGHT = 4, CW_LEFT = 8;  // coarse windows: 64 right cells, 128 left cells
+constexpr int CW_RIGHT = 4, CW_LEFT = 7;  // coarse windows: 64 right cells, 112 left cells
+static_assert(CW_RIGHT <= (int)G2_RIGHT_WORDS && CW_LEFT <= (int)G2_LEFT_WORDS, "windows within the records");
 
 // The counts are at most 16 and the class scores small, so the products are
 // 24-bit multiply-adds (full rate; a 32-bit multiply is quarter rate).
@@ -1538,11 +1539,11 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.init();
     bool has = false, item = false, rok = false;
     uint32_t R = 0, s_cur = 0, s_end = 0, iidx = 0;
-    uint32_t A0[CW_RIGHT], LA0[CW_LEFT];
+    uint32_t A0[CW_RIGHT], LA0[G2_LEFT_WORDS];
 #pragma unroll
     for (int k = 0; k < CW_RIGHT; k++) A0[k] = 0;
 #pragma unroll
-    for (int k = 0; k < CW_LEFT; k++) LA0[k] = 0;
+    for (int k = 0; k < (int)G2_LEFT_WORDS; k++) LA0[k] = 0;
     for (;;) {
         bool fresh_r = false, fresh_q = false;
         uint32_t ridx = 0;
@@ -1563,7 +1564,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
         const uint4 v0 = rec[0], v1 = rec[1], v2 = rec[2], v3 = rec[3];
         s_cur += has ? 1u : 0u;
         const uint32_t B[CW_RIGHT] = {v0.x, v0.y, v0.z, v0.w};
-        const uint32_t LB[CW_LEFT] = {v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
+        const uint32_t LB[G2_LEFT_WORDS] = {v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
         int32_t Ur = 0, mxr = 0, Br = 0, Ul = 0, mxl = 0, Bl = 0;
         bool dr = false, dl = false;
 #pragma unroll
@@ -1661,11 +1662,11 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     const s16x2 zero = splat16(0), lowest = splat16(-32768);
     bool has = false, item = false, rok = false;
     uint32_t R = 0, s_cur = 0, s_end = 0, iidx = 0;
-    uint32_t A0[CW_RIGHT], LA0[CW_LEFT];
+    uint32_t A0[CW_RIGHT], LA0[G2_LEFT_WORDS];
 #pragma unroll
     for (int k = 0; k < CW_RIGHT; k++) A0[k] = 0;
 #pragma unroll
-    for (int k = 0; k < CW_LEFT; k++) LA0[k] = 0;
+    for (int k = 0; k < (int)G2_LEFT_WORDS; k++) LA0[k] = 0;
     for (;;) {
         bool fresh_r = false, fresh_q = false;
         uint32_t ridx = 0;
@@ -1689,8 +1690,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
         const uint4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3];
         const uint4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3];
         const uint32_t QA[CW_RIGHT] = {a0.x, a0.y, a0.z, a0.w}, QB[CW_RIGHT] = {b0.x, b0.y, b0.z, b0.w};
-        const uint32_t LQA[CW_LEFT] = {a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
-        const uint32_t LQB[CW_LEFT] = {b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+        const uint32_t LQA[G2_LEFT_WORDS] = {a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+        const uint32_t LQB[G2_LEFT_WORDS] = {b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
         s16x2 Ur = zero, mxr = zero, mgr = zero, Br = lowest, Ul = zero, mxl = zero, mgl = zero, Bl = lowest;
 #pragma unroll
         for (int k = 0; k < CW_LEFT; k++) {
@@ -1721,19 +1722,58 @@ struct CandSource : GateOut {
     const uint4 *list;
     unsigned long long *work;
     uint32_t n, claim, c_cur, c_end;
+    bool dry;
 
     __device__ __forceinline__ bool take() {
+        if (dry) return false;
         uint32_t leader = (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1);
         unsigned long long v = 0;
         if (lane_id() == leader) v = atomicAdd(work, (unsigned long long)claim);
         const uint64_t start = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
                                (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
-        if (start >= (uint64_t)n) return false;
+        if (start >= (uint64_t)n) {
+            dry = true;
+            return false;
+        }
         c_cur = (uint32_t)start;
         c_end = start + claim < (uint64_t)n ? (uint32_t)(start + claim) : n;
         return true;
     }
-    __device__ __forceinline__ void init() { c_cur = c_end = 0; }
+    // A fixed share of the list instead of claims: wave w of `waves` takes
+    // entries [n w / waves, n (w + 1) / waves) and no atomic (a claim's
+    // returned value would make the wave wait on every load in flight).
+    __device__ __forceinline__ void share(uint32_t w, uint32_t waves) {
+        c_cur = (uint32_t)((uint64_t)n * w / waves);
+        c_end = (uint32_t)((uint64_t)n * (w + 1) / waves);
+        dry = true;
+    }
+    __device__ __forceinline__ void init() {
+        c_cur = c_end = 0;
+        dry = false;
+    }
+    // Hands each lane with `want` its next candidate entry; whether it got one.
+    // (The load is issued after the loop: inside it, the compiler would wait
+    // for every load in flight before each pass.)
+    __device__ __forceinline__ bool claim_entry(bool want, uint4 &e) {
+        bool got = false;
+        uint32_t at = 0;
+        uint64_t need = __ballot(want);
+        while (need) {
+            if (c_cur >= c_end) {
+                if (!take()) break;
+                continue;
+            }
+            const uint32_t avail = c_end - c_cur, rank = mbcnt(need);
+            if (want && !got && rank < avail) {
+                at = c_cur + rank;
+                got = true;
+            }
+            c_cur += min((uint32_t)__popcll(need), avail);
+            need = __ballot(want && !got);
+        }
+        if (got) e = list[at];
+        return got;
+    }
     __device__ __forceinline__ const uint4 *record(const QRec *recs, uint32_t p, uint32_t &var) const {
         var = p >> CAND_VAR_SHIFT;
         return (const uint4 *)(recs + (p & ((1u << CAND_VAR_SHIFT) - 1)));
@@ -1791,6 +1831,142 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.set_lists(nullptr, s_dq[threadIdx.x / WAVE]);
     src.init();
     kmer_gate_merged<2, SUMS>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+}
+
+// The merged gate (RW = 2) over the coarse gate's candidates with its global
+// loads issued ahead, in three stages per lane: a claimed candidate entry
+// (loaded in one iteration), its next hit's reference and query records
+// (loads issued in the next, from the landed entry), and the current hit
+// (promoted from those records once the lane is free). Every load is issued
+// one iteration before its data is used, so the counter wait at the top of an
+// iteration finds them landed while the wave computed. Cells, decisions and
+// lists are those of kmer_gate_merged<2, SUMS> over CandSource; a finished
+// hit's results are appended at the top of the next iteration.
+template <int SUMS>
+__device__ __forceinline__ void cand_gate_ahead(CandSource &src, const uint4 *__restrict__ rrec,
+                                                const QRec *__restrict__ recs, const uint8_t *s_map,
+                                                int32_t xdrop, int32_t thresh) {
+    constexpr int RW = 2;
+    const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
+    // current hit
+    bool has = false;
+    HitSides st;
+    uint32_t R = 0, Q = 0, tag = 0;
+    uint32_t A0[G2_RIGHT_WORDS], LA0[G2_LEFT_WORDS], B[G2_RIGHT_WORDS], LB[G2_LEFT_WORDS];
+    // next hit: its records (loads issued last iteration)
+    bool nxt = false;
+    uint32_t nR = 0, nvar = 0;
+    const uint4 zero4 = make_uint4(0, 0, 0, 0);
+    uint4 T0 = zero4, T1 = zero4, T2 = zero4, P0 = zero4, P1 = zero4, P2 = zero4, P3 = zero4;
+    // claimed entry (load issued last iteration)
+    bool ent = false;
+    uint4 E = zero4;
+    // the hit finished in the previous iteration
+    bool e_pass = false, e_defer = false;
+    uint32_t eR = 0, eQ = 0, etag = 0;
+#pragma unroll
+    for (int j = 0; j < (int)G2_RIGHT_WORDS; j++) A0[j] = B[j] = 0;
+#pragma unroll
+    for (int j = 0; j < (int)G2_LEFT_WORDS; j++) LA0[j] = LB[j] = 0;
+
+    for (;;) {
+        // The records' pad word and the entry's last word are read nowhere;
+        // keeping them live to here stops the compiler from reusing their
+        // registers while the loads are in flight (which would wait on them).
+        asm volatile("" ::"v"(P3.w), "v"(E.w));
+        src.emit(true, e_pass, etag, eR, eQ);
+        src.defer(e_defer, etag, eR, eQ);
+        e_pass = e_defer = false;
+
+        // 1. a free lane takes its next hit
+        bool defer = false;
+        if (!has && nxt) {
+            A0[0] = T0.x, A0[1] = T0.y, A0[2] = T0.z, A0[3] = T0.w;
+            LA0[0] = T1.x, LA0[1] = T1.y, LA0[2] = T1.z, LA0[3] = T1.w;
+            LA0[4] = T2.x, LA0[5] = T2.y, LA0[6] = T2.z, LA0[7] = T2.w;
+            B[0] = P0.x, B[1] = P0.y, B[2] = P0.z, B[3] = P0.w;
+            LB[0] = P1.x, LB[1] = P1.y, LB[2] = P1.z, LB[3] = P1.w;
+            LB[4] = P2.x, LB[5] = P2.y, LB[6] = P2.z, LB[7] = P2.w;
+            R = nR;
+            Q = P3.x;
+            tag = P3.y + nvar;
+            has = true;
+            nxt = false;
+            st.start();
+            if (P3.z == 0) defer = true;
+        }
+        // 2. the landed entry's records, for the lane's next hit
+        if (!nxt && ent) {
+            const uint4 *rr = rrec + 3 * (size_t)E.y;
+            T0 = rr[0], T1 = rr[1], T2 = rr[2];
+            uint32_t var;
+            const uint4 *rec = src.record(recs, E.z, var);
+            P0 = rec[0], P1 = rec[1], P2 = rec[2], P3 = rec[3];
+            nR = E.x;
+            nvar = var;
+            nxt = true;
+            ent = false;
+        }
+        // 3. a lane without an entry claims one
+        if (src.claim_entry(!ent, E)) ent = true;
+        if (__ballot(has || nxt || ent) == 0) break;
+
+        // 4. the current hit's next stretches
+        bool done = false, pass = false;
+        if (has && !defer) {
+            int32_t er = st.rcont ? st.er_keep : xdrop, cr = st.rcont ? st.cr_keep : 0, eminr = er;
+            map_words<RW, G2_RIGHT_WORDS, SUMS>(s_map, A0, B, st.rcont, off_fwd, er, cr, eminr);
+            int32_t el = st.cont ? st.el_keep : xdrop, cl = st.cont ? st.cl_keep : 0, eminl = el;
+            map_stretch<8, SUMS>(s_map, LA0, LB, st.cont, off_rev, el, cl, eminl);
+            st.decide<RW>(er, side_sum<SUMS>(cr), eminr, el, side_sum<SUMS>(cl), eminl, xdrop, thresh, defer, done,
+                          pass);
+        }
+        if (has && (done || defer)) {
+            e_pass = done && pass;
+            e_defer = defer;
+            eR = R;
+            eQ = Q;
+            etag = tag;
+            has = false;
+            st.cont = st.rcont = false;
+        }
+    }
+    src.finish();
+}
+
+// cand_gate_ahead over the coarse gate's candidates: 512-thread blocks, four
+// waves per SIMD (the 128 VGPRs its in-flight records need); same arguments
+// and lists as k_bgate_exact.
+template <int SUMS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_bgate_exact_ahead(
+    const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh,
+    const QRec *__restrict__ recs, const uint32_t *__restrict__ sval, const uint4 *__restrict__ rrec,
+    const uint4 *__restrict__ cand, const uint32_t *__restrict__ cand_n, uint32_t cap_cand, uint32_t claim,
+    unsigned long long *__restrict__ work, SurvRec *__restrict__ surv, uint32_t *__restrict__ surv_n,
+    uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow,
+    uint32_t *__restrict__ err) {
+    __shared__ uint32_t s_map[G2_MAP_WORDS];
+    __shared__ uint4 s_dq[512 / WAVE][WQ_CAP];
+    load_map_table(s_map, pairtab, 512);
+    CandSource src;
+    src.list = cand;
+    src.work = work;
+    (void)rplane;
+    (void)sval;
+    src.n = __builtin_amdgcn_readfirstlane(min(*cand_n, cap_cand));
+    src.claim = claim;
+    src.surv = surv;
+    src.surv_n = surv_n;
+    src.cap_surv = cap_surv;
+    src.slow = slow;
+    src.slow_n = slow_n;
+    src.cap_slow = cap_slow;
+    src.err = err;
+    src.set_lists(nullptr, s_dq[threadIdx.x / WAVE]);
+    src.init();
+    (void)work;
+    src.share(blockIdx.x * (512 / WAVE) + threadIdx.x / WAVE, gridDim.x * (512 / WAVE));
+    cand_gate_ahead<SUMS>(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
 // Block bounds of a seed's batch: arena offset and length of its block.

@@ -89,8 +89,15 @@ extern "C" int hspz_autotune_batch(hspz_engine *h, const hspz_pass_params *p, co
     for (int order = 0; order < 2; order++) measure({saved.step, saved.threads, saved.claim, 0, order});
     long bi = best_index();
     const int order = bi >= 0 ? knobs[bi].order : saved.order;
-    // stage 1: step x block x claim at the resident grid
-    const int steps[] = {16, 32, 64, 96, 128}, threads[] = {512, 1024}, claims[] = {16, 64, 256};
+    // stage 1: step x block x claim at the resident grid. Behind the coarse
+    // bound (k-mer-major order; the stage-0 pass in that order tells) the step
+    // does not apply (the candidates' gate has its own) and a wave's claim
+    // sets how much work each claim's wait on its loads in flight covers, so
+    // only blocks and the larger claims are swept.
+    const bool coarse = order == 1 && h->coarse_used != 0;
+    const std::vector<int> steps = coarse ? std::vector<int>{saved.step} : std::vector<int>{16, 32, 64, 96, 128};
+    const std::vector<int> claims = coarse ? std::vector<int>{64, 256, 1024} : std::vector<int>{16, 64, 256};
+    const int threads[] = {512, 1024};
     for (int st : steps)
         for (int th : threads)
             for (int cl : claims) measure({st, th, cl, 0, order});

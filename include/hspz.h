@@ -183,7 +183,7 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           Seed-major order runs both as 64 */
 #define HSPZ_FLAG_BATCH_BLOCK_THREADS 3 /* threads per gate block: 512 or 1024 (default) */
 #define HSPZ_FLAG_BATCH_CLAIM_SEEDS 4  /* seeds a wave claims from the work counter at a time (1..4096,
-                                          default 64) */
+                                          default 256) */
 #define HSPZ_FLAG_BATCH_ORDER 5        /* hit order of the batched gate: 0 seed-major, 1 k-mer-major (a
                                           reference window is reused by every query seed of its k-mer),
                                           2 (default) k-mer-major when the batch holds at least one seed
@@ -342,7 +342,9 @@ typedef struct hspz_tune_cell {
  * hspz_filter_batch over `chunks` for both hit orders at the engine's knobs,
  * then, in the faster order, for every cell of a grid over step width
  * {16, 32, 64, 96, 128} cells x block size {512, 1024} x wave claim {16, 64, 256} seeds at
- * the resident grid, then over grid sizes {1/2, 3/4, 1, 3/2, 2} x resident
+ * the resident grid (in k-mer-major order behind the coarse bound, whose
+ * candidates' gate has its own step: block size {512, 1024} x wave claim
+ * {64, 256, 1024} at the engine's step), then over grid sizes {1/2, 3/4, 1, 3/2, 2} x resident
  * for the best of those, `reps` times each, and records every cell it measured
  * in cells[0..*n_cells) (up to cap; *n_cells counts all of them). The engine
  * keeps the cell with the least gate_ms + after_ms, which is also returned in

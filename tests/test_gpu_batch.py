@@ -439,14 +439,20 @@ def test_batch_autotune(gpu_available):
     target, blocks, sub, index, pos, chunks = _engine_and_chunks(0xB006, 8)
     with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
         before, st0 = eng.filter_batch(chunks)
+        step0 = eng.get_flag("batch_step_cells")
         cells, best = eng.autotune(chunks, reps=1)
-        assert len(cells) == 36
         assert all(c["status"] == 0 for c in cells)
         assert len({c["hits"] for c in cells}) == 1 and cells[0]["hits"] == st0["hits"]
         assert [c["order"] for c in cells[:2]] == [0, 1]
         assert len({c["order"] for c in cells[2:]}) == 1
-        assert {(c["step_cells"], c["block_threads"], c["claim_seeds"]) for c in cells[2:32]} == {
-            (s, t, c) for s in (16, 32, 64, 96, 128) for t in (512, 1024) for c in (16, 64, 256)}
+        if cells[2]["order"] == 1:  # behind the coarse bound: blocks x larger claims at the engine's step
+            assert len(cells) == 12
+            assert {(c["step_cells"], c["block_threads"], c["claim_seeds"]) for c in cells[2:8]} == {
+                (step0, t, c) for t in (512, 1024) for c in (64, 256, 1024)}
+        else:
+            assert len(cells) == 36
+            assert {(c["step_cells"], c["block_threads"], c["claim_seeds"]) for c in cells[2:32]} == {
+                (s, t, c) for s in (16, 32, 64, 96, 128) for t in (512, 1024) for c in (16, 64, 256)}
         fastest = min(c["gate_ms"] + c["after_ms"] for c in cells)
         assert best["gate_ms"] + best["after_ms"] == fastest
         assert eng.get_flag("batch_step_cells") == best["step_cells"]
@@ -524,11 +530,14 @@ def test_batch_coarse_forms(gpu_available, name, acgt, xdrop, used):
     chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
     with E.Engine(index, pos, S.encode(target), sub, xdrop=xdrop, hspthresh=thresh, max_hits=16711680) as eng:
         eng.set_flag("batch_order", 1)
+        # the candidates' gate with its loads issued ahead (lookahead on) and without
         for threads in (1024, 512):
             eng.set_flag("batch_block_threads", threads)
-            for coarse, u in ((1, used), (2, 1), (0, 0)):
+            for coarse, u, ahead in ((1, used, 1), (1, used, 0), (2, 1, 1), (2, 1, 0), (0, 0, 1)):
                 eng.set_flag("batch_coarse", coarse)
+                eng.set_flag("batch_lookahead", ahead)
                 got, _ = eng.filter_batch(chunks)
                 assert eng.get_flag("batch_coarse_used") == u
                 for i, (g, w) in enumerate(zip(got, want)):
-                    assert same_intervals(g, w), "chunk %d differs in coarse mode %d, %d threads" % (i, coarse, threads)
+                    assert same_intervals(g, w), "chunk %d differs in coarse mode %d, lookahead %d, %d threads" % (
+                        i, coarse, ahead, threads)
