@@ -480,10 +480,13 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                 uint32_t, uint32_t *);
         // the paired form (two hits per lane, packed 16-bit sums) where the class scores fit
         const bool pair = h->coarse_gate == 1 && h->coarse_fits16;
+        const bool t1k = h->bgate_threads == 1024, pv = h->coarse_pv, c0 = h->coarse_cc.c0 != 0;
         const CGateFn cgate_fn =
-            !pair ? (h->bgate_threads == 1024 ? k_bgate_coarse<1024> : k_bgate_coarse<512>)
-            : h->coarse_pv ? (h->bgate_threads == 1024 ? k_bgate_coarse2<1024, true> : k_bgate_coarse2<512, true>)
-                           : (h->bgate_threads == 1024 ? k_bgate_coarse2<1024, false> : k_bgate_coarse2<512, false>);
+            !pair ? (t1k ? k_bgate_coarse<1024> : k_bgate_coarse<512>)
+            : t1k ? (pv ? (c0 ? k_bgate_coarse2<1024, true, true> : k_bgate_coarse2<1024, true, false>)
+                        : (c0 ? k_bgate_coarse2<1024, false, true> : k_bgate_coarse2<1024, false, false>))
+                  : (pv ? (c0 ? k_bgate_coarse2<512, true, true> : k_bgate_coarse2<512, true, false>)
+                        : (c0 ? k_bgate_coarse2<512, false, true> : k_bgate_coarse2<512, false, false>));
         XGateFn xgate_fn;
         // the candidates' gate with its loads issued ahead (512-thread blocks) unless the lookahead is off
         const bool xahead = h->lookahead;

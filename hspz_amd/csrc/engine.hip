@@ -102,7 +102,7 @@ struct hspz_engine {
     int coarse_used = 0;               // the form the last batched pass ran (0: none)
     bool coarse_fits16 = false;        // every |class score| <= COARSE_PAIR_MAX_CLASS
     bool coarse_pv = false;            // the transversion class adds to the positive part
-    CoarseCoef coarse_cc = {0, 0, 0, 0, 0, 0};
+    CoarseCoef coarse_cc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t last_cand = 0;            // candidates of the last batched pass
     int gate_blocks = 2048;
     // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
@@ -605,10 +605,28 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
                 }
             const int32_t pm = std::max(um, 0), pt = std::max(ut, 0), pv = std::max(uv, 0);
             // nw mismatches, nv of them transversions: nm = 16 - nw, nt = nw - nv
-            h->coarse_cc = {16 * um, ut - um, uv - ut, 16 * pm, pt - pm, pv - pt};
+            CoarseCoef &cc = h->coarse_cc;
+            cc = {16 * um, ut - um, uv - ut, 16 * pm, pt - pm, pv - pt, 0, 0, 0, 0, 0, 0, 0, 0};
+            // paired form: matches and transversions counted, biases folding 16 ut away where they exist
+            cc.a1 = um - ut;
+            cc.a2 = uv - ut;
+            cc.qm = pm - pt;
+            cc.qv = pv - pt;
+            cc.c0 = 16 * ut;
+            bool bias = false;
+            for (int32_t reach = 0; reach <= 256 && !bias; reach++)  // smallest |bm| + |bv| first
+                for (int32_t bm = -reach; bm <= reach && !bias; bm++)
+                    for (int32_t bv : {reach - std::abs(bm), std::abs(bm) - reach})
+                        if (!bias && (int64_t)cc.a1 * bm + (int64_t)cc.a2 * bv == 16 * (int64_t)ut) {
+                            cc.bm = bm;
+                            cc.bv = bv;
+                            bias = true;
+                        }
+            if (bias) cc.c0 = 0;
+            cc.pc = 16 * pt - cc.qm * cc.bm - cc.qv * cc.bv;
             const int32_t a = std::max({std::abs(um), std::abs(ut), std::abs(uv)});
-            h->coarse_fits16 = a <= COARSE_PAIR_MAX_CLASS;
-            h->coarse_pv = pv - pt != 0;
+            h->coarse_fits16 = a <= COARSE_PAIR_MAX_CLASS && 128 * a + std::abs(cc.pc) <= 32767;
+            h->coarse_pv = cc.qv != 0;
         }
         if (ok) {
             uint32_t tab[512];

@@ -1476,8 +1476,17 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
 // A 16-cell word with nw mismatches, nv of them transversions: its class-score
 // sum is s0 + sw * nw + sv * nv, the sum of its positive class scores
 // p0 + pw * nw + pv * nv.
+// The paired form counts matches nm and transversions nv instead (nt = 16 -
+// nm - nv): sum 16 ut + a1 nm + a2 nv (a1 = um - ut, a2 = uv - ut), positive
+// part 16 pt + qm nm + qv nv. Where small integers bm, bv with a1 bm + a2 bv
+// = 16 ut exist, the counts are taken as nm' = nm + bm and nv' = nv + bv
+// (free: the popcount instruction adds an accumulator), so a word's sum is
+// a1 nm' + a2 nv' with no constant term (c0 = 0; else c0 = 16 ut and no
+// biases). The tracked bound U + qm nm' + qv nv' is the true one less the
+// constant pc = 16 pt - qm bm - qv bv, added back at the decision.
 struct CoarseCoef {
     int32_t s0, sw, sv, p0, pw, pv;
+    int32_t a1, a2, c0, qm, qv, bm, bv, pc;  // paired form; c0 = 16 ut, or 0 with biases
 };
 
 constexpr int CW_RIGHT = 4, CW_LEFT = 7;  // coarse windows: 64 right cells, 112 left cells
@@ -1592,8 +1601,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
 // hits. The bound B is taken over every window word, drop or not (a larger
 // value than the maximum up to the drop, so still an upper bound of the true
 // maximum). Values in range: with every class score within +-a, a side's
-// prefix sums over 8 words lie within +-128a and U - max within [-256a, 16a],
-// so a <= 127 keeps every compared value within 16 bits (the host checks it);
+// prefix sums over 8 words lie within +-128a, U - max within [-256a, 16a] and
+// the tracked bound within +-(128a + |pc|), so a <= 127 and 128a + |pc| <=
+// 32767 keep every compared value within 16 bits (the host checks both);
 // sums and products wrap like the 16-bit ring they are, exact wherever the
 // result is in range.
 typedef short s16x2 __attribute__((ext_vector_type(2)));
@@ -1603,25 +1613,33 @@ __device__ __forceinline__ s16x2 splat16(int32_t v) { return as_s16x2(((uint32_t
 constexpr int32_t COARSE_PAIR_MAX_CLASS = 127;  // largest |class score| the packed form holds
 
 struct CoarsePair {
-    s16x2 s0, sw, sv, pw, pv;
+    s16x2 a1, a2, c0, qm, qv;
+    int32_t bm, bv;
 };
 
+// Lanes of `lo` in the low half and of `hi` in the high half (a byte select).
+__device__ __forceinline__ uint32_t pack16(uint32_t lo, uint32_t hi) {
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+
 // One 16-cell word of two chains (query words qa and qb against reference
-// word r); U, mx, mg (minimum of U - running maximum) and B as in
-// coarse_word, per half. PV: the transversion class adds to the positive part.
-template <bool PV>
+// word r); U, mx, mg (minimum of U - running maximum) and B (the bound less
+// pc) per half. PV: the transversion class adds to the positive part. C0: the
+// sums carry a constant (no bias pair exists).
+template <bool PV, bool C0>
 __device__ __forceinline__ void coarse_pair_word(uint32_t r, uint32_t qa, uint32_t qb, const CoarsePair &cp,
                                                  s16x2 &U, s16x2 &mx, s16x2 &mg, s16x2 &B) {
     constexpr uint32_t M = 0x55555555u;
     const uint32_t xa = r ^ qa, xb = r ^ qb;
-    const uint32_t nvp = (uint32_t)__popc(xa & M) | (uint32_t)__popc(xb & M) << 16;
-    const uint32_t tsa = (xa >> 1) & ~xa & M, tsb = (xb >> 1) & ~xb & M;
-    const uint32_t nwp = (uint32_t)__popc(tsa) + ((uint32_t)__popc(tsb) << 16) + nvp;
-    const s16x2 nv = as_s16x2(nvp), nw = as_s16x2(nwp);
-    s16x2 bnd = U + cp.pw * nw;
-    if constexpr (PV) bnd = bnd + cp.pv * nv;
+    const uint32_t nvp = pack16(__popc(xa & M) + cp.bv, __popc(xb & M) + cp.bv);
+    const uint32_t ma = ~(xa | (xa >> 1)) & M, mb = ~(xb | (xb >> 1)) & M;  // matching cells
+    const uint32_t nmp = pack16(__popc(ma) + cp.bm, __popc(mb) + cp.bm);
+    const s16x2 nv = as_s16x2(nvp), nm = as_s16x2(nmp);
+    s16x2 bnd = U + cp.qm * nm;
+    if constexpr (PV) bnd = bnd + cp.qv * nv;
     B = __builtin_elementwise_max(B, bnd);
-    const s16x2 Un = U + cp.s0 + cp.sw * nw + cp.sv * nv;
+    s16x2 Un = U + cp.a1 * nm + cp.a2 * nv;
+    if constexpr (C0) Un = Un + cp.c0;
     mg = __builtin_elementwise_min(mg, Un - mx);
     mx = __builtin_elementwise_max(mx, Un);
     U = Un;
@@ -1629,7 +1647,7 @@ __device__ __forceinline__ void coarse_pair_word(uint32_t r, uint32_t qa, uint32
 
 // k_bgate_coarse with two hits per lane and iteration (the second half idle
 // when the item has one record left); same arguments, lists and decisions.
-template <int BT, bool PV>
+template <int BT, bool PV, bool C0>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_coarse2(
     const QRec *__restrict__ recs, const uint32_t *__restrict__ skm, const uint32_t *__restrict__ sval,
     const uint32_t *__restrict__ goff, uint32_t nk, const uint32_t *__restrict__ index,
@@ -1658,7 +1676,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.err = err;
     src.set_lists(s_q[0][threadIdx.x / WAVE], s_q[1][threadIdx.x / WAVE]);
     src.init();
-    const CoarsePair cp = {splat16(cc.s0), splat16(cc.sw), splat16(cc.sv), splat16(cc.pw), splat16(cc.pv)};
+    const CoarsePair cp = {splat16(cc.a1), splat16(cc.a2), splat16(cc.c0), splat16(cc.qm), splat16(cc.qv), cc.bm,
+                           cc.bv};
+    const s16x2 pc = splat16(cc.pc);
     const s16x2 zero = splat16(0), lowest = splat16(-32768);
     bool has = false, item = false, rok = false;
     uint32_t R = 0, s_cur = 0, s_end = 0, iidx = 0;
@@ -1695,13 +1715,16 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
         s16x2 Ur = zero, mxr = zero, mgr = zero, Br = lowest, Ul = zero, mxl = zero, mgl = zero, Bl = lowest;
 #pragma unroll
         for (int k = 0; k < CW_LEFT; k++) {
-            if (k < CW_RIGHT) coarse_pair_word<PV>(A0[k], QA[k], QB[k], cp, Ur, mxr, mgr, Br);
-            coarse_pair_word<PV>(LA0[k], LQA[k], LQB[k], cp, Ul, mxl, mgl, Bl);
+            if (k < CW_RIGHT) coarse_pair_word<PV, C0>(A0[k], QA[k], QB[k], cp, Ur, mxr, mgr, Br);
+            coarse_pair_word<PV, C0>(LA0[k], LQA[k], LQB[k], cp, Ul, mxl, mgl, Bl);
         }
+        // both sides dropped: the larger of the two minima below -X; the bound's
+        // total (each side's bound at least 0, the sum within 0..65535) in u16
+        const s16x2 dd = __builtin_elementwise_max(mgr, mgl);
+        const s16x2 tt = __builtin_elementwise_max(Br + pc, zero) + __builtin_elementwise_max(Bl + pc, zero);
         const int32_t X = xdrop;
-        const bool da = mgr.x < -X && mgl.x < -X, db = mgr.y < -X && mgl.y < -X;
-        const int32_t ta = max(Br.x + cc.p0, 0) + max(Bl.x + cc.p0, 0);
-        const int32_t tb = max(Br.y + cc.p0, 0) + max(Bl.y + cc.p0, 0);
+        const bool da = dd.x < -X, db = dd.y < -X;
+        const int32_t ta = (uint16_t)tt.x, tb = (uint16_t)tt.y;
         const bool oka = rok && a3.z != 0, okb = rok && b3.z != 0;
         const bool pa = has && oka && (!da || packed_keep(ta, thresh));
         const bool pb = two && okb && (!db || packed_keep(tb, thresh));
