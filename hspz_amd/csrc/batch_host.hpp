@@ -149,7 +149,8 @@ double hits_per_position(const hspz_engine *h, uint32_t per) {
 // memory leaves room for them and a margin; otherwise the k-mer-major gate
 // reads its windows from the plane.
 int ensure_ref_records(hspz_engine *h, bool want) {
-    if (!want || h->rrec.p || h->rrec_declined || h->pos_len == 0) return 0;
+    // (an engine whose target is shared reads the holder's records or none)
+    if (!want || h->rrec.p || h->rrec_declined || h->pos_len == 0 || h->resident) return 0;
     size_t need = h->pos_len * 48;
     size_t f = 0, t = 0;
     HIP_TRY(hipMemGetInfo(&f, &t));

@@ -8,7 +8,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <memory>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/hspz.h"
@@ -49,6 +52,19 @@ struct Batch {
 }  // namespace
 
 struct hspz_batch_state;  // batch_host.hpp
+
+// The target side of engines that share it (hspz_engine_share): the device
+// buffers of the reference, index, pos table, planes, clean bits and reference
+// records, freed when the last engine holding them is.
+struct ResidentTarget {
+    int device = 0;
+    std::vector<void *> bufs;
+    ~ResidentTarget() {
+        if (bufs.empty()) return;
+        (void)hipSetDevice(device);
+        for (void *p : bufs) (void)hipFree(p);
+    }
+};
 
 namespace {
 constexpr size_t H_SMALL_BYTES = 4096;  // pinned host words for counters and chunk limits
@@ -127,6 +143,21 @@ struct hspz_engine {
     // (looked up once per device at engine creation) and the test cap on its tile
     uint8_t *scratch = nullptr;
     uint32_t noalloc_tile_hits = 0;
+    // asynchronous seed uploads (hspz_upload_seeds_async): a copy stream, per
+    // slot the copy's completion and the last compute reading the slot
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_ready[2] = {nullptr, nullptr}, compute_done[2] = {nullptr, nullptr};
+    bool copy_pending[2] = {false, false};
+    uint64_t seed_uploads = 0;
+    // pinned host staging pool (hspz_pinned_take / hspz_pinned_give)
+    std::mutex pinned_mu;
+    std::vector<std::pair<uint64_t *, size_t>> pinned_free;  // returned buffers and their capacity (u64)
+    std::unordered_map<uint64_t *, size_t> pinned_out;        // handed-out buffers
+    bool pinned_ok = false;                                   // a pinned buffer was handed out
+    bool dump_raw = false;  // HSPZ_FLAG_DUMP_RAW: hspz_seed_and_filter keeps its pre-dedup records
+    // the resident target when shared between engines (hspz_engine_share): it
+    // owns ref, index, pos, ref2, refx, clean and rrec, freed with its last holder
+    std::shared_ptr<ResidentTarget> resident;
 };
 
 namespace {
@@ -267,9 +298,34 @@ int chunk_limits(const std::vector<uint32_t> &hit_num, uint32_t max_hits,
     return 0;
 }
 
+// The slot's pending asynchronous seed copy, ordered before the compute
+// stream's next work (the copy completes before any later kernel starts).
+int join_seed_copy(hspz_engine *h, int slot) {
+    if (!h->copy_pending[slot]) return 0;
+    HIP_TRY(hipStreamWaitEvent(h->stream, h->copy_ready[slot], 0));
+    h->copy_pending[slot] = false;
+    return 0;
+}
+
+// The slot's pending asynchronous seed copy, waited for on the host.
+int wait_seed_copy(hspz_engine *h, int slot) {
+    if (!h->copy_pending[slot]) return 0;
+    HIP_TRY(hipEventSynchronize(h->copy_ready[slot]));
+    h->copy_pending[slot] = false;
+    return 0;
+}
+
+// Marks the compute stream's last work on the slot (an asynchronous upload
+// into the slot waits for it).
+int mark_slot_computed(hspz_engine *h, int slot) {
+    if (h->compute_done[slot]) HIP_TRY(hipEventRecord(h->compute_done[slot], h->stream));
+    return 0;
+}
+
 int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_segment_pair> &res,
-                         uint32_t *num_hits_out, uint32_t *raw_out) {
+                         uint32_t *num_hits_out, uint32_t *raw_out, std::vector<hspz_segment_pair> *raw_list) {
     HIP_TRY(hipSetDevice(h->device));
+    if (int rc = join_seed_copy(h, slot)) return rc;
     uint32_t n = h->seed_len[slot];
     *num_hits_out = 0;
     *raw_out = 0;
@@ -373,6 +429,7 @@ int seed_and_filter_impl(hspz_engine *h, int slot, int rev, std::vector<hspz_seg
         HIP_TRY(hipMemcpyAsync(res.data() + base, h->reduced.p, (size_t)nraw * 16, hipMemcpyDeviceToHost, h->stream));
         HIP_TRY(hipStreamSynchronize(h->stream));
         h->pipeline_syncs++;
+        if (raw_list) raw_list->insert(raw_list->end(), res.begin() + base, res.end());
         size_t kept = dedup_and_order(res.data() + base, nraw);
         res.resize(base + kept);
     }
@@ -497,16 +554,20 @@ namespace {
 int device_index_build(hspz_engine *h, const Shape &s, uint32_t step);
 
 // Engine::new (gpu/mod.rs:1024-1128). With `build` set, the index is built
-// on the device from the reference (cfg's index arrays are ignored).
+// on the device from the reference (cfg's index arrays are ignored). With
+// `shared` set, the engine reads that engine's resident target (its
+// ResidentTarget holder) instead of uploading or building one: cfg's
+// reference and index arrays are ignored.
 int engine_create(int device, const hspz_engine_config *cfg, const Shape *build, uint32_t build_step,
-                  hspz_engine **out) {
+                  hspz_engine **out, const hspz_engine *shared = nullptr) {
     clear_error();
     if (!cfg || !out) return set_error(HSPZ_E_ARG, "null config or output");
     *out = nullptr;
     int ndev = hspz_device_count();
     if (ndev <= 0) return set_error(HSPZ_E_NO_DEVICE, "no HIP device visible");
     if (device < 0 || device >= ndev) return set_error(HSPZ_E_ARG, "device %d out of range", device);
-    if (!build) {
+    if (shared) {
+    } else if (!build) {
         if (!cfg->index_table || cfg->index_len == 0 || (cfg->index_len & (cfg->index_len - 1)))
             return set_error(HSPZ_E_ARG, "index_table must hold 4^k entries");
         if (cfg->pos_len && !cfg->pos_table) return set_error(HSPZ_E_ARG, "null pos_table");
@@ -561,10 +622,26 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
     if ((rc = reserve(h, h->total, 256))) return fail(rc);
     if (hipMemcpy(h->sub.p, cfg->sub_mat, 256, hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_error(-1, "reference upload failed"));
+    if (shared) {  // the target's buffers are the holder's, never released or counted here
+        h->resident = shared->resident;
+        h->ref = shared->ref;
+        h->index = shared->index;
+        h->pos = shared->pos;
+        h->ref2 = shared->ref2;
+        h->refx = shared->refx;
+        h->clean = shared->clean;
+        h->rrec = shared->rrec;
+        h->rrec_declined = !shared->rrec.p;
+        h->ref_len = shared->ref_len;
+        h->index_len = shared->index_len;
+        h->pos_len = shared->pos_len;
+    } else {
     h->ref_len = (uint32_t)cfg->ref_len;
     if ((rc = upload_seq(h, h->ref, cfg->ref_seq, cfg->ref_len))) return fail(rc);
     h->reference_uploads++;
-    if (build) {
+    }
+    if (shared) {
+    } else if (build) {
         if ((rc = device_index_build(h, *build, build_step))) return fail(rc);
     } else {
         if ((rc = reserve(h, h->index, cfg->index_len * 4))) return fail(rc);
@@ -656,8 +733,8 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
             if ((rc = reserve(h, h->slow_n, 256))) return fail(rc);
             if (hipMemcpy(h->pairtab.p, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess)
                 return fail(set_error(-1, "pair table upload failed"));
-            if ((rc = build_planes(h, h->ref, h->ref_len, h->ref2, h->refx))) return fail(rc);
-            if (h->pos_len) {
+            if (!shared && (rc = build_planes(h, h->ref, h->ref_len, h->ref2, h->refx))) return fail(rc);
+            if (!shared && h->pos_len) {
                 if ((rc = reserve(h, h->clean, ((h->pos_len + 31) / 32 + 2) * 4))) return fail(rc);
                 hipLaunchKernelGGL(k_clean_bits, dim3(8192), dim3(256), 0, h->stream, h->pos.as<uint32_t>(),
                                    (uint32_t)h->pos_len, h->refx.as<uint32_t>(), h->ref_len, h->seed_size,
@@ -684,6 +761,50 @@ extern "C" int hspz_engine_new_device_index(int device, const hspz_engine_config
     return engine_create(device, cfg, &s, step ? step : 1, out);
 }
 
+extern "C" int hspz_engine_share(hspz_engine *base, hspz_engine **out) {
+    clear_error();
+    if (!base || !out) return set_error(HSPZ_E_ARG, "null argument");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(base->device));
+    HIP_TRY(hipStreamSynchronize(base->stream));
+    if (!base->resident) {
+        // the reference records are built on first use; build them now so that
+        // every engine reading this target reads the one copy
+        if (int rc = ensure_ref_records(base, base->rrec_mode >= 0)) return rc;
+        auto rt = std::make_shared<ResidentTarget>();
+        rt->device = base->device;
+        for (DevBuf *b : {&base->ref, &base->index, &base->pos, &base->ref2, &base->refx, &base->clean, &base->rrec})
+            if (b->p) rt->bufs.push_back(b->p);
+        base->resident = rt;
+    }
+    hspz_engine_config cfg;
+    memset(&cfg, 0, sizeof cfg);
+    cfg.sub_mat = base->sub_host;
+    cfg.seed_size = base->seed_size;
+    cfg.xdrop = base->xdrop;
+    cfg.hspthresh = base->hspthresh;
+    cfg.noentropy = base->noentropy;
+    cfg.max_hits = base->max_hits;
+    hspz_engine *h = nullptr;
+    if (int rc = engine_create(base->device, &cfg, nullptr, 0, &h, base)) return rc;
+    // the same gate configuration
+    h->gate_blocks = base->gate_blocks;
+    h->bgate_blocks = base->bgate_blocks;
+    h->bgate_step = base->bgate_step;
+    h->bgate_threads = base->bgate_threads;
+    h->bgate_claim = base->bgate_claim;
+    h->batch_order = base->batch_order;
+    h->rrec_mode = base->rrec_mode;
+    h->lookahead = base->lookahead;
+    h->coarse_gate = base->coarse_gate;
+    h->batch_graph = base->batch_graph;
+    h->device_dedup = base->device_dedup;
+    h->list_cap = base->list_cap;
+    h->batch_max_positions = base->batch_max_positions;
+    *out = h;
+    return 0;
+}
+
 extern "C" int hspz_engine_index(const hspz_engine *h, uint32_t *index_table, size_t index_len,
                                  uint32_t **pos_table, size_t *pos_len) {
     clear_error();
@@ -706,6 +827,10 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->resident) {  // the target's buffers go with their last holder
+        for (DevBuf *b : {&h->ref, &h->index, &h->pos, &h->ref2, &h->refx, &h->clean, &h->rrec}) *b = DevBuf();
+        h->resident.reset();
+    }
     DevBuf *bufs[] = {&h->ref, &h->index, &h->pos, &h->sub, &h->qf, &h->qr, &h->enc_tabs, &h->rawq, &h->encseg,
                       &h->seeds[0], &h->seeds[1],
                       &h->ref2, &h->refx, &h->qf2, &h->qfx, &h->qr2, &h->qrx, &h->pairtab, &h->slow, &h->slow_n, &h->clean, &h->rrec,
@@ -729,6 +854,16 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
         }
         delete h->batch;
     }
+    for (int sl = 0; sl < 2; sl++) {
+        if (h->copy_ready[sl]) (void)hipEventDestroy(h->copy_ready[sl]);
+        if (h->compute_done[sl]) (void)hipEventDestroy(h->compute_done[sl]);
+    }
+    if (h->copy_stream) {
+        (void)hipStreamSynchronize(h->copy_stream);
+        (void)hipStreamDestroy(h->copy_stream);
+    }
+    for (auto &b : h->pinned_free) (void)hipHostFree(b.first);
+    for (auto &b : h->pinned_out) (void)hipHostFree(b.first);
     if (h->h_small) (void)hipHostFree(h->h_small);
     if (h->ev_gate0) (void)hipEventDestroy(h->ev_gate0);
     if (h->ev_gate1) (void)hipEventDestroy(h->ev_gate1);
@@ -797,10 +932,126 @@ extern "C" int hspz_upload_seeds(hspz_engine *h, int slot, const uint64_t *seeds
     HIP_TRY(hipSetDevice(h->device));
     for (size_t i = 0; i < n; i++)
         if ((seeds[i] >> 32) >= h->index_len) return set_error(HSPZ_E_ARG, "seed %zu has k-mer beyond the index", i);
+    if (int rc = wait_seed_copy(h, slot)) return rc;
     if (int rc = reserve(h, h->seeds[slot], std::max<size_t>(n, 1) * 8)) return rc;
     if (n) HIP_TRY(hipMemcpyAsync(h->seeds[slot].p, seeds, n * 8, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->seed_len[slot] = (uint32_t)n;
+    h->seed_uploads++;
+    return 0;
+}
+
+extern "C" int hspz_upload_seeds_async(hspz_engine *h, int slot, const uint64_t *seeds, size_t n) {
+    clear_error();
+    if (!h || slot < 0 || slot > 1 || (n && !seeds)) return set_error(HSPZ_E_ARG, "bad slot or seeds");
+    if (n > 0xFFFFFFFFull) return set_error(HSPZ_E_ARG, "more than 2^32 seeds");
+    HIP_TRY(hipSetDevice(h->device));
+    for (size_t i = 0; i < n; i++)
+        if ((seeds[i] >> 32) >= h->index_len) return set_error(HSPZ_E_ARG, "seed %zu has k-mer beyond the index", i);
+    if (!h->copy_stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+        for (int s = 0; s < 2; s++) {
+            HIP_TRY(hipEventCreateWithFlags(&h->copy_ready[s], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&h->compute_done[s], hipEventDisableTiming));
+        }
+    }
+    // the slot's previous copy has landed (its host array is free again), and
+    // a growing slot buffer is only released once no queued kernel reads it
+    if (int rc = wait_seed_copy(h, slot)) return rc;
+    if (h->seeds[slot].cap < std::max<size_t>(n, 1) * 8) HIP_TRY(hipStreamSynchronize(h->stream));
+    if (int rc = reserve(h, h->seeds[slot], std::max<size_t>(n, 1) * 8)) return rc;
+    h->seed_len[slot] = (uint32_t)n;
+    if (n == 0) return 0;
+    // ordered after the compute stream's last work on the slot, so the copy
+    // cannot overwrite seeds a queued kernel still reads
+    HIP_TRY(hipStreamWaitEvent(h->copy_stream, h->compute_done[slot], 0));
+    HIP_TRY(hipMemcpyAsync(h->seeds[slot].p, seeds, n * 8, hipMemcpyHostToDevice, h->copy_stream));
+    HIP_TRY(hipEventRecord(h->copy_ready[slot], h->copy_stream));
+    h->copy_pending[slot] = true;
+    h->seed_uploads++;
+    return 0;
+}
+
+extern "C" int hspz_sync_seeds(hspz_engine *h, int slot) {
+    clear_error();
+    if (!h || slot < 0 || slot > 1) return set_error(HSPZ_E_ARG, "bad slot");
+    HIP_TRY(hipSetDevice(h->device));
+    return wait_seed_copy(h, slot);
+}
+
+extern "C" int hspz_pinned_take(hspz_engine *h, size_t n, uint64_t **buf) {
+    clear_error();
+    if (!h || !buf) return set_error(HSPZ_E_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(h->pinned_mu);
+    long best = -1;
+    for (size_t i = 0; i < h->pinned_free.size(); i++)
+        if (h->pinned_free[i].second >= n && (best < 0 || h->pinned_free[i].second < h->pinned_free[best].second))
+            best = (long)i;
+    uint64_t *p = nullptr;
+    size_t cap = 0;
+    if (best >= 0) {
+        p = h->pinned_free[best].first;
+        cap = h->pinned_free[best].second;
+        h->pinned_free.erase(h->pinned_free.begin() + best);
+    } else {
+        HIP_TRY(hipSetDevice(h->device));
+        cap = std::max<size_t>(n, 1);
+        if (hipHostMalloc((void **)&p, cap * 8, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(HSPZ_E_OOM, "pinned host allocation of %zu bytes failed", cap * 8);
+        }
+    }
+    h->pinned_out[p] = cap;
+    h->pinned_ok = true;
+    *buf = p;
+    return 0;
+}
+
+extern "C" int hspz_pinned_give(hspz_engine *h, uint64_t *buf) {
+    clear_error();
+    if (!h || !buf) return set_error(HSPZ_E_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(h->pinned_mu);
+    auto it = h->pinned_out.find(buf);
+    if (it == h->pinned_out.end()) return set_error(HSPZ_E_ARG, "buffer did not come from hspz_pinned_take");
+    h->pinned_free.emplace_back(buf, it->second);
+    h->pinned_out.erase(it);
+    return 0;
+}
+
+extern "C" int hspz_check_seed_bytes(hspz_engine *h, int slot, const uint64_t *expected, size_t n) {
+    clear_error();
+    if (!h || slot < 0 || slot > 1 || (n && !expected)) return set_error(HSPZ_E_ARG, "bad slot or seeds");
+    HIP_TRY(hipSetDevice(h->device));
+    if (int rc = wait_seed_copy(h, slot)) return rc;
+    if (h->seed_len[slot] != n)
+        return set_error(HSPZ_E_ARG, "device seed count differs: %u != %zu", h->seed_len[slot], n);
+    if (n == 0) return 0;
+    std::vector<uint64_t> got(n);
+    HIP_TRY(hipMemcpyAsync(got.data(), h->seeds[slot].p, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < n; i++)
+        if (got[i] != expected[i])
+            return set_error(HSPZ_E_ARG, "device seed differs at %zu: %016llx != %016llx", i,
+                             (unsigned long long)got[i], (unsigned long long)expected[i]);
+    return 0;
+}
+
+extern "C" int hspz_launch_overhead_ms(hspz_engine *h, uint32_t iters, double *ms) {
+    clear_error();
+    if (!h || !ms || iters == 0) return set_error(HSPZ_E_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(h->device));
+    const uint32_t *sink = h->total.as<uint32_t>();
+    uint32_t *out = h->total.as<uint32_t>() + 1;
+    for (int i = 0; i < 16; i++) hipLaunchKernelGGL(k_noop, dim3(1), dim3(64), 0, h->stream, sink, out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 0; i < iters; i++) hipLaunchKernelGGL(k_noop, dim3(1), dim3(64), 0, h->stream, sink, out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    h->launches += 16 + iters;
+    *ms = secs * 1000.0 / iters;
     return 0;
 }
 
@@ -815,6 +1066,7 @@ extern "C" int hspz_generate_seeds(hspz_engine *h, int slot, int rev, uint32_t s
     if (shape_pos[kmer_size - 1] + 1 > h->seed_size)
         return set_error(HSPZ_E_SHAPE, "device seed shape does not match Engine seed size");
     HIP_TRY(hipSetDevice(h->device));
+    if (int rc = wait_seed_copy(h, slot)) return rc;  // the slot's buffer is rewritten (and may grow)
     *n_out = 0;
     h->seed_len[slot] = 0;
     uint32_t span = end > start ? end - start : 0;
@@ -844,7 +1096,7 @@ extern "C" int hspz_generate_seeds(hspz_engine *h, int slot, int rev, uint32_t s
     }
     h->seed_len[slot] = (uint32_t)nseeds;
     *n_out = (uint32_t)nseeds;
-    return 0;
+    return mark_slot_computed(h, slot);
 }
 
 extern "C" int hspz_seed_and_filter(hspz_engine *h, int slot, int rev, hspz_filter_output *out) {
@@ -853,9 +1105,10 @@ extern "C" int hspz_seed_and_filter(hspz_engine *h, int slot, int rev, hspz_filt
     memset(out, 0, sizeof *out);
     if (h->query_len == 0)
         return set_error(HSPZ_E_NO_QUERY, "Engine has no query: swap_query() must be called before seed_and_filter()");
-    std::vector<hspz_segment_pair> res;
+    std::vector<hspz_segment_pair> res, raw_list;
     uint32_t nh = 0, raw = 0;
-    int rc = seed_and_filter_impl(h, slot, rev, res, &nh, &raw);
+    int rc = seed_and_filter_impl(h, slot, rev, res, &nh, &raw, h->dump_raw ? &raw_list : nullptr);
+    if (rc == 0) rc = mark_slot_computed(h, slot);
     if (rc) return rc;
     out->num_hits = nh;
     out->raw_hsps = raw;
@@ -865,14 +1118,24 @@ extern "C" int hspz_seed_and_filter(hspz_engine *h, int slot, int rev, hspz_filt
         if (!out->hsps) return set_error(HSPZ_E_OOM, "host output allocation");
         memcpy(out->hsps, res.data(), res.size() * 16);
     }
+    if (!raw_list.empty()) {
+        out->raw = (hspz_segment_pair *)aligned_alloc(16, raw_list.size() * 16);
+        if (!out->raw) {
+            hspz_filter_output_free(out);
+            return set_error(HSPZ_E_OOM, "host output allocation");
+        }
+        memcpy(out->raw, raw_list.data(), raw_list.size() * 16);
+        out->n_raw = raw_list.size();
+    }
     return 0;
 }
 
 extern "C" void hspz_filter_output_free(hspz_filter_output *out) {
     if (!out) return;
     free(out->hsps);
-    out->hsps = nullptr;
-    out->n_hsps = 0;
+    free(out->raw);
+    out->hsps = out->raw = nullptr;
+    out->n_hsps = out->n_raw = 0;
 }
 
 extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
@@ -908,6 +1171,7 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
             return set_error(HSPZ_E_ARG, "list capacity must be 0 or in [64, 2^28]");
         h->list_cap = (uint32_t)value;
         break;
+    case HSPZ_FLAG_DUMP_RAW: h->dump_raw = value != 0; break;
     case HSPZ_FLAG_BATCH_COARSE:
         if (value < 0 || value > 2) return set_error(HSPZ_E_ARG, "coarse mode must be 0, 1 or 2");
         h->coarse_gate = value;
@@ -923,7 +1187,7 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
     case HSPZ_FLAG_REF_RECORDS:
         if (value < -1 || value > 1) return set_error(HSPZ_E_ARG, "ref_records must be -1, 0 or 1");
         h->rrec_mode = value;
-        if (value <= 0 && h->rrec.p) {
+        if (value <= 0 && h->rrec.p && !h->resident) {  // a shared target keeps its records
             (void)hipStreamSynchronize(h->stream);
             std::lock_guard<std::mutex> lk(capture_mutex());
             release(h, h->rrec);
@@ -953,6 +1217,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_NOALLOC_TILE_HITS: *value = (int)h->noalloc_tile_hits; break;
     case HSPZ_FLAG_BATCH_MAX_POSITIONS: *value = (int)h->batch_max_positions; break;
     case HSPZ_FLAG_BATCH_COARSE: *value = h->coarse_gate; break;
+    case HSPZ_FLAG_DUMP_RAW: *value = h->dump_raw ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_COARSE_USED: *value = h->coarse_used; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }
@@ -986,6 +1251,8 @@ extern "C" int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value) {
     case HSPZ_STAT_BATCH_PASSES: *value = h->batch_passes_last; break;
     case HSPZ_STAT_DEVICE_BYTES: *value = h->dev_bytes; break;
     case HSPZ_STAT_BATCH_CANDIDATES: *value = h->last_cand; break;
+    case HSPZ_STAT_SEED_UPLOADS: *value = h->seed_uploads; break;
+    case HSPZ_STAT_PINNED_ACTIVE: *value = h->pinned_ok ? 1 : 0; break;
     default: return set_error(HSPZ_E_ARG, "unknown stat %d", which);
     }
     return 0;
@@ -1035,7 +1302,7 @@ extern "C" int hspz_run_pass(hspz_engine *h, const hspz_pass_params *p, hspz_seg
         st.seeds += ns;
         std::vector<hspz_segment_pair> res;
         uint32_t nh = 0, raw = 0;
-        if (int e = seed_and_filter_impl(h, 0, b.rev, res, &nh, &raw)) return e;
+        if (int e = seed_and_filter_impl(h, 0, b.rev, res, &nh, &raw, nullptr)) return e;
         st.hits += nh;
         st.raw_hsps += raw;
         auto &dst = b.rev ? rc[b.interval] : fw[b.interval];

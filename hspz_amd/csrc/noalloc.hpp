@@ -216,6 +216,7 @@ int noalloc_compact(hspz_engine *h, const uint8_t *flags, uint32_t n, uint32_t *
 int seed_and_filter_noalloc(hspz_engine *h, int slot, int rev, hspz_segment_pair *out, size_t cap,
                             size_t *n_out, uint32_t *num_hits_out, uint32_t *raw_out) {
     HIP_TRY(hipSetDevice(h->device));
+    if (int rc = join_seed_copy(h, slot)) return rc;
     uint8_t *base = h->scratch;
     if (!base) return set_error(HSPZ_E_ARG, "engine has no scratch address");
     *n_out = 0;
@@ -374,6 +375,7 @@ extern "C" int hspz_seed_and_filter_noalloc(hspz_engine *h, int slot, int rev, h
         return set_error(HSPZ_E_NO_QUERY, "Engine has no query: swap_query() must be called before seed_and_filter()");
     uint32_t nh = 0, raw = 0;
     int rc = seed_and_filter_noalloc(h, slot, rev, out, cap, n_out, &nh, &raw);
+    if (rc == 0) rc = mark_slot_computed(h, slot);
     if (num_hits) *num_hits = nh;
     if (raw_hsps) *raw_hsps = raw;
     return rc;

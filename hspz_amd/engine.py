@@ -26,11 +26,11 @@ STAT = {"max_hits": 0, "launches": 1, "pipeline_syncs": 2, "reference_uploads": 
         "batch_prep_ns": 12, "batch_device_ns": 13, "batch_read_ns": 14, "batch_post_ns": 15,
         "index_build_ns": 16, "pos_len": 17, "index_len": 18,
         "batch_reruns": 19, "batch_passes": 20, "device_bytes": 21,
-        "batch_candidates": 22}
+        "batch_candidates": 22, "seed_uploads": 23, "pinned_active": 24}
 FLAG = {"batch_graph": 0, "batch_gate_blocks": 1, "batch_step_cells": 2, "batch_block_threads": 3,
         "batch_claim_seeds": 4, "batch_order": 5, "batch_order_used": 6, "batch_device_dedup": 7,
         "ref_records": 8, "batch_lookahead": 9, "batch_list_cap": 10, "noalloc_tile_hits": 11,
-        "batch_max_positions": 12, "batch_coarse": 13, "batch_coarse_used": 14}
+        "batch_max_positions": 12, "batch_coarse": 13, "batch_coarse_used": 14, "dump_raw": 15}
 
 # Every symbol include/hspz.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -42,7 +42,9 @@ EXPORTS = [
     "hspz_batch_output_free", "hspz_server_new", "hspz_server_submit", "hspz_server_wait",
     "hspz_request_result_free", "hspz_server_get_stats", "hspz_server_free", "hspz_set_flag",
     "hspz_get_flag", "hspz_autotune_batch", "hspz_engine_new_device_index", "hspz_engine_index",
-    "hspz_swap_query_raw", "hspz_engine_query", "hspz_seed_and_filter_noalloc",
+    "hspz_swap_query_raw", "hspz_engine_query", "hspz_seed_and_filter_noalloc", "hspz_engine_share",
+    "hspz_upload_seeds_async", "hspz_sync_seeds", "hspz_pinned_take", "hspz_pinned_give",
+    "hspz_check_seed_bytes", "hspz_launch_overhead_ms",
 ]
 
 
@@ -63,7 +65,7 @@ class EngineConfig(C.Structure):
 
 class FilterOutput(C.Structure):
     _fields_ = [("hsps", C.c_void_p), ("n_hsps", C.c_size_t), ("num_hits", C.c_uint32),
-                ("raw_hsps", C.c_uint32)]
+                ("raw_hsps", C.c_uint32), ("raw", C.c_void_p), ("n_raw", C.c_size_t)]
 
 
 class PassParams(C.Structure):
@@ -181,6 +183,13 @@ def lib():
         L.hspz_server_free.restype = None
         L.hspz_free.argtypes = [vp]
         L.hspz_free.restype = None
+        L.hspz_engine_share.argtypes = [vp, C.POINTER(vp)]
+        L.hspz_upload_seeds_async.argtypes = [vp, C.c_int, vp, sz]
+        L.hspz_sync_seeds.argtypes = [vp, C.c_int]
+        L.hspz_pinned_take.argtypes = [vp, sz, C.POINTER(vp)]
+        L.hspz_pinned_give.argtypes = [vp, vp]
+        L.hspz_check_seed_bytes.argtypes = [vp, C.c_int, vp, sz]
+        L.hspz_launch_overhead_ms.argtypes = [vp, u32, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -348,15 +357,65 @@ class Engine:
                                          int(transitions), C.byref(n)))
         return int(n.value)
 
-    def seed_and_filter(self, slot: int, rev: bool):
-        """Returns (hsps structured array, num_hits, raw_hsps)."""
+    def seed_and_filter(self, slot: int, rev: bool, with_raw: bool = False):
+        """Returns (hsps structured array, num_hits, raw_hsps), and with
+        with_raw the pre-dedup records as a fourth item (dump_raw must be set,
+        else it is empty)."""
         out = FilterOutput()
         _check(lib().hspz_seed_and_filter(self._h, slot, int(rev), C.byref(out)))
         res = np.zeros(out.n_hsps, dtype=SP_DTYPE)
         if out.n_hsps:
             C.memmove(_ptr(res), out.hsps, out.n_hsps * 16)
+        raw = np.zeros(out.n_raw, dtype=SP_DTYPE)
+        if out.n_raw:
+            C.memmove(_ptr(raw), out.raw, out.n_raw * 16)
         lib().hspz_filter_output_free(C.byref(out))
+        if with_raw:
+            return res, int(out.num_hits), int(out.raw_hsps), raw
         return res, int(out.num_hits), int(out.raw_hsps)
+
+    def upload_seeds_async(self, slot: int, seeds):
+        """hspz_upload_seeds_async: `seeds` must stay alive and unchanged until
+        the slot's copy completes (the next seed_and_filter, generate_seeds or
+        upload on the slot, or sync_seeds(slot)). Pass a PinnedSeeds array for
+        a truly asynchronous copy."""
+        s = seeds if isinstance(seeds, np.ndarray) and seeds.dtype == np.uint64 and seeds.flags.c_contiguous \
+            else np.ascontiguousarray(seeds, dtype=np.uint64)
+        _check(lib().hspz_upload_seeds_async(self._h, slot, _ptr(s), len(s)))
+        self._inflight = getattr(self, "_inflight", {})
+        self._inflight[slot] = s  # kept alive until the slot's copy is known complete
+
+    def sync_seeds(self, slot: int):
+        _check(lib().hspz_sync_seeds(self._h, slot))
+
+    def pinned_take(self, n: int) -> np.ndarray:
+        """hspz_pinned_take: a uint64 array of n seeds over a pinned pool buffer
+        (valid until pinned_give or the engine is closed)."""
+        p = C.c_void_p()
+        _check(lib().hspz_pinned_take(self._h, n, C.byref(p)))
+        buf = (C.c_uint64 * max(n, 1)).from_address(p.value)
+        return np.ctypeslib.as_array(buf)[:n]
+
+    def pinned_give(self, arr: np.ndarray):
+        _check(lib().hspz_pinned_give(self._h, C.c_void_p(arr.ctypes.data)))
+
+    def check_seed_bytes(self, slot: int, expected):
+        e = np.ascontiguousarray(expected, dtype=np.uint64)
+        _check(lib().hspz_check_seed_bytes(self._h, slot, _ptr(e), len(e)))
+
+    def launch_overhead_ms(self, iters: int = 1000) -> float:
+        ms = C.c_double()
+        _check(lib().hspz_launch_overhead_ms(self._h, iters, C.byref(ms)))
+        return float(ms.value)
+
+    def share(self) -> "Engine":
+        """hspz_engine_share: a second engine over this engine's resident target."""
+        h = C.c_void_p()
+        _check(lib().hspz_engine_share(self._h, C.byref(h)))
+        e = Engine.__new__(Engine)
+        e._h = h
+        e.seed_size = self.seed_size
+        return e
 
     def seed_and_filter_noalloc(self, slot: int, rev: bool, cap: int):
         """hspz_seed_and_filter_noalloc into a caller array of cap records.

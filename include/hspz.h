@@ -67,6 +67,11 @@ typedef struct hspz_filter_output {
     size_t n_hsps;
     uint32_t num_hits;
     uint32_t raw_hsps;
+    /* FilterOutput::raw (gpu/mod.rs:55-67): the records surviving the score
+     * threshold before dedup, in the order each max_hits chunk produced them,
+     * kept only when HSPZ_FLAG_DUMP_RAW is set (else NULL / 0). */
+    hspz_segment_pair *raw;
+    size_t n_raw;
 } hspz_filter_output;
 
 typedef struct hspz_engine hspz_engine;
@@ -82,6 +87,16 @@ int hspz_engine_new(int device, const hspz_engine_config *cfg, hspz_engine **out
  * pattern's span. */
 int hspz_engine_new_device_index(int device, const hspz_engine_config *cfg, const char *pattern, uint32_t step,
                                  hspz_engine **out);
+/* A second engine on base's device over the same resident target (the
+ * reference bin the reference keeps resident across work units, gpu/mod.rs:
+ * 1130-1148): it has its own stream, query side, seed slots and work buffers
+ * and base's scoring and gate configuration, and reads base's device
+ * reference, index, pos table, packed planes, clean bits and reference
+ * records (built now if the engine would use them) instead of holding a
+ * copy. The target's device memory is held once, by whichever of the
+ * engines sharing it is freed last; HSPZ_STAT_DEVICE_BYTES counts it in base
+ * only. Engines sharing a target may run on different host threads. */
+int hspz_engine_share(hspz_engine *base, hspz_engine **out);
 /* Copies the engine's resident index to the host: index_len must be 4^k;
  * *pos_table is library-allocated (hspz_free). No counterpart in the
  * reference; it lets a caller reuse or check a device-built index. */
@@ -107,6 +122,35 @@ int hspz_engine_query(const hspz_engine *h, uint8_t *fwd, uint8_t *rc, size_t le
  * (kmer << 32) | query_position in seed-stream order (seed.rs:385-398).
  * Copied synchronously, so the host array may be reused on return. */
 int hspz_upload_seeds(hspz_engine *h, int slot, const uint64_t *seeds, size_t n);
+/* Engine::upload_seeds with async_seed_copy (gpu/mod.rs:1246-1309): the copy
+ * runs on the engine's copy stream, ordered after the compute stream's last
+ * work on the slot, and the next hspz_seed_and_filter (or its no-alloc
+ * variant) on the slot waits for it on the device; the call returns at once.
+ * Slot lifetime: the host array must stay allocated and unchanged until that
+ * slot's copy has completed, which it has once hspz_seed_and_filter /
+ * hspz_seed_and_filter_noalloc / hspz_generate_seeds / hspz_upload_seeds* on
+ * the same slot or hspz_sync_seeds(h, slot) has returned. Issue it one batch
+ * ahead (the other slot) for the copy to hide behind compute. A pinned host
+ * array (hspz_pinned_take) is what makes the copy asynchronous. */
+int hspz_upload_seeds_async(hspz_engine *h, int slot, const uint64_t *seeds, size_t n);
+/* Waits on the host for the slot's pending asynchronous copy. No counterpart
+ * in the reference (its caller synchronises the copy stream itself). */
+int hspz_sync_seeds(hspz_engine *h, int slot);
+/* Engine::take_pinned / give_pinned (gpu/mod.rs:1165-1187): a pinned host
+ * staging buffer of at least n seeds from the engine's pool (page-locked
+ * once, reused across passes), and its return to the pool; buffers are freed
+ * with the engine. Give a buffer back only after its copy has completed
+ * (the slot lifetime above). HSPZ_STAT_PINNED_ACTIVE reports whether one was
+ * handed out (Engine::pinned_seeds_active). */
+int hspz_pinned_take(hspz_engine *h, size_t n, uint64_t **buf);
+int hspz_pinned_give(hspz_engine *h, uint64_t *buf);
+/* Engine::check_seed_bytes (gpu/mod.rs:1464-1495): compares the slot's seeds
+ * (as hspz_generate_seeds or an upload left them) with `expected` byte for
+ * byte; HSPZ_E_ARG with the count or first differing index otherwise. */
+int hspz_check_seed_bytes(hspz_engine *h, int slot, const uint64_t *expected, size_t n);
+/* Engine::launch_overhead_ms (gpu/mod.rs:1210-1236): mean wall-clock cost of
+ * one launch of an empty kernel over `iters` launches after 16 warm-up ones. */
+int hspz_launch_overhead_ms(hspz_engine *h, uint32_t iters, double *ms);
 
 /* Engine::generate_seeds (gpu/mod.rs:1312-1463): device seeding of query
  * positions [start, end) on the forward (rev=0) or reverse-complement block.
@@ -170,6 +214,8 @@ int hspz_seed_and_filter_noalloc(hspz_engine *h, int slot, int rev, hspz_segment
 #define HSPZ_STAT_DEVICE_BYTES 21     /* device bytes the engine holds now (resident reference side and
                                         grow-only work buffers; PEAK_DEVICE_BYTES is the most it has held) */
 #define HSPZ_STAT_BATCH_CANDIDATES 22 /* hits of the last batched pass the coarse bound passed on */
+#define HSPZ_STAT_SEED_UPLOADS 23     /* seed uploads (hspz_upload_seeds and _async) */
+#define HSPZ_STAT_PINNED_ACTIVE 24    /* 1 once hspz_pinned_take handed out a pinned buffer */
 int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 
 /* Engine switches (the reference's pub flags, gpu/mod.rs:836-949, have no
@@ -228,6 +274,8 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           runs the two-cell maps. The lists do not change */
 #define HSPZ_FLAG_BATCH_COARSE_USED 14 /* read-only: the coarse bound form the last batched pass ran:
                                           0 none, 1 one hit per lane, 2 two hits per lane */
+#define HSPZ_FLAG_DUMP_RAW 15          /* Engine::dump_raw (gpu/mod.rs:55-67): 1 = hspz_seed_and_filter keeps
+                                          the pre-dedup records in hspz_filter_output.raw; 0 (default) */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

@@ -533,14 +533,13 @@ def test_batch_coarse_forms(gpu_available, name, acgt, xdrop, used):
     chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
     with E.Engine(index, pos, S.encode(target), sub, xdrop=xdrop, hspthresh=thresh, max_hits=16711680) as eng:
         eng.set_flag("batch_order", 1)
-        # the candidates' gate with its loads issued ahead (lookahead on) and without
         for threads in (1024, 512):
             eng.set_flag("batch_block_threads", threads)
-            for coarse, u, ahead in ((1, used, 1), (1, used, 0), (2, 1, 1), (2, 1, 0), (0, 0, 1)):
+            for coarse, u in ((1, used), (2, 1), (0, 0)):
                 eng.set_flag("batch_coarse", coarse)
-                eng.set_flag("batch_lookahead", ahead)
-                got, _ = eng.filter_batch(chunks)
+                got, st = eng.filter_batch(chunks)
                 assert eng.get_flag("batch_coarse_used") == u
+                if coarse:  # homologous chunks: the bound keeps many hits, never more than there are
+                    assert 0 < eng.stat("batch_candidates") <= st["hits"]
                 for i, (g, w) in enumerate(zip(got, want)):
-                    assert same_intervals(g, w), "chunk %d differs in coarse mode %d, lookahead %d, %d threads" % (
-                        i, coarse, ahead, threads)
+                    assert same_intervals(g, w), "chunk %d differs in coarse mode %d, %d threads" % (i, coarse, threads)

@@ -128,7 +128,7 @@ def test_c5_serving_1gbp_poisson(gpu_available):
     reqs = [_raw([("r%d" % i, p)]) for i, p in enumerate(pieces)]
     _say("  generated in %.1fs: %d requests, %d bp" % (time.time() - t0, len(reqs), sum(map(len, pieces))))
     e1 = E.Engine(None, None, ref_enc, sub, max_hits=MAX_HITS, device_index="12of19")
-    e2 = E.Engine(None, None, ref_enc, sub, max_hits=MAX_HITS, device_index="12of19")
+    e2 = e1.share()  # one resident target for both serving engines
     gaps = np.random.default_rng(0xC5).exponential(1.0 / 400, len(reqs))
     ids = [None] * len(reqs)
     with E.Server([e1, e2], max_batch_requests=16) as srv:
