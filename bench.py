@@ -49,14 +49,20 @@ HBM_PEAK_GBS = 8000.0
 W_PACKED = 0.25 + 1.0 / 512
 
 
-def profiled_traffic(hits_per_launch, order_used):
-    """HBM bytes per launch of the batched gate (the k-mer-major gate and the
-    deferred-hit gates),
-    FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from the committed rocprofv3 counter
-    summary of this same workload; None when this run's launch is not that one
-    (another hit order or hit count)."""
-    path = os.path.join(ROOT, "profiles", "kmer_gate_1gbp_target.md")
-    if order_used != 1 or not os.path.exists(path):
+def gate_profile(order_used, coarse_used):
+    """The committed rocprofv3 summary of this gate path on the bench workload."""
+    if not order_used:
+        return "profiles/seed_gate_1gbp_target.md"
+    return "profiles/coarse_gate_1gbp_target.md" if coarse_used else "profiles/kmer_gate_1gbp_target.md"
+
+
+def profiled_traffic(hits_per_launch, order_used, coarse_used):
+    """HBM bytes per launch of the batched gate (its kernels for every hit and
+    the deferred-hit gates), FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from the
+    committed rocprofv3 counter summary of this same workload and path; None
+    when this run's launch is not that one (another hit count)."""
+    path = os.path.join(ROOT, gate_profile(order_used, coarse_used))
+    if not os.path.exists(path):
         return None
     m = re.search(r"traffic-line: \S+ hits=(\d+) .*bytes_per_launch=(\d+)", open(path).read())
     if not m or int(m.group(1)) != hits_per_launch:
@@ -93,7 +99,7 @@ def main():
     ap.add_argument("--tune-reps", type=int, default=1)
     ap.add_argument("--no-io", action="store_true", help="skip the host input/output timing")
     ap.add_argument("--gate", default="",
-                    help="step_cells,block_threads,claim_seeds,grid_blocks[,order[,lookahead]]] for the batched gate "
+                    help="step_cells,block_threads,claim_seeds,grid_blocks[,order[,lookahead[,coarse]]] for the batched gate "
                          "(applied after the autotuner; use with --tune-mbp 0 to fix the configuration)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once and exit: the plain copy of known size "
@@ -212,12 +218,13 @@ def main():
         step, thr, claim, grid = vals[:4]
         order = vals[4] if len(vals) > 4 else 2
         ahead = vals[5] if len(vals) > 5 else eng.get_flag("batch_lookahead")
+        coarse = vals[6] if len(vals) > 6 else eng.get_flag("batch_coarse")
         for name, v in (("batch_step_cells", step), ("batch_block_threads", thr),
                         ("batch_claim_seeds", claim), ("batch_gate_blocks", grid), ("batch_order", order),
-                        ("batch_lookahead", ahead)):
+                        ("batch_lookahead", ahead), ("batch_coarse", coarse)):
             eng.set_flag(name, v)
-        log("gate fixed: step %d threads %d claim %d grid %d order %d lookahead %d"
-            % (step, thr, claim, grid, order, ahead))
+        log("gate fixed: step %d threads %d claim %d grid %d order %d lookahead %d coarse %d"
+            % (step, thr, claim, grid, order, ahead, coarse))
 
     for i in range(args.warmup):
         res_b, st = batched()
@@ -234,8 +241,12 @@ def main():
     cand_step = eng.stat("batch_candidates")  # of the last timed step
     g = {k: eng.stat(k) - g0[k] for k in g0}
     order_used = eng.get_flag("batch_order_used")
+    coarse_used = eng.get_flag("batch_coarse_used")
     if not order_used:
         gate_kernel = "k_bgate_packed"
+    elif coarse_used:
+        gate_kernel = "%s (coarse bound over every hit) + k_bgate_exact_ahead (its candidates)" % (
+            "k_bgate_coarse2" if coarse_used == 2 else "k_bgate_coarse")
     elif (eng.get_flag("batch_step_cells") >= 96 and eng.get_flag("batch_block_threads") == 512
           and eng.get_flag("batch_lookahead") and eng.get_flag("ref_records") == 2):
         gate_kernel = "k_bgate_ahead"
@@ -280,14 +291,19 @@ def main():
         per_hit = 4.0 + 2.0 * W_PACKED * cells_per_hit
         secs = g["gate_ns"] * 1e-9
         achieved = g["gate_hits"] * per_hit / secs / 1e9
-        traffic = profiled_traffic(st_b["hits"] if st_b else 0, order_used)
+        traffic = profiled_traffic(st_b["hits"] if st_b else 0, order_used, coarse_used)
+        prof = gate_profile(order_used, coarse_used)
+        if not order_used:
+            limit = "memory fetch: each hit fetches its own reference lines (%s)" % prof
+        elif coarse_used:
+            limit = ("VALU issue: the coarse bound over every hit keeps 95%% of the SIMDs' issue slots busy "
+                     "(2.7 wave-instructions per hit); HBM at 13.4 B/hit (%s)" % prof)
+        else:
+            limit = "VALU issue: the two-cell maps over every hit saturate the vector ALU (%s)" % prof
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": "profiles/kmer_gate_1gbp_target.md (FETCH_SIZE x2.0 + WRITE_SIZE, "
-                                  "same workload)" if traffic else None,
-                "binding_limit": "VALU issue: the vector ALU is saturated by the X-drop cell pairs; HBM at "
-                                 "7.8 B/hit (profiles/kmer_gate_1gbp_target.md)"
-                                 if order_used else "L2-miss bandwidth of random reference lines",
+                "traffic_source": "%s (FETCH_SIZE x2.0 + WRITE_SIZE, same workload)" % prof if traffic else None,
+                "binding_limit": limit,
                 "kernel": "%s (+ k_bgate_long, k_bgate_wave for deferred hits)" % gate_kernel,
                 "hit_order": "k-mer-major" if order_used else "seed-major",
                 "avg_launch_us": round(g["gate_ns"] / max(g["gate_launches"], 1) / 1e3, 1),
