@@ -465,7 +465,7 @@ def c4_workload(args):
                    max_hits=max_hits, device_index="12of19")
     log("engine up in %.1fs: device index %.0f ms, %d positions, %.1f GB held"
         % (time.time() - t1, eng.stat("index_build_ns") / 1e6, eng.stat("pos_len"), eng.stat("device_bytes") / 1e9))
-    tot_bp, tot_s, hits, hsps, raw_hsps, passes, per = 0, 0.0, 0, 0, 0, 0, []
+    tot_bp, tot_s, hits, hsps, raw_hsps, passes, per, parts = 0, 0.0, 0, 0, 0, 0, [], []
     first = None
     for k in range(args.c4_chunks):
         t2 = time.time()
@@ -473,9 +473,11 @@ def c4_workload(args):
         if first is None:
             first = q
         gen = time.time() - t2
+        p0 = {n: eng.stat(n) for n in ("batch_prep_ns", "batch_device_ns", "batch_read_ns", "batch_post_ns")}
         t3 = time.perf_counter()
         res, st = eng.filter_batch([(q, None)])
         dt = time.perf_counter() - t3
+        parts.append({n[6:-3]: round((eng.stat(n) - v) / 1e9, 3) for n, v in p0.items()})
         tot_bp += len(q)
         tot_s += dt
         hits += st["hits"]
@@ -483,8 +485,9 @@ def c4_workload(args):
         raw_hsps += st["raw_hsps"]
         passes += eng.stat("batch_passes")
         per.append(round(dt, 3))
-        log("c4 chunk %d: %d bp (generated in %.1fs) in %.2fs = %.2f Mbp/s; %d hits, %d HSPs, %d device passes"
-            % (k, len(q), gen, dt, len(q) / 1e6 / dt, st["hits"], st["hsps"], eng.stat("batch_passes")))
+        log("c4 chunk %d: %d bp (generated in %.1fs) in %.2fs = %.2f Mbp/s; %d hits, %d HSPs, %d device passes; "
+            "seconds by part %s" % (k, len(q), gen, dt, len(q) / 1e6 / dt, st["hits"], st["hsps"],
+                                    eng.stat("batch_passes"), parts[-1]))
     # parity sample
     blk, sl = first[10_000_000:11_000_000], first[20_000_000:20_020_000]
     res_b, _ = eng.filter_batch([(blk, None), (sl, None)])
@@ -511,6 +514,7 @@ def c4_workload(args):
                       "target_bp": bl, "query_bp": tot_bp, "chunks": args.c4_chunks, "max_hits": max_hits},
            "positions": eng.stat("pos_len"), "hits": hits, "hsps": hsps, "raw_hsps": raw_hsps,
            "device_passes": passes, "overflow_reruns": eng.stat("batch_reruns"), "seconds_per_chunk": per,
+           "seconds_by_part": parts,
            "peak_device_bytes": eng.stat("peak_device_bytes"), "device_bytes_held": eng.stat("device_bytes"),
            "ref_records": eng.get_flag("ref_records") == 2,
            "index_build_ms": round(eng.stat("index_build_ns") / 1e6, 1),

@@ -51,6 +51,14 @@ bool grow_list_cap(uint32_t cap, uint32_t count, uint32_t *out) {
 
 }  // namespace
 
+// Test hook (not part of the ABI in include/hspz.h): the staged read's piece
+// size, so that a GPU test covers many pieces with a small output.
+extern "C" int hspz_test_set_read_piece(hspz_engine *h, size_t bytes) {
+    if (!h || bytes < 4096 || bytes % 16) return 1;
+    h->read_piece = bytes;
+    return 0;
+}
+
 // Test hook (not part of the ABI in include/hspz.h): grow_list_cap for the
 // CPU unit test of the growth arithmetic; 1 when the grown list holds count.
 extern "C" int hspz_test_grow_list_cap(uint32_t cap, uint32_t count, uint32_t *out) {
@@ -236,13 +244,75 @@ int device_dedup(hspz_engine *h, BatchBufs &B, uint32_t n, uint32_t *kept) {
 // in output order): kept HSPs with their scope keys (local batch index *
 // MAX_CHUNKS + max_hits chunk), grouped by ascending scope and deduplicated
 // when *deduped comes back true.
+// Copies n bytes of device memory to pageable host memory through two pinned
+// staging halves: the DMA of one piece overlaps the host threads that copy
+// the previous piece out (and take the destination's first-touch page faults
+// in parallel). Small reads go directly.
+int read_to_host(hspz_engine *h, void *dst, const void *src, size_t n) {
+    const size_t READ_STAGE_BYTES = h->read_piece;  // bytes per staging half (64 MiB)
+    if (n < READ_STAGE_BYTES / 8) {
+        HIP_TRY(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (h->read_stage && h->read_stage_piece != READ_STAGE_BYTES) {
+        (void)hipHostFree(h->read_stage);
+        h->read_stage = nullptr;
+    }
+    if (!h->read_stage) {
+        if (hipHostMalloc((void **)&h->read_stage, 2 * READ_STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            h->read_stage = nullptr;
+            HIP_TRY(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            return 0;
+        }
+        h->read_stage_piece = READ_STAGE_BYTES;
+    }
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto copy_out = [&](size_t off, size_t len, const uint8_t *from) {
+        std::vector<std::thread> th;
+        const size_t per = (len + nth - 1) / nth;
+        for (unsigned t = 0; t < nth && (size_t)t * per < len; t++) {
+            const size_t a = (size_t)t * per, b = std::min(len, a + per);
+            th.emplace_back([=] { memcpy((uint8_t *)dst + off + a, from + a, b - a); });
+        }
+        for (auto &x : th) x.join();
+    };
+    const size_t pieces = (n + READ_STAGE_BYTES - 1) / READ_STAGE_BYTES;
+    for (size_t k = 0; k <= pieces; k++) {
+        if (k < pieces) {  // this piece's DMA into its half (that half's last piece was copied out already)
+            const size_t off = k * READ_STAGE_BYTES, len = std::min(READ_STAGE_BYTES, n - off);
+            HIP_TRY(hipMemcpyAsync(h->read_stage + (k & 1) * READ_STAGE_BYTES, (const uint8_t *)src + off, len,
+                                   hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(hipEventRecord(h->ev_stage[k & 1], h->stream));
+        }
+        if (k > 0) {  // the previous piece has landed: out to the destination while this one copies
+            const size_t j = k - 1, off = j * READ_STAGE_BYTES, len = std::min(READ_STAGE_BYTES, n - off);
+            HIP_TRY(hipEventSynchronize(h->ev_stage[j & 1]));
+            copy_out(off, len, h->read_stage + (j & 1) * READ_STAGE_BYTES);
+        }
+    }
+    return 0;
+}
+
+// Where a device-deduplicated pass puts its lists: appended to a growing
+// host array (already in the call's output order: scopes ascend in batch
+// order, and a list's batches are consecutive), with each batch's record
+// count in batch_counts. Passes deduplicated on the host fill hsps/scopes.
+struct DirectOut {
+    hspz_segment_pair *buf = nullptr;
+    size_t len = 0, cap = 0;
+    std::vector<uint32_t> batch_counts;
+};
+
 int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                const hspz_query_chunk *chunks, size_t n_chunks, const std::vector<HostBatch> &hb,
                std::vector<hspz_segment_pair> &hsps, std::vector<uint32_t> &scopes,
-               hspz_pass_stats *st, bool *deduped, bool use_graph) {
+               hspz_pass_stats *st, bool *deduped, bool use_graph, DirectOut *dout) {
     *deduped = false;
     BatchBufs &B = h->batch->b;
-    const auto t_begin = std::chrono::steady_clock::now();
+    auto t_begin = std::chrono::steady_clock::now();
     // ---- layout: blocks (chunk c fwd = 2c, rc = 2c + 1), each after QPAD code-7 bytes
     std::vector<uint32_t> blk_off(2 * n_chunks), blk_len(2 * n_chunks);
     uint64_t off = QPAD;
@@ -696,6 +766,11 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             bs->cap_slow = nd;
             bs->cap_cand = nc;
             h->batch_reruns++;
+            // the failed attempt's time is device time, not the rerun's preparation
+            const auto t_now = std::chrono::steady_clock::now();
+            h->batch_device_ns += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t_now - t_launch)
+                                      .count();
+            t_begin = t_now;
             continue;
         }
         std::vector<uint64_t> bh(nb);
@@ -710,11 +785,39 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             src_scope = B.mscope.p;
             *deduped = true;
         }
-        hsps.resize(nout);
-        scopes.resize(nout);
-        if (nout) {
-            HIP_TRY(hipMemcpyAsync(hsps.data(), src_hsp, (size_t)nout * 16, hipMemcpyDeviceToHost, h->stream));
-            HIP_TRY(hipMemcpyAsync(scopes.data(), src_scope, (size_t)nout * 4, hipMemcpyDeviceToHost, h->stream));
+        if (dout && h->device_dedup) {
+            // straight into the call's output: each batch's count from the sorted
+            // scopes (B.starts is free once the pass has run), then the records
+            dout->batch_counts.assign(nb, 0);
+            if (nout) {
+                *deduped = true;
+                uint32_t *bstart = B.starts.as<uint32_t>();
+                hipLaunchKernelGGL(k_scope_starts, dim3(ceil_div(nb + 1, 256)), dim3(256), 0, h->stream,
+                                   (const uint32_t *)src_scope, nout, nb, MAX_CHUNKS, bstart);
+                HIP_TRY(hipGetLastError());
+                std::vector<uint32_t> bs0(nb + 1);
+                HIP_TRY(hipMemcpyAsync(bs0.data(), bstart, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, h->stream));
+                if (dout->len + nout > dout->cap) {
+                    const size_t cap = std::max(dout->len + nout, dout->cap + dout->cap / 2);
+                    void *nbuf = realloc(dout->buf, cap * 16);
+                    if (!nbuf) return set_error(HSPZ_E_OOM, "host output allocation of %zu records", cap);
+                    dout->buf = (hspz_segment_pair *)nbuf;
+                    dout->cap = cap;
+                }
+                HIP_TRY(hipStreamSynchronize(h->stream));
+                if ((rc = read_to_host(h, dout->buf + dout->len, src_hsp, (size_t)nout * 16))) return rc;
+                for (uint32_t b = 0; b < nb; b++) dout->batch_counts[b] = bs0[b + 1] - bs0[b];
+                dout->len += nout;
+            }
+            hsps.clear();
+            scopes.clear();
+        } else {
+            hsps.resize(nout);
+            scopes.resize(nout);
+            if (nout) {
+                HIP_TRY(hipMemcpyAsync(hsps.data(), src_hsp, (size_t)nout * 16, hipMemcpyDeviceToHost, h->stream));
+                HIP_TRY(hipMemcpyAsync(scopes.data(), src_scope, (size_t)nout * 4, hipMemcpyDeviceToHost, h->stream));
+            }
         }
         HIP_TRY(hipStreamSynchronize(h->stream));
         const auto t_read = std::chrono::steady_clock::now();
@@ -838,6 +941,14 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
     h->batch_passes_last = passes.size();
     hspz_pass_stats st = {0, 0, 0, 0, 0};
     double post_ns = 0;
+    // device dedup: every pass appends its lists, in output order, to one array
+    DirectOut dout;
+    std::vector<size_t> dcount(nlists, 0);
+    struct FreeOnExit {
+        DirectOut &d;
+        ~FreeOnExit() { free(d.buf); }
+    } free_on_exit{dout};
+    const bool direct = h->device_dedup;
     for (const auto &ps : passes) {
         const size_t cf = all[ps.first].chunk, cl = all[ps.second - 1].chunk;
         std::vector<HostBatch> hb(all.begin() + ps.first, all.begin() + ps.second);
@@ -845,11 +956,22 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
         std::vector<hspz_segment_pair> raw;
         std::vector<uint32_t> scopes;
         bool deduped = false;
+        dout.batch_counts.clear();
         // one pass per call replays a captured graph; several run eagerly (their shapes differ)
         if (int rc = batch_pass(h, s, p, chunks + cf, cl - cf + 1, hb, raw, scopes, &st, &deduped,
-                                passes.size() == 1))
+                                passes.size() == 1, direct ? &dout : nullptr))
             return rc;
         const auto t_post = std::chrono::steady_clock::now();
+        if (direct) {
+            for (size_t b = 0; b < dout.batch_counts.size(); b++) {
+                const HostBatch &x = all[ps.first + b];
+                dcount[2 * (ibase[x.chunk] + x.interval) + x.rev] += dout.batch_counts[b];
+            }
+            post_ns += (double)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                           std::chrono::steady_clock::now() - t_post)
+                           .count();
+            continue;
+        }
         // group by scope (local batch * MAX_CHUNKS + chunk): ascending scope = output order
         std::vector<uint32_t> order(raw.size());
         std::vector<hspz_segment_pair> grouped;
@@ -895,6 +1017,31 @@ extern "C" int hspz_filter_batch(hspz_engine *h, const hspz_pass_params *p, cons
                        .count();
     }
     const auto t_out = std::chrono::steady_clock::now();
+    if (direct) {
+        st.hsps = dout.len;
+        out->counts = (size_t *)malloc(std::max<size_t>(nlists, 1) * sizeof(size_t));
+        out->chunk_intervals = (size_t *)malloc(std::max<size_t>(n_chunks, 1) * sizeof(size_t));
+        if (!out->counts || !out->chunk_intervals) {
+            hspz_batch_output_free(out);
+            return set_error(HSPZ_E_OOM, "host output allocation");
+        }
+        if (!dout.buf) dout.buf = (hspz_segment_pair *)malloc(16);  // an empty call still returns an array
+        if (!dout.buf) {
+            hspz_batch_output_free(out);
+            return set_error(HSPZ_E_OOM, "host output allocation");
+        }
+        out->hsps = dout.buf;  // malloc / realloc: 16-byte aligned, released by free
+        dout.buf = nullptr;
+        for (size_t i = 0; i < nlists; i++) out->counts[i] = dcount[i];
+        for (size_t c = 0; c < n_chunks; c++) out->chunk_intervals[c] = civ[c];
+        out->n_hsps = dout.len;
+        out->n_chunks = n_chunks;
+        h->batch_post_ns += (uint64_t)post_ns + (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                                    std::chrono::steady_clock::now() - t_out)
+                                                    .count();
+        if (stats) *stats = st;
+        return 0;
+    }
     size_t total = 0;
     for (auto &l : lists) total += l.size();
     st.hsps = total;

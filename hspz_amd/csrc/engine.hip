@@ -74,6 +74,7 @@ struct hspz_engine {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_gate0 = nullptr, ev_gate1 = nullptr, ev_post1 = nullptr;
+    hipEvent_t ev_stage[2] = {nullptr, nullptr};  // read_to_host's staging halves
     // reference side (resident)
     DevBuf ref, index, pos, sub;
     uint32_t ref_len = 0;
@@ -155,6 +156,9 @@ struct hspz_engine {
     std::unordered_map<uint64_t *, size_t> pinned_out;        // handed-out buffers
     bool pinned_ok = false;                                   // a pinned buffer was handed out
     bool dump_raw = false;  // HSPZ_FLAG_DUMP_RAW: hspz_seed_and_filter keeps its pre-dedup records
+    // pinned staging for reading large batched outputs (two halves, used in turn)
+    uint8_t *read_stage = nullptr;
+    size_t read_piece = 64u << 20, read_stage_piece = 0;  // piece size; the staging's piece size
     // the resident target when shared between engines (hspz_engine_share): it
     // owns ref, index, pos, ref2, refx, clean and rrec, freed with its last holder
     std::shared_ptr<ResidentTarget> resident;
@@ -589,7 +593,9 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
     };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_error(-1, "stream"));
     if (hipEventCreate(&h->ev_gate0) != hipSuccess || hipEventCreate(&h->ev_gate1) != hipSuccess ||
-        hipEventCreate(&h->ev_post1) != hipSuccess)
+        hipEventCreate(&h->ev_post1) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_stage[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_stage[1], hipEventDisableTiming) != hipSuccess)
         return fail(set_error(-1, "events"));
     if (hipHostMalloc((void **)&h->h_small, H_SMALL_BYTES, hipHostMallocDefault) != hipSuccess)
         return fail(set_error(HSPZ_E_OOM, "pinned scratch"));
@@ -862,12 +868,15 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
         (void)hipStreamSynchronize(h->copy_stream);
         (void)hipStreamDestroy(h->copy_stream);
     }
+    if (h->read_stage) (void)hipHostFree(h->read_stage);
     for (auto &b : h->pinned_free) (void)hipHostFree(b.first);
     for (auto &b : h->pinned_out) (void)hipHostFree(b.first);
     if (h->h_small) (void)hipHostFree(h->h_small);
     if (h->ev_gate0) (void)hipEventDestroy(h->ev_gate0);
     if (h->ev_gate1) (void)hipEventDestroy(h->ev_gate1);
     if (h->ev_post1) (void)hipEventDestroy(h->ev_post1);
+    for (hipEvent_t e : h->ev_stage)
+        if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }

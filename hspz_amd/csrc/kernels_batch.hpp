@@ -1992,6 +1992,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     cand_gate_ahead<SUMS>(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
+// First index of each batch's records in a scope-sorted list (scope = batch
+// * cpb + max_hits chunk): starts[b] = lower bound of b * cpb, for b in
+// [0, nb] (starts[nb] = n).
+__global__ void k_scope_starts(const uint32_t *__restrict__ scopes, uint32_t n, uint32_t nb, uint32_t cpb,
+                               uint32_t *__restrict__ starts) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nb) return;
+    const uint64_t key = (uint64_t)b * cpb;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (scopes[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    starts[b] = lo;
+}
+
 // Block bounds of a seed's batch: arena offset and length of its block.
 struct BlockOf {
     const uint64_t *seeds;

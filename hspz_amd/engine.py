@@ -190,6 +190,7 @@ def lib():
         L.hspz_pinned_give.argtypes = [vp, vp]
         L.hspz_check_seed_bytes.argtypes = [vp, C.c_int, vp, sz]
         L.hspz_launch_overhead_ms.argtypes = [vp, u32, C.POINTER(C.c_double)]
+        L.hspz_test_set_read_piece.argtypes = [vp, sz]
         _lib = L
     return _lib
 

@@ -543,3 +543,28 @@ def test_batch_coarse_forms(gpu_available, name, acgt, xdrop, used):
                     assert 0 < eng.stat("batch_candidates") <= st["hits"]
                 for i, (g, w) in enumerate(zip(got, want)):
                     assert same_intervals(g, w), "chunk %d differs in coarse mode %d, %d threads" % (i, coarse, threads)
+
+
+def test_batch_staged_read(gpu_available):
+    """The device-deduplicated lists read through the pinned staging halves in
+    many pieces (a 4 KiB piece for the test; 64 MiB in use) equal the direct
+    read and the oracle."""
+    _say("case batch: output read through staged pieces")
+    target = synth.random_bases(0x57A6, 300_000).tobytes()
+    blocks = _chunks_from(target, 24, 0x57A6, lo=5_000, hi=60_000, div=(0.0, 0.1))
+    sub = O.sub_mat("", 910)
+    index, pos = O.seed_table(target, O.shape("12of19"), 1)
+    want = []
+    for f, r in blocks:
+        o = O.Oracle(target, f, r, sub, max_hits=16711680, index=(index, pos))
+        want.append(o.run_pass(threads=16)[0])
+    chunks = [(S.encode(f), S.encode(r)) for f, r in blocks]
+    with E.Engine(index, pos, S.encode(target), sub, max_hits=16711680) as eng:
+        direct, st = eng.filter_batch(chunks)
+        assert st["hsps"] > 1000  # at least four 4 KiB pieces of 256 records
+        assert E.lib().hspz_test_set_read_piece(eng._h, 4096) == 0
+        for pmax in (0, 1):  # one device pass, and a pass per batch
+            eng.set_flag("batch_max_positions", pmax)
+            staged, _ = eng.filter_batch(chunks)
+            for i, (a, b, w) in enumerate(zip(staged, direct, want)):
+                assert same_intervals(a, b) and same_intervals(a, w), "chunk %d (passes %d)" % (i, pmax)
