@@ -607,14 +607,14 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                B.cand.as<uint4>(), ctr + CTR_CAND, bs->cap_cand, B.slow.as<SurvRec>(),
                                ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
             hipLaunchKernelGGL(xgate_fn, dim3(xgrid), dim3(xthreads), 0, h->stream, h->ref2.as<uint32_t>(),
-                               h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
+                               h->pairtab.as<uint32_t>() + PAIRTAB_GRAY, h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
                                B.kvalo.as<uint32_t>(), h->rrec.as<uint4>(), B.cand.as<uint4>(), ctr + CTR_CAND,
                                bs->cap_cand, (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK2),
                                B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(),
                                ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
         } else if (kmer_order)
             hipLaunchKernelGGL(kgate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
-                               h->pairtab.as<uint32_t>(), h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
+                               h->pairtab.as<uint32_t>() + PAIRTAB_GRAY, h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
                                B.skm.as<uint32_t>(), B.kvalo.as<uint32_t>(), B.gcnt.as<uint32_t>(), nkm,
                                h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->clean.as<uint32_t>(),
                                h->rrec.p ? h->rrec.as<uint4>() : nullptr, h->seed_size, (uint32_t)h->bgate_claim,

@@ -118,8 +118,8 @@ struct hspz_engine {
                                        // (0 off, 1 paired form where the class scores fit, 2 single-hit form)
     int coarse_used = 0;               // the form the last batched pass ran (0: none)
     bool coarse_fits16 = false;        // every |class score| <= COARSE_PAIR_MAX_CLASS
-    bool coarse_pv = false;            // the transversion class adds to the positive part
-    CoarseCoef coarse_cc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool coarse_pv = false;            // the set-bit count adds to the coarse bound's positive part
+    CoarseCoef coarse_cc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t last_cand = 0;            // candidates of the last batched pass
     int gate_blocks = 2048;
     // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
@@ -687,32 +687,29 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
                     u = std::max(u, v);
                 }
             const int32_t pm = std::max(um, 0), pt = std::max(ut, 0), pv = std::max(uv, 0);
-            // nw mismatches, nv of them transversions: nm = 16 - nw, nt = nw - nv
+            // over the Gray-coded words: nd differing cells, np set xor bits,
+            // nt = np - nd transitions, nv = 2 nd - np transversions, 16 - nd matches
             CoarseCoef &cc = h->coarse_cc;
-            cc = {16 * um, ut - um, uv - ut, 16 * pm, pt - pm, pv - pt, 0, 0, 0, 0, 0, 0, 0, 0};
-            // paired form: matches and transversions counted, biases folding 16 ut away where they exist
-            cc.a1 = um - ut;
-            cc.a2 = uv - ut;
-            cc.qm = pm - pt;
-            cc.qv = pv - pt;
-            cc.c0 = 16 * ut;
+            cc = {16 * um, 2 * uv - ut - um, ut - uv, 16 * pm, 2 * pv - pt - pm, pt - pv, 0, 0, 0, 0};
+            // paired form: biases folding the constant 16 um into the counts where they exist
+            cc.c0 = cc.s0;
             bool bias = false;
-            for (int32_t reach = 0; reach <= 256 && !bias; reach++)  // smallest |bm| + |bv| first
-                for (int32_t bm = -reach; bm <= reach && !bias; bm++)
-                    for (int32_t bv : {reach - std::abs(bm), std::abs(bm) - reach})
-                        if (!bias && (int64_t)cc.a1 * bm + (int64_t)cc.a2 * bv == 16 * (int64_t)ut) {
-                            cc.bm = bm;
-                            cc.bv = bv;
+            for (int32_t reach = 0; reach <= 256 && !bias; reach++)  // smallest |bd| + |bp| first
+                for (int32_t bd = -reach; bd <= reach && !bias; bd++)
+                    for (int32_t bp : {reach - std::abs(bd), std::abs(bd) - reach})
+                        if (!bias && (int64_t)cc.sd * bd + (int64_t)cc.sp * bp == (int64_t)cc.s0) {
+                            cc.bd = bd;
+                            cc.bp = bp;
                             bias = true;
                         }
             if (bias) cc.c0 = 0;
-            cc.pc = 16 * pt - cc.qm * cc.bm - cc.qv * cc.bv;
+            cc.pc = cc.p0 - cc.pd * cc.bd - cc.pp * cc.bp;
             const int32_t a = std::max({std::abs(um), std::abs(ut), std::abs(uv)});
             h->coarse_fits16 = a <= COARSE_PAIR_MAX_CLASS && 128 * a + std::abs(cc.pc) <= 32767;
-            h->coarse_pv = cc.qv != 0;
+            h->coarse_pv = cc.pp != 0;
         }
         if (ok) {
-            uint32_t tab[512];
+            uint32_t tab[1024];
             const int32_t X = cfg->xdrop;
             // the separator forces a drop against every code (as build_sub_mat's -10 * xdrop does)
             h->sep_drops = true;
@@ -734,6 +731,12 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
                 int32_t bf = X + std::min(hi, 0), br = X + std::min(lo, 0);
                 tab[key] = ((uint32_t)a & 0xFFFFu) | ((uint32_t)bf << 16);
                 tab[256 + key] = ((uint32_t)a & 0xFFFFu) | ((uint32_t)br << 16);
+            }
+            // the same maps keyed by Gray-coded cells, for the gates that read records
+            for (uint32_t key = 0; key < 256; key++) {
+                const uint32_t g = key ^ ((key >> 1) & 0x55u);  // each 2-bit field through its Gray code
+                tab[512 + key] = tab[g];
+                tab[768 + key] = tab[256 + g];
             }
             if ((rc = reserve(h, h->pairtab, sizeof tab))) return fail(rc);
             if ((rc = reserve(h, h->slow_n, 256))) return fail(rc);

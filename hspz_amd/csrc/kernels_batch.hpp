@@ -405,6 +405,20 @@ __device__ __forceinline__ uint32_t nib_swap(uint32_t x) {
     return ((x >> 4) & 0x0F0F0F0Fu) | ((x << 4) & 0xF0F0F0F0u);
 }
 
+// The 2-bit Gray code of every cell of a packed word (code c -> c ^ (c >> 1):
+// A0 C1 G3 T2). It is its own inverse and moves no cell, so it commutes with
+// the word alignments and swaps above. The reference and query records hold
+// their words in this code: the xor of two cells is then 00 for a match, 11
+// for a transition and 01 or 10 for a transversion, so a word's class counts
+// come from one unmasked popcount (matches 0, transversions 1, transitions 2
+// per cell) and one masked one (cells that differ). The two-cell maps that
+// read records index the Gray-keyed maps of the pair table (PAIRTAB_GRAY).
+__device__ __forceinline__ uint32_t gray2(uint32_t w) { return w ^ ((w >> 1) & 0x55555555u); }
+
+// Offset of the Gray-keyed maps in the engine's pair table (the plane-keyed
+// ones come first).
+constexpr uint32_t PAIRTAB_GRAY = 512;
+
 // Per pos_table entry: 1 when the anchor R = pos + seed_size has the packed
 // gate's margins and no exception code in its window (the reference-side
 // test of packed_gate_loop, done once per engine).
@@ -448,10 +462,11 @@ __global__ void k_ref_records(const uint32_t *__restrict__ pos, uint32_t n, cons
             for (int k = 0; k < 13; k++) W[k] = rp[k];  // word 8 holds base R
             uint32_t sr = 2 * (R & 15);
 #pragma unroll
-            for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) A[k] = __builtin_amdgcn_alignbit(W[9 + k], W[8 + k], sr);
+            for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) A[k] = gray2(__builtin_amdgcn_alignbit(W[9 + k], W[8 + k], sr));
 #pragma unroll
             for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
-                A[G2_RIGHT_WORDS + k] = nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sr)));
+                A[G2_RIGHT_WORDS + k] =
+                    gray2(nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sr))));
         }
         uint4 *o = out + 3 * (size_t)i;
         o[0] = make_uint4(A[0], A[1], A[2], A[3]);
@@ -584,10 +599,10 @@ __global__ __launch_bounds__(256) void k_qrecords(const uint64_t *__restrict__ s
             uint32_t sq2 = 2 * (Q & 15);
 #pragma unroll
             for (int k = 0; k < (int)G2_RIGHT_WORDS; k++)
-                r.right[k] = __builtin_amdgcn_alignbit(W[9 + k], W[8 + k], sq2);
+                r.right[k] = gray2(__builtin_amdgcn_alignbit(W[9 + k], W[8 + k], sq2));
 #pragma unroll
             for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
-                r.left[k] = nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sq2)));
+                r.left[k] = gray2(nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sq2))));
             r.ok = ok ? 1u : 0u;
             const uint4 *s = (const uint4 *)&r;
 #pragma unroll
@@ -845,6 +860,11 @@ __device__ __forceinline__ void kmer_gate_loop(Src &src, const uint32_t *__restr
             for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
                 LA0[k] = nib_swap(__builtin_bswap32(
                     __builtin_amdgcn_alignbit(LA0[k], k + 1 < (int)G2_LEFT_WORDS ? LA0[k + 1] : lend, sr)));
+            // the query records are Gray-coded: so are the reference words they meet
+#pragma unroll
+            for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) A0[k] = gray2(A0[k]);
+#pragma unroll
+            for (int k = 0; k < (int)G2_LEFT_WORDS; k++) LA0[k] = gray2(LA0[k]);
         }
         if (fresh_q) {
             uint32_t var;
@@ -1176,6 +1196,11 @@ __device__ __forceinline__ void kmer_gate_merged(Src &src, const uint32_t *__res
             for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
                 LA0[k] = nib_swap(__builtin_bswap32(
                     __builtin_amdgcn_alignbit(LA0[k], k + 1 < (int)G2_LEFT_WORDS ? LA0[k + 1] : lend, sr)));
+            // the query records are Gray-coded: so are the reference words they meet
+#pragma unroll
+            for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) A0[k] = gray2(A0[k]);
+#pragma unroll
+            for (int k = 0; k < (int)G2_LEFT_WORDS; k++) LA0[k] = gray2(LA0[k]);
         }
         if (fresh_q) {
             uint32_t var;
@@ -1471,38 +1496,41 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
 // class scores of the word) over the words up to that boundary: B. A hit
 // whose sides both drop within the windows and whose B_right + B_left fails
 // the gate's keep test (monotone in the total) fails the exact test too. Per
-// word: the classes come from r ^ q as two masks and two popcounts.
+// word: the classes come from the xor of the two Gray-coded words (gray2) as
+// two popcounts.
 
-// A 16-cell word with nw mismatches, nv of them transversions: its class-score
-// sum is s0 + sw * nw + sv * nv, the sum of its positive class scores
-// p0 + pw * nw + pv * nv.
-// The paired form counts matches nm and transversions nv instead (nt = 16 -
-// nm - nv): sum 16 ut + a1 nm + a2 nv (a1 = um - ut, a2 = uv - ut), positive
-// part 16 pt + qm nm + qv nv. Where small integers bm, bv with a1 bm + a2 bv
-// = 16 ut exist, the counts are taken as nm' = nm + bm and nv' = nv + bv
-// (free: the popcount instruction adds an accumulator), so a word's sum is
-// a1 nm' + a2 nv' with no constant term (c0 = 0; else c0 = 16 ut and no
-// biases). The tracked bound U + qm nm' + qv nv' is the true one less the
-// constant pc = 16 pt - qm bm - qv bv, added back at the decision.
+// A 16-cell word whose Gray-coded xor x has nd = popc((x | x >> 1) & odd-cell
+// mask) differing cells and np = popc(x) set bits (a transversion sets one, a
+// transition two): nt = np - nd transitions, nv = 2 nd - np transversions,
+// 16 - nd matches. Its class-score sum is s0 + sd * nd + sp * np (s0 = 16 um,
+// sd = 2 uv - ut - um, sp = ut - uv), the sum of its positive class scores
+// p0 + pd * nd + pp * np (the same with pm, pt, pv = max(u, 0)).
+// The paired form takes the same counts with biases: where small integers
+// bd, bp with sd bd + sp bp = s0 exist, the counts are taken as nd' = nd + bd
+// and np' = np + bp (free: the popcount instruction adds an accumulator), so a
+// word's sum is sd nd' + sp np' with no constant term (c0 = 0; else c0 = s0
+// and no biases). The tracked bound U + pd nd' + pp np' is the true one less
+// the constant pc = p0 - pd bd - pp bp, added back at the decision.
 struct CoarseCoef {
-    int32_t s0, sw, sv, p0, pw, pv;
-    int32_t a1, a2, c0, qm, qv, bm, bv, pc;  // paired form; c0 = 16 ut, or 0 with biases
+    int32_t s0, sd, sp, p0, pd, pp;
+    int32_t c0, bd, bp, pc;  // paired form; c0 = s0, or 0 with biases
 };
 
 constexpr int CW_RIGHT = 4, CW_LEFT = 7;  // coarse windows: 64 right cells, 112 left cells
 static_assert(CW_RIGHT <= (int)G2_RIGHT_WORDS && CW_LEFT <= (int)G2_LEFT_WORDS, "windows within the records");
 
-// The counts are at most 16 and the class scores small, so the products are
+// Differing cells of a Gray-coded xor: one bit per cell at its even position.
+__device__ __forceinline__ uint32_t differ_bits(uint32_t x) { return (x | (x >> 1)) & 0x55555555u; }
+
+// The counts are at most 32 and the class scores small, so the products are
 // 24-bit multiply-adds (full rate; a 32-bit multiply is quarter rate).
 __device__ __forceinline__ void coarse_word(uint32_t r, uint32_t q, const CoarseCoef &cc, int32_t xdrop,
                                             int32_t &U, int32_t &mx, int32_t &B, bool &dropped) {
     const uint32_t x = r ^ q;
-    const uint32_t tv = x & 0x55555555u;              // odd xor: transversion
-    const uint32_t ts = (x >> 1) & ~x & 0x55555555u;  // xor 2: transition
-    const int32_t nv = __popc(tv), nw = __popc(ts) + nv;
-    const int32_t bnd = U + cc.p0 + __mul24(cc.pw, nw) + __mul24(cc.pv, nv);
+    const int32_t nd = __popc(differ_bits(x)), np = __popc(x);
+    const int32_t bnd = U + cc.p0 + __mul24(cc.pd, nd) + __mul24(cc.pp, np);
     B = max(B, dropped ? (int32_t)0x80000000 : bnd);
-    U += cc.s0 + __mul24(cc.sw, nw) + __mul24(cc.sv, nv);
+    U += cc.s0 + __mul24(cc.sd, nd) + __mul24(cc.sp, np);
     mx = max(mx, U);
     dropped = dropped || mx - U > xdrop;
 }
@@ -1613,8 +1641,8 @@ __device__ __forceinline__ s16x2 splat16(int32_t v) { return as_s16x2(((uint32_t
 constexpr int32_t COARSE_PAIR_MAX_CLASS = 127;  // largest |class score| the packed form holds
 
 struct CoarsePair {
-    s16x2 a1, a2, c0, qm, qv;
-    int32_t bm, bv;
+    s16x2 sd, sp, c0, pd, pp;
+    int32_t bd, bp;
 };
 
 // Lanes of `lo` in the low half and of `hi` in the high half (a byte select).
@@ -1622,23 +1650,20 @@ __device__ __forceinline__ uint32_t pack16(uint32_t lo, uint32_t hi) {
     return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
-// One 16-cell word of two chains (query words qa and qb against reference
-// word r); U, mx, mg (minimum of U - running maximum) and B (the bound less
-// pc) per half. PV: the transversion class adds to the positive part. C0: the
-// sums carry a constant (no bias pair exists).
-template <bool PV, bool C0>
+// One 16-cell word of two chains (Gray-coded query words qa and qb against
+// reference word r); U, mx, mg (minimum of U - running maximum) and B (the
+// bound less pc) per half. PP: the set-bit count adds to the positive part.
+// C0: the sums carry a constant (no bias pair exists).
+template <bool PP, bool C0>
 __device__ __forceinline__ void coarse_pair_word(uint32_t r, uint32_t qa, uint32_t qb, const CoarsePair &cp,
                                                  s16x2 &U, s16x2 &mx, s16x2 &mg, s16x2 &B) {
-    constexpr uint32_t M = 0x55555555u;
     const uint32_t xa = r ^ qa, xb = r ^ qb;
-    const uint32_t nvp = pack16(__popc(xa & M) + cp.bv, __popc(xb & M) + cp.bv);
-    const uint32_t ma = ~(xa | (xa >> 1)) & M, mb = ~(xb | (xb >> 1)) & M;  // matching cells
-    const uint32_t nmp = pack16(__popc(ma) + cp.bm, __popc(mb) + cp.bm);
-    const s16x2 nv = as_s16x2(nvp), nm = as_s16x2(nmp);
-    s16x2 bnd = U + cp.qm * nm;
-    if constexpr (PV) bnd = bnd + cp.qv * nv;
+    const s16x2 np = as_s16x2(pack16(__popc(xa) + cp.bp, __popc(xb) + cp.bp));
+    const s16x2 nd = as_s16x2(pack16(__popc(differ_bits(xa)) + cp.bd, __popc(differ_bits(xb)) + cp.bd));
+    s16x2 bnd = U + cp.pd * nd;
+    if constexpr (PP) bnd = bnd + cp.pp * np;
     B = __builtin_elementwise_max(B, bnd);
-    s16x2 Un = U + cp.a1 * nm + cp.a2 * nv;
+    s16x2 Un = U + cp.sd * nd + cp.sp * np;
     if constexpr (C0) Un = Un + cp.c0;
     mg = __builtin_elementwise_min(mg, Un - mx);
     mx = __builtin_elementwise_max(mx, Un);
@@ -1647,7 +1672,7 @@ __device__ __forceinline__ void coarse_pair_word(uint32_t r, uint32_t qa, uint32
 
 // k_bgate_coarse with two hits per lane and iteration (the second half idle
 // when the item has one record left); same arguments, lists and decisions.
-template <int BT, bool PV, bool C0>
+template <int BT, bool PP, bool C0>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_coarse2(
     const QRec *__restrict__ recs, const uint32_t *__restrict__ skm, const uint32_t *__restrict__ sval,
     const uint32_t *__restrict__ goff, uint32_t nk, const uint32_t *__restrict__ index,
@@ -1676,8 +1701,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.err = err;
     src.set_lists(s_q[0][threadIdx.x / WAVE], s_q[1][threadIdx.x / WAVE]);
     src.init();
-    const CoarsePair cp = {splat16(cc.a1), splat16(cc.a2), splat16(cc.c0), splat16(cc.qm), splat16(cc.qv), cc.bm,
-                           cc.bv};
+    const CoarsePair cp = {splat16(cc.sd), splat16(cc.sp), splat16(cc.c0), splat16(cc.pd), splat16(cc.pp), cc.bd,
+                           cc.bp};
     const s16x2 pc = splat16(cc.pc);
     const s16x2 zero = splat16(0), lowest = splat16(-32768);
     bool has = false, item = false, rok = false;
@@ -1715,8 +1740,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
         s16x2 Ur = zero, mxr = zero, mgr = zero, Br = lowest, Ul = zero, mxl = zero, mgl = zero, Bl = lowest;
 #pragma unroll
         for (int k = 0; k < CW_LEFT; k++) {
-            if (k < CW_RIGHT) coarse_pair_word<PV, C0>(A0[k], QA[k], QB[k], cp, Ur, mxr, mgr, Br);
-            coarse_pair_word<PV, C0>(LA0[k], LQA[k], LQB[k], cp, Ul, mxl, mgl, Bl);
+            if (k < CW_RIGHT) coarse_pair_word<PP, C0>(A0[k], QA[k], QB[k], cp, Ur, mxr, mgr, Br);
+            coarse_pair_word<PP, C0>(LA0[k], LQA[k], LQB[k], cp, Ul, mxl, mgl, Bl);
         }
         // both sides dropped: the larger of the two minima below -X; the bound's
         // total (each side's bound at least 0, the sum within 0..65535) in u16

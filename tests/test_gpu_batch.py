@@ -506,8 +506,8 @@ def test_batch_score_sum_forms(gpu_available, scale, xdrop):
     # a transversion pair scoring +20: the transversion class adds to the positive part
     ("positive transversion", [91, 20, -31, -123, 20, 100, -125, -31, -31, -125, 100, -114, -123, -31, -114, 91],
      910, 2),
-    # class bests 101 / -31 / -112: a1 = 132 and a2 = -81 share the factor 3, which 16 x -31 lacks,
-    # so no count biases exist and the paired sums carry their constant
+    # class bests 101 / -31 / -112: the count coefficients -294 and 81 share the factor 3, which
+    # 16 x 101 lacks, so no count biases exist and the paired sums carry their constant
     ("no bias pair", [91, -112, -31, -123, -112, 101, -125, -31, -31, -125, 101, -112, -123, -31, -112, 91], 910, 2),
     # x2: class scores beyond 127, so the single-hit form even when the paired one is asked for
     ("hoxd70 x2", [2 * v for v in [91, -114, -31, -123, -114, 100, -125, -31, -31, -125, 100, -114, -123, -31,
