@@ -99,7 +99,8 @@ def main():
     ap.add_argument("--tune-reps", type=int, default=1)
     ap.add_argument("--no-io", action="store_true", help="skip the host input/output timing")
     ap.add_argument("--gate", default="",
-                    help="step_cells,block_threads,claim_seeds,grid_blocks[,order[,lookahead[,coarse]]] for the batched gate "
+                    help="step_cells,block_threads,claim_seeds,grid_blocks[,order[,lookahead[,coarse[,cand_planes]]]] "
+                         "for the batched gate "
                          "(applied after the autotuner; use with --tune-mbp 0 to fix the configuration)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once and exit: the plain copy of known size "
@@ -219,12 +220,13 @@ def main():
         order = vals[4] if len(vals) > 4 else 2
         ahead = vals[5] if len(vals) > 5 else eng.get_flag("batch_lookahead")
         coarse = vals[6] if len(vals) > 6 else eng.get_flag("batch_coarse")
+        planes = vals[7] if len(vals) > 7 else eng.get_flag("batch_cand_planes")
         for name, v in (("batch_step_cells", step), ("batch_block_threads", thr),
                         ("batch_claim_seeds", claim), ("batch_gate_blocks", grid), ("batch_order", order),
-                        ("batch_lookahead", ahead), ("batch_coarse", coarse)):
+                        ("batch_lookahead", ahead), ("batch_coarse", coarse), ("batch_cand_planes", planes)):
             eng.set_flag(name, v)
-        log("gate fixed: step %d threads %d claim %d grid %d order %d lookahead %d coarse %d"
-            % (step, thr, claim, grid, order, ahead, coarse))
+        log("gate fixed: step %d threads %d claim %d grid %d order %d lookahead %d coarse %d cand_planes %d"
+            % (step, thr, claim, grid, order, ahead, coarse, planes))
 
     for i in range(args.warmup):
         res_b, st = batched()

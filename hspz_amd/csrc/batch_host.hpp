@@ -493,11 +493,9 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                            h->stream, ar, B.blk_off.as<uint32_t>(), B.desc.as<BatchDesc>(), B.poff.as<uint32_t>(),
                            nb, B.ids.as<uint32_t>(), ctr + CTR_NVALID, sa, p->transitions, B.seeds.as<uint64_t>());
         // counts, u64 prefix, per-batch chunks
-        hipLaunchKernelGGL(k_bnum_hits, dim3(std::min<uint32_t>(ceil_div(cap_seeds, 256), 16384)), dim3(256), 0,
-                           h->stream, B.seeds.as<uint64_t>(), ctr + CTR_NVALID, per, h->index.as<uint32_t>(),
-                           B.counts.as<uint32_t>());
-        hipLaunchKernelGGL(k_tile_sum_u64, dim3(cap_tiles), dim3(TILE_THREADS), 0, h->stream, B.counts.as<uint32_t>(),
-                           ctr + CTR_NVALID, per, B.tiles.as<uint64_t>());
+        hipLaunchKernelGGL(k_bnum_tiles, dim3(cap_tiles), dim3(TILE_THREADS), 0, h->stream, B.seeds.as<uint64_t>(),
+                           ctr + CTR_NVALID, per, h->index.as<uint32_t>(), B.counts.as<uint32_t>(),
+                           B.tiles.as<uint64_t>());
         hipLaunchKernelGGL(k_scan_tiles_u64, dim3(1), dim3(SCAN_THREADS), 0, h->stream, B.tiles.as<uint64_t>(),
                            ctr + CTR_NVALID, per);
         hipLaunchKernelGGL(k_tile_apply_u64, dim3(cap_tiles), dim3(TILE_THREADS), 0, h->stream,
@@ -548,7 +546,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         typedef void (*XGateFn)(const uint32_t *, const uint32_t *, int32_t, int32_t, const QRec *,
                                 const uint32_t *, const uint4 *, const uint4 *, const uint32_t *, uint32_t, uint32_t,
                                 unsigned long long *, SurvRec *, uint32_t *, uint32_t, SurvRec *, uint32_t *,
-                                uint32_t, uint32_t *);
+                                uint32_t, uint32_t *, const uint32_t *, uint32_t);
         // the paired form (two hits per lane, packed 16-bit sums) where the class scores fit
         const bool pair = h->coarse_gate == 1 && h->coarse_fits16;
         const bool t1k = h->bgate_threads == 1024, pv = h->coarse_pv, c0 = h->coarse_cc.c0 != 0;
@@ -562,7 +560,13 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         // the candidates' gate with its loads issued ahead (512-thread blocks) unless the lookahead is off
         const bool xahead = h->lookahead;
         const int xthreads = xahead ? 512 : h->bgate_threads;
-        if (xahead)
+        // with the lookahead, the candidates' windows come from the planes unless asked otherwise
+        const bool xplanes = xahead && h->cand_planes;
+        if (xplanes)
+            xgate_fn = h->sum_mode == SUM_RAW      ? k_bgate_exact_ahead<SUM_RAW, true>
+                       : h->sum_mode == SUM_TREE16 ? k_bgate_exact_ahead<SUM_TREE16, true>
+                                                   : k_bgate_exact_ahead<SUM_TREE8, true>;
+        else if (xahead)
             xgate_fn = h->sum_mode == SUM_RAW      ? k_bgate_exact_ahead<SUM_RAW>
                        : h->sum_mode == SUM_TREE16 ? k_bgate_exact_ahead<SUM_TREE16>
                                                    : k_bgate_exact_ahead<SUM_TREE8>;
@@ -607,11 +611,12 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                B.cand.as<uint4>(), ctr + CTR_CAND, bs->cap_cand, B.slow.as<SurvRec>(),
                                ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
             hipLaunchKernelGGL(xgate_fn, dim3(xgrid), dim3(xthreads), 0, h->stream, h->ref2.as<uint32_t>(),
-                               h->pairtab.as<uint32_t>() + PAIRTAB_GRAY, h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
-                               B.kvalo.as<uint32_t>(), h->rrec.as<uint4>(), B.cand.as<uint4>(), ctr + CTR_CAND,
-                               bs->cap_cand, (uint32_t)h->bgate_claim, (unsigned long long *)(ctr + CTR_WORK2),
-                               B.surv.as<SurvRec>(), ctr + CTR_SURV, bs->cap_surv, B.slow.as<SurvRec>(),
-                               ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
+                               h->pairtab.as<uint32_t>() + (xplanes ? 0 : PAIRTAB_GRAY), h->xdrop, h->hspthresh,
+                               B.qrec.as<QRec>(), B.kvalo.as<uint32_t>(), h->rrec.as<uint4>(), B.cand.as<uint4>(),
+                               ctr + CTR_CAND, bs->cap_cand, (uint32_t)h->bgate_claim,
+                               (unsigned long long *)(ctr + CTR_WORK2), B.surv.as<SurvRec>(), ctr + CTR_SURV,
+                               bs->cap_surv, B.slow.as<SurvRec>(), ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR,
+                               B.plane.as<uint32_t>(), per);
         } else if (kmer_order)
             hipLaunchKernelGGL(kgate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
                                h->pairtab.as<uint32_t>() + PAIRTAB_GRAY, h->xdrop, h->hspthresh, B.qrec.as<QRec>(),

@@ -100,37 +100,79 @@ __global__ void k_bseed_scatter(const uint8_t *__restrict__ arena, const uint32_
 }
 
 // Bucket sizes of all seeds (count read from the device).
-__global__ void k_bnum_hits(const uint64_t *__restrict__ seeds, const uint32_t *__restrict__ nvalid,
-                            uint32_t per, const uint32_t *__restrict__ index, uint32_t *__restrict__ out) {
-    const uint32_t n = *nvalid * per;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t k = (uint32_t)(seeds[i] >> 32);
-        out[i] = index[k] - (k ? index[k - 1] : 0);
-    }
+// Sum of one u64 per thread over the block (every thread gets it).
+template <int THREADS>
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v) {
+    __shared__ uint64_t part[THREADS / WAVE];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    if (lane_id() == 0) part[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    uint64_t t = 0;
+#pragma unroll
+    for (int w = 0; w < THREADS / WAVE; w++) t += part[w];
+    __syncthreads();
+    return t;
 }
 
-// u64 inclusive scan of u32 counts: tile sums, one-block tile scan, apply.
-__global__ __launch_bounds__(TILE_THREADS) void k_tile_sum_u64(const uint32_t *__restrict__ in,
-                                                               const uint32_t *__restrict__ nvalid,
-                                                               uint32_t per,
-                                                               uint64_t *__restrict__ tile_sums) {
-    const uint32_t n = *nvalid * per;
-    uint32_t ntiles = (n + TILE - 1) / TILE;
-    if (blockIdx.x >= ntiles) return;
-    uint32_t base = blockIdx.x * TILE + threadIdx.x * TILE_ITEMS;
-    uint64_t s = 0;
-    for (int i = 0; i < TILE_ITEMS; i++)
-        if (base + i < n) s += in[base + i];
-    __shared__ uint64_t red[TILE_THREADS / WAVE];
+// Exclusive block scan of one u64 per thread.
+template <int THREADS>
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v) {
+    constexpr int NW = THREADS / WAVE;
+    __shared__ uint64_t wsum[NW];
+    const uint32_t lane = lane_id(), w = threadIdx.x / WAVE;
+    uint64_t inc = v;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, WAVE);
-    if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (int w = 0; w < TILE_THREADS / WAVE; w++) t += red[w];
-        tile_sums[blockIdx.x] = t;
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint64_t up = __shfl_up(inc, off, WAVE);
+        if (lane >= (uint32_t)off) inc += up;
     }
+    if (lane == WAVE - 1) wsum[w] = inc;
+    __syncthreads();
+    uint64_t base = 0;
+#pragma unroll
+    for (int k = 0; k < NW; k++) base += k < (int)w ? wsum[k] : 0;
+    __syncthreads();
+    return base + inc - v;
+}
+
+// Hit counts of the valid seed slots (bucket sizes of their k-mers) and the
+// per-tile sums of the u64 scan: a block per tile of TILE slots, whose items
+// lie at tile start + i * TILE_THREADS + thread, so every load and store is
+// coalesced and each thread has all its seed and index loads in flight at
+// once.
+__global__ __launch_bounds__(TILE_THREADS) void k_bnum_tiles(const uint64_t *__restrict__ seeds,
+                                                             const uint32_t *__restrict__ nvalid, uint32_t per,
+                                                             const uint32_t *__restrict__ index,
+                                                             uint32_t *__restrict__ out,
+                                                             uint64_t *__restrict__ tile_sums) {
+    const uint32_t n = *nvalid * per;
+    if (blockIdx.x >= (n + TILE - 1) / TILE) return;
+    const uint32_t base = blockIdx.x * TILE + threadIdx.x;
+    uint32_t k[TILE_ITEMS];
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; i++) {
+        const uint32_t at = base + i * TILE_THREADS;
+        k[i] = at < n ? (uint32_t)(seeds[at] >> 32) : 0u;
+    }
+    uint32_t hi[TILE_ITEMS], lo[TILE_ITEMS];
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; i++) {
+        hi[i] = index[k[i]];
+        lo[i] = index[k[i] ? k[i] - 1 : 0];
+    }
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; i++) {
+        const uint32_t at = base + i * TILE_THREADS;
+        const uint32_t c = hi[i] - (k[i] ? lo[i] : 0u);
+        if (at < n) {
+            out[at] = c;
+            s += c;
+        }
+    }
+    const uint64_t t = block_sum_u64<TILE_THREADS>(s);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = t;
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles_u64(uint64_t *__restrict__ sums,
@@ -159,28 +201,41 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles_u64(uint64_t *__res
     }
 }
 
+// Phase 3 of the u64 scan: each tile's inclusive prefix of the counts, plus
+// the tile's offset. Items are read and written coalesced (item i * THREADS +
+// thread) and transposed through LDS, so each thread scans 16 consecutive
+// items (rows padded by one word per 16: no bank conflicts).
 __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_u64(const uint32_t *__restrict__ in,
                                                                  const uint32_t *__restrict__ nvalid,
                                                                  uint32_t per,
                                                                  const uint64_t *__restrict__ offs,
                                                                  uint64_t *__restrict__ out) {
+    constexpr uint32_t PAD = TILE + TILE / TILE_ITEMS;
+    __shared__ uint64_t s_v[PAD];
     const uint32_t n = *nvalid * per;
-    uint32_t ntiles = (n + TILE - 1) / TILE;
-    if (blockIdx.x >= ntiles) return;
-    uint32_t base = blockIdx.x * TILE + threadIdx.x * TILE_ITEMS;
-    uint32_t v[TILE_ITEMS];
-    uint32_t s = 0;
+    if (blockIdx.x >= (n + TILE - 1) / TILE) return;
+    const uint32_t t0 = blockIdx.x * TILE;
+    auto slot = [](uint32_t x) { return x + x / TILE_ITEMS; };
 #pragma unroll
     for (int i = 0; i < TILE_ITEMS; i++) {
-        v[i] = base + i < n ? in[base + i] : 0;
-        s += v[i];
+        const uint32_t x = i * TILE_THREADS + threadIdx.x;
+        s_v[slot(x)] = t0 + x < n ? in[t0 + x] : 0u;
     }
-    uint32_t tot;
-    uint64_t run = (uint64_t)block_excl_scan<TILE_THREADS>(s, &tot) + offs[blockIdx.x];
+    __syncthreads();
+    uint64_t v[TILE_ITEMS], run = 0;
+#pragma unroll
+    for (int j = 0; j < TILE_ITEMS; j++) {
+        run += s_v[slot(threadIdx.x * TILE_ITEMS + j)];
+        v[j] = run;
+    }
+    const uint64_t off = block_excl_scan_u64<TILE_THREADS>(run) + offs[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < TILE_ITEMS; j++) s_v[slot(threadIdx.x * TILE_ITEMS + j)] = v[j] + off;
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < TILE_ITEMS; i++) {
-        run += v[i];
-        if (base + i < n) out[base + i] = run;
+        const uint32_t x = i * TILE_THREADS + threadIdx.x;
+        if (t0 + x < n) out[t0 + x] = s_v[slot(x)];
     }
 }
 
@@ -1536,7 +1591,8 @@ __device__ __forceinline__ void coarse_word(uint32_t r, uint32_t q, const Coarse
 }
 
 // Candidate record: (R, reference-record index, query-record index |
-// variant << CAND_VAR_SHIFT, 0), everything k_bgate_exact loads directly.
+// variant << CAND_VAR_SHIFT, Q): what k_bgate_exact loads directly, or, for
+// the plane-reading candidates' gate, where its windows lie.
 constexpr uint32_t CAND_VAR_SHIFT = 32 - GV_SHIFT;
 
 // The coarse gate over the k-mer-major walk (KmerSource, reference records
@@ -1612,7 +1668,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
         const bool ok = rok && v3.z != 0;
         const bool defer = has && !ok;
         const bool pass = has && ok && (!(dr && dl) || packed_keep(max(Br, 0) + max(Bl, 0), thresh));
-        src.sq.push(pass, make_uint4(R, iidx, qi | var << CAND_VAR_SHIFT, 0u), err);
+        src.sq.push(pass, make_uint4(R, iidx, qi | var << CAND_VAR_SHIFT, v3.x), err);
         src.defer(defer, v3.y + var, R, v3.x);
         has = false;
     }
@@ -1753,8 +1809,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
         const bool oka = rok && a3.z != 0, okb = rok && b3.z != 0;
         const bool pa = has && oka && (!da || packed_keep(ta, thresh));
         const bool pb = two && okb && (!db || packed_keep(tb, thresh));
-        src.sq.push(pa, make_uint4(R, iidx, qa | va << CAND_VAR_SHIFT, 0u), err);
-        src.sq.push(pb, make_uint4(R, iidx, qb | vb << CAND_VAR_SHIFT, 0u), err);
+        src.sq.push(pa, make_uint4(R, iidx, qa | va << CAND_VAR_SHIFT, a3.x), err);
+        src.sq.push(pb, make_uint4(R, iidx, qb | vb << CAND_VAR_SHIFT, b3.x), err);
         src.defer(has && !oka, a3.y + va, R, a3.x);
         src.defer(two && !okb, b3.y + vb, R, b3.x);
         has = false;
@@ -1859,7 +1915,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     const uint4 *__restrict__ cand, const uint32_t *__restrict__ cand_n, uint32_t cap_cand, uint32_t claim,
     unsigned long long *__restrict__ work, SurvRec *__restrict__ surv, uint32_t *__restrict__ surv_n,
     uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow,
-    uint32_t *__restrict__ err) {
+    uint32_t *__restrict__ err, const uint32_t *__restrict__ qplane, uint32_t per) {
+    (void)qplane;
+    (void)per;
     __shared__ uint32_t s_map[G2_MAP_WORDS];
     __shared__ uint4 s_dq[BT / WAVE][WQ_CAP];
     load_map_table(s_map, pairtab, BT);
@@ -1890,10 +1948,38 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
 // iteration finds them landed while the wave computed. Cells, decisions and
 // lists are those of kmer_gate_merged<2, SUMS> over CandSource; a finished
 // hit's results are appended at the top of the next iteration.
-template <int SUMS>
+// The 13 plane words around anchor P (word 8 holds base P) of a window whose
+// margins were checked when the hit became a candidate.
+__device__ __forceinline__ void load_window(const uint32_t *__restrict__ plane, uint32_t P, uint32_t (&W)[13]) {
+    const uint32_t *wp = plane + (P >> 4) - G2_LEFT_WORDS;
+#pragma unroll
+    for (int k = 0; k < 13; k++) W[k] = wp[k];
+}
+
+// A window's right and left streams as the merged gate consumes them (the
+// layout of the records, in plane code).
+__device__ __forceinline__ void window_streams(const uint32_t (&W)[13], uint32_t P, uint32_t (&A)[G2_RIGHT_WORDS],
+                                               uint32_t (&LA)[G2_LEFT_WORDS]) {
+    const uint32_t sh = 2 * (P & 15);
+#pragma unroll
+    for (int k = 0; k < (int)G2_RIGHT_WORDS; k++) A[k] = __builtin_amdgcn_alignbit(W[9 + k], W[8 + k], sh);
+#pragma unroll
+    for (int k = 0; k < (int)G2_LEFT_WORDS; k++)
+        LA[k] = nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sh)));
+}
+
+// The merged 96-cell-step gate over the coarse gate's candidates, every load
+// issued one iteration ahead. PLANES = false: a candidate's windows are its
+// reference and query records (Gray-coded; the Gray-keyed maps). PLANES =
+// true: they are read from the 2-bit planes at R and Q (the reference plane
+// is 250 MB at 1 Gbp, the pass's query plane a few MB, against 48 B per
+// indexed position of reference records), in plane code (the plane-keyed
+// maps), and the hit's seed slot is rank * per + variant.
+template <int SUMS, bool PLANES = false>
 __device__ __forceinline__ void cand_gate_ahead(CandSource &src, const uint4 *__restrict__ rrec,
-                                                const QRec *__restrict__ recs, const uint8_t *s_map,
-                                                int32_t xdrop, int32_t thresh) {
+                                                const QRec *__restrict__ recs, const uint32_t *__restrict__ rplane,
+                                                const uint32_t *__restrict__ qplane, uint32_t per,
+                                                const uint8_t *s_map, int32_t xdrop, int32_t thresh) {
     constexpr int RW = 2;
     const uint32_t off_fwd = (lane_id() & 31) * 4, off_rev = off_fwd + 128;
     // current hit
@@ -1906,6 +1992,10 @@ __device__ __forceinline__ void cand_gate_ahead(CandSource &src, const uint4 *__
     uint32_t nR = 0, nvar = 0;
     const uint4 zero4 = make_uint4(0, 0, 0, 0);
     uint4 T0 = zero4, T1 = zero4, T2 = zero4, P0 = zero4, P1 = zero4, P2 = zero4, P3 = zero4;
+    uint32_t WR[13], WQ[13];  // PLANES: the next hit's window words
+    uint32_t nQ = 0;
+#pragma unroll
+    for (int k = 0; k < 13; k++) WR[k] = WQ[k] = 0;
     // claimed entry (load issued last iteration)
     bool ent = false;
     uint4 E = zero4;
@@ -1921,7 +2011,8 @@ __device__ __forceinline__ void cand_gate_ahead(CandSource &src, const uint4 *__
         // The records' pad word and the entry's last word are read nowhere;
         // keeping them live to here stops the compiler from reusing their
         // registers while the loads are in flight (which would wait on them).
-        asm volatile("" ::"v"(P3.w), "v"(E.w));
+        if constexpr (PLANES) asm volatile("" ::"v"(E.y));
+        else asm volatile("" ::"v"(P3.w), "v"(E.w));
         src.emit(true, e_pass, etag, eR, eQ);
         src.defer(e_defer, etag, eR, eQ);
         e_pass = e_defer = false;
@@ -1929,29 +2020,45 @@ __device__ __forceinline__ void cand_gate_ahead(CandSource &src, const uint4 *__
         // 1. a free lane takes its next hit
         bool defer = false;
         if (!has && nxt) {
-            A0[0] = T0.x, A0[1] = T0.y, A0[2] = T0.z, A0[3] = T0.w;
-            LA0[0] = T1.x, LA0[1] = T1.y, LA0[2] = T1.z, LA0[3] = T1.w;
-            LA0[4] = T2.x, LA0[5] = T2.y, LA0[6] = T2.z, LA0[7] = T2.w;
-            B[0] = P0.x, B[1] = P0.y, B[2] = P0.z, B[3] = P0.w;
-            LB[0] = P1.x, LB[1] = P1.y, LB[2] = P1.z, LB[3] = P1.w;
-            LB[4] = P2.x, LB[5] = P2.y, LB[6] = P2.z, LB[7] = P2.w;
+            if constexpr (PLANES) {
+                window_streams(WR, nR, A0, LA0);
+                window_streams(WQ, nQ, B, LB);
+                Q = nQ;
+                tag = nvar;
+            } else {
+                A0[0] = T0.x, A0[1] = T0.y, A0[2] = T0.z, A0[3] = T0.w;
+                LA0[0] = T1.x, LA0[1] = T1.y, LA0[2] = T1.z, LA0[3] = T1.w;
+                LA0[4] = T2.x, LA0[5] = T2.y, LA0[6] = T2.z, LA0[7] = T2.w;
+                B[0] = P0.x, B[1] = P0.y, B[2] = P0.z, B[3] = P0.w;
+                LB[0] = P1.x, LB[1] = P1.y, LB[2] = P1.z, LB[3] = P1.w;
+                LB[4] = P2.x, LB[5] = P2.y, LB[6] = P2.z, LB[7] = P2.w;
+                Q = P3.x;
+                tag = P3.y + nvar;
+                if (P3.z == 0) defer = true;
+            }
             R = nR;
-            Q = P3.x;
-            tag = P3.y + nvar;
             has = true;
             nxt = false;
             st.start();
-            if (P3.z == 0) defer = true;
         }
-        // 2. the landed entry's records, for the lane's next hit
+        // 2. the landed entry's windows, for the lane's next hit
         if (!nxt && ent) {
-            const uint4 *rr = rrec + 3 * (size_t)E.y;
-            T0 = rr[0], T1 = rr[1], T2 = rr[2];
-            uint32_t var;
-            const uint4 *rec = src.record(recs, E.z, var);
-            P0 = rec[0], P1 = rec[1], P2 = rec[2], P3 = rec[3];
+            if constexpr (PLANES) {
+                // every candidate's windows are clean (the coarse gate deferred the others)
+                load_window(rplane, E.x, WR);
+                load_window(qplane, E.w, WQ);
+                const uint32_t rank = E.z & ((1u << CAND_VAR_SHIFT) - 1);
+                nvar = rank * per + (E.z >> CAND_VAR_SHIFT);
+                nQ = E.w;
+            } else {
+                const uint4 *rr = rrec + 3 * (size_t)E.y;
+                T0 = rr[0], T1 = rr[1], T2 = rr[2];
+                uint32_t var;
+                const uint4 *rec = src.record(recs, E.z, var);
+                P0 = rec[0], P1 = rec[1], P2 = rec[2], P3 = rec[3];
+                nvar = var;
+            }
             nR = E.x;
-            nvar = var;
             nxt = true;
             ent = false;
         }
@@ -1985,21 +2092,20 @@ __device__ __forceinline__ void cand_gate_ahead(CandSource &src, const uint4 *__
 // cand_gate_ahead over the coarse gate's candidates: 512-thread blocks, four
 // waves per SIMD (the 128 VGPRs its in-flight records need); same arguments
 // and lists as k_bgate_exact.
-template <int SUMS>
+template <int SUMS, bool PLANES = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_bgate_exact_ahead(
     const uint32_t *__restrict__ rplane, const uint32_t *__restrict__ pairtab, int32_t xdrop, int32_t thresh,
     const QRec *__restrict__ recs, const uint32_t *__restrict__ sval, const uint4 *__restrict__ rrec,
     const uint4 *__restrict__ cand, const uint32_t *__restrict__ cand_n, uint32_t cap_cand, uint32_t claim,
     unsigned long long *__restrict__ work, SurvRec *__restrict__ surv, uint32_t *__restrict__ surv_n,
     uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow,
-    uint32_t *__restrict__ err) {
+    uint32_t *__restrict__ err, const uint32_t *__restrict__ qplane, uint32_t per) {
     __shared__ uint32_t s_map[G2_MAP_WORDS];
     __shared__ uint4 s_dq[512 / WAVE][WQ_CAP];
     load_map_table(s_map, pairtab, 512);
     CandSource src;
     src.list = cand;
     src.work = work;
-    (void)rplane;
     (void)sval;
     src.n = __builtin_amdgcn_readfirstlane(min(*cand_n, cap_cand));
     src.claim = claim;
@@ -2014,7 +2120,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     src.init();
     (void)work;
     src.share(blockIdx.x * (512 / WAVE) + threadIdx.x / WAVE, gridDim.x * (512 / WAVE));
-    cand_gate_ahead<SUMS>(src, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
+    cand_gate_ahead<SUMS, PLANES>(src, rrec, recs, rplane, qplane, per, (const uint8_t *)s_map, xdrop, thresh);
 }
 
 // First index of each batch's records in a scope-sorted list (scope = batch

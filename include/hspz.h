@@ -276,6 +276,9 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           0 none, 1 one hit per lane, 2 two hits per lane */
 #define HSPZ_FLAG_DUMP_RAW 15          /* Engine::dump_raw (gpu/mod.rs:55-67): 1 = hspz_seed_and_filter keeps
                                           the pre-dedup records in hspz_filter_output.raw; 0 (default) */
+#define HSPZ_FLAG_BATCH_CAND_PLANES 16 /* where the two-cell maps behind the coarse bound read a candidate's
+                                          windows: 1 (default) the 2-bit planes at R and Q, 0 its reference
+                                          and query records. The lists do not change */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

@@ -42,15 +42,18 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
             for i, (a, g) in enumerate(zip(again, got)):
                 assert same_intervals(a, g), "chunk %d differs in hit order %d" % (i, order)
         # k-mer-major order without the coarse bound ahead of the two-cell maps, with
-        # it in its paired form (HOXD70's class scores fit 16 bits) and single-hit form
+        # it in its paired form (HOXD70's class scores fit 16 bits) and single-hit form,
+        # the candidates' windows read from the planes and from the records
         eng.set_flag("batch_order", 1)
-        for coarse, used in ((0, 0), (1, 2), (2, 1)):
+        for coarse, used, planes in ((0, 0, 1), (1, 2, 1), (2, 1, 1), (1, 2, 0), (2, 1, 0)):
             eng.set_flag("batch_coarse", coarse)
+            eng.set_flag("batch_cand_planes", planes)
             again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
             assert eng.get_flag("batch_coarse_used") == used
             for i, (a, g) in enumerate(zip(again, got)):
-                assert same_intervals(a, g), "chunk %d differs with coarse mode %d" % (i, coarse)
+                assert same_intervals(a, g), "chunk %d differs with coarse mode %d, planes %d" % (i, coarse, planes)
         eng.set_flag("batch_coarse", 1)
+        eng.set_flag("batch_cand_planes", 1)
         eng.set_flag("batch_order", 2)
         # the call split into device passes between batches gives the same lists:
         # every batch alone (a batch is never split, so 1 position per pass runs each
@@ -535,14 +538,16 @@ def test_batch_coarse_forms(gpu_available, name, acgt, xdrop, used):
         eng.set_flag("batch_order", 1)
         for threads in (1024, 512):
             eng.set_flag("batch_block_threads", threads)
-            for coarse, u in ((1, used), (2, 1), (0, 0)):
+            for coarse, u, planes in ((1, used, 1), (1, used, 0), (2, 1, 1), (2, 1, 0), (0, 0, 1)):
                 eng.set_flag("batch_coarse", coarse)
+                eng.set_flag("batch_cand_planes", planes)
                 got, st = eng.filter_batch(chunks)
                 assert eng.get_flag("batch_coarse_used") == u
                 if coarse:  # homologous chunks: the bound keeps many hits, never more than there are
                     assert 0 < eng.stat("batch_candidates") <= st["hits"]
                 for i, (g, w) in enumerate(zip(got, want)):
-                    assert same_intervals(g, w), "chunk %d differs in coarse mode %d, %d threads" % (i, coarse, threads)
+                    assert same_intervals(g, w), "chunk %d differs in coarse mode %d, planes %d, %d threads" % (
+                        i, coarse, planes, threads)
 
 
 def test_batch_staged_read(gpu_available):
