@@ -70,9 +70,11 @@ namespace {
 // Device counters of one pass; CTR_WORK is the 64-bit seed-claim counter of
 // the batched gate (two slots, 8-byte aligned).
 // CTR_CAND counts the coarse gate's candidates and CTR_WORK2 (64-bit, slots
-// 10-11) is the candidate-claim counter of k_bgate_exact.
+// 10-11) is the candidate-claim counter of k_bgate_exact. CTR_LONG counts the
+// survivors the lane materializer hands to the wave materializer, CTR_MWORK
+// its survivor-claim counter.
 enum { CTR_NVALID = 0, CTR_SURV = 1, CTR_SLOW = 2, CTR_ERR = 3, CTR_KEPT = 4, CTR_EXACT = 5, CTR_WORK = 6,
-       CTR_KGAPS = 8, CTR_CAND = 9, CTR_WORK2 = 10, CTR_N = 12 };
+       CTR_KGAPS = 8, CTR_CAND = 9, CTR_WORK2 = 10, CTR_LONG = 12, CTR_MWORK = 13, CTR_N = 14 };
 
 typedef void (*BGateFn)(const uint32_t *, const uint32_t *, uint32_t, const uint32_t *, const uint32_t *, uint32_t,
                         const uint32_t *, int32_t, int32_t, const uint64_t *, const uint32_t *, uint32_t,
@@ -650,10 +652,27 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         // materializer and compaction of kept records
         hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(ceil_div(bs->cap_surv / 4 + 4, 256), 4096)), dim3(256), 0,
                            h->stream, (uint32_t *)B.keep.p, bs->cap_surv / 4 + 4, 0u);
-        hipLaunchKernelGGL(k_bmaterialize, dim3(4096), dim3(256), 0, h->stream, r, h->ref_len, ar, bo,
-                           B.starts.as<uint32_t>(), B.nchunks.as<uint32_t>(), h->sub.as<int32_t>(), h->xdrop,
-                           h->hspthresh, h->noentropy, 1.3862943649291992, B.surv.as<SurvRec>(), ctr + CTR_SURV,
-                           bs->cap_surv, B.mhsp.as<SegmentPair>(), B.mscope.as<uint32_t>(), B.keep.as<uint8_t>());
+        // one lane per survivor, then one wave per survivor for the ones a lane
+        // handed on (B.kids is free until the compaction below)
+        if (h->lane_materialize) {
+            hipLaunchKernelGGL(k_bmaterialize_lane, dim3(8 * h->num_cus), dim3(256), 0, h->stream, r, h->ref_len,
+                               ar, bo, B.starts.as<uint32_t>(), B.nchunks.as<uint32_t>(), h->sub.as<int32_t>(),
+                               h->xdrop, h->hspthresh, h->noentropy, 1.3862943649291992, B.surv.as<SurvRec>(),
+                               ctr + CTR_SURV, bs->cap_surv, B.mhsp.as<SegmentPair>(), B.mscope.as<uint32_t>(),
+                               B.keep.as<uint8_t>(), B.kids.as<uint32_t>(), ctr + CTR_LONG, bs->cap_surv,
+                               ctr + CTR_MWORK, ctr + CTR_ERR);
+            hipLaunchKernelGGL(k_bmaterialize, dim3(4096), dim3(256), 0, h->stream, r, h->ref_len, ar, bo,
+                               B.starts.as<uint32_t>(), B.nchunks.as<uint32_t>(), h->sub.as<int32_t>(), h->xdrop,
+                               h->hspthresh, h->noentropy, 1.3862943649291992, B.surv.as<SurvRec>(),
+                               ctr + CTR_SURV, bs->cap_surv, B.mhsp.as<SegmentPair>(), B.mscope.as<uint32_t>(),
+                               B.keep.as<uint8_t>(), B.kids.as<uint32_t>(), ctr + CTR_LONG, bs->cap_surv);
+        } else {
+            hipLaunchKernelGGL(k_bmaterialize, dim3(4096), dim3(256), 0, h->stream, r, h->ref_len, ar, bo,
+                               B.starts.as<uint32_t>(), B.nchunks.as<uint32_t>(), h->sub.as<int32_t>(), h->xdrop,
+                               h->hspthresh, h->noentropy, 1.3862943649291992, B.surv.as<SurvRec>(),
+                               ctr + CTR_SURV, bs->cap_surv, B.mhsp.as<SegmentPair>(), B.mscope.as<uint32_t>(),
+                               B.keep.as<uint8_t>(), nullptr, nullptr, 0u);
+        }
         hipLaunchKernelGGL(k_tile_count_u8, dim3(kt), dim3(TILE_THREADS), 0, h->stream, B.keep.as<uint8_t>(),
                            bs->cap_surv, h->tiles.as<uint32_t>());
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, h->stream, h->tiles.as<uint32_t>(), kt,
@@ -679,7 +698,8 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             memcpy(key.ptrs, ptrs, sizeof ptrs);
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow | (uint64_t)bs->cap_cand << 32,
                                cap_seeds | (uint64_t)coarse << 40 | (uint64_t)xgrid << 41, per, h->max_hits,
-                               (uint64_t)grid | (uint64_t)h->bgate_claim << 32, (uint64_t)p->transitions, nt, kt,
+                               (uint64_t)grid | (uint64_t)h->bgate_claim << 32,
+                               (uint64_t)p->transitions | (uint64_t)h->lane_materialize << 8, nt, kt,
                                cap_tiles, (uint64_t)h->bgate_step | (uint64_t)h->bgate_threads << 32,
                                (uint64_t)(uintptr_t)h->sub.p, s.kmer_size, (uint64_t)(uintptr_t)h->ref.p,
                                (uint64_t)kmer_order, ntk, (uint64_t)(uintptr_t)(const void *)kgate_fn,

@@ -116,6 +116,7 @@ struct hspz_engine {
     uint64_t batch_passes_last = 0;    // device passes of the last hspz_filter_batch call
     int coarse_gate = 1;               // HSPZ_FLAG_BATCH_COARSE: class-score bound ahead of the two-cell maps
     bool cand_planes = true;           // HSPZ_FLAG_BATCH_CAND_PLANES: candidates' windows from the planes
+    bool lane_materialize = true;      // HSPZ_FLAG_BATCH_LANE_MATERIALIZE: a survivor per lane first
                                        // (0 off, 1 paired form where the class scores fit, 2 single-hit form)
     int coarse_used = 0;               // the form the last batched pass ran (0: none)
     bool coarse_fits16 = false;        // every |class score| <= COARSE_PAIR_MAX_CLASS
@@ -808,6 +809,7 @@ extern "C" int hspz_engine_share(hspz_engine *base, hspz_engine **out) {
     h->lookahead = base->lookahead;
     h->coarse_gate = base->coarse_gate;
     h->cand_planes = base->cand_planes;
+    h->lane_materialize = base->lane_materialize;
     h->batch_graph = base->batch_graph;
     h->device_dedup = base->device_dedup;
     h->list_cap = base->list_cap;
@@ -1187,6 +1189,7 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
         break;
     case HSPZ_FLAG_DUMP_RAW: h->dump_raw = value != 0; break;
     case HSPZ_FLAG_BATCH_CAND_PLANES: h->cand_planes = value != 0; break;
+    case HSPZ_FLAG_BATCH_LANE_MATERIALIZE: h->lane_materialize = value != 0; break;
     case HSPZ_FLAG_BATCH_COARSE:
         if (value < 0 || value > 2) return set_error(HSPZ_E_ARG, "coarse mode must be 0, 1 or 2");
         h->coarse_gate = value;
@@ -1234,6 +1237,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_COARSE: *value = h->coarse_gate; break;
     case HSPZ_FLAG_DUMP_RAW: *value = h->dump_raw ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_CAND_PLANES: *value = h->cand_planes ? 1 : 0; break;
+    case HSPZ_FLAG_BATCH_LANE_MATERIALIZE: *value = h->lane_materialize ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_COARSE_USED: *value = h->coarse_used; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }

@@ -2327,7 +2327,8 @@ __global__ __launch_bounds__(256) void k_bgate_wave(const uint8_t *__restrict__ 
     }
 }
 
-// find_hsps for the batched survivors (one wave per record): exact extension
+// find_hsps for the batched survivors (one wave per record, or per listed
+// record: the ones k_bmaterialize_lane handed on): exact extension
 // with positions in the chunk-local block, entropy, keep test. Output record
 // i: HSP in block coordinates, its scope key (batch * MAX_CHUNKS + chunk) and
 // the keep flag.
@@ -2336,14 +2337,17 @@ __global__ __launch_bounds__(256) void k_bmaterialize(
     const uint32_t *__restrict__ starts, const uint32_t *__restrict__ nchunks,
     const int32_t *__restrict__ sub, int32_t xdrop, int32_t thresh, int32_t noentropy, double log4,
     const SurvRec *__restrict__ surv, const uint32_t *__restrict__ surv_n, uint32_t cap_surv,
-    SegmentPair *__restrict__ hsp, uint32_t *__restrict__ scope, uint8_t *__restrict__ keep) {
+    SegmentPair *__restrict__ hsp, uint32_t *__restrict__ scope, uint8_t *__restrict__ keep,
+    const uint32_t *__restrict__ ids, const uint32_t *__restrict__ ids_n, uint32_t cap_ids) {
     __shared__ int32_t s_sub[64];
     if (threadIdx.x < 64) s_sub[threadIdx.x] = sub[threadIdx.x];
     __syncthreads();
     const uint32_t lane = lane_id();
-    const uint32_t n = min(*surv_n, cap_surv);
+    // every survivor, or (ids) the survivors listed there
+    const uint32_t n = ids ? min(*ids_n, cap_ids) : min(*surv_n, cap_surv);
     const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
-    for (uint32_t i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; i < n; i += nwaves) {
+    for (uint32_t j = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; j < n; j += nwaves) {
+        const uint32_t i = ids ? ids[j] : j;
         SurvRec rec = surv[i];
         uint32_t b = bo.batch_of(rec.seed);
         uint32_t blk = bo.batches[b].block;
@@ -2396,6 +2400,201 @@ __global__ __launch_bounds__(256) void k_bmaterialize(
                 scope[i] = b * MAX_CHUNKS + ch;
             }
             keep[i] = kept ? 1 : 0;
+        }
+    }
+}
+
+// Sixteen byte codes from p on (p may lie a few bytes outside the sequence:
+// the reference and the arena are padded with code 7 on both sides) as four
+// words, word g byte j = the code at p + 4 g + j, from five aligned loads.
+__device__ __forceinline__ void bytes16(const uint8_t *__restrict__ base, int64_t p, uint32_t (&v)[4]) {
+    const uintptr_t u = (uintptr_t)(base + p);
+    const uint32_t *w = (const uint32_t *)(u & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(u & 3);
+    uint32_t x[5];
+#pragma unroll
+    for (int g = 0; g < 5; g++) x[g] = w[g];
+#pragma unroll
+    for (int g = 0; g < 4; g++) v[g] = __builtin_amdgcn_alignbyte(x[g + 1], x[g], sh);
+}
+
+// Cells one lane extends per side before it hands the survivor to the wave
+// materializer instead.
+constexpr uint32_t LANE_EXTEND_CAP = 1024;
+// Survivors a wave of the lane materializer claims at a time.
+constexpr uint32_t LANE_CLAIM = 64;
+
+// find_hsps for the batched survivors, one lane per survivor: the exact
+// X-drop extension of wave_extend<true> written as the scalar loop it scans
+// (kernels.rs:1279-1543: running sum, maximum with its earliest position, stop
+// at the first cell more than xdrop below the maximum or at a sequence end),
+// sixteen cells per iteration, then the entropy counts over the extent and the
+// same f64 keep test and record as k_bmaterialize. Waves claim LANE_CLAIM
+// survivors at a time and hand the next one to a lane as soon as the lane
+// finishes (right side, left side, entropy counts), so lanes with short
+// extensions do not wait for long ones. A side longer than LANE_EXTEND_CAP
+// cells sends the survivor to `ids` for the wave-per-record kernel (which
+// writes its keep flag; the caller zero-filled keep).
+__global__ __launch_bounds__(256) void k_bmaterialize_lane(
+    const uint8_t *__restrict__ ref, uint32_t ref_len, const uint8_t *__restrict__ arena, BlockOf bo,
+    const uint32_t *__restrict__ starts, const uint32_t *__restrict__ nchunks,
+    const int32_t *__restrict__ sub, int32_t xdrop, int32_t thresh, int32_t noentropy, double log4,
+    const SurvRec *__restrict__ surv, const uint32_t *__restrict__ surv_n, uint32_t cap_surv,
+    SegmentPair *__restrict__ hsp, uint32_t *__restrict__ scope, uint8_t *__restrict__ keep,
+    uint32_t *__restrict__ ids, uint32_t *__restrict__ ids_n, uint32_t cap_ids, uint32_t *__restrict__ work,
+    uint32_t *__restrict__ err) {
+    __shared__ int32_t s_sub[64];
+    if (threadIdx.x < 64) s_sub[threadIdx.x] = sub[threadIdx.x];
+    __syncthreads();
+    const uint32_t n = min(*surv_n, cap_surv);
+    uint32_t c_cur = 0, c_end = 0;
+    bool dry = false;
+    bool act = false;
+    uint32_t i = 0, b = 0, R = 0, Q = 0, k = 0, lim = 0, phase = 0;
+    const uint8_t *qry = arena;
+    int32_t score = 0, best = 0, bpos = 0, Mr = 0, rpos = 0, total = 0, extent = 0;
+    uint32_t cnt0 = 0, cnt1 = 0, cnt2 = 0, cnt3 = 0;
+    for (;;) {
+        // lanes without a survivor take the next ones of the wave's claims
+        uint64_t need = __ballot(!act);
+        while (need) {
+            if (c_cur >= c_end) {
+                if (dry) break;
+                uint32_t v = 0;
+                if (lane_id() == (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1))
+                    v = atomicAdd(work, LANE_CLAIM);
+                v = __builtin_amdgcn_readfirstlane(__shfl(v, (int)(__ffsll((unsigned long long)__ballot(true)) - 1), WAVE));
+                if (v >= n) {
+                    dry = true;
+                    break;
+                }
+                c_cur = v;
+                c_end = min(v + LANE_CLAIM, n);
+            }
+            const uint32_t rank = mbcnt(need);
+            if (!act && rank < c_end - c_cur) {
+                i = c_cur + rank;
+                const SurvRec rec = surv[i];
+                b = bo.batch_of(rec.seed);
+                const uint32_t blk = bo.batches[b].block, q0 = bo.blk_off[blk], qlen = bo.blk_len[blk];
+                qry = arena + q0;
+                R = rec.R;
+                Q = rec.Q - q0;
+                phase = 0;
+                k = 0;
+                score = best = 0;
+                bpos = -1;
+                lim = min(ref_len - R, qlen - Q);
+                act = true;
+            }
+            c_cur += min((uint32_t)__popcll(need), c_end - c_cur);
+            need = __ballot(!act);
+        }
+        if (__ballot(act) == 0) break;
+        if (!act) continue;
+        bool fin = false;
+        if (phase < 2) {
+            // right (phase 0): cells R + k, Q + k; left (phase 1): R - 1 - k, Q - 1 - k
+            const bool lft = phase == 1;
+            uint32_t rw[4], qw[4];
+            bytes16(ref, lft ? (int64_t)R - 16 - k : (int64_t)R + k, rw);
+            bytes16(qry, lft ? (int64_t)Q - 16 - k : (int64_t)Q + k, qw);
+            bool stop = false;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                // left: word 3 - g byte-swapped holds cells 4 g .. 4 g + 3 in walk order
+                const uint32_t r4 = lft ? __builtin_bswap32(rw[3 - g]) : rw[g];
+                const uint32_t q4 = lft ? __builtin_bswap32(qw[3 - g]) : qw[g];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t cell = k + 4 * g + j;
+                    if (!stop && cell < lim) {
+                        score += s_sub[((r4 >> (8 * j)) & 0xFFu) * 8 + ((q4 >> (8 * j)) & 0xFFu)];
+                        const int32_t pos = (int32_t)cell + (lft ? 1 : 0);
+                        bpos = score > best ? pos : bpos;
+                        best = max(best, score);
+                        stop = (int32_t)((uint32_t)best - (uint32_t)score) > xdrop;
+                    } else {
+                        stop = true;  // a sequence end: the cells past it score 0 and change nothing
+                    }
+                }
+            }
+            k += 16;
+            if (!stop && k >= LANE_EXTEND_CAP) {  // too long for a lane: the wave materializer takes it
+                const uint32_t at = atomicAdd(ids_n, 1u);
+                if (at < cap_ids) ids[at] = i;
+                else atomicOr(err, 8u);
+                act = false;
+            } else if (stop && !lft) {
+                Mr = best;
+                rpos = bpos;
+                phase = 1;
+                k = 0;
+                score = best = bpos = 0;
+                lim = min(R, Q);
+            } else if (stop) {
+                total = (int32_t)((uint32_t)Mr + (uint32_t)best);
+                extent = (int32_t)((uint32_t)rpos + (uint32_t)bpos);
+                rpos = bpos;  // from here: the left extent
+                if (total >= thresh && (int64_t)total <= 3 * (int64_t)thresh && noentropy == 0) {
+                    phase = 2;
+                    k = 0;
+                    lim = (uint32_t)(extent + 1);
+                    cnt0 = cnt1 = cnt2 = cnt3 = 0;
+                } else {
+                    fin = true;
+                }
+            }
+        } else {
+            // entropy counts over positions 0..extent from R - left extent
+            const uint32_t rs = R - (uint32_t)rpos, qs = Q - (uint32_t)rpos;
+            uint32_t rw[4], qw[4];
+            bytes16(ref, (int64_t)rs + k, rw);
+            bytes16(qry, (int64_t)qs + k, qw);
+#pragma unroll
+            for (int g = 0; g < 4; g++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t r = (rw[g] >> (8 * j)) & 0xFFu, q = (qw[g] >> (8 * j)) & 0xFFu;
+                    const bool m = k + 4 * g + j < lim && r == q;
+                    cnt0 += m && r == 0 ? 1u : 0u;
+                    cnt1 += m && r == 1 ? 1u : 0u;
+                    cnt2 += m && r == 2 ? 1u : 0u;
+                    cnt3 += m && r == 3 ? 1u : 0u;
+                }
+            k += 16;
+            fin = k >= lim;
+        }
+        if (fin) {
+            double entropy = 1.0;
+            if (phase == 2 && cnt0 + cnt1 + cnt2 + cnt3 >= 20) {
+                const uint32_t cnt[4] = {cnt0, cnt1, cnt2, cnt3};
+                const double denom = (double)(extent + 1);
+                double acc = 0.0;
+                for (int m = 0; m < 4; m++) {
+                    double pb = (double)cnt[m] / denom;
+                    double l = cnt[m] != 0 ? ref_ln(pb) : 0.0;
+                    double t = pb * l;
+                    acc = acc + t;
+                }
+                entropy = -acc / log4;
+            }
+            const uint32_t leftx = (uint32_t)rpos;
+            const double gated = (double)(float)total * entropy;
+            const bool kept = trunc_sat(gated) >= thresh;
+            if (kept) {
+                SegmentPair o;
+                o.ref_start = R - leftx;
+                o.query_start = Q - leftx;
+                o.len = (uint32_t)extent;
+                o.score = entropy > 0.0 ? trunc_sat((double)total * entropy) : 0;
+                hsp[i] = o;
+                const uint32_t nc = nchunks[b];
+                const uint32_t ch = nc > 1 ? find_le(starts + (uint64_t)b * MAX_CHUNKS, nc, surv[i].seed) : 0;
+                scope[i] = b * MAX_CHUNKS + ch;
+            }
+            keep[i] = kept ? 1 : 0;
+            act = false;
         }
     }
 }

@@ -279,6 +279,9 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 #define HSPZ_FLAG_BATCH_CAND_PLANES 16 /* where the two-cell maps behind the coarse bound read a candidate's
                                           windows: 1 (default) the 2-bit planes at R and Q, 0 its reference
                                           and query records. The lists do not change */
+#define HSPZ_FLAG_BATCH_LANE_MATERIALIZE 17 /* 1 (default): the batched materializer extends one survivor per
+                                          lane (sides longer than 2048 cells go to one wave per survivor);
+                                          0: one wave per survivor throughout. The lists do not change */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

@@ -54,6 +54,12 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
                 assert same_intervals(a, g), "chunk %d differs with coarse mode %d, planes %d" % (i, coarse, planes)
         eng.set_flag("batch_coarse", 1)
         eng.set_flag("batch_cand_planes", 1)
+        # the materializer one wave per survivor throughout, instead of a lane per survivor first
+        eng.set_flag("batch_lane_materialize", 0)
+        again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
+        for i, (a, g) in enumerate(zip(again, got)):
+            assert same_intervals(a, g), "chunk %d differs with the wave materializer" % i
+        eng.set_flag("batch_lane_materialize", 1)
         eng.set_flag("batch_order", 2)
         # the call split into device passes between batches gives the same lists:
         # every batch alone (a batch is never split, so 1 position per pass runs each
