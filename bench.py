@@ -248,7 +248,7 @@ def main():
         gate_kernel = "k_bgate_packed"
     elif coarse_used:
         gate_kernel = "%s (coarse bound over every hit) + k_bgate_exact_ahead (its candidates)" % (
-            "k_bgate_coarse2" if coarse_used == 2 else "k_bgate_coarse")
+            ("k_bgate_coarse", "k_bgate_coarse2", "k_bgate_coarse2p")[coarse_used - 1])
     elif (eng.get_flag("batch_step_cells") >= 96 and eng.get_flag("batch_block_threads") == 512
           and eng.get_flag("batch_lookahead") and eng.get_flag("ref_records") == 2):
         gate_kernel = "k_bgate_ahead"
@@ -312,8 +312,8 @@ def main():
                 "hits_per_s": round(g["gate_hits"] / secs, 1),
                 "deferred_frac": round(g["gate_deferred"] / max(g["gate_hits"], 1), 5),
                 "cells_per_hit": round(cells_per_hit, 3), "bytes_per_hit": round(per_hit, 2),
-                "coarse_bound": {"form": ("off", "one hit per lane", "two hits per lane")[
-                                     eng.get_flag("batch_coarse_used")],
+                "coarse_bound": {"form": ("off", "one hit per lane", "two query records per lane",
+                                          "two reference positions per lane")[eng.get_flag("batch_coarse_used")],
                                  "candidates_per_step": cand_step,
                                  "candidate_frac": round(cand_step / max(st_b["hits"], 1), 5)}}
 

@@ -550,10 +550,18 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                 unsigned long long *, SurvRec *, uint32_t *, uint32_t, SurvRec *, uint32_t *,
                                 uint32_t, uint32_t *, const uint32_t *, uint32_t);
         // the paired form (two hits per lane, packed 16-bit sums) where the class scores fit
-        const bool pair = h->coarse_gate == 1 && h->coarse_fits16;
+        // 1: two reference positions per lane sharing each query record; 3: two query
+        // records per lane sharing one reference position; 2: one hit per lane
+        const bool pair = (h->coarse_gate == 1 || h->coarse_gate == 3) && h->coarse_fits16;
+        const bool items2 = pair && h->coarse_gate == 1;
         const bool t1k = h->bgate_threads == 1024, pv = h->coarse_pv, c0 = h->coarse_cc.c0 != 0;
         const CGateFn cgate_fn =
             !pair ? (t1k ? k_bgate_coarse<1024> : k_bgate_coarse<512>)
+            : items2
+                ? (t1k ? (pv ? (c0 ? k_bgate_coarse2p<1024, true, true> : k_bgate_coarse2p<1024, true, false>)
+                             : (c0 ? k_bgate_coarse2p<1024, false, true> : k_bgate_coarse2p<1024, false, false>))
+                       : (pv ? (c0 ? k_bgate_coarse2p<512, true, true> : k_bgate_coarse2p<512, true, false>)
+                             : (c0 ? k_bgate_coarse2p<512, false, true> : k_bgate_coarse2p<512, false, false>)))
             : t1k ? (pv ? (c0 ? k_bgate_coarse2<1024, true, true> : k_bgate_coarse2<1024, true, false>)
                         : (c0 ? k_bgate_coarse2<1024, false, true> : k_bgate_coarse2<1024, false, false>))
                   : (pv ? (c0 ? k_bgate_coarse2<512, true, true> : k_bgate_coarse2<512, true, false>)
@@ -601,7 +609,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         }
         h->bgate_grid_used = grid;
         h->batch_order_used = kmer_order ? 1 : 0;
-        h->coarse_used = !coarse ? 0 : pair ? 2 : 1;
+        h->coarse_used = !coarse ? 0 : !pair ? 1 : items2 ? 3 : 2;
         auto enqueue_gate = [&]() {
         if (coarse) {
             // coarse bound over every hit, then the two-cell maps over its candidates

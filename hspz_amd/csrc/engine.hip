@@ -1191,7 +1191,7 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
     case HSPZ_FLAG_BATCH_CAND_PLANES: h->cand_planes = value != 0; break;
     case HSPZ_FLAG_BATCH_LANE_MATERIALIZE: h->lane_materialize = value != 0; break;
     case HSPZ_FLAG_BATCH_COARSE:
-        if (value < 0 || value > 2) return set_error(HSPZ_E_ARG, "coarse mode must be 0, 1 or 2");
+        if (value < 0 || value > 3) return set_error(HSPZ_E_ARG, "coarse mode must be 0, 1, 2 or 3");
         h->coarse_gate = value;
         break;
     case HSPZ_FLAG_BATCH_MAX_POSITIONS:

@@ -1818,6 +1818,138 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.finish();
 }
 
+// One 16-cell word of two chains that share the query word q and differ in
+// the reference word (ra for the low halves, rb for the high halves);
+// otherwise coarse_pair_word.
+template <bool PP, bool C0>
+__device__ __forceinline__ void coarse_pair_word_q(uint32_t ra, uint32_t rb, uint32_t q, const CoarsePair &cp,
+                                                   s16x2 &U, s16x2 &mx, s16x2 &mg, s16x2 &B) {
+    coarse_pair_word<PP, C0>(q, ra, rb, cp, U, mx, mg, B);
+}
+
+// The coarse bound with two reference positions per lane: a lane takes two
+// consecutive items of a k-mer group (the second absent at the group's end)
+// and walks the group's query records once for both, one record per
+// iteration, so the hits of an iteration share their query words (hit a: the
+// first item in the low halves, hit b: the second in the high halves). Both
+// items of a lane meet the same records, so they finish together and a lane
+// is refilled once per group walk; only a group with an odd number of
+// positions leaves a half idle. Same arguments, lists and decisions as
+// k_bgate_coarse2.
+template <int BT, bool PP, bool C0>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256, 8))) void k_bgate_coarse2p(
+    const QRec *__restrict__ recs, const uint32_t *__restrict__ skm, const uint32_t *__restrict__ sval,
+    const uint32_t *__restrict__ goff, uint32_t nk, const uint32_t *__restrict__ index,
+    const uint32_t *__restrict__ pos, const uint32_t *__restrict__ clean, const uint4 *__restrict__ rrec,
+    uint32_t seed_size, uint32_t claim, unsigned long long *__restrict__ work, CoarseCoef cc, int32_t xdrop,
+    int32_t thresh, uint4 *__restrict__ cand, uint32_t *__restrict__ cand_n, uint32_t cap_cand,
+    SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow, uint32_t *__restrict__ err) {
+    __shared__ uint4 s_q[2][BT / WAVE][WQ_CAP];
+    KmerSource src;
+    src.skm = skm;
+    src.sval = sval;
+    src.goff = goff;
+    src.index = index;
+    src.pos = pos;
+    src.clean = clean;
+    src.work = work;
+    src.total = __builtin_amdgcn_readfirstlane(goff[nk]);
+    src.claim = claim;
+    src.seed_size = seed_size;
+    src.surv = (SurvRec *)cand;
+    src.surv_n = cand_n;
+    src.cap_surv = cap_cand;
+    src.slow = slow;
+    src.slow_n = slow_n;
+    src.cap_slow = cap_slow;
+    src.err = err;
+    src.set_lists(s_q[0][threadIdx.x / WAVE], s_q[1][threadIdx.x / WAVE]);
+    src.init();
+    const CoarsePair cp = {splat16(cc.sd), splat16(cc.sp), splat16(cc.c0), splat16(cc.pd), splat16(cc.pp), cc.bd,
+                           cc.bp};
+    const s16x2 pc = splat16(cc.pc);
+    const s16x2 zero = splat16(0), lowest = splat16(-32768);
+    // the lane's items: reference positions e (hit a) and e + 1 (hit b, if `two`)
+    bool has = false, item = false, two = false, roka = false, rokb = false;
+    uint32_t Ra = 0, Rb = 0, s_cur = 0, s_end = 0, e = 0;
+    uint32_t A[CW_RIGHT], LA[CW_LEFT], B[CW_RIGHT], LB[CW_LEFT];
+#pragma unroll
+    for (int k = 0; k < CW_RIGHT; k++) A[k] = B[k] = 0;
+#pragma unroll
+    for (int k = 0; k < CW_LEFT; k++) LA[k] = LB[k] = 0;
+    for (;;) {
+        // a lane continues with its items' next query record, else takes a new pair of items
+        if (!has && item) {
+            if (s_cur < s_end) has = true;
+            else item = false;
+        }
+        bool fresh = false;
+        uint64_t need = __ballot(!has);
+        while (need) {
+            if (src.j >= src.n) {
+                if (!src.next_group()) break;
+                continue;
+            }
+            const uint32_t avail = src.n - src.j, rank = mbcnt(need);
+            if (!has && 2 * rank < avail) {
+                e = src.b0 + src.j + 2 * rank;
+                two = 2 * rank + 1 < avail;
+                s_cur = src.g0;
+                s_end = src.g1;
+                has = item = fresh = true;
+            }
+            src.j += min(2 * (uint32_t)__popcll(need), avail);
+            need = __ballot(!has);
+        }
+        if (__ballot(has) == 0) break;
+        if (fresh) {
+            const uint32_t pa = pos[e], pb = two ? pos[e + 1] : 0u;
+            Ra = pa + seed_size;
+            Rb = pb + seed_size;
+            roka = ((clean[e >> 5] >> (e & 31)) & 1u) != 0;
+            rokb = two && ((clean[(e + 1) >> 5] >> ((e + 1) & 31)) & 1u) != 0;
+            const uint4 *ra = rrec + 3 * (size_t)e;
+            const uint4 a0 = ra[0], a1 = ra[1], a2 = ra[2];
+            const uint4 *rb = rrec + 3 * (size_t)(two ? e + 1 : e);
+            const uint4 b0 = rb[0], b1 = rb[1], b2 = rb[2];
+            A[0] = a0.x, A[1] = a0.y, A[2] = a0.z, A[3] = a0.w;
+            LA[0] = a1.x, LA[1] = a1.y, LA[2] = a1.z, LA[3] = a1.w, LA[4] = a2.x, LA[5] = a2.y, LA[6] = a2.z;
+            B[0] = b0.x, B[1] = b0.y, B[2] = b0.z, B[3] = b0.w;
+            LB[0] = b1.x, LB[1] = b1.y, LB[2] = b1.z, LB[3] = b1.w, LB[4] = b2.x, LB[5] = b2.y, LB[6] = b2.z;
+        }
+        const uint32_t g = src.sval[has ? s_cur : 0u];
+        s_cur += has ? 1u : 0u;
+        const uint32_t var = g & ((1u << GV_SHIFT) - 1), qi = g >> GV_SHIFT;
+        const uint4 *rq = (const uint4 *)(recs + qi);
+        const uint4 q0 = rq[0], q1 = rq[1], q2 = rq[2], q3 = rq[3];
+        const uint32_t QW[CW_RIGHT] = {q0.x, q0.y, q0.z, q0.w};
+        const uint32_t LQ[CW_LEFT] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z};
+        s16x2 Ur = zero, mxr = zero, mgr = zero, Br = lowest, Ul = zero, mxl = zero, mgl = zero, Bl = lowest;
+#pragma unroll
+        for (int k = 0; k < CW_LEFT; k++) {
+            if (k < CW_RIGHT) coarse_pair_word_q<PP, C0>(A[k], B[k], QW[k], cp, Ur, mxr, mgr, Br);
+            coarse_pair_word_q<PP, C0>(LA[k], LB[k], LQ[k], cp, Ul, mxl, mgl, Bl);
+        }
+        const s16x2 dd = __builtin_elementwise_max(mgr, mgl);
+        const s16x2 tt = __builtin_elementwise_max(Br + pc, zero) + __builtin_elementwise_max(Bl + pc, zero);
+        const int32_t X = xdrop;
+        const bool da = dd.x < -X, db = dd.y < -X;
+        const int32_t ta = (uint16_t)tt.x, tb = (uint16_t)tt.y;
+        const bool qok = q3.z != 0;
+        const bool hb = has && two;
+        const bool oka = roka && qok, okb = rokb && qok;
+        const bool pa = has && oka && (!da || packed_keep(ta, thresh));
+        const bool pb = hb && okb && (!db || packed_keep(tb, thresh));
+        const uint32_t qv = qi | var << CAND_VAR_SHIFT;
+        src.sq.push(pa, make_uint4(Ra, e, qv, q3.x), err);
+        src.sq.push(pb, make_uint4(Rb, e + 1, qv, q3.x), err);
+        src.defer(has && !oka, q3.y + var, Ra, q3.x);
+        src.defer(hb && !okb, q3.y + var, Rb, q3.x);
+        has = false;
+    }
+    src.finish();
+}
+
 // Hit source over the coarse gate's candidates: waves claim `claim`
 // candidates at a time from a grid-wide counter and hand them to idle lanes,
 // each one hit on its own item. A lane's s_cur is the candidate's query-record

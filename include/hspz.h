@@ -269,11 +269,14 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           scored by the best score of its class: match, transition,
                                           transversion; counted per 16-cell word) and only hits the bound
                                           does not reject run the two-cell maps; two hits per lane in packed
-                                          16-bit sums when every class score is within +-127, else one hit
-                                          per lane in 32-bit sums. 2: always the one-hit form. 0: every hit
-                                          runs the two-cell maps. The lists do not change */
+                                          16-bit sums when every class score is within +-127 (two reference
+                                          positions sharing each query record), else one hit per lane in
+                                          32-bit sums. 3: the packed form with two query records sharing one
+                                          reference position. 2: always the one-hit form. 0: every hit runs
+                                          the two-cell maps. The lists do not change */
 #define HSPZ_FLAG_BATCH_COARSE_USED 14 /* read-only: the coarse bound form the last batched pass ran:
-                                          0 none, 1 one hit per lane, 2 two hits per lane */
+                                          0 none, 1 one hit per lane, 2 two query records per lane, 3 two
+                                          reference positions per lane */
 #define HSPZ_FLAG_DUMP_RAW 15          /* Engine::dump_raw (gpu/mod.rs:55-67): 1 = hspz_seed_and_filter keeps
                                           the pre-dedup records in hspz_filter_output.raw; 0 (default) */
 #define HSPZ_FLAG_BATCH_CAND_PLANES 16 /* where the two-cell maps behind the coarse bound read a candidate's

@@ -42,10 +42,10 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
             for i, (a, g) in enumerate(zip(again, got)):
                 assert same_intervals(a, g), "chunk %d differs in hit order %d" % (i, order)
         # k-mer-major order without the coarse bound ahead of the two-cell maps, with
-        # it in its paired form (HOXD70's class scores fit 16 bits) and single-hit form,
+        # it in its paired forms (HOXD70's class scores fit 16 bits) and single-hit form,
         # the candidates' windows read from the planes and from the records
         eng.set_flag("batch_order", 1)
-        for coarse, used, planes in ((0, 0, 1), (1, 2, 1), (2, 1, 1), (1, 2, 0), (2, 1, 0)):
+        for coarse, used, planes in ((0, 0, 1), (1, 3, 1), (3, 2, 1), (2, 1, 1), (1, 3, 0), (2, 1, 0)):
             eng.set_flag("batch_coarse", coarse)
             eng.set_flag("batch_cand_planes", planes)
             again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
@@ -511,21 +511,22 @@ def test_batch_score_sum_forms(gpu_available, scale, xdrop):
 
 
 @pytest.mark.parametrize("name,acgt,xdrop,used", [
-    ("hoxd70", [91, -114, -31, -123, -114, 100, -125, -31, -31, -125, 100, -114, -123, -31, -114, 91], 910, 2),
+    ("hoxd70", [91, -114, -31, -123, -114, 100, -125, -31, -31, -125, 100, -114, -123, -31, -114, 91], 910, 3),
     # a transversion pair scoring +20: the transversion class adds to the positive part
     ("positive transversion", [91, 20, -31, -123, 20, 100, -125, -31, -31, -125, 100, -114, -123, -31, -114, 91],
-     910, 2),
+     910, 3),
     # class bests 101 / -31 / -112: the count coefficients -294 and 81 share the factor 3, which
     # 16 x 101 lacks, so no count biases exist and the paired sums carry their constant
-    ("no bias pair", [91, -112, -31, -123, -112, 101, -125, -31, -31, -125, 101, -112, -123, -31, -112, 91], 910, 2),
+    ("no bias pair", [91, -112, -31, -123, -112, 101, -125, -31, -31, -125, 101, -112, -123, -31, -112, 91], 910, 3),
     # x2: class scores beyond 127, so the single-hit form even when the paired one is asked for
     ("hoxd70 x2", [2 * v for v in [91, -114, -31, -123, -114, 100, -125, -31, -31, -125, 100, -114, -123, -31,
                                    -114, 91]], 1820, 1),
 ])
 def test_batch_coarse_forms(gpu_available, name, acgt, xdrop, used):
-    """The coarse bound's two forms (two hits per lane in packed 16-bit sums;
-    one hit per lane in 32-bit sums) and no bound, against the oracle, on
-    matrices that select each form and both positive-part variants."""
+    """The coarse bound's forms (two hits per lane in packed 16-bit sums, over
+    two reference positions or two query records; one hit per lane in 32-bit
+    sums) and no bound, against the oracle, on matrices that select each form
+    and both positive-part variants."""
     _say("case batch: coarse bound forms, %s" % name)
     thresh = 3000 * (2 if "x2" in name else 1)
     target = synth.random_bases(0xC0A5 + len(name), 200_000).tobytes()
@@ -544,7 +545,8 @@ def test_batch_coarse_forms(gpu_available, name, acgt, xdrop, used):
         eng.set_flag("batch_order", 1)
         for threads in (1024, 512):
             eng.set_flag("batch_block_threads", threads)
-            for coarse, u, planes in ((1, used, 1), (1, used, 0), (2, 1, 1), (2, 1, 0), (0, 0, 1)):
+            for coarse, u, planes in ((1, used, 1), (1, used, 0), (3, 2 if used == 3 else 1, 1), (2, 1, 1),
+                                      (2, 1, 0), (0, 0, 1)):
                 eng.set_flag("batch_coarse", coarse)
                 eng.set_flag("batch_cand_planes", planes)
                 got, st = eng.filter_batch(chunks)
