@@ -70,6 +70,16 @@ def profiled_traffic(hits_per_launch, order_used, coarse_used):
     return int(m.group(2))
 
 
+def profiled_valu(order_used, coarse_used):
+    """(VALU wave-instructions per hit, VALU issue fraction) of the coarse
+    bound from the committed profile of this path, or None."""
+    path = os.path.join(ROOT, gate_profile(order_used, coarse_used))
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"valu-line: \S+ valu_per_hit=([0-9.]+) valu_issue=([0-9.]+)", open(path).read())
+    return (float(m.group(1)), float(m.group(2))) if m else None
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -298,8 +308,11 @@ def main():
         if not order_used:
             limit = "memory fetch: each hit fetches its own reference lines (%s)" % prof
         elif coarse_used:
-            limit = ("VALU issue: the coarse bound over every hit keeps 95%% of the SIMDs' issue slots busy "
-                     "(2.7 wave-instructions per hit); HBM at 13.4 B/hit (%s)" % prof)
+            pv = profiled_valu(order_used, coarse_used)
+            limit = ("VALU issue: the coarse bound over every hit keeps %.0f%% of the SIMDs' issue slots busy "
+                     "(%.2f wave-instructions per hit); HBM traffic %s (%s)"
+                     % (100 * pv[1], pv[0], "%.1f B/hit" % (traffic / st_b["hits"]) if traffic else "not profiled",
+                        prof) if pv else "VALU issue: the coarse bound over every hit (%s)" % prof)
         else:
             limit = "VALU issue: the two-cell maps over every hit saturate the vector ALU (%s)" % prof
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
