@@ -2071,15 +2071,6 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     kmer_gate_merged<2, SUMS>(src, rplane, rrec, recs, (const uint8_t *)s_map, xdrop, thresh);
 }
 
-// The merged gate (RW = 2) over the coarse gate's candidates with its global
-// loads issued ahead, in three stages per lane: a claimed candidate entry
-// (loaded in one iteration), its next hit's reference and query records
-// (loads issued in the next, from the landed entry), and the current hit
-// (promoted from those records once the lane is free). Every load is issued
-// one iteration before its data is used, so the counter wait at the top of an
-// iteration finds them landed while the wave computed. Cells, decisions and
-// lists are those of kmer_gate_merged<2, SUMS> over CandSource; a finished
-// hit's results are appended at the top of the next iteration.
 // The 13 plane words around anchor P (word 8 holds base P) of a window whose
 // margins were checked when the hit became a candidate.
 __device__ __forceinline__ void load_window(const uint32_t *__restrict__ plane, uint32_t P, uint32_t (&W)[13]) {
@@ -2100,8 +2091,15 @@ __device__ __forceinline__ void window_streams(const uint32_t (&W)[13], uint32_t
         LA[k] = nib_swap(__builtin_bswap32(__builtin_amdgcn_alignbit(W[8 - k], W[7 - k], sh)));
 }
 
-// The merged 96-cell-step gate over the coarse gate's candidates, every load
-// issued one iteration ahead. PLANES = false: a candidate's windows are its
+// The merged gate (RW = 2) over the coarse gate's candidates with its global
+// loads issued ahead, in three stages per lane: a claimed candidate entry
+// (loaded in one iteration), its next hit's windows (loads issued in the
+// next, from the landed entry), and the current hit (promoted from those
+// windows once the lane is free). Every load is issued one iteration before
+// its data is used, so the counter wait at the top of an iteration finds them
+// landed while the wave computed. Cells, decisions and lists are those of
+// kmer_gate_merged<2, SUMS> over CandSource; a finished hit's results are
+// appended at the top of the next iteration. PLANES = false: a candidate's windows are its
 // reference and query records (Gray-coded; the Gray-keyed maps). PLANES =
 // true: they are read from the 2-bit planes at R and Q (the reference plane
 // is 250 MB at 1 Gbp, the pass's query plane a few MB, against 48 B per
