@@ -204,14 +204,15 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles_u64(uint64_t *__res
 // Phase 3 of the u64 scan: each tile's inclusive prefix of the counts, plus
 // the tile's offset. Items are read and written coalesced (item i * THREADS +
 // thread) and transposed through LDS, so each thread scans 16 consecutive
-// items (rows padded by one word per 16: no bank conflicts).
+// items (one spare slot per 16 items spreads a wave's transposed reads over
+// the banks).
 __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_u64(const uint32_t *__restrict__ in,
                                                                  const uint32_t *__restrict__ nvalid,
                                                                  uint32_t per,
                                                                  const uint64_t *__restrict__ offs,
                                                                  uint64_t *__restrict__ out) {
-    constexpr uint32_t PAD = TILE + TILE / TILE_ITEMS;
-    __shared__ uint64_t s_v[PAD];
+    constexpr uint32_t SLOTS = TILE + TILE / TILE_ITEMS;
+    __shared__ uint64_t s_v[SLOTS];
     const uint32_t n = *nvalid * per;
     if (blockIdx.x >= (n + TILE - 1) / TILE) return;
     const uint32_t t0 = blockIdx.x * TILE;
