@@ -257,8 +257,10 @@ def main():
     if not order_used:
         gate_kernel = "k_bgate_packed"
     elif coarse_used:
-        gate_kernel = "%s (coarse bound over every hit) + k_bgate_exact_ahead (its candidates)" % (
-            ("k_bgate_coarse", "k_bgate_coarse2", "k_bgate_coarse2p")[coarse_used - 1])
+        gate_kernel = "%s (coarse bound over every hit) + %sk_bgate_exact_ahead (its candidates)" % (
+            ("k_bgate_coarse", "k_bgate_coarse2", "k_bgate_coarse2p")[coarse_used - 1],
+            "k_cand_hist, k_cand_scatter (binned by reference position) + "
+            if eng.get_flag("batch_cand_binned") else "")
     elif (eng.get_flag("batch_step_cells") >= 96 and eng.get_flag("batch_block_threads") == 512
           and eng.get_flag("batch_lookahead") and eng.get_flag("ref_records") == 2):
         gate_kernel = "k_bgate_ahead"
@@ -328,6 +330,7 @@ def main():
                 "coarse_bound": {"form": ("off", "one hit per lane", "two query records per lane",
                                           "two reference positions per lane")[eng.get_flag("batch_coarse_used")],
                                  "candidates_per_step": cand_step,
+                                 "candidates_binned": bool(eng.get_flag("batch_cand_binned")),
                                  "candidate_frac": round(cand_step / max(st_b["hits"], 1), 5)}}
 
     out = {

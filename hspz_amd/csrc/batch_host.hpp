@@ -30,7 +30,7 @@ struct HostBatch {
 struct BatchBufs {
     DevBuf arena, plane, coarse, desc, poff, blk_off, blk_len, valid, ids, voff, seeds, counts, prefix,
         tiles, nchunks, starts, bhits, surv, slow, mhsp, mscope, keep, kids, ohsp, oscope, ctr, gcnt, sq, skm, qrec,
-        kvalo, ktmp, exact, kkey, kgaps, cand;
+        kvalo, ktmp, exact, kkey, kgaps, cand, cand2, cbins;
 };
 
 // Largest survivor or deferred-hit list of one pass, in records (16 B each).
@@ -308,6 +308,10 @@ struct DirectOut {
     std::vector<uint32_t> batch_counts;
 };
 
+// Blocks of the candidate binning (histogram and scatter alike): one per CU,
+// whose scatter tile (8 records per thread, 128 KiB) fills most of its LDS.
+uint32_t cand_scatter_blocks(const hspz_engine *h) { return (uint32_t)std::max(h->num_cus, 1); }
+
 int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                const hspz_query_chunk *chunks, size_t n_chunks, const std::vector<HostBatch> &hb,
                std::vector<hspz_segment_pair> &hsps, std::vector<uint32_t> &scopes,
@@ -447,6 +451,17 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         if (kmer_order) {
             if ((rc = reserve(h, B.gcnt, ((size_t)nkm + 1) * 4))) return rc;
             if (h->coarse_gate && (rc = reserve(h, B.cand, (size_t)bs->cap_cand * 16))) return rc;
+            // the binned copy of the candidates, when the device has room for it
+            if (h->coarse_gate && h->cand_planes && h->lookahead && h->cand_bins &&
+                B.cand2.cap < (size_t)bs->cap_cand * 16) {
+                size_t f = 0, t = 0;
+                HIP_TRY(hipMemGetInfo(&f, &t));
+                if ((uint64_t)f >= (uint64_t)bs->cap_cand * 18 + (8ull << 30)) {
+                    if ((rc = reserve(h, B.cand2, (size_t)bs->cap_cand * 16))) return rc;
+                    // bin sizes, bin bases, then one row of per-block counts per bin
+                    if ((rc = reserve(h, B.cbins, (size_t)(2 + cand_scatter_blocks(h)) * CAND_BINS * 4))) return rc;
+                }
+            }
             if ((rc = reserve(h, B.kgaps, (size_t)kgap_capacity(nkm) * sizeof(uint4)))) return rc;
             if ((rc = reserve(h, B.kkey, cap_seeds * 4 + 4))) return rc;   // sort keys (u32)
             if ((rc = reserve(h, B.sq, cap_seeds * 4 + 4))) return rc;     // sort values (u32), in
@@ -548,7 +563,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         typedef void (*XGateFn)(const uint32_t *, const uint32_t *, int32_t, int32_t, const QRec *,
                                 const uint32_t *, const uint4 *, const uint4 *, const uint32_t *, uint32_t, uint32_t,
                                 unsigned long long *, SurvRec *, uint32_t *, uint32_t, SurvRec *, uint32_t *,
-                                uint32_t, uint32_t *, const uint32_t *, uint32_t);
+                                uint32_t, uint32_t *, const uint32_t *, uint32_t, uint32_t);
         // the paired form (two hits per lane, packed 16-bit sums) where the class scores fit
         // 1: two reference positions per lane sharing each query record; 3: two query
         // records per lane sharing one reference position; 2: one hit per lane
@@ -572,6 +587,12 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         const int xthreads = xahead ? 512 : h->bgate_threads;
         // with the lookahead, the candidates' windows come from the planes unless asked otherwise
         const bool xplanes = xahead && h->cand_planes;
+        // and binned by reference position, when the binned copy fits
+        const uint32_t cblk = cand_scatter_blocks(h);
+        const bool binned = coarse && xplanes && h->cand_bins && B.cand2.p && B.cand2.cap >= (size_t)bs->cap_cand * 16 &&
+                            B.cbins.p && B.cbins.cap >= (size_t)(2 + cblk) * CAND_BINS * 4;
+        uint32_t cand_shift = 0;  // bins of 2^shift reference bases, at most CAND_BINS of them
+        while ((h->ref_len >> cand_shift) >= CAND_BINS) cand_shift++;
         if (xplanes)
             xgate_fn = h->sum_mode == SUM_RAW      ? k_bgate_exact_ahead<SUM_RAW, true>
                        : h->sum_mode == SUM_TREE16 ? k_bgate_exact_ahead<SUM_TREE16, true>
@@ -610,6 +631,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         h->bgate_grid_used = grid;
         h->batch_order_used = kmer_order ? 1 : 0;
         h->coarse_used = !coarse ? 0 : !pair ? 1 : items2 ? 3 : 2;
+        h->cand_binned = binned ? 1 : 0;
         auto enqueue_gate = [&]() {
         if (coarse) {
             // coarse bound over every hit, then the two-cell maps over its candidates
@@ -620,13 +642,23 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                (unsigned long long *)(ctr + CTR_WORK), h->coarse_cc, h->xdrop, h->hspthresh,
                                B.cand.as<uint4>(), ctr + CTR_CAND, bs->cap_cand, B.slow.as<SurvRec>(),
                                ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
+            if (binned) {  // the candidates in reference-position bins, for the plane reads
+                uint32_t *sizes = B.cbins.as<uint32_t>(), *bases = sizes + CAND_BINS, *table = bases + CAND_BINS;
+                hipLaunchKernelGGL(k_cand_hist, dim3(cblk), dim3(CAND_BINS), 0, h->stream, B.cand.as<uint4>(),
+                                   ctr + CTR_CAND, bs->cap_cand, cand_shift, table);
+                hipLaunchKernelGGL(k_cand_rows, dim3(CAND_BINS), dim3(256), 0, h->stream, table, cblk, sizes);
+                hipLaunchKernelGGL(k_cand_scan, dim3(1), dim3(CAND_BINS), 0, h->stream, sizes, bases);
+                hipLaunchKernelGGL(k_cand_scatter<8>, dim3(cblk), dim3(CAND_BINS), 0, h->stream, B.cand.as<uint4>(),
+                                   ctr + CTR_CAND, bs->cap_cand, cand_shift, table, bases, B.cand2.as<uint4>());
+            }
             hipLaunchKernelGGL(xgate_fn, dim3(xgrid), dim3(xthreads), 0, h->stream, h->ref2.as<uint32_t>(),
                                h->pairtab.as<uint32_t>() + (xplanes ? 0 : PAIRTAB_GRAY), h->xdrop, h->hspthresh,
-                               B.qrec.as<QRec>(), B.kvalo.as<uint32_t>(), h->rrec.as<uint4>(), B.cand.as<uint4>(),
+                               B.qrec.as<QRec>(), B.kvalo.as<uint32_t>(), h->rrec.as<uint4>(),
+                               binned ? B.cand2.as<uint4>() : B.cand.as<uint4>(),
                                ctr + CTR_CAND, bs->cap_cand, (uint32_t)h->bgate_claim,
                                (unsigned long long *)(ctr + CTR_WORK2), B.surv.as<SurvRec>(), ctr + CTR_SURV,
                                bs->cap_surv, B.slow.as<SurvRec>(), ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR,
-                               B.plane.as<uint32_t>(), per);
+                               B.plane.as<uint32_t>(), per, binned ? 1u : 0u);
         } else if (kmer_order)
             hipLaunchKernelGGL(kgate_fn, dim3(grid), dim3(h->bgate_threads), 0, h->stream, h->ref2.as<uint32_t>(),
                                h->pairtab.as<uint32_t>() + PAIRTAB_GRAY, h->xdrop, h->hspthresh, B.qrec.as<QRec>(),
@@ -701,7 +733,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                   B.keep.p, B.kids.p, B.ohsp.p, B.oscope.p, B.ctr.p, h->tiles.p, h->ref2.p,
                                   h->refx.p, h->pairtab.p, h->index.p, h->pos.p, B.gcnt.p, B.sq.p, B.skm.p,
                                   h->clean.p, B.qrec.p, h->rrec.p, B.kvalo.p, B.exact.p, B.kkey.p, B.kgaps.p,
-                                  B.cand.p};
+                                  B.cand.p, binned ? B.cand2.p : nullptr, binned ? B.cbins.p : nullptr};
             static_assert(sizeof ptrs <= sizeof key.ptrs, "graph key pointer slots");
             memcpy(key.ptrs, ptrs, sizeof ptrs);
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow | (uint64_t)bs->cap_cand << 32,

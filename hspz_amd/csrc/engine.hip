@@ -117,6 +117,8 @@ struct hspz_engine {
     int coarse_gate = 1;               // HSPZ_FLAG_BATCH_COARSE: class-score bound ahead of the two-cell maps
     bool cand_planes = true;           // HSPZ_FLAG_BATCH_CAND_PLANES: candidates' windows from the planes
     bool lane_materialize = true;      // HSPZ_FLAG_BATCH_LANE_MATERIALIZE: a survivor per lane first
+    bool cand_bins = true;             // HSPZ_FLAG_BATCH_CAND_BINS: candidates binned by reference position
+    int cand_binned = 0;               // whether the last batched pass binned them
                                        // (0 off, 1 paired form where the class scores fit, 2 single-hit form)
     int coarse_used = 0;               // the form the last batched pass ran (0: none)
     bool coarse_fits16 = false;        // every |class score| <= COARSE_PAIR_MAX_CLASS
@@ -810,6 +812,7 @@ extern "C" int hspz_engine_share(hspz_engine *base, hspz_engine **out) {
     h->coarse_gate = base->coarse_gate;
     h->cand_planes = base->cand_planes;
     h->lane_materialize = base->lane_materialize;
+    h->cand_bins = base->cand_bins;
     h->batch_graph = base->batch_graph;
     h->device_dedup = base->device_dedup;
     h->list_cap = base->list_cap;
@@ -855,7 +858,8 @@ extern "C" void hspz_engine_free(hspz_engine *h) {
         DevBuf *bb[] = {&B.arena, &B.plane, &B.coarse, &B.desc, &B.poff, &B.blk_off, &B.blk_len, &B.valid, &B.ids,
                         &B.voff, &B.seeds, &B.counts, &B.prefix, &B.tiles, &B.nchunks, &B.starts, &B.bhits, &B.surv,
                         &B.slow, &B.mhsp, &B.mscope, &B.keep, &B.kids, &B.ohsp, &B.oscope, &B.ctr, &B.gcnt,
-                        &B.sq, &B.skm, &B.qrec, &B.kvalo, &B.ktmp, &B.exact, &B.kkey, &B.kgaps, &B.cand};
+                        &B.sq, &B.skm, &B.qrec, &B.kvalo, &B.ktmp, &B.exact, &B.kkey, &B.kgaps, &B.cand,
+                        &B.cand2, &B.cbins};
         for (DevBuf *b : bb) release(*b);
         hspz_batch_state *bs = h->batch;
         DevBuf *dd[] = {&bs->dd_key_a, &bs->dd_key_b, &bs->dd_id_a, &bs->dd_id_b, &bs->dd_keep, &bs->dd_kids,
@@ -1190,6 +1194,7 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
     case HSPZ_FLAG_DUMP_RAW: h->dump_raw = value != 0; break;
     case HSPZ_FLAG_BATCH_CAND_PLANES: h->cand_planes = value != 0; break;
     case HSPZ_FLAG_BATCH_LANE_MATERIALIZE: h->lane_materialize = value != 0; break;
+    case HSPZ_FLAG_BATCH_CAND_BINS: h->cand_bins = value != 0; break;
     case HSPZ_FLAG_BATCH_COARSE:
         if (value < 0 || value > 3) return set_error(HSPZ_E_ARG, "coarse mode must be 0, 1, 2 or 3");
         h->coarse_gate = value;
@@ -1238,6 +1243,8 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_DUMP_RAW: *value = h->dump_raw ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_CAND_PLANES: *value = h->cand_planes ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_LANE_MATERIALIZE: *value = h->lane_materialize ? 1 : 0; break;
+    case HSPZ_FLAG_BATCH_CAND_BINS: *value = h->cand_bins ? 1 : 0; break;
+    case HSPZ_FLAG_BATCH_CAND_BINNED: *value = h->cand_binned; break;
     case HSPZ_FLAG_BATCH_COARSE_USED: *value = h->coarse_used; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }

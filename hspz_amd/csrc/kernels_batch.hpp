@@ -1951,6 +1951,131 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     src.finish();
 }
 
+// Candidates binned by reference position before the candidates' gate reads
+// their windows from the planes: bin = R >> shift, at most CAND_BINS bins (a
+// shift that makes each bin's reference-plane segment a few hundred KiB, which
+// the L2 of an XCD holds while its waves work through the bin).
+constexpr uint32_t CAND_BINS = 1024;
+// XCDs of the device: consecutive workgroups go to consecutive XCDs.
+constexpr uint32_t XCDS = 8;
+// Entries of a wave's chunk in interleaved shares of a binned list.
+constexpr uint32_t CAND_CHUNK = 64;
+
+__device__ __forceinline__ uint32_t cand_bin(uint32_t R, uint32_t shift) { return min(R >> shift, CAND_BINS - 1); }
+
+// Block b's share [c0, c1) of a list of n entries; the histogram and the
+// scatter split the list alike.
+__device__ __forceinline__ void cand_share(uint32_t n, uint32_t b, uint32_t nb, uint32_t &c0, uint32_t &c1) {
+    c0 = (uint32_t)((uint64_t)n * b / nb);
+    c1 = (uint32_t)((uint64_t)n * (b + 1) / nb);
+}
+
+// Per-block bin sizes: block b counts the bins of its share of the candidate
+// list into table[bin * gridDim.x + b]; CAND_BINS threads.
+__global__ __launch_bounds__(CAND_BINS) void k_cand_hist(const uint4 *__restrict__ cand,
+                                                         const uint32_t *__restrict__ cand_n, uint32_t cap,
+                                                         uint32_t shift, uint32_t *__restrict__ table) {
+    __shared__ uint32_t h[CAND_BINS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t c0, c1;
+    cand_share(min(*cand_n, cap), blockIdx.x, gridDim.x, c0, c1);
+    const uint32_t *r = (const uint32_t *)cand;  // word 0 of each record is R
+    for (uint32_t i = c0 + threadIdx.x; i < c1; i += CAND_BINS) atomicAdd(&h[cand_bin(r[4 * (size_t)i], shift)], 1u);
+    __syncthreads();
+    table[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Each bin's row of the table (one entry per scatter block) to its exclusive
+// prefix sums, and the bin's size to sizes[bin]; one 256-thread block per bin.
+__global__ __launch_bounds__(256) void k_cand_rows(uint32_t *__restrict__ table, uint32_t nblk,
+                                                   uint32_t *__restrict__ sizes) {
+    uint32_t *row = table + (size_t)blockIdx.x * nblk;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nblk; b0 += 256) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < nblk ? row[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<256>(v, &tot);
+        if (i < nblk) row[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) sizes[blockIdx.x] = carry;
+}
+
+// The bins' first entries: bases = exclusive scan of sizes; one block of
+// CAND_BINS threads.
+__global__ __launch_bounds__(CAND_BINS) void k_cand_scan(const uint32_t *__restrict__ sizes,
+                                                         uint32_t *__restrict__ bases) {
+    uint32_t tot;
+    bases[threadIdx.x] = block_excl_scan<CAND_BINS>(sizes[threadIdx.x], &tot);
+}
+
+// The candidates in bin order: block b writes the records of its share that
+// fall in a bin from bases[bin] + table[bin * gridDim.x + b] on. Each tile of
+// CPT records per thread is counting-sorted by bin in LDS first, so that a
+// bin's records of the tile go out as one run of consecutive stores (order
+// within a bin is free: the gate's decisions do not depend on it); the next
+// tile's records are loaded while the current one is sorted.
+template <int CPT>
+__global__ __launch_bounds__(CAND_BINS) void k_cand_scatter(const uint4 *__restrict__ cand,
+                                                            const uint32_t *__restrict__ cand_n, uint32_t cap,
+                                                            uint32_t shift, const uint32_t *__restrict__ table,
+                                                            const uint32_t *__restrict__ bases,
+                                                            uint4 *__restrict__ out) {
+    constexpr uint32_t TILE = CPT * CAND_BINS;
+    __shared__ uint4 s_rec[TILE];
+    __shared__ uint32_t s_cnt[CAND_BINS], s_start[CAND_BINS], s_cur[CAND_BINS];
+    const uint32_t t = threadIdx.x;
+    s_cur[t] = bases[t] + table[(size_t)t * gridDim.x + blockIdx.x];
+    uint32_t c0, c1;
+    cand_share(min(*cand_n, cap), blockIdx.x, gridDim.x, c0, c1);
+    uint4 nxt[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const uint32_t i = c0 + t + k * CAND_BINS;
+        nxt[k] = i < c1 ? cand[i] : make_uint4(0, 0, 0, 0);
+    }
+    for (uint32_t t0 = c0; t0 < c1; t0 += TILE) {
+        const uint32_t m = min(TILE, c1 - t0);
+        uint4 rec[CPT];
+        uint32_t bin[CPT], rank[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; k++) {
+            rec[k] = nxt[k];
+            const uint32_t i = t0 + TILE + t + k * CAND_BINS;
+            nxt[k] = i < c1 ? cand[i] : make_uint4(0, 0, 0, 0);
+        }
+        s_cnt[t] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < CPT; k++) {
+            bin[k] = cand_bin(rec[k].x, shift);
+            rank[k] = t + k * CAND_BINS < m ? atomicAdd(&s_cnt[bin[k]], 1u) : 0u;
+        }
+        __syncthreads();
+        const uint32_t cnt = s_cnt[t];
+        uint32_t tot;
+        s_start[t] = block_excl_scan<CAND_BINS>(cnt, &tot);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < CPT; k++)
+            if (t + k * CAND_BINS < m) s_rec[s_start[bin[k]] + rank[k]] = rec[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < CPT; k++) {
+            const uint32_t j = t + k * CAND_BINS;
+            if (j < m) {
+                const uint4 r = s_rec[j];
+                const uint32_t b = cand_bin(r.x, shift);
+                out[s_cur[b] + j - s_start[b]] = r;
+            }
+        }
+        __syncthreads();
+        s_cur[t] += cnt;
+    }
+}
+
 // Hit source over the coarse gate's candidates: waves claim `claim`
 // candidates at a time from a grid-wide counter and hand them to idle lanes,
 // each one hit on its own item. A lane's s_cur is the candidate's query-record
@@ -1959,10 +2084,22 @@ struct CandSource : GateOut {
     const uint4 *list;
     unsigned long long *work;
     uint32_t n, claim, c_cur, c_end;
+    uint64_t s_next;                   // interleaved shares: the wave's next chunk,
+    uint32_t s_step, s_chunk, s_hi;    // the chunks' stride and size, the share's end (s_step 0: claims)
     bool dry;
 
     __device__ __forceinline__ bool take() {
         if (dry) return false;
+        if (s_step) {
+            if (s_next >= s_hi) {
+                dry = true;
+                return false;
+            }
+            c_cur = (uint32_t)s_next;
+            c_end = s_next + s_chunk < (uint64_t)s_hi ? (uint32_t)(s_next + s_chunk) : s_hi;
+            s_next += s_step;
+            return true;
+        }
         uint32_t leader = (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1);
         unsigned long long v = 0;
         if (lane_id() == leader) v = atomicAdd(work, (unsigned long long)claim);
@@ -1984,8 +2121,21 @@ struct CandSource : GateOut {
         c_end = (uint32_t)((uint64_t)n * (w + 1) / waves);
         dry = true;
     }
+    // Interleaved shares of [lo, hi): wave w of `waves` takes the chunks of
+    // `chunk` entries w, w + waves, w + 2 waves, ..., so that the waves work on
+    // neighbouring entries at any time (no atomic either).
+    __device__ __forceinline__ void interleave(uint32_t lo, uint32_t hi, uint32_t w, uint32_t waves,
+                                               uint32_t chunk) {
+        c_cur = c_end = 0;
+        s_next = (uint64_t)lo + (uint64_t)w * chunk;
+        s_step = waves * chunk;
+        s_chunk = chunk;
+        s_hi = hi;
+        dry = false;
+    }
     __device__ __forceinline__ void init() {
         c_cur = c_end = 0;
+        s_step = 0;
         dry = false;
     }
     // Hands each lane with `want` its next candidate entry; whether it got one.
@@ -2048,9 +2198,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
     const uint4 *__restrict__ cand, const uint32_t *__restrict__ cand_n, uint32_t cap_cand, uint32_t claim,
     unsigned long long *__restrict__ work, SurvRec *__restrict__ surv, uint32_t *__restrict__ surv_n,
     uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow,
-    uint32_t *__restrict__ err, const uint32_t *__restrict__ qplane, uint32_t per) {
+    uint32_t *__restrict__ err, const uint32_t *__restrict__ qplane, uint32_t per, uint32_t interleave) {
     (void)qplane;
     (void)per;
+    (void)interleave;
     __shared__ uint32_t s_map[G2_MAP_WORDS];
     __shared__ uint4 s_dq[BT / WAVE][WQ_CAP];
     load_map_table(s_map, pairtab, BT);
@@ -2230,7 +2381,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const uint4 *__restrict__ cand, const uint32_t *__restrict__ cand_n, uint32_t cap_cand, uint32_t claim,
     unsigned long long *__restrict__ work, SurvRec *__restrict__ surv, uint32_t *__restrict__ surv_n,
     uint32_t cap_surv, SurvRec *__restrict__ slow, uint32_t *__restrict__ slow_n, uint32_t cap_slow,
-    uint32_t *__restrict__ err, const uint32_t *__restrict__ qplane, uint32_t per) {
+    uint32_t *__restrict__ err, const uint32_t *__restrict__ qplane, uint32_t per, uint32_t interleave) {
     __shared__ uint32_t s_map[G2_MAP_WORDS];
     __shared__ uint4 s_dq[512 / WAVE][WQ_CAP];
     load_map_table(s_map, pairtab, 512);
@@ -2250,7 +2401,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     src.set_lists(nullptr, s_dq[threadIdx.x / WAVE]);
     src.init();
     (void)work;
-    src.share(blockIdx.x * (512 / WAVE) + threadIdx.x / WAVE, gridDim.x * (512 / WAVE));
+    // Blocks are dealt to the eight XCDs in turn. With binned candidates, the
+    // waves of one XCD interleave small chunks of one eighth of the list, so
+    // that they move through its bins together and the XCD's L2 holds the
+    // reference-plane lines of the bin they are in.
+    if (interleave && gridDim.x % XCDS == 0) {
+        const uint32_t x = blockIdx.x % XCDS, per_x = gridDim.x / XCDS * (512 / WAVE);
+        const uint32_t w = blockIdx.x / XCDS * (512 / WAVE) + threadIdx.x / WAVE;
+        src.interleave((uint32_t)((uint64_t)src.n * x / XCDS), (uint32_t)((uint64_t)src.n * (x + 1) / XCDS), w,
+                       per_x, CAND_CHUNK);
+    } else {
+        src.share(blockIdx.x * (512 / WAVE) + threadIdx.x / WAVE, gridDim.x * (512 / WAVE));
+    }
     cand_gate_ahead<SUMS, PLANES>(src, rrec, recs, rplane, qplane, per, (const uint8_t *)s_map, xdrop, thresh);
 }
 

@@ -31,7 +31,8 @@ FLAG = {"batch_graph": 0, "batch_gate_blocks": 1, "batch_step_cells": 2, "batch_
         "batch_claim_seeds": 4, "batch_order": 5, "batch_order_used": 6, "batch_device_dedup": 7,
         "ref_records": 8, "batch_lookahead": 9, "batch_list_cap": 10, "noalloc_tile_hits": 11,
         "batch_max_positions": 12, "batch_coarse": 13, "batch_coarse_used": 14, "dump_raw": 15,
-        "batch_cand_planes": 16, "batch_lane_materialize": 17}
+        "batch_cand_planes": 16, "batch_lane_materialize": 17,
+        "batch_cand_bins": 18, "batch_cand_binned": 19}
 
 # Every symbol include/hspz.h declares (checked by tests/test_abi.py).
 EXPORTS = [

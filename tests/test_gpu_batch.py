@@ -50,10 +50,19 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
             eng.set_flag("batch_cand_planes", planes)
             again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
             assert eng.get_flag("batch_coarse_used") == used
+            # candidates binned by reference position whenever their windows come from the planes
+            assert eng.get_flag("batch_cand_binned") == (1 if coarse and planes else 0)
             for i, (a, g) in enumerate(zip(again, got)):
                 assert same_intervals(a, g), "chunk %d differs with coarse mode %d, planes %d" % (i, coarse, planes)
         eng.set_flag("batch_coarse", 1)
         eng.set_flag("batch_cand_planes", 1)
+        # the planes' candidates in the coarse gate's order, unbinned
+        eng.set_flag("batch_cand_bins", 0)
+        again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
+        assert eng.get_flag("batch_cand_binned") == 0
+        for i, (a, g) in enumerate(zip(again, got)):
+            assert same_intervals(a, g), "chunk %d differs with unbinned candidates" % i
+        eng.set_flag("batch_cand_bins", 1)
         # the materializer one wave per survivor throughout, instead of a lane per survivor first
         eng.set_flag("batch_lane_materialize", 0)
         again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
@@ -551,11 +560,20 @@ def test_batch_coarse_forms(gpu_available, name, acgt, xdrop, used):
                 eng.set_flag("batch_cand_planes", planes)
                 got, st = eng.filter_batch(chunks)
                 assert eng.get_flag("batch_coarse_used") == u
+                assert eng.get_flag("batch_cand_binned") == (1 if coarse and planes else 0)
                 if coarse:  # homologous chunks: the bound keeps many hits, never more than there are
                     assert 0 < eng.stat("batch_candidates") <= st["hits"]
                 for i, (g, w) in enumerate(zip(got, want)):
                     assert same_intervals(g, w), "chunk %d differs in coarse mode %d, planes %d, %d threads" % (
                         i, coarse, planes, threads)
+        # the planes' candidates unbinned
+        eng.set_flag("batch_coarse", 1)
+        eng.set_flag("batch_cand_planes", 1)
+        eng.set_flag("batch_cand_bins", 0)
+        got, _ = eng.filter_batch(chunks)
+        assert eng.get_flag("batch_cand_binned") == 0
+        for i, (g, w) in enumerate(zip(got, want)):
+            assert same_intervals(g, w), "chunk %d differs with unbinned candidates" % i
 
 
 def test_batch_staged_read(gpu_available):
