@@ -112,6 +112,8 @@ def main():
                     help="step_cells,block_threads,claim_seeds,grid_blocks[,order[,lookahead[,coarse[,cand_planes]]]] "
                          "for the batched gate "
                          "(applied after the autotuner; use with --tune-mbp 0 to fix the configuration)")
+    ap.add_argument("--no-cand-bins", action="store_true",
+                    help="run the candidates' gate on the coarse bound's own order, unbinned")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once and exit: the plain copy of known size "
                          "that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE readings")
@@ -224,6 +226,8 @@ def main():
                           for c in cells],
                 "chosen": [best[k] for k in cols]}
 
+    if args.no_cand_bins:
+        eng.set_flag("batch_cand_bins", 0)
     if args.gate:
         vals = [int(x) for x in args.gate.split(",")]
         step, thr, claim, grid = vals[:4]
@@ -483,7 +487,9 @@ def c4_workload(args):
                    max_hits=max_hits, device_index="12of19")
     log("engine up in %.1fs: device index %.0f ms, %d positions, %.1f GB held"
         % (time.time() - t1, eng.stat("index_build_ns") / 1e6, eng.stat("pos_len"), eng.stat("device_bytes") / 1e9))
-    tot_bp, tot_s, hits, hsps, raw_hsps, passes, per, parts = 0, 0.0, 0, 0, 0, 0, [], []
+    if args.no_cand_bins:
+        eng.set_flag("batch_cand_bins", 0)
+    tot_bp, tot_s, hits, hsps, raw_hsps, passes, per, parts, binned = 0, 0.0, 0, 0, 0, 0, [], [], []
     first = None
     for k in range(args.c4_chunks):
         t2 = time.time()
@@ -503,6 +509,7 @@ def c4_workload(args):
         raw_hsps += st["raw_hsps"]
         passes += eng.stat("batch_passes")
         per.append(round(dt, 3))
+        binned.append(bool(eng.get_flag("batch_cand_binned")))  # the chunk's last pass
         log("c4 chunk %d: %d bp (generated in %.1fs) in %.2fs = %.2f Mbp/s; %d hits, %d HSPs, %d device passes; "
             "seconds by part %s" % (k, len(q), gen, dt, len(q) / 1e6 / dt, st["hits"], st["hsps"],
                                     eng.stat("batch_passes"), parts[-1]))
@@ -532,6 +539,7 @@ def c4_workload(args):
                       "target_bp": bl, "query_bp": tot_bp, "chunks": args.c4_chunks, "max_hits": max_hits},
            "positions": eng.stat("pos_len"), "hits": hits, "hsps": hsps, "raw_hsps": raw_hsps,
            "device_passes": passes, "overflow_reruns": eng.stat("batch_reruns"), "seconds_per_chunk": per,
+           "candidates_binned": binned,
            "seconds_by_part": parts,
            "peak_device_bytes": eng.stat("peak_device_bytes"), "device_bytes_held": eng.stat("device_bytes"),
            "ref_records": eng.get_flag("ref_records") == 2,

@@ -330,6 +330,9 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         }
     if (off >= 0xFFFFFFFFull - 2 * PAD) return set_error(HSPZ_E_ARG, "batch arena beyond 4 GiB");
     uint32_t arena_len = (uint32_t)off;
+    // the coarse bound's candidates binned by reference position: every pass, or
+    // (the default) passes whose query plane stays in an XCD's L2
+    const bool bin_want = h->cand_bins == 2 || (h->cand_bins == 1 && arena_len <= CAND_BIN_MAX_QUERY);
     // encoded chunks are copied into a host image of the arena; raw chunks
     // (rc == NULL) are concatenated and encoded into it on the device
     bool any_encoded = false;
@@ -452,7 +455,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             if ((rc = reserve(h, B.gcnt, ((size_t)nkm + 1) * 4))) return rc;
             if (h->coarse_gate && (rc = reserve(h, B.cand, (size_t)bs->cap_cand * 16))) return rc;
             // the binned copy of the candidates, when the device has room for it
-            if (h->coarse_gate && h->cand_planes && h->lookahead && h->cand_bins &&
+            if (h->coarse_gate && h->cand_planes && h->lookahead && bin_want &&
                 B.cand2.cap < (size_t)bs->cap_cand * 16) {
                 size_t f = 0, t = 0;
                 HIP_TRY(hipMemGetInfo(&f, &t));
@@ -589,7 +592,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         const bool xplanes = xahead && h->cand_planes;
         // and binned by reference position, when the binned copy fits
         const uint32_t cblk = cand_scatter_blocks(h);
-        const bool binned = coarse && xplanes && h->cand_bins && B.cand2.p && B.cand2.cap >= (size_t)bs->cap_cand * 16 &&
+        const bool binned = coarse && xplanes && bin_want && B.cand2.p && B.cand2.cap >= (size_t)bs->cap_cand * 16 &&
                             B.cbins.p && B.cbins.cap >= (size_t)(2 + cblk) * CAND_BINS * 4;
         uint32_t cand_shift = 0;  // bins of 2^shift reference bases, at most CAND_BINS of them
         while ((h->ref_len >> cand_shift) >= CAND_BINS) cand_shift++;

@@ -117,7 +117,7 @@ struct hspz_engine {
     int coarse_gate = 1;               // HSPZ_FLAG_BATCH_COARSE: class-score bound ahead of the two-cell maps
     bool cand_planes = true;           // HSPZ_FLAG_BATCH_CAND_PLANES: candidates' windows from the planes
     bool lane_materialize = true;      // HSPZ_FLAG_BATCH_LANE_MATERIALIZE: a survivor per lane first
-    bool cand_bins = true;             // HSPZ_FLAG_BATCH_CAND_BINS: candidates binned by reference position
+    int cand_bins = 1;                 // HSPZ_FLAG_BATCH_CAND_BINS: 0 off, 1 small passes, 2 every pass
     int cand_binned = 0;               // whether the last batched pass binned them
                                        // (0 off, 1 paired form where the class scores fit, 2 single-hit form)
     int coarse_used = 0;               // the form the last batched pass ran (0: none)
@@ -1194,7 +1194,10 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
     case HSPZ_FLAG_DUMP_RAW: h->dump_raw = value != 0; break;
     case HSPZ_FLAG_BATCH_CAND_PLANES: h->cand_planes = value != 0; break;
     case HSPZ_FLAG_BATCH_LANE_MATERIALIZE: h->lane_materialize = value != 0; break;
-    case HSPZ_FLAG_BATCH_CAND_BINS: h->cand_bins = value != 0; break;
+    case HSPZ_FLAG_BATCH_CAND_BINS:
+        if (value < 0 || value > 2) return set_error(HSPZ_E_ARG, "candidate binning must be 0, 1 or 2");
+        h->cand_bins = value;
+        break;
     case HSPZ_FLAG_BATCH_COARSE:
         if (value < 0 || value > 3) return set_error(HSPZ_E_ARG, "coarse mode must be 0, 1, 2 or 3");
         h->coarse_gate = value;
@@ -1243,7 +1246,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_DUMP_RAW: *value = h->dump_raw ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_CAND_PLANES: *value = h->cand_planes ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_LANE_MATERIALIZE: *value = h->lane_materialize ? 1 : 0; break;
-    case HSPZ_FLAG_BATCH_CAND_BINS: *value = h->cand_bins ? 1 : 0; break;
+    case HSPZ_FLAG_BATCH_CAND_BINS: *value = h->cand_bins; break;
     case HSPZ_FLAG_BATCH_CAND_BINNED: *value = h->cand_binned; break;
     case HSPZ_FLAG_BATCH_COARSE_USED: *value = h->coarse_used; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);

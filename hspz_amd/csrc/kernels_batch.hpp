@@ -1956,6 +1956,11 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
 // shift that makes each bin's reference-plane segment a few hundred KiB, which
 // the L2 of an XCD holds while its waves work through the bin).
 constexpr uint32_t CAND_BINS = 1024;
+// Query bases of a pass up to which binning pays (a 3 MiB query plane): the
+// binned order splits the pairs of hits that share a query window, so the
+// pass's query plane has to stay in an XCD's L2 beside the bins' reference
+// lines (4 MiB).
+constexpr uint32_t CAND_BIN_MAX_QUERY = 12u << 20;
 // XCDs of the device: consecutive workgroups go to consecutive XCDs.
 constexpr uint32_t XCDS = 8;
 // Entries of a wave's chunk in interleaved shares of a binned list.

@@ -285,13 +285,14 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
 #define HSPZ_FLAG_BATCH_LANE_MATERIALIZE 17 /* 1 (default): the batched materializer extends one survivor per
                                           lane (sides longer than 2048 cells go to one wave per survivor);
                                           0: one wave per survivor throughout. The lists do not change */
-#define HSPZ_FLAG_BATCH_CAND_BINS 18   /* 1 (default): when the two-cell maps read the planes, the coarse
-                                          bound's candidates are first binned by reference position (1024
-                                          bins over the target) and each XCD's waves work through one
-                                          eighth of the bins together, so that the plane lines they read stay
-                                          in that XCD's L2; needs a second candidate list in device memory
-                                          and is skipped when that does not fit. 0: off. The lists do not
-                                          change */
+#define HSPZ_FLAG_BATCH_CAND_BINS 18   /* when the two-cell maps read the planes, bin the coarse bound's
+                                          candidates by reference position first (1024 bins over the
+                                          target; each XCD's waves work through one eighth of the bins
+                                          together, so that the plane lines they read stay in its L2).
+                                          1 (default): in passes of at most 12 Mi query bases (a query
+                                          plane that stays in L2 beside them); 2: every pass; 0: never.
+                                          Needs a second candidate list in device memory and is skipped
+                                          when that does not fit. The lists do not change */
 #define HSPZ_FLAG_BATCH_CAND_BINNED 19 /* read-only: 1 if the last batched pass binned its candidates */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS

@@ -62,6 +62,8 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
         assert eng.get_flag("batch_cand_binned") == 0
         for i, (a, g) in enumerate(zip(again, got)):
             assert same_intervals(a, g), "chunk %d differs with unbinned candidates" % i
+        with pytest.raises(E.HspzError):
+            eng.set_flag("batch_cand_bins", 3)
         eng.set_flag("batch_cand_bins", 1)
         # the materializer one wave per survivor throughout, instead of a lane per survivor first
         eng.set_flag("batch_lane_materialize", 0)

@@ -112,6 +112,15 @@ def test_c2_full_size(gpu_available):
         # 10 intervals x 40 batches x 2 strands, plus the few the indels' extra length adds
         assert 800 <= pst["batches"] <= 810 and pst["hits"] > 1e10 and bst["hits"] > pst["hits"]
         assert same_intervals(got[0], plain), "C2: batched differs from plain"
+        # its passes hold more query bases than candidate binning takes by default; forced, it
+        # bins them (multi-tile scatters over large lists) and the lists stay the same
+        eng.set_flag("batch_cand_bins", 2)
+        t1 = time.perf_counter()
+        forced, _ = eng.filter_batch([(fwd, None)])
+        _say("  batched with the candidates binned: %.2fs" % (time.perf_counter() - t1))
+        assert eng.get_flag("batch_cand_binned") == 1
+        assert same_intervals(forced[0], plain), "C2: batched with binned candidates differs from plain"
+        eng.set_flag("batch_cand_bins", 1)
         index, pos = eng.index()
     (w, _, h), = _oracle_all(t, ref_enc, index, pos, sub, [(sl_f, sl_r)])
     assert same_intervals(got[1], w) and E.hash_pass(got[1]) == h, "C2 slice differs from the oracle"
