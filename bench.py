@@ -112,8 +112,9 @@ def main():
                     help="step_cells,block_threads,claim_seeds,grid_blocks[,order[,lookahead[,coarse[,cand_planes]]]] "
                          "for the batched gate "
                          "(applied after the autotuner; use with --tune-mbp 0 to fix the configuration)")
-    ap.add_argument("--no-cand-bins", action="store_true",
-                    help="run the candidates' gate on the coarse bound's own order, unbinned")
+    ap.add_argument("--cand-bins", type=int, choices=[0, 1, 2], default=None,
+                    help="binning of the coarse bound's candidates by reference position: 0 never, "
+                         "1 passes of at most 12 Mi query bases (the engine's default), 2 every pass")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once and exit: the plain copy of known size "
                          "that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE readings")
@@ -226,8 +227,8 @@ def main():
                           for c in cells],
                 "chosen": [best[k] for k in cols]}
 
-    if args.no_cand_bins:
-        eng.set_flag("batch_cand_bins", 0)
+    if args.cand_bins is not None:
+        eng.set_flag("batch_cand_bins", args.cand_bins)
     if args.gate:
         vals = [int(x) for x in args.gate.split(",")]
         step, thr, claim, grid = vals[:4]
@@ -487,8 +488,8 @@ def c4_workload(args):
                    max_hits=max_hits, device_index="12of19")
     log("engine up in %.1fs: device index %.0f ms, %d positions, %.1f GB held"
         % (time.time() - t1, eng.stat("index_build_ns") / 1e6, eng.stat("pos_len"), eng.stat("device_bytes") / 1e9))
-    if args.no_cand_bins:
-        eng.set_flag("batch_cand_bins", 0)
+    if args.cand_bins is not None:
+        eng.set_flag("batch_cand_bins", args.cand_bins)
     tot_bp, tot_s, hits, hsps, raw_hsps, passes, per, parts, binned = 0, 0.0, 0, 0, 0, 0, [], [], []
     first = None
     for k in range(args.c4_chunks):
