@@ -80,6 +80,14 @@ def profiled_valu(order_used, coarse_used):
     return (float(m.group(1)), float(m.group(2))) if m else None
 
 
+def set_flags(eng, pairs):
+    """Engine flags from NAME=VALUE strings."""
+    for kv in pairs:
+        name, _, v = kv.partition("=")
+        eng.set_flag(name.strip(), int(v))
+        log("flag %s = %d" % (name.strip(), int(v)))
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -115,6 +123,9 @@ def main():
     ap.add_argument("--cand-bins", type=int, choices=[0, 1, 2], default=None,
                     help="binning of the coarse bound's candidates by reference position: 0 never, "
                          "1 passes of at most 12 Mi query bases (the engine's default), 2 every pass")
+    ap.add_argument("--flag", action="append", default=[], metavar="NAME=VALUE",
+                    help="set an engine flag (hspz_amd.engine.FLAG names, e.g. batch_max_positions=11500000) "
+                         "before the timed region; repeatable")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once and exit: the plain copy of known size "
                          "that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE readings")
@@ -229,6 +240,7 @@ def main():
 
     if args.cand_bins is not None:
         eng.set_flag("batch_cand_bins", args.cand_bins)
+    set_flags(eng, args.flag)
     if args.gate:
         vals = [int(x) for x in args.gate.split(",")]
         step, thr, claim, grid = vals[:4]
@@ -490,6 +502,7 @@ def c4_workload(args):
         % (time.time() - t1, eng.stat("index_build_ns") / 1e6, eng.stat("pos_len"), eng.stat("device_bytes") / 1e9))
     if args.cand_bins is not None:
         eng.set_flag("batch_cand_bins", args.cand_bins)
+    set_flags(eng, args.flag)
     tot_bp, tot_s, hits, hsps, raw_hsps, passes, per, parts, binned = 0, 0.0, 0, 0, 0, 0, [], [], []
     first = None
     for k in range(args.c4_chunks):

@@ -178,21 +178,23 @@ int ensure_ref_records(hspz_engine *h, bool want) {
     return 0;
 }
 
-// Stable radix sort of (key, u32 value) pairs on the engine's stream.
+// Stable radix sort of (key, u32 value) pairs on the engine's stream, over
+// the key's low `bits` bits (the rest are zero).
 template <class K>
-int dd_sort(hspz_engine *h, DevBuf &tmp, const K *ki, K *ko, const uint32_t *vi, uint32_t *vo, uint32_t n) {
+int dd_sort(hspz_engine *h, DevBuf &tmp, const K *ki, K *ko, const uint32_t *vi, uint32_t *vo, uint32_t n,
+            uint32_t bits = 8 * sizeof(K)) {
     size_t bytes = 0;
-    HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, ki, ko, vi, vo, n, 0, 8 * sizeof(K), h->stream));
+    HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, ki, ko, vi, vo, n, 0, bits, h->stream));
     if (int rc = reserve(h, tmp, bytes + 256)) return rc;
-    HIP_TRY(rocprim::radix_sort_pairs(tmp.p, bytes, ki, ko, vi, vo, n, 0, 8 * sizeof(K), h->stream));
+    HIP_TRY(rocprim::radix_sort_pairs(tmp.p, bytes, ki, ko, vi, vo, n, 0, bits, h->stream));
     return 0;
 }
 
 // dedup_and_order of every scope of the pass's kept list (ohsp/oscope, n
 // records) on the device; the result, grouped by ascending scope and in the
 // reference's output order within each scope, lands in mhsp/mscope and its
-// length in *kept.
-int device_dedup(hspz_engine *h, BatchBufs &B, uint32_t n, uint32_t *kept) {
+// length in *kept. Scopes are below nscopes.
+int device_dedup(hspz_engine *h, BatchBufs &B, uint32_t n, uint64_t nscopes, uint32_t *kept) {
     hspz_batch_state *bs = h->batch;
     int rc;
     if ((rc = reserve(h, bs->dd_key_a, (size_t)n * 8 + 8)) || (rc = reserve(h, bs->dd_key_b, (size_t)n * 8 + 8)) ||
@@ -207,13 +209,16 @@ int device_dedup(hspz_engine *h, BatchBufs &B, uint32_t n, uint32_t *kept) {
     uint32_t *ia = bs->dd_id_a.as<uint32_t>(), *ib = bs->dd_id_b.as<uint32_t>();
     uint32_t *k32a = (uint32_t *)ka, *k32b = (uint32_t *)kb;
     auto grid = [](uint32_t m) { return dim3(std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(m, 256), 4096))); };
+    // (scope << 32 | low word) keys: only the scope's bits above the low word
+    uint32_t sbits = 32;
+    while (sbits < 64 && (nscopes - 1) >> (sbits - 32)) sbits++;
     // order 1 over all records: (scope, diagonal, ref_start, len, -score)
     hipLaunchKernelGGL(k_dd_key_len_score, grid(n), dim3(256), 0, h->stream, hsp, nullptr, n, ka, ia);
     if ((rc = dd_sort<uint64_t>(h, bs->dd_tmp, ka, kb, ia, ib, n))) return rc;
     hipLaunchKernelGGL(k_dd_key_ref, grid(n), dim3(256), 0, h->stream, hsp, ib, n, k32a);
     if ((rc = dd_sort<uint32_t>(h, bs->dd_tmp, k32a, k32b, ib, ia, n))) return rc;
     hipLaunchKernelGGL(k_dd_key_scope, grid(n), dim3(256), 0, h->stream, hsp, scope, ia, n, 1, ka);
-    if ((rc = dd_sort<uint64_t>(h, bs->dd_tmp, ka, kb, ia, ib, n))) return rc;
+    if ((rc = dd_sort<uint64_t>(h, bs->dd_tmp, ka, kb, ia, ib, n, sbits))) return rc;
     // containment against the previous record, compaction
     hipLaunchKernelGGL(k_dd_mark, grid(n), dim3(256), 0, h->stream, hsp, scope, ib, n, bs->dd_keep.as<uint8_t>());
     hipLaunchKernelGGL(k_tile_count_u8, dim3(nt), dim3(TILE_THREADS), 0, h->stream, bs->dd_keep.as<uint8_t>(), n,
@@ -233,7 +238,7 @@ int device_dedup(hspz_engine *h, BatchBufs &B, uint32_t n, uint32_t *kept) {
     hipLaunchKernelGGL(k_dd_key_ref, grid(k), dim3(256), 0, h->stream, hsp, ia, k, k32a);
     if ((rc = dd_sort<uint32_t>(h, bs->dd_tmp, k32a, k32b, ia, ib, k))) return rc;
     hipLaunchKernelGGL(k_dd_key_scope, grid(k), dim3(256), 0, h->stream, hsp, scope, ib, k, 0, ka);
-    if ((rc = dd_sort<uint64_t>(h, bs->dd_tmp, ka, kb, ib, ia, k))) return rc;
+    if ((rc = dd_sort<uint64_t>(h, bs->dd_tmp, ka, kb, ib, ia, k, sbits))) return rc;
     hipLaunchKernelGGL(k_dd_gather, grid(k), dim3(256), 0, h->stream, hsp, scope, ia, k, B.mhsp.as<SegmentPair>(),
                        B.mscope.as<uint32_t>());
     HIP_TRY(hipGetLastError());
@@ -841,14 +846,10 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             t_begin = t_now;
             continue;
         }
-        std::vector<uint64_t> bh(nb);
-        HIP_TRY(hipMemcpyAsync(bh.data(), B.bhits.p, (size_t)nb * 8, hipMemcpyDeviceToHost, h->stream));
-        uint64_t hits = 0;
-        for (uint64_t x : bh) hits += x;
         uint32_t nk = hc[CTR_KEPT], nout = nk;
         const void *src_hsp = B.ohsp.p, *src_scope = B.oscope.p;
         if (nk && h->device_dedup) {
-            if ((rc = device_dedup(h, B, nk, &nout))) return rc;
+            if ((rc = device_dedup(h, B, nk, (uint64_t)nb * MAX_CHUNKS, &nout))) return rc;
             src_hsp = B.mhsp.p;
             src_scope = B.mscope.p;
             *deduped = true;
@@ -887,6 +888,12 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                 HIP_TRY(hipMemcpyAsync(scopes.data(), src_scope, (size_t)nout * 4, hipMemcpyDeviceToHost, h->stream));
             }
         }
+        // the batches' hit counts, behind the dedup on the stream (a copy to
+        // pageable memory returns once it has landed)
+        std::vector<uint64_t> bh(nb);
+        HIP_TRY(hipMemcpyAsync(bh.data(), B.bhits.p, (size_t)nb * 8, hipMemcpyDeviceToHost, h->stream));
+        uint64_t hits = 0;
+        for (uint64_t x : bh) hits += x;
         HIP_TRY(hipStreamSynchronize(h->stream));
         const auto t_read = std::chrono::steady_clock::now();
         auto ns = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
