@@ -640,6 +640,9 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
         h->batch_order_used = kmer_order ? 1 : 0;
         h->coarse_used = !coarse ? 0 : !pair ? 1 : items2 ? 3 : 2;
         h->cand_binned = binned ? 1 : 0;
+        // the coarse coefficients, with the long-gate routing off when the flag says so
+        CoarseCoef ccl = h->coarse_cc;
+        if (!h->coarse_long) ccl.lm = 32768;  // above every 16-bit sum
         auto enqueue_gate = [&]() {
         if (coarse) {
             // coarse bound over every hit, then the two-cell maps over its candidates
@@ -647,7 +650,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
                                B.skm.as<uint32_t>(), B.kvalo.as<uint32_t>(), B.gcnt.as<uint32_t>(), nkm,
                                h->index.as<uint32_t>(), h->pos.as<uint32_t>(), h->clean.as<uint32_t>(),
                                h->rrec.as<uint4>(), h->seed_size, (uint32_t)h->bgate_claim,
-                               (unsigned long long *)(ctr + CTR_WORK), h->coarse_cc, h->xdrop, h->hspthresh,
+                               (unsigned long long *)(ctr + CTR_WORK), ccl, h->xdrop, h->hspthresh,
                                B.cand.as<uint4>(), ctr + CTR_CAND, bs->cap_cand, B.slow.as<SurvRec>(),
                                ctr + CTR_SLOW, bs->cap_slow, ctr + CTR_ERR);
             if (binned) {  // the candidates in reference-position bins, for the plane reads
@@ -747,7 +750,7 @@ int batch_pass(hspz_engine *h, const Shape &s, const hspz_pass_params *p,
             uint64_t vals[] = {nb, npos, arena_len, bs->cap_surv, bs->cap_slow | (uint64_t)bs->cap_cand << 32,
                                cap_seeds | (uint64_t)coarse << 40 | (uint64_t)xgrid << 41, per, h->max_hits,
                                (uint64_t)grid | (uint64_t)h->bgate_claim << 32,
-                               (uint64_t)p->transitions | (uint64_t)h->lane_materialize << 8, nt, kt,
+                               (uint64_t)p->transitions | (uint64_t)h->lane_materialize << 8 | (uint64_t)h->coarse_long << 9, nt, kt,
                                cap_tiles, (uint64_t)h->bgate_step | (uint64_t)h->bgate_threads << 32,
                                (uint64_t)(uintptr_t)h->sub.p, s.kmer_size, (uint64_t)(uintptr_t)h->ref.p,
                                (uint64_t)kmer_order, ntk, (uint64_t)(uintptr_t)(const void *)kgate_fn,

@@ -123,7 +123,8 @@ struct hspz_engine {
     int coarse_used = 0;               // the form the last batched pass ran (0: none)
     bool coarse_fits16 = false;        // every |class score| <= COARSE_PAIR_MAX_CLASS
     bool coarse_pv = false;            // the set-bit count adds to the coarse bound's positive part
-    CoarseCoef coarse_cc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    CoarseCoef coarse_cc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool coarse_long = true;           // HSPZ_FLAG_BATCH_COARSE_LONG
     uint64_t last_cand = 0;            // candidates of the last batched pass
     int gate_blocks = 2048;
     // batched gate configuration (hspz_set_flag, hspz_autotune_batch)
@@ -694,7 +695,9 @@ int engine_create(int device, const hspz_engine_config *cfg, const Shape *build,
             // over the Gray-coded words: nd differing cells, np set xor bits,
             // nt = np - nd transitions, nv = 2 nd - np transversions, 16 - nd matches
             CoarseCoef &cc = h->coarse_cc;
-            cc = {16 * um, 2 * uv - ut - um, ut - uv, 16 * pm, 2 * pv - pt - pm, pt - pv, 0, 0, 0, 0};
+            cc = {16 * um, 2 * uv - ut - um, ut - uv, 16 * pm, 2 * pv - pt - pm, pt - pv, 0, 0, 0, 0, 0};
+            // ten best matches' worth of class-score sum on a side marks a likely long extension
+            cc.lm = std::min(std::max(10 * um, 1), 32767);
             // paired form: biases folding the constant 16 um into the counts where they exist
             cc.c0 = cc.s0;
             bool bias = false;
@@ -812,6 +815,7 @@ extern "C" int hspz_engine_share(hspz_engine *base, hspz_engine **out) {
     h->coarse_gate = base->coarse_gate;
     h->cand_planes = base->cand_planes;
     h->lane_materialize = base->lane_materialize;
+    h->coarse_long = base->coarse_long;
     h->cand_bins = base->cand_bins;
     h->batch_graph = base->batch_graph;
     h->device_dedup = base->device_dedup;
@@ -1194,6 +1198,7 @@ extern "C" int hspz_set_flag(hspz_engine *h, int which, int value) {
     case HSPZ_FLAG_DUMP_RAW: h->dump_raw = value != 0; break;
     case HSPZ_FLAG_BATCH_CAND_PLANES: h->cand_planes = value != 0; break;
     case HSPZ_FLAG_BATCH_LANE_MATERIALIZE: h->lane_materialize = value != 0; break;
+    case HSPZ_FLAG_BATCH_COARSE_LONG: h->coarse_long = value != 0; break;
     case HSPZ_FLAG_BATCH_CAND_BINS:
         if (value < 0 || value > 2) return set_error(HSPZ_E_ARG, "candidate binning must be 0, 1 or 2");
         h->cand_bins = value;
@@ -1248,6 +1253,7 @@ extern "C" int hspz_get_flag(const hspz_engine *h, int which, int *value) {
     case HSPZ_FLAG_BATCH_LANE_MATERIALIZE: *value = h->lane_materialize ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_CAND_BINS: *value = h->cand_bins; break;
     case HSPZ_FLAG_BATCH_CAND_BINNED: *value = h->cand_binned; break;
+    case HSPZ_FLAG_BATCH_COARSE_LONG: *value = h->coarse_long ? 1 : 0; break;
     case HSPZ_FLAG_BATCH_COARSE_USED: *value = h->coarse_used; break;
     default: return set_error(HSPZ_E_ARG, "unknown flag %d", which);
     }

@@ -1570,6 +1570,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
 struct CoarseCoef {
     int32_t s0, sd, sp, p0, pd, pp;
     int32_t c0, bd, bp, pc;  // paired form; c0 = s0, or 0 with biases
+    int32_t lm;              // paired form: class-score sum that sends an undropped candidate to the long gate
 };
 
 constexpr int CW_RIGHT = 4, CW_LEFT = 7;  // coarse windows: 64 right cells, 112 left cells
@@ -1942,10 +1943,17 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(2 * BT / 256
         const bool pa = has && oka && (!da || packed_keep(ta, thresh));
         const bool pb = hb && okb && (!db || packed_keep(tb, thresh));
         const uint32_t qv = qi | var << CAND_VAR_SHIFT;
-        src.sq.push(pa, make_uint4(Ra, e, qv, q3.x), err);
-        src.sq.push(pb, make_uint4(Rb, e + 1, qv, q3.x), err);
-        src.defer(has && !oka, q3.y + var, Ra, q3.x);
-        src.defer(hb && !okb, q3.y + var, Rb, q3.x);
+        // A candidate with a side whose u-process never dropped and a side
+        // whose class-score sum reached cc.lm is most likely a homologous
+        // extension that outlives the two-cell maps' windows (which would
+        // defer it after loading them): it goes straight to the deferred-hit
+        // gate, which decides every hit from its seed.
+        const s16x2 uu = __builtin_elementwise_max(Ur, Ul);
+        const bool la = pa && !da && uu.x >= cc.lm, lb = pb && !db && uu.y >= cc.lm;
+        src.sq.push(pa && !la, make_uint4(Ra, e, qv, q3.x), err);
+        src.sq.push(pb && !lb, make_uint4(Rb, e + 1, qv, q3.x), err);
+        src.defer((has && !oka) || la, q3.y + var, Ra, q3.x);
+        src.defer((hb && !okb) || lb, q3.y + var, Rb, q3.x);
         has = false;
     }
     src.finish();

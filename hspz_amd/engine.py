@@ -32,7 +32,7 @@ FLAG = {"batch_graph": 0, "batch_gate_blocks": 1, "batch_step_cells": 2, "batch_
         "ref_records": 8, "batch_lookahead": 9, "batch_list_cap": 10, "noalloc_tile_hits": 11,
         "batch_max_positions": 12, "batch_coarse": 13, "batch_coarse_used": 14, "dump_raw": 15,
         "batch_cand_planes": 16, "batch_lane_materialize": 17,
-        "batch_cand_bins": 18, "batch_cand_binned": 19}
+        "batch_cand_bins": 18, "batch_cand_binned": 19, "batch_coarse_long": 20}
 
 # Every symbol include/hspz.h declares (checked by tests/test_abi.py).
 EXPORTS = [

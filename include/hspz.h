@@ -294,6 +294,12 @@ int hspz_get_stat(const hspz_engine *h, int which, uint64_t *value);
                                           Needs a second candidate list in device memory and is skipped
                                           when that does not fit. The lists do not change */
 #define HSPZ_FLAG_BATCH_CAND_BINNED 19 /* read-only: 1 if the last batched pass binned its candidates */
+#define HSPZ_FLAG_BATCH_COARSE_LONG 20 /* 1 (default): in the paired coarse forms, a candidate with a side the
+                                          bound did not see drop and a side whose class-score sum reached
+                                          ten best matches goes straight to the deferred-hit gate instead
+                                          of the two-cell maps (a homologous extension that would outlive
+                                          their windows); 0: every candidate runs the two-cell maps. The
+                                          lists do not change */
 int hspz_set_flag(hspz_engine *h, int which, int value);
 /* Current value of a flag (the resolved grid for HSPZ_FLAG_BATCH_GATE_BLOCKS
  * once a batched pass has run). */

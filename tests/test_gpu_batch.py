@@ -65,6 +65,15 @@ def _run(target: bytes, query_blocks, max_hits=16711680, ambiguous="", noentropy
         with pytest.raises(E.HspzError):
             eng.set_flag("batch_cand_bins", 3)
         eng.set_flag("batch_cand_bins", 1)
+        # every candidate through the two-cell maps, none sent straight to the deferred-hit gate:
+        # the same lists, from at least as many candidates
+        routed = eng.stat("batch_candidates")
+        eng.set_flag("batch_coarse_long", 0)
+        again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
+        assert eng.stat("batch_candidates") >= routed
+        for i, (a, g) in enumerate(zip(again, got)):
+            assert same_intervals(a, g), "chunk %d differs without the long-gate routing" % i
+        eng.set_flag("batch_coarse_long", 1)
         # the materializer one wave per survivor throughout, instead of a lane per survivor first
         eng.set_flag("batch_lane_materialize", 0)
         again, _ = eng.filter_batch(chunks, wga_chunk=wga_chunk, interval_size=interval_size)
@@ -560,15 +569,18 @@ def test_batch_coarse_forms(gpu_available, name, acgt, xdrop, used):
                                       (2, 1, 0), (0, 0, 1)):
                 eng.set_flag("batch_coarse", coarse)
                 eng.set_flag("batch_cand_planes", planes)
+                d0 = eng.stat("gate_deferred")
                 got, st = eng.filter_batch(chunks)
                 assert eng.get_flag("batch_coarse_used") == u
                 assert eng.get_flag("batch_cand_binned") == (1 if coarse and planes else 0)
-                if coarse:  # homologous chunks: the bound keeps many hits, never more than there are
-                    assert 0 < eng.stat("batch_candidates") <= st["hits"]
+                if coarse:  # homologous chunks: the bound keeps many hits (to the two-cell maps or,
+                    # sides outliving the windows, the deferred-hit gate), never more than there are
+                    assert eng.stat("batch_candidates") <= st["hits"]
+                    assert eng.stat("batch_candidates") + eng.stat("gate_deferred") - d0 > 0
                 for i, (g, w) in enumerate(zip(got, want)):
                     assert same_intervals(g, w), "chunk %d differs in coarse mode %d, planes %d, %d threads" % (
                         i, coarse, planes, threads)
-        # the planes' candidates unbinned
+        # the planes' candidates unbinned, then without the long-gate routing
         eng.set_flag("batch_coarse", 1)
         eng.set_flag("batch_cand_planes", 1)
         eng.set_flag("batch_cand_bins", 0)
@@ -576,6 +588,18 @@ def test_batch_coarse_forms(gpu_available, name, acgt, xdrop, used):
         assert eng.get_flag("batch_cand_binned") == 0
         for i, (g, w) in enumerate(zip(got, want)):
             assert same_intervals(g, w), "chunk %d differs with unbinned candidates" % i
+        eng.set_flag("batch_cand_bins", 1)
+        routed = eng.stat("batch_candidates")
+        eng.set_flag("batch_coarse_long", 0)
+        got, _ = eng.filter_batch(chunks)
+        # homologous chunks: in the two-reference-positions form some candidates went straight to
+        # the deferred-hit gate (the other forms send none there)
+        if used == 3:
+            assert eng.stat("batch_candidates") > routed
+        else:
+            assert eng.stat("batch_candidates") == routed
+        for i, (g, w) in enumerate(zip(got, want)):
+            assert same_intervals(g, w), "chunk %d differs without the long-gate routing" % i
 
 
 def test_batch_staged_read(gpu_available):
